@@ -1,0 +1,324 @@
+"""Python handles on the two CPU checkers (test infrastructure only).
+
+``CareCpu``  - our C++ restatement (oracle/carecpu.cpp), always available once built.
+``CareRef``  - the reference engine compiled from its own sources (oracle/_ref),
+               available where it was built; it never reads the reference tree at
+               run time.
+
+Only tests, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use
+these, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CPU_LIB = os.path.join(HERE, "_build", "libcarecpu.so")
+REF_LIB = os.path.join(HERE, "_ref", "libcareref.so")
+
+_cpu = None
+_ref = None
+
+DEFAULTS = dict(kmerlength=20, num_hash_maps=48, estimated_coverage=30.0, use_quality_scores=0,
+                exclude_ambiguous=0, paired=0, min_overlap=30, max_error_rate=0.2,
+                min_overlap_ratio=0.3, estimated_errorrate=0.06, m_coverage=0.6,
+                paired_filter_threshold=0.06)
+
+
+class CpuOpts(ctypes.Structure):
+    _fields_ = [("kmerlength", ctypes.c_int), ("num_hash_maps", ctypes.c_int),
+                ("estimated_coverage", ctypes.c_float), ("use_quality_scores", ctypes.c_int),
+                ("exclude_ambiguous", ctypes.c_int), ("paired", ctypes.c_int),
+                ("min_overlap", ctypes.c_int), ("max_error_rate", ctypes.c_float),
+                ("min_overlap_ratio", ctypes.c_float), ("estimated_errorrate", ctypes.c_float),
+                ("m_coverage", ctypes.c_float), ("paired_filter_threshold", ctypes.c_float)]
+
+
+class RefOpts(ctypes.Structure):
+    _fields_ = [("kmerlength", ctypes.c_int), ("num_hash_maps", ctypes.c_int),
+                ("estimated_coverage", ctypes.c_float), ("threads", ctypes.c_int),
+                ("use_quality_scores", ctypes.c_int), ("exclude_ambiguous", ctypes.c_int),
+                ("paired", ctypes.c_int), ("min_overlap", ctypes.c_int),
+                ("max_error_rate", ctypes.c_float), ("min_overlap_ratio", ctypes.c_float),
+                ("estimated_errorrate", ctypes.c_float), ("m_coverage", ctypes.c_float),
+                ("paired_filter_threshold", ctypes.c_float), ("fixed_number_of_reads", ctypes.c_longlong)]
+
+
+class Trace(ctypes.Structure):
+    _fields_ = [("cap_candidates", ctypes.c_int), ("cap_columns", ctypes.c_int),
+                ("num_candidates", ctypes.c_int), ("candidate_ids", ctypes.c_void_p),
+                ("fwd_alignments", ctypes.c_void_p), ("rc_alignments", ctypes.c_void_p),
+                ("orientation", ctypes.c_void_p), ("is_paired", ctypes.c_void_p),
+                ("num_filtered", ctypes.c_int), ("num_final", ctypes.c_int),
+                ("final_ids", ctypes.c_void_p), ("final_shifts", ctypes.c_void_p),
+                ("final_weights", ctypes.c_void_p), ("refinement_rounds", ctypes.c_int),
+                ("num_columns", ctypes.c_int), ("anchor_column_begin", ctypes.c_int),
+                ("counts", ctypes.c_void_p), ("weights", ctypes.c_void_p), ("coverage", ctypes.c_void_p),
+                ("consensus", ctypes.c_void_p), ("support", ctypes.c_void_p),
+                ("orig_weights", ctypes.c_void_p), ("orig_coverage", ctypes.c_void_p),
+                ("avg_support", ctypes.c_float), ("min_support", ctypes.c_float),
+                ("min_coverage", ctypes.c_int), ("max_coverage", ctypes.c_int),
+                ("is_hq", ctypes.c_int), ("is_corrected", ctypes.c_int), ("corrected", ctypes.c_void_p)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "carecpu"], check=True)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data
+
+
+def cpu_lib():
+    global _cpu
+    if _cpu is None:
+        if not os.path.exists(CPU_LIB):
+            build()
+        lib = ctypes.CDLL(CPU_LIB)
+        vp, sz, u64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int
+        lib.carecpu_create.argtypes = [ctypes.POINTER(CpuOpts), u64, vp, sz, vp, vp, sz, vp, i32]
+        lib.carecpu_create.restype = vp
+        lib.carecpu_destroy.argtypes = [vp]
+        lib.carecpu_results_per_map_threshold.argtypes = [ctypes.c_float]
+        lib.carecpu_signature.argtypes = [vp, i32, i32, i32, vp]
+        lib.carecpu_signatures.argtypes = [vp, u64, u64, vp]
+        lib.carecpu_bucket.argtypes = [vp, i32, u64, ctypes.POINTER(ctypes.c_void_p)]
+        lib.carecpu_candidates.argtypes = [vp, ctypes.c_uint32, vp, i32]
+        lib.carecpu_align.argtypes = [vp, i32, vp, i32, i32, ctypes.c_float, ctypes.c_float, vp]
+        lib.carecpu_orientation.argtypes = [vp, vp, i32, i32, ctypes.c_float, i32, ctypes.c_float]
+        lib.carecpu_trace_anchor.argtypes = [vp, ctypes.c_uint32, ctypes.POINTER(Trace)]
+        lib.carecpu_correct.argtypes = [vp, u64, u64, i32]
+        lib.carecpu_correct.restype = sz
+        lib.carecpu_records_bytes.argtypes = [vp]
+        lib.carecpu_records_bytes.restype = sz
+        lib.carecpu_records_data.argtypes = [vp]
+        lib.carecpu_records_data.restype = vp
+        lib.carecpu_last_seconds.argtypes = [vp]
+        lib.carecpu_last_seconds.restype = ctypes.c_double
+        _cpu = lib
+    return _cpu
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_LIB)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_LIB)
+        vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
+        lib.ref_create.argtypes = [ctypes.POINTER(RefOpts), sz, vp, sz, vp, vp, sz, vp]
+        lib.ref_create.restype = vp
+        lib.ref_destroy.argtypes = [vp]
+        lib.ref_num_maps.argtypes = [vp]
+        lib.ref_correct.argtypes = [vp, i32, ctypes.c_longlong]
+        lib.ref_correct.restype = sz
+        lib.ref_correct_timed.argtypes = [vp, i32, ctypes.c_longlong, ctypes.POINTER(ctypes.c_longlong)]
+        lib.ref_correct_timed.restype = ctypes.c_double
+        lib.ref_last_seconds.argtypes = [vp]
+        lib.ref_last_seconds.restype = ctypes.c_double
+        lib.ref_records_bytes.argtypes = [vp]
+        lib.ref_records_bytes.restype = sz
+        lib.ref_records_data.argtypes = [vp]
+        lib.ref_records_data.restype = vp
+        lib.ref_hash.argtypes = [vp, i32, i32, i32, vp]
+        lib.ref_candidates.argtypes = [vp, ctypes.c_uint, vp, i32]
+        lib.ref_align.argtypes = [vp, i32, vp, i32, i32, ctypes.c_float, ctypes.c_float, vp]
+        lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
+        lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 9
+        _ref = lib
+    return _ref
+
+
+def _opts(cls, **kw):
+    o = cls()
+    d = dict(DEFAULTS)
+    d.update(kw)
+    for name, _ in cls._fields_:
+        if name in d:
+            setattr(o, name, d[name])
+    return o
+
+
+class CareCpu:
+    """Our CPU restatement over a ReadStorage."""
+
+    def __init__(self, rs, threads: int = 0, **opts):
+        self.lib = cpu_lib()
+        self.rs = rs
+        opts.setdefault("paired", int(rs.paired))
+        opts.setdefault("use_quality_scores", int(rs.quals is not None))
+        self.opts = _opts(CpuOpts, **opts)
+        q = rs.quals
+        self.h = self.lib.carecpu_create(ctypes.byref(self.opts), rs.n, rs.enc.ctypes.data, rs.pitch_ints,
+                                         rs.lengths.ctypes.data, _p(q), 0 if q is None else q.shape[1],
+                                         rs.ambiguous.ctypes.data, threads)
+        if not self.h:
+            raise RuntimeError("carecpu_create failed")
+
+    def close(self):
+        if self.h:
+            self.lib.carecpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def signatures(self, first=0, n=None):
+        n = self.rs.n - first if n is None else n
+        out = np.zeros((n, self.opts.num_hash_maps), np.uint64)
+        self.lib.carecpu_signatures(self.h, first, n, out.ctypes.data)
+        return out
+
+    def candidates(self, read_id):
+        cap = 48 * 1024
+        buf = np.zeros(cap, np.uint32)
+        n = self.lib.carecpu_candidates(self.h, read_id, buf.ctypes.data, cap)
+        return buf[:n].copy()
+
+    def trace(self, read_id, cap_candidates=8192, cap_columns=4096):
+        t = Trace()
+        bufs = dict(candidate_ids=np.zeros(cap_candidates, np.uint32),
+                    fwd_alignments=np.zeros((cap_candidates, 5), np.int32),
+                    rc_alignments=np.zeros((cap_candidates, 5), np.int32),
+                    orientation=np.zeros(cap_candidates, np.int8),
+                    is_paired=np.zeros(cap_candidates, np.uint8),
+                    final_ids=np.zeros(cap_candidates, np.uint32),
+                    final_shifts=np.zeros(cap_candidates, np.int32),
+                    final_weights=np.zeros(cap_candidates, np.float32),
+                    counts=np.zeros((cap_columns, 4), np.int32),
+                    weights=np.zeros((cap_columns, 4), np.float32),
+                    coverage=np.zeros(cap_columns, np.int32),
+                    consensus=np.zeros(cap_columns, np.uint8),
+                    support=np.zeros(cap_columns, np.float32),
+                    orig_weights=np.zeros(cap_columns, np.float32),
+                    orig_coverage=np.zeros(cap_columns, np.int32),
+                    corrected=np.zeros(int(self.rs.lengths[read_id]), np.uint8))
+        t.cap_candidates = cap_candidates
+        t.cap_columns = cap_columns
+        for k, v in bufs.items():
+            setattr(t, k, v.ctypes.data)
+        rc = self.lib.carecpu_trace_anchor(self.h, read_id, ctypes.byref(t))
+        if rc != 0:
+            raise RuntimeError(f"trace failed {rc}")
+        nc, nf, ncol = t.num_candidates, t.num_final, t.num_columns
+        res = {k: getattr(t, k) for k in ("num_candidates", "num_filtered", "num_final", "refinement_rounds",
+                                          "num_columns", "anchor_column_begin", "avg_support", "min_support",
+                                          "min_coverage", "max_coverage", "is_hq", "is_corrected")}
+        for k in ("candidate_ids", "fwd_alignments", "rc_alignments", "orientation", "is_paired"):
+            res[k] = bufs[k][:nc].copy()
+        for k in ("final_ids", "final_shifts", "final_weights"):
+            res[k] = bufs[k][:nf].copy()
+        for k in ("counts", "weights", "coverage", "consensus", "support", "orig_weights", "orig_coverage"):
+            res[k] = bufs[k][:ncol].copy()
+        res["corrected"] = bufs["corrected"].tobytes() if t.is_corrected else None
+        return res
+
+    def correct(self, first=0, n=None, threads=0):
+        n = self.rs.n - first if n is None else n
+        cnt = self.lib.carecpu_correct(self.h, first, n, threads)
+        nbytes = self.lib.carecpu_records_bytes(self.h)
+        data = ctypes.string_at(self.lib.carecpu_records_data(self.h), nbytes) if nbytes else b""
+        return cnt, data
+
+    def last_seconds(self):
+        return self.lib.carecpu_last_seconds(self.h)
+
+
+class CareRef:
+    """The reference engine (compiled from source) over the same ReadStorage."""
+
+    def __init__(self, rs, threads: int = 8, **opts):
+        self.lib = ref_lib()
+        self.rs = rs
+        opts.setdefault("paired", int(rs.paired))
+        opts.setdefault("use_quality_scores", int(rs.quals is not None))
+        opts.setdefault("threads", threads)
+        opts.setdefault("fixed_number_of_reads", 0)
+        self.opts = _opts(RefOpts, **opts)
+        q = rs.quals
+        self.h = self.lib.ref_create(ctypes.byref(self.opts), rs.n, rs.enc.ctypes.data, rs.pitch_ints,
+                                     rs.lengths.ctypes.data, _p(q), 0 if q is None else q.shape[1],
+                                     rs.ambiguous.ctypes.data)
+        if not self.h:
+            raise RuntimeError("ref_create failed")
+
+    def close(self):
+        if self.h:
+            self.lib.ref_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def correct(self, threads=8, num=0):
+        cnt = self.lib.ref_correct(self.h, threads, num)
+        nbytes = self.lib.ref_records_bytes(self.h)
+        data = ctypes.string_at(self.lib.ref_records_data(self.h), nbytes) if nbytes else b""
+        return cnt, data
+
+    def correct_timed(self, threads, num_anchors):
+        recs = ctypes.c_longlong(0)
+        secs = self.lib.ref_correct_timed(self.h, threads, num_anchors, ctypes.byref(recs))
+        return secs, recs.value
+
+    def candidates(self, read_id):
+        cap = 48 * 1024
+        buf = np.zeros(cap, np.uint32)
+        n = self.lib.ref_candidates(self.h, read_id, buf.ctypes.data, cap)
+        return buf[:n].copy()
+
+
+def ref_hash(row, length, k=20, nmaps=48):
+    out = np.zeros(nmaps, np.uint64)
+    row = np.ascontiguousarray(row, np.uint32)
+    ref_lib().ref_hash(row.ctypes.data, int(length), k, nmaps, out.ctypes.data)
+    return out
+
+
+def cpu_hash(row, length, k=20, nmaps=48):
+    out = np.zeros(nmaps, np.uint64)
+    row = np.ascontiguousarray(row, np.uint32)
+    cpu_lib().carecpu_signature(row.ctypes.data, int(length), k, nmaps, out.ctypes.data)
+    return out
+
+
+def ref_align(a, la, c, lc, min_overlap=30, max_error_rate=0.2, ratio=0.3):
+    out = np.zeros(5, np.int32)
+    a = np.ascontiguousarray(a, np.uint32)
+    c = np.ascontiguousarray(c, np.uint32)
+    ref_lib().ref_align(a.ctypes.data, la, c.ctypes.data, lc, min_overlap, max_error_rate, ratio, out.ctypes.data)
+    return out
+
+
+def cpu_align(a, la, c, lc, min_overlap=30, max_error_rate=0.2, ratio=0.3):
+    out = np.zeros(5, np.int32)
+    a = np.ascontiguousarray(a, np.uint32)
+    c = np.ascontiguousarray(c, np.uint32)
+    cpu_lib().carecpu_align(a.ctypes.data, la, c.ctypes.data, lc, min_overlap, max_error_rate, ratio, out.ctypes.data)
+    return out
+
+
+def parse_records(data: bytes):
+    """Splits serialized records into a list of (read_id, flags, payload)."""
+    out = []
+    p = 0
+    mv = memoryview(data)
+    while p < len(data):
+        rid = int.from_bytes(mv[p:p + 4], "little")
+        flags = int.from_bytes(mv[p + 4:p + 8], "little")
+        nb = flags & ((1 << 29) - 1)
+        out.append((rid, flags, bytes(mv[p + 8:p + 8 + nb])))
+        p += 8 + nb
+    return out

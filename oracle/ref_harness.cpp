@@ -1,0 +1,396 @@
+// Test infrastructure only: a thin C-ABI driver around the reference CARE engine,
+// compiled from the reference sources where they lie (see oracle/Makefile).
+// It is the checker for oracle/carecpu.cpp and for the HIP library; nothing in
+// the product links it.
+//
+// What it drives (all reference code, unmodified):
+//   * care::OrdinaryCpuMinhasher built by constructCpuMinhasherFromCpuReadStorage
+//     (src/cpuminhasherconstruction.cpp:168-212)
+//   * care::cpu::correct_cpu (src/correct_cpu.cpp:47-348) for end-to-end records
+//   * care::CpuErrorCorrector::process/processMulti (include/corrector.hpp:145,325)
+//     in an OpenMP loop without the 1 s ProgressThread, for timing
+//   * CPUSequenceHasher::hashInto (include/cpusequencehasher.hpp:88-125)
+//   * cpu::shd::cpuShiftedHammingDistancePopcount2Bit (include/cpu_alignment.hpp:411)
+//   * chooseBestAlignmentOrientation (include/alignmentorientation.hpp:14-92)
+//   * MultipleSequenceAlignment::build (src/msa.cpp:9-54)
+//
+// The only code of ours here is the CpuReadStorage implementation over plain
+// arrays (the interface is include/cpureadstorage.hpp:11-84) and the C-ABI glue.
+
+#include <config.hpp>
+#include <options.hpp>
+#include <cpureadstorage.hpp>
+#include <cpuminhasher.hpp>
+#include <cpuminhasherconstruction.hpp>
+#include <correct_cpu.hpp>
+#include <corrector.hpp>
+#include <cpusequencehasher.hpp>
+#include <cpu_alignment.hpp>
+#include <alignmentorientation.hpp>
+#include <msa.hpp>
+#include <correctedsequence.hpp>
+#include <serializedobjectstorage.hpp>
+#include <sequencehelpers.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include <omp.h>
+
+namespace {
+
+struct ArrayReadStorage : public care::CpuReadStorage {
+    std::size_t n = 0;
+    std::size_t pitchInts = 0;
+    std::size_t qualPitch = 0;
+    int minLen = 0;
+    int maxLen = 0;
+    bool paired = false;
+    bool hasQual = false;
+    std::vector<unsigned int> seqs;
+    std::vector<int> lens;
+    std::vector<char> quals;
+    std::vector<std::uint8_t> ambig;
+    std::vector<care::read_number> ambigIds;
+
+    void areSequencesAmbiguous(bool* result, const care::read_number* ids, int num) const override {
+        for(int i = 0; i < num; i++) result[i] = ambig[ids[i]] != 0;
+    }
+    void gatherSequences(unsigned int* out, std::size_t outPitch, const care::read_number* ids, int num) const override {
+        for(int i = 0; i < num; i++){
+            const std::size_t l = std::min(outPitch, pitchInts);
+            std::copy_n(seqs.data() + pitchInts * ids[i], l, out + outPitch * i);
+        }
+    }
+    void gatherContiguousSequences(unsigned int* out, std::size_t outPitch, care::read_number first, int num) const override {
+        for(int i = 0; i < num; i++){
+            const std::size_t l = std::min(outPitch, pitchInts);
+            std::copy_n(seqs.data() + pitchInts * (first + i), l, out + outPitch * i);
+        }
+    }
+    void gatherQualities(char* out, std::size_t outPitch, const care::read_number* ids, int num) const override {
+        for(int i = 0; i < num; i++){
+            const std::size_t l = std::min(outPitch, qualPitch);
+            std::copy_n(quals.data() + qualPitch * ids[i], l, out + outPitch * i);
+        }
+    }
+    void gatherEncodedQualities(unsigned int*, std::size_t, const care::read_number*, int) const override {
+        throw std::runtime_error("not used");
+    }
+    void gatherContiguousEncodedQualities(unsigned int*, std::size_t, care::read_number, int) const override {
+        throw std::runtime_error("not used");
+    }
+    void gatherSequenceLengths(int* out, const care::read_number* ids, int num) const override {
+        for(int i = 0; i < num; i++) out[i] = lens[ids[i]];
+    }
+    void getIdsOfAmbiguousReads(care::read_number* ids) const override {
+        std::copy(ambigIds.begin(), ambigIds.end(), ids);
+    }
+    std::int64_t getNumberOfReadsWithN() const override { return ambigIds.size(); }
+    MemoryUsage getMemoryInfo() const override {
+        MemoryUsage m{};
+        m.host = seqs.size() * 4 + lens.size() * 4 + quals.size();
+        return m;
+    }
+    care::read_number getNumberOfReads() const override { return n; }
+    bool canUseQualityScores() const override { return hasQual; }
+    int getSequenceLengthLowerBound() const override { return minLen; }
+    int getSequenceLengthUpperBound() const override { return maxLen; }
+    bool isPairedEnd() const override { return paired; }
+    int getQualityBits() const override { return 8; }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Plain-data option block; field meaning follows care::ProgramOptions
+// (include/options.hpp:36-123).
+struct ref_opts {
+    int kmerlength;
+    int num_hash_maps;
+    float estimated_coverage;
+    int threads;
+    int use_quality_scores;
+    int exclude_ambiguous;
+    int paired;
+    int min_overlap;
+    float max_error_rate;
+    float min_overlap_ratio;
+    float estimated_errorrate;
+    float m_coverage;
+    float paired_filter_threshold;
+    long long fixed_number_of_reads;  // 0 = all
+};
+
+struct ref_ctx {
+    ArrayReadStorage rs;
+    care::ProgramOptions po;
+    std::unique_ptr<care::CpuMinhasher> mh;
+    std::vector<std::uint8_t> records;
+    std::vector<std::size_t> recordOffsets;
+    double seconds = 0.0;
+};
+
+static void fill_program_options(care::ProgramOptions& po, const ref_opts* o, int maxLen){
+    po.kmerlength = o->kmerlength;
+    po.numHashFunctions = o->num_hash_maps;
+    po.estimatedCoverage = o->estimated_coverage;
+    po.threads = o->threads;
+    po.useQualityScores = o->use_quality_scores != 0;
+    po.excludeAmbiguousReads = o->exclude_ambiguous != 0;
+    po.pairType = o->paired ? care::SequencePairType::PairedEnd : care::SequencePairType::SingleEnd;
+    po.min_overlap = o->min_overlap;
+    po.maxErrorRate = o->max_error_rate;
+    po.min_overlap_ratio = o->min_overlap_ratio;
+    po.estimatedErrorrate = o->estimated_errorrate;
+    po.m_coverage = o->m_coverage;
+    po.pairedFilterThreshold = o->paired_filter_threshold;
+    po.batchsize = 16;  // forced by src/main_correct_cpu.cpp:105
+    po.fixedNumberOfReads = o->fixed_number_of_reads;
+    po.memoryTotalLimit = std::size_t(48) << 30;
+    po.memoryForHashtables = std::size_t(40) << 30;
+    po.hashtableLoadfactor = 0.8f;
+    po.tempdirectory = "/tmp";
+    po.maximum_sequence_length = maxLen;
+    po.correctionType = care::CorrectionType::Classic;
+    po.correctionTypeCands = care::CorrectionType::Classic;
+}
+
+// Creates the read storage and the reference minhasher (tables built by the
+// reference's own construction routine).
+ref_ctx* ref_create(const ref_opts* o, std::size_t n, const unsigned int* enc, std::size_t pitchInts,
+                    const int* lens, const char* quals, std::size_t qualPitch, const std::uint8_t* ambig){
+    auto* c = new ref_ctx;
+    c->rs.n = n;
+    c->rs.pitchInts = pitchInts;
+    c->rs.seqs.assign(enc, enc + n * pitchInts);
+    c->rs.lens.assign(lens, lens + n);
+    c->rs.minLen = n ? *std::min_element(lens, lens + n) : 0;
+    c->rs.maxLen = n ? *std::max_element(lens, lens + n) : 0;
+    c->rs.paired = o->paired != 0;
+    if(quals){
+        c->rs.hasQual = true;
+        c->rs.qualPitch = qualPitch;
+        c->rs.quals.assign(quals, quals + n * qualPitch);
+    }
+    c->rs.ambig.assign(n, 0);
+    if(ambig){
+        for(std::size_t i = 0; i < n; i++){
+            c->rs.ambig[i] = ambig[i];
+            if(ambig[i]) c->rs.ambigIds.push_back(i);
+        }
+    }
+    fill_program_options(c->po, o, c->rs.maxLen);
+    auto pair = care::constructCpuMinhasherFromCpuReadStorage(c->po, c->rs, care::CpuMinhasherType::Ordinary);
+    c->mh = std::move(pair.first);
+    return c;
+}
+
+void ref_destroy(ref_ctx* c){ delete c; }
+
+int ref_num_maps(ref_ctx* c){ return c->mh->getNumberOfMaps(); }
+
+// End to end through care::cpu::correct_cpu; records are then ordered by read id
+// (STEP3 does the same, src/dispatch_care_correct_cpu.cpp:409).
+std::size_t ref_correct(ref_ctx* c, int threads, long long fixedNumberOfReads){
+    c->po.threads = threads;
+    c->po.fixedNumberOfReads = fixedNumberOfReads;
+    auto t0 = std::chrono::steady_clock::now();
+    care::SerializedObjectStorage storage = care::cpu::correct_cpu(c->po, *c->mh, c->rs);
+    auto t1 = std::chrono::steady_clock::now();
+    c->seconds = std::chrono::duration<double>(t1 - t0).count();
+
+    const std::size_t num = storage.getNumElements();
+    std::vector<std::pair<care::read_number, std::size_t>> order(num);
+    for(std::size_t i = 0; i < num; i++){
+        order[i] = {care::EncodedTempCorrectedSequence::parseReadId(storage.getPointer(i)), i};
+    }
+    std::sort(order.begin(), order.end());
+    c->records.clear();
+    c->recordOffsets.clear();
+    for(const auto& p : order){
+        const std::uint8_t* ptr = storage.getPointer(p.second);
+        std::uint32_t flags;
+        std::memcpy(&flags, ptr + 4, 4);
+        const std::size_t bytes = 8 + (flags & ((1u << 29) - 1));
+        c->recordOffsets.push_back(c->records.size());
+        c->records.insert(c->records.end(), ptr, ptr + bytes);
+    }
+    c->recordOffsets.push_back(c->records.size());
+    return num;
+}
+
+// Timed run of the reference engine (CpuErrorCorrector::process / processMulti)
+// under an OpenMP loop over 16-read batches, like src/correct_cpu.cpp:134-299 but
+// without the ProgressThread whose 1 s sleep granularity distorts short timings.
+// Records are encoded as in src/correct_cpu.cpp:255-273 and counted, not kept.
+double ref_correct_timed(ref_ctx* c, int threads, long long numAnchors, long long* numRecords){
+    c->po.threads = threads;
+    omp_set_num_threads(threads);
+    const std::size_t total = numAnchors > 0 ? std::min<std::size_t>(numAnchors, c->rs.n) : c->rs.n;
+    care::ReadCorrectionFlags flags(c->rs.n);
+    care::ClfAgent clf_(c->po);
+    std::atomic<std::size_t> next{0};
+    std::atomic<long long> recs{0};
+    auto t0 = std::chrono::steady_clock::now();
+    #pragma omp parallel
+    {
+        care::ClfAgent clf = clf_;
+        const std::size_t pitch = SequenceHelpers::getEncodedNumInts2Bit(c->rs.maxLen);
+        const std::size_t qpitch = c->rs.maxLen;
+        care::CpuErrorCorrector ec(pitch, c->rs.maxLen, qpitch, c->po, *c->mh, c->rs, flags, clf);
+        const int bs = 16;
+        std::vector<care::read_number> ids(bs);
+        std::vector<unsigned int> enc(bs * pitch);
+        std::vector<char> q(bs * qpitch);
+        std::vector<int> lens(bs);
+        long long local = 0;
+        while(true){
+            const std::size_t b = next.fetch_add(bs);
+            if(b >= total) break;
+            const int num = int(std::min<std::size_t>(bs, total - b));
+            ids.resize(num);
+            std::iota(ids.begin(), ids.end(), care::read_number(b));
+            c->rs.gatherSequenceLengths(lens.data(), ids.data(), num);
+            c->rs.gatherSequences(enc.data(), pitch, ids.data(), num);
+            if(c->po.useQualityScores) c->rs.gatherQualities(q.data(), qpitch, ids.data(), num);
+            care::CorrectionOutput out;
+            if(c->rs.paired){
+                care::CpuErrorCorrectorMultiInput in{};
+                for(int i = 0; i < num; i++){
+                    in.anchorReadIds.push_back(ids[i]);
+                    in.encodedAnchors.push_back(enc.data() + i * pitch);
+                    in.anchorQualityscores.push_back(q.data() + i * qpitch);
+                    in.anchorLengths.push_back(lens[i]);
+                }
+                auto outs = ec.processMulti(in);
+                for(auto& o : outs){
+                    if(o.hasAnchorCorrection) out.anchorCorrections.emplace_back(std::move(o.anchorCorrection));
+                }
+            }else{
+                for(int i = 0; i < num; i++){
+                    care::CpuErrorCorrectorInput in;
+                    in.anchorReadId = ids[i];
+                    in.encodedAnchor = enc.data() + i * pitch;
+                    in.anchorQualityscores = q.data() + i * qpitch;
+                    in.anchorLength = lens[i];
+                    auto o = ec.process(in);
+                    if(o.hasAnchorCorrection) out.anchorCorrections.emplace_back(std::move(o.anchorCorrection));
+                }
+            }
+            care::EncodedCorrectionOutput encoded = out;
+            for(const auto& e : encoded.encodedAnchorCorrections){
+                if(!(e.isHQ() && e.useEdits() && e.getNumEdits() == 0)) local++;
+            }
+        }
+        recs += local;
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    if(numRecords) *numRecords = recs.load();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+double ref_last_seconds(ref_ctx* c){ return c->seconds; }
+std::size_t ref_records_bytes(ref_ctx* c){ return c->records.size(); }
+const std::uint8_t* ref_records_data(ref_ctx* c){ return c->records.data(); }
+
+// Minhash signature of one read (reference CPUSequenceHasher, map ids 0..nmaps-1).
+void ref_hash(const unsigned int* enc, int len, int k, int nmaps, std::uint64_t* out){
+    care::CPUSequenceHasher<care::kmer_type> hasher;
+    hasher.hashInto(out, enc, len, k, nmaps, 0);
+}
+
+// Candidate ids of one anchor exactly as CpuErrorCorrector::determineCandidateReadIds
+// computes them (include/corrector.hpp:571-650), using the reference minhasher.
+int ref_candidates(ref_ctx* c, unsigned int readId, unsigned int* out, int cap){
+    const std::size_t pitch = SequenceHelpers::getEncodedNumInts2Bit(c->rs.maxLen);
+    std::vector<unsigned int> enc(pitch);
+    c->rs.gatherSequences(enc.data(), pitch, &readId, 1);
+    int len = c->rs.lens[readId];
+    care::MinhasherHandle h = c->mh->makeMinhasherHandle();
+    int numPer = 0, total = 0;
+    std::vector<care::read_number> ids;
+    bool containsN = c->rs.ambig[readId] != 0;
+    if(!(c->po.excludeAmbiguousReads && containsN)){
+        c->mh->determineNumValues(h, enc.data(), pitch, &len, 1, &numPer, total);
+        ids.resize(total);
+        int offsets[2];
+        c->mh->retrieveValues(h, 1, total, ids.data(), &numPer, offsets);
+        std::sort(ids.begin(), ids.end());
+        ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+        auto it = std::lower_bound(ids.begin(), ids.end(), readId);
+        if(it != ids.end() && *it == readId) ids.erase(it);
+        if(c->po.excludeAmbiguousReads){
+            ids.erase(std::remove_if(ids.begin(), ids.end(), [&](care::read_number r){ return c->rs.ambig[r] != 0; }), ids.end());
+        }
+    }
+    c->mh->destroyHandle(h);
+    const int n = int(ids.size());
+    for(int i = 0; i < n && i < cap; i++) out[i] = ids[i];
+    return n;
+}
+
+// One shifted-Hamming alignment (reference routine) of a 2-bit anchor and a 2-bit
+// candidate; out = {score, overlap, shift, nOps, isValid}.
+void ref_align(const unsigned int* a, int la, const unsigned int* cnd, int lc, int min_overlap,
+               float maxErrorRate, float min_overlap_ratio, int* out){
+    care::cpu::shd::CpuAlignmentHandle handle;
+    care::cpu::shd::AlignmentResult r;
+    const int pitch = SequenceHelpers::getEncodedNumInts2Bit(lc);
+    care::cpu::shd::cpuShiftedHammingDistancePopcount2Bit(handle, &r, a, la, cnd, pitch, &lc, 1,
+                                                          min_overlap, maxErrorRate, min_overlap_ratio);
+    out[0] = r.score; out[1] = r.overlap; out[2] = r.shift; out[3] = r.nOps; out[4] = r.isValid ? 1 : 0;
+}
+
+int ref_orientation(int fo, int ro, int fn, int rn, int fv, int rv, int la, int lc,
+                    float min_overlap_ratio, int min_overlap, float estimatedErrorrate){
+    care::cpu::shd::AlignmentResult f{}, r{};
+    f.overlap = fo; f.nOps = fn; f.isValid = fv != 0;
+    r.overlap = ro; r.nOps = rn; r.isValid = rv != 0;
+    return int(care::chooseBestAlignmentOrientation(f, r, la, lc, min_overlap_ratio, min_overlap, estimatedErrorrate));
+}
+
+// Builds a reference MSA from decoded rows and returns column data.
+// out arrays sized >= ncols (query with ncols_out first using NULL outputs).
+int ref_msa(int useQ, int la, int nc, const char* anchor, const char* aq, const char* cands, int cpitch,
+            const char* cq, int qpitch, const int* clens, const int* shifts, const float* wf,
+            int* beginAnchor, int* countsACGT, float* weightsACGT, int* coverage, char* consensus,
+            float* support, float* origW, int* origCov){
+    care::cpu::QualityScoreConversion conv;
+    care::MultipleSequenceAlignment msa(&conv);
+    care::MultipleSequenceAlignment::InputData in;
+    in.useQualityScores = useQ != 0;
+    in.anchorLength = la;
+    in.nCandidates = nc;
+    in.candidatesPitch = cpitch;
+    in.candidateQualitiesPitch = qpitch;
+    in.anchor = anchor;
+    in.candidates = cands;
+    in.anchorQualities = aq;
+    in.candidateQualities = cq;
+    in.candidateLengths = clens;
+    in.candidateShifts = shifts;
+    in.candidateDefaultWeightFactors = wf;
+    msa.build(in);
+    const int ncol = msa.nColumns;
+    if(beginAnchor) *beginAnchor = msa.anchorColumnsBegin_incl;
+    if(countsACGT){
+        for(int i = 0; i < ncol; i++){
+            countsACGT[4*i+0] = msa.countsA[i]; countsACGT[4*i+1] = msa.countsC[i];
+            countsACGT[4*i+2] = msa.countsG[i]; countsACGT[4*i+3] = msa.countsT[i];
+            weightsACGT[4*i+0] = msa.weightsA[i]; weightsACGT[4*i+1] = msa.weightsC[i];
+            weightsACGT[4*i+2] = msa.weightsG[i]; weightsACGT[4*i+3] = msa.weightsT[i];
+            coverage[i] = msa.coverage[i]; consensus[i] = msa.consensus[i]; support[i] = msa.support[i];
+            origW[i] = msa.origWeights[i]; origCov[i] = msa.origCoverages[i];
+        }
+    }
+    return ncol;
+}
+
+}  // extern "C"
