@@ -1,0 +1,143 @@
+/* MI355X (gfx950) CARE error-correction engine: C-ABI.
+ *
+ * Drop-in boundary for the CARE correction stage behind RabbitQCPlus's
+ * --correctWithCare. The reference runs this stage as
+ *
+ *   SerializedObjectStorage care::cpu::correct_cpu(const ProgramOptions&,
+ *                                                  CpuMinhasher&, CpuReadStorage&)
+ *       (reference include/correct_cpu.hpp:17-21, called from
+ *        src/dispatch_care_correct_cpu.cpp:363)
+ *
+ * over a read storage (include/cpureadstorage.hpp:11-84) and a minhasher built from
+ * it (src/cpuminhasherconstruction.cpp:168-212). This library takes the same read
+ * storage as plain arrays, builds device-resident minhash tables, and produces the
+ * same serialized correction records (include/correctedsequence.hpp:151-387),
+ * ordered by read id, with the kernels running on one GPU.
+ *
+ * Conventions: every function returns CARE_OK (0) or a CARE_E* code; the message of
+ * the last failure on a context is available from care_hip_last_error(). No C++
+ * exception crosses this boundary. Calls on one context must not overlap; each
+ * context owns one device, one stream and all device memory it allocates.
+ */
+#ifndef CARE_HIP_H
+#define CARE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    CARE_OK = 0,
+    CARE_EINVAL = 1,      /* bad argument / unsupported configuration */
+    CARE_ESTATE = 2,      /* call out of order (e.g. correct before build_tables) */
+    CARE_EHIP = 3,        /* HIP runtime error */
+    CARE_ENOMEM = 4,      /* device or host allocation failed */
+    CARE_ENODEV = 5       /* no usable device */
+};
+
+/* Option block; names and defaults follow care::ProgramOptions
+ * (reference include/options.hpp:36-77). */
+typedef struct care_opts {
+    int kmerlength;                 /* k, 1..32 (reference default 20; 0 = auto: 20 if max length < 160 else 32,
+                                       src/dispatch_care_correct_cpu.cpp:283-297) */
+    int num_hash_maps;              /* 1..48 (--hashmaps) */
+    float estimated_coverage;       /* --coverage */
+    int use_quality_scores;         /* --useQualityScores */
+    int exclude_ambiguous;          /* --excludeAmbiguous */
+    int paired;                     /* --pairmode PE: reads 2i, 2i+1 are mates */
+    int min_overlap;                /* 30 */
+    float max_error_rate;           /* 0.2 */
+    float min_overlap_ratio;        /* 0.3 */
+    float estimated_errorrate;      /* 0.06 */
+    float m_coverage;               /* 0.6 */
+    float paired_filter_threshold;  /* 0.06 */
+    int batch_size;                 /* anchors per device batch, 0 = automatic */
+    int keep_intermediates;         /* 1: keep per-batch stage outputs for care_hip_inspect_* */
+} care_opts;
+
+typedef struct care_ctx care_ctx;
+
+/* Stage timings of the last care_hip_correct call, in milliseconds (device events). */
+typedef struct care_timings {
+    double total_ms;
+    double candidates_ms;     /* lookup + sort/unique/remove-self + compaction */
+    double align_ms;          /* shifted-Hamming alignment, both orientations */
+    double msa_ms;            /* filters + MSA + refinement + decision + record encoding */
+    double output_ms;         /* record compaction + device-to-host copy */
+    double align_kernel_ms;   /* sum of alignment kernel durations */
+    uint64_t anchors;
+    uint64_t candidates;      /* anchor/candidate pairs aligned */
+    uint64_t records;
+    uint64_t record_bytes;
+    uint64_t bucket_values;   /* values gathered from the tables */
+    uint64_t batches;
+} care_timings;
+
+void care_hip_default_opts(care_opts* o);
+
+int care_hip_create(const care_opts* opts, int device, care_ctx** out);
+void care_hip_destroy(care_ctx* ctx);
+const char* care_hip_last_error(const care_ctx* ctx);
+
+/* Read storage (reference CpuReadStorage contract, include/cpureadstorage.hpp:16-79):
+ * 2-bit rows of pitch_ints u32 (A=0 C=1 G=2 T=3, 16 bases per word, first base in
+ * the high bits), lengths, optional Phred+33 qualities in rows of qual_pitch bytes
+ * (bytes past a read's length are read as written), and one ambiguity flag per read
+ * (1 = the read contained non-ACGT bases before replacement). Copied to the device. */
+int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints,
+                       const int32_t* lengths, const uint8_t* quals, size_t qual_pitch,
+                       uint64_t num_reads, const uint8_t* ambiguous);
+
+/* Minhash signatures of all reads and the device-resident tables: per map, the
+ * ascending read ids of every signature value shared by 2..threshold reads,
+ * threshold = max(10, int(int(coverage) * 2.5f)) (reference
+ * include/minhasherlimit.hpp:10-16, include/groupbykey.hpp:99-250). */
+int care_hip_build_tables(care_ctx* ctx, int* maps_built);
+
+/* Corrects anchors [first, first+n) (replaces cpu::correct_cpu for that range).
+ * On success the serialized records are held by the context: num_records records,
+ * num_bytes bytes, in read-id order. For paired input first and n must be even. */
+int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_records, size_t* num_bytes);
+
+/* Copies the records of the last care_hip_correct call (cap must be >= num_bytes). */
+int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap);
+
+int care_hip_get_timings(const care_ctx* ctx, care_timings* out);
+
+/* ---- stage inspection (parity tests) ---- */
+
+/* Signatures of reads [first, first+n), row-major [n][num_hash_maps]. */
+int care_hip_inspect_signatures(care_ctx* ctx, uint64_t first, uint64_t n, uint64_t* out);
+
+/* With keep_intermediates=1, the stage outputs of the last batch of the last
+ * care_hip_correct call (anchors [batch_first, batch_first+batch_n)):
+ *   candidate_offsets [batch_n+1]      prefix sums of per-anchor candidate counts
+ *   candidate_ids     [total]          ascending per anchor, self and duplicates removed
+ *   alignments        [total][11]      fwd {score,overlap,shift,nOps,isValid},
+ *                                      rc {score,overlap,shift,nOps,isValid}, orientation
+ *   paired            [total]          pair flags (paired input)
+ *   anchor_info       [batch_n][12]    num_after_filters, num_final, num_columns,
+ *                                      anchor_column_begin, is_hq, is_corrected,
+ *                                      min_coverage, max_coverage, refinement_rounds,
+ *                                      avg_support (float bits), min_support (float bits),
+ *                                      record_bytes
+ * Any output pointer may be NULL. */
+int care_hip_inspect_batch(care_ctx* ctx, uint64_t* batch_first, uint64_t* batch_n, uint64_t* total,
+                           uint32_t* candidate_offsets, uint32_t* candidate_ids, int32_t* alignments,
+                           uint8_t* paired, int32_t* anchor_info);
+
+/* With keep_intermediates=1: final MSA of anchor `index` (0-based within the last
+ * batch): columns [num_columns] of counts[4], weights[4], coverage, consensus
+ * (ASCII), support; cap_columns bounds the writes. Returns CARE_EINVAL if absent. */
+int care_hip_inspect_msa(care_ctx* ctx, uint64_t index, int cap_columns, int* num_columns,
+                         int32_t* counts, float* weights, int32_t* coverage, uint8_t* consensus,
+                         float* support);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1569 @@
+// CARE per-anchor error correction on MI355X (gfx950): kernels and C-ABI.
+//
+// Pipeline per batch of anchors (each stage one kernel, all on the context stream):
+//   k_candidates   workgroup per anchor: 48 table lookups, gather of the bucket
+//                  values into LDS, bitonic sort, unique / remove-self / ambiguity
+//                  filter                         (reference include/corrector.hpp:571-650)
+//   k_compact      per-anchor candidate lists -> one dense list + anchor index
+//   k_pair_flags   mate-pair flags for paired input (include/util.hpp:1016-1200)
+//   k_align        one wavefront per (anchor, candidate): both orientations, lanes over
+//                  shifts on HiLo bit planes held in LDS, popcount mismatches, wave
+//                  argmin with scan-order tie-break (src/cpu_alignment.cpp:32-497,
+//                  include/alignmentorientation.hpp:14-92)
+//   k_msa          workgroup per anchor: orientation + mismatch-ratio filters, MSA
+//                  columns in LDS (thread per column, candidates accumulated in list
+//                  order so float sums match the reference), consensus, up to 5
+//                  refinement rounds, HQ decision, corrected anchor, record encoding
+//                  (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734)
+//   k_gather       variable-size records -> contiguous buffer in read-id order
+//
+// Set-up kernels: k_signatures (minhash of every read, include/cpusequencehasher.hpp:
+// 88-125), k_hilo (HiLo planes of every read and its reverse complement,
+// include/sequencehelpers.hpp:646-689), table build = radix sort + run-length
+// encode + bucket filter + open-addressing insert (include/groupbykey.hpp:99-250).
+//
+// Float arithmetic follows the reference expression by expression and is compiled
+// without contraction (-ffp-contract=off), so weights, supports and decisions are
+// bit-identical to the reference's x86 build.
+
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "care_hip.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr int kMaxMaps = 48;         // include/config.hpp:30
+constexpr int kBlock = 256;          // threads per workgroup of the per-anchor kernels
+constexpr int kWave = 64;
+constexpr int kMaxHiLoWords = 32;    // reads up to 1024 bases
+constexpr int kMaxSortEntries = 32768;  // per-anchor gathered bucket values held in LDS
+constexpr int kAnchorInfo = 12;
+
+struct Slot {            // one open-addressing slot of a per-map table (16 bytes)
+    uint64_t key;
+    uint32_t offset;     // first value of the bucket within the map's value array
+    uint32_t count;      // bucket size; 0 = empty slot
+};
+
+struct AlignBest {       // chosen alignment of a candidate (orientation 3 = none)
+    int shift;
+    int overlap;
+    int nOps;
+    int orientation;
+};
+
+// ---------------------------------------------------------------- device helpers
+
+__device__ __forceinline__ uint64_t fmix64(uint64_t x){
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+__device__ __forceinline__ int baseAt(const uint32_t* row, int i){
+    return int((row[i >> 4] >> (30 - 2 * (i & 15))) & 3u);
+}
+
+__device__ __forceinline__ bool feq(float l, float r){
+    const float absdiff = l - r < 0 ? -(l - r) : l - r;
+    return absdiff < 1e-5f;
+}
+__device__ __forceinline__ bool fleq(float l, float r){ return (l < r) || feq(l, r); }
+__device__ __forceinline__ bool fgeq(float l, float r){ return (l > r) || feq(l, r); }
+
+__device__ __forceinline__ uint64_t waveMinU64(uint64_t v){
+    #pragma unroll
+    for(int o = 32; o > 0; o >>= 1){
+        const uint64_t w = __shfl_xor(v, o, kWave);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// Block-wide exclusive sum over kBlock threads; `tmp` holds kBlock/kWave+1 ints.
+__device__ __forceinline__ int blockExclusiveSum(int v, int* tmp, int& total){
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    int x = v;
+    #pragma unroll
+    for(int o = 1; o < kWave; o <<= 1){
+        const int y = __shfl_up(x, o, kWave);
+        if(lane >= o) x += y;
+    }
+    if(lane == kWave - 1) tmp[wid] = x;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        int acc = 0;
+        for(int w = 0; w < kBlock / kWave; w++){ const int t = tmp[w]; tmp[w] = acc; acc += t; }
+        tmp[kBlock / kWave] = acc;
+    }
+    __syncthreads();
+    const int res = x - v + tmp[wid];
+    total = tmp[kBlock / kWave];
+    __syncthreads();
+    return res;
+}
+
+template<class Op>
+__device__ __forceinline__ int blockReduce(int v, int* tmp, Op op, int identity){
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    #pragma unroll
+    for(int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, kWave));
+    if(lane == 0) tmp[wid] = v;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        int acc = identity;
+        for(int w = 0; w < kBlock / kWave; w++) acc = op(acc, tmp[w]);
+        tmp[kBlock / kWave] = acc;
+    }
+    __syncthreads();
+    const int res = tmp[kBlock / kWave];
+    __syncthreads();
+    return res;
+}
+
+__device__ __forceinline__ float blockMinF(float v, float* tmp){
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    #pragma unroll
+    for(int o = 32; o > 0; o >>= 1){ const float w = __shfl_xor(v, o, kWave); v = w < v ? w : v; }
+    if(lane == 0) tmp[wid] = v;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        float acc = tmp[0];
+        for(int w = 1; w < kBlock / kWave; w++) acc = tmp[w] < acc ? tmp[w] : acc;
+        tmp[kBlock / kWave] = acc;
+    }
+    __syncthreads();
+    const float res = tmp[kBlock / kWave];
+    __syncthreads();
+    return res;
+}
+
+struct OpSum { __device__ int operator()(int a, int b) const { return a + b; } };
+struct OpMin { __device__ int operator()(int a, int b) const { return a < b ? a : b; } };
+struct OpMax { __device__ int operator()(int a, int b) const { return a > b ? a : b; } };
+struct OpOr  { __device__ int operator()(int a, int b) const { return a | b; } };
+
+// ---------------------------------------------------------------- set-up kernels
+
+// Minhash signatures: one thread per read, all maps; output map-major [map][read].
+// Canonical k-mer = min(forward, reverse complement) (include/sequencehelpers.hpp:844-923);
+// sig[m] = (min over k-mers of fmix64(kmer + m)) & mask (include/cpusequencehasher.hpp:88-125).
+__global__ void __launch_bounds__(256)
+k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n,
+             int k, int M, uint64_t* __restrict__ sig){
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(r >= n) return;
+    const uint64_t mask = ~0ull >> ((32 - k) * 2);
+    uint64_t mins[kMaxMaps];
+    #pragma unroll
+    for(int m = 0; m < kMaxMaps; m++) mins[m] = ~0ull;
+    const int L = len[r];
+    const uint32_t* row = enc + pitch * r;
+    if(L >= k){
+        uint64_t fwd = 0, rc = 0;
+        const int top = 2 * k - 2;
+        uint32_t word = 0;
+        for(int i = 0; i < L; i++){
+            if((i & 15) == 0) word = row[i >> 4];
+            const uint64_t b = (word >> (30 - 2 * (i & 15))) & 3u;
+            fwd = ((fwd << 2) | b) & mask;
+            rc = (rc >> 2) | ((3ull - b) << top);
+            if(i >= k - 1){
+                const uint64_t canon = fwd < rc ? fwd : rc;
+                #pragma unroll
+                for(int m = 0; m < kMaxMaps; m++){
+                    if(m < M){
+                        const uint64_t h = fmix64(canon + uint64_t(m));
+                        mins[m] = h < mins[m] ? h : mins[m];
+                    }
+                }
+            }
+        }
+    }
+    #pragma unroll
+    for(int m = 0; m < kMaxMaps; m++){
+        if(m < M) sig[uint64_t(m) * n + r] = mins[m] & mask;
+    }
+}
+
+// HiLo planes (include/sequencehelpers.hpp:646-689): bit 31-(i%32) of word i/32 holds
+// the high (hi plane) / low (lo plane) bit of base i. Layout per read:
+// [fwd hi W][fwd lo W][rc hi W][rc lo W].
+__global__ void __launch_bounds__(256)
+k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n, int W,
+       uint32_t* __restrict__ hilo){
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(t >= n * uint64_t(W)) return;
+    const uint64_t r = t / W;
+    const int w = int(t % W);
+    const int L = len[r];
+    const uint32_t* row = enc + pitch * r;
+    uint32_t fh = 0, fl = 0, rh = 0, rl = 0;
+    for(int j = 0; j < 32; j++){
+        const int i = w * 32 + j;
+        if(i < L){
+            const int b = baseAt(row, i);
+            const int rb = 3 - baseAt(row, L - 1 - i);
+            fh |= uint32_t(b >> 1) << (31 - j);
+            fl |= uint32_t(b & 1) << (31 - j);
+            rh |= uint32_t(rb >> 1) << (31 - j);
+            rl |= uint32_t(rb & 1) << (31 - j);
+        }
+    }
+    uint32_t* out = hilo + r * uint64_t(4 * W);
+    out[w] = fh;
+    out[W + w] = fl;
+    out[2 * W + w] = rh;
+    out[3 * W + w] = rl;
+}
+
+__global__ void k_iota(uint32_t* v, uint64_t n){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < n) v[i] = uint32_t(i);
+}
+
+// Buckets that survive the 2..threshold rule (include/groupbykey.hpp:176-215).
+__global__ void k_count_kept(const uint32_t* counts, const int* numRuns, int thr, uint32_t* kept){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= *numRuns) return;
+    const uint32_t c = counts[i];
+    if(c >= 2 && c <= uint32_t(thr)) atomicAdd(kept, 1u);
+}
+
+__global__ void k_insert(const uint64_t* keys, const uint32_t* counts, const uint32_t* offsets, const int* numRuns,
+                         int thr, Slot* slots, uint32_t capMask){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= *numRuns) return;
+    const uint32_t c = counts[i];
+    if(c < 2 || c > uint32_t(thr)) return;
+    const uint64_t key = keys[i];
+    uint32_t h = uint32_t(fmix64(key)) & capMask;
+    // keys are unique, so claiming an empty slot (count 0 -> c) is the only race
+    for(;;){
+        if(atomicCAS(&slots[h].count, 0u, c) == 0u){
+            slots[h].key = key;
+            slots[h].offset = offsets[i];
+            return;
+        }
+        h = (h + 1) & capMask;
+    }
+}
+
+// ---------------------------------------------------------------- candidates
+
+struct CandParams {
+    const uint64_t* sig; uint64_t n; int M; int k; int thr; int excludeAmbiguous;
+    const int* len; const uint8_t* ambig;
+    const Slot* const* slots; const uint32_t* caps; const uint32_t* values;
+    uint64_t b0; int B; int capPerAnchor;
+    uint32_t* raw; uint32_t* uniq; unsigned long long* gathered;
+};
+
+__device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint64_t key, uint32_t* offset){
+    uint32_t h = uint32_t(fmix64(key)) & (cap - 1);
+    for(;;){
+        const Slot s = slots[h];
+        if(s.count == 0) return 0;
+        if(s.key == key){ *offset = s.offset; return s.count; }
+        h = (h + 1) & (cap - 1);
+    }
+}
+
+// Workgroup per anchor: candidate read ids, ascending, unique, self removed
+// (include/corrector.hpp:571-650; include/ordinaryminhasher.hpp:135-221,437-449).
+__global__ void __launch_bounds__(kBlock)
+k_candidates(CandParams p){
+    extern __shared__ uint32_t buf[];
+    __shared__ uint32_t sOff[kMaxMaps];
+    __shared__ uint32_t sPre[kMaxMaps + 1];
+    __shared__ int tmp[kBlock / kWave + 1];
+    const int i = blockIdx.x;
+    const uint64_t a = p.b0 + i;
+    const int tid = threadIdx.x;
+    const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
+    if(tid < kMaxMaps){
+        uint32_t cnt = 0, off = 0;
+        if(valid && tid < p.M){
+            const uint64_t key = p.sig[uint64_t(tid) * p.n + a];
+            cnt = lookup(p.slots[tid], p.caps[tid], key, &off);
+            if(cnt > uint32_t(p.thr)) cnt = 0;
+        }
+        sOff[tid] = off;
+        sPre[tid + 1] = cnt;
+    }
+    __syncthreads();
+    if(tid == 0){
+        sPre[0] = 0;
+        for(int m = 0; m < kMaxMaps; m++) sPre[m + 1] += sPre[m];
+    }
+    __syncthreads();
+    const int T = int(sPre[kMaxMaps]);
+    int P = 1;
+    while(P < T) P <<= 1;
+    for(int idx = tid; idx < P; idx += kBlock){
+        uint32_t v = 0xFFFFFFFFu;
+        if(idx < T){
+            int lo = 0, hi = kMaxMaps - 1;       // map m with sPre[m] <= idx < sPre[m+1]
+            while(lo < hi){
+                const int mid = (lo + hi + 1) >> 1;
+                if(int(sPre[mid]) <= idx) lo = mid; else hi = mid - 1;
+            }
+            v = p.values[uint64_t(lo) * p.n + sOff[lo] + (idx - sPre[lo])];
+        }
+        buf[idx] = v;
+    }
+    __syncthreads();
+    for(int kk = 2; kk <= P; kk <<= 1){
+        for(int j = kk >> 1; j > 0; j >>= 1){
+            for(int idx = tid; idx < P; idx += kBlock){
+                const int ixj = idx ^ j;
+                if(ixj > idx){
+                    const uint32_t x = buf[idx], y = buf[ixj];
+                    const bool up = (idx & kk) == 0;
+                    if((x > y) == up){ buf[idx] = y; buf[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    uint32_t* out = p.raw + uint64_t(i) * p.capPerAnchor;
+    int running = 0;
+    for(int base = 0; base < T; base += kBlock){
+        const int idx = base + tid;
+        bool keep = false;
+        uint32_t v = 0;
+        if(idx < T){
+            v = buf[idx];
+            keep = (idx == 0 || buf[idx - 1] != v) && uint64_t(v) != a
+                   && !(p.excludeAmbiguous && p.ambig[v]);
+        }
+        int total;
+        const int pos = blockExclusiveSum(keep ? 1 : 0, tmp, total);
+        if(keep) out[running + pos] = v;
+        running += total;
+    }
+    if(tid == 0){
+        p.uniq[i] = uint32_t(running);
+        atomicAdd(p.gathered, (unsigned long long)T);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_compact(const uint32_t* raw, int capPerAnchor, const uint32_t* offsets, uint32_t* ids, uint32_t* anchorOf){
+    const int i = blockIdx.x;
+    const uint32_t beg = offsets[i];
+    const uint32_t cnt = offsets[i + 1] - beg;
+    const uint32_t* src = raw + uint64_t(i) * capPerAnchor;
+    for(uint32_t j = threadIdx.x; j < cnt; j += kBlock){
+        ids[beg + j] = src[j];
+        anchorOf[beg + j] = uint32_t(i);
+    }
+}
+
+// Pair flags of mates 2p, 2p+1: merge of the two sorted candidate lists
+// (include/util.hpp:1016-1141 via findPositionsOfPairedReadIds :1153-1200).
+__global__ void k_pair_flags(const uint32_t* offsets, const uint32_t* ids, uint8_t* flags, int numPairs){
+    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+    if(pidx >= numPairs) return;
+    const uint32_t b1 = offsets[2 * pidx], e1 = offsets[2 * pidx + 1], e2 = offsets[2 * pidx + 2];
+    const uint32_t* c1 = ids + b1;
+    const uint32_t* c2 = ids + e1;
+    uint8_t* f1 = flags + b1;
+    uint8_t* f2 = flags + e1;
+    const uint32_t n1 = e1 - b1, n2 = e2 - e1;
+    for(uint32_t q = 0; q < n1; q++) f1[q] = 0;
+    for(uint32_t q = 0; q < n2; q++) f2[q] = 0;
+    uint32_t i = 0, j = 0;
+    while(i < n1 && j < n2){
+        const int x1 = (i + 1 < n1 && c1[i] / 2 == c1[i + 1] / 2) ? 2 : 1;
+        const int x2 = (j + 1 < n2 && c2[j] / 2 == c2[j + 1] / 2) ? 2 : 1;
+        const bool same = c1[i] / 2 == c2[j] / 2;
+        if(x1 == 1 && x2 == 1){
+            if(same){
+                if(c1[i] != c2[j]){ f1[i] = 1; f2[j] = 1; }
+                i++; j++;
+            }else{
+                if(c1[i] < c2[j]) i++; else j++;
+            }
+        }else if(x1 == 2 && x2 == 2){
+            if(same){
+                f1[i] = 1; f2[j] = 1; f1[i + 1] = 1; f2[j + 1] = 1;
+                i += 2; j += 2;
+            }else{
+                if(c1[i] < c2[j]) i += 2; else j += 2;
+            }
+        }else if(x1 == 2 && x2 == 1){
+            if(same){
+                if(c1[i] == c2[j]){ i++; f1[i] = 1; f2[j] = 1; i++; j++; }
+                else { f1[i] = 1; f2[j] = 1; i += 2; j++; }
+            }else{
+                if(c1[i] < c2[j]) i += 2; else j++;
+            }
+        }else{
+            if(same){
+                if(c1[i] == c2[j]){ j++; f1[i] = 1; f2[j] = 1; i++; j++; }
+                else { f1[i] = 1; f2[j] = 1; i++; j += 2; }
+            }else{
+                if(c1[i] < c2[j]) i++; else j += 2;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- alignment
+
+struct AlignParams {
+    const uint32_t* hilo; int W; const int* len;
+    const uint32_t* ids; const uint32_t* anchorOf; uint64_t b0; uint32_t total;
+    int minOverlap; float maxErrorRate; float minOverlapRatio; float orientErrorRate;
+    AlignBest* best; int* dbg;   // dbg: [total][11] or null
+};
+
+struct ShdResult { int score, overlap, shift, nOps, valid; };
+
+// Best shift of one orientation (src/cpu_alignment.cpp:32-497). Lanes take shifts in
+// scan order (0..La-minoverlap, then -1..-(Lc-minoverlap)) 64 at a time. A shift is
+// admissible iff mismatches < int(float(overlap)*maxErrorRate); its score is
+// mismatches + La + Lc - 2*overlap; the reference's strict-'<' scan keeps the first
+// admissible shift of minimum score, and its early break / pruning never discards a
+// shift that could beat the running best, so argmin over (score, scan index) of the
+// admissible shifts gives the same (score, shift). The running best of earlier
+// chunks bounds each lane's mismatch count.
+__device__ ShdResult alignOne(const uint32_t* Ahi, const uint32_t* Alo, const uint32_t* Chi, const uint32_t* Clo,
+                              int La, int Lc, const AlignParams& p, int lane){
+    const int total = La + Lc;
+    const int minoverlap = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
+    const int F = max(La - minoverlap + 1, 0);
+    const int Bn = max(Lc - minoverlap, 0);
+    const int S = F + Bn;
+    int best = total;
+    int bestIdx = -1;
+    for(int base = 0; base < S; base += kWave){
+        const int t = base + lane;
+        int score = INT_MAX;
+        if(t < S){
+            const int s = t < F ? t : -(t - F + 1);
+            const int ov = s >= 0 ? min(La - s, Lc) : min(La, Lc + s);
+            const int maxErr = min(int(float(ov) * p.maxErrorRate), best - total + 2 * ov);
+            if(maxErr > 0 && ov > 0){
+                const uint32_t* Xh = s >= 0 ? Ahi : Chi;
+                const uint32_t* Xl = s >= 0 ? Alo : Clo;
+                const uint32_t* Yh = s >= 0 ? Chi : Ahi;
+                const uint32_t* Yl = s >= 0 ? Clo : Alo;
+                const int sh = s >= 0 ? s : -s;
+                const int ps = sh >> 5;
+                const int bs = sh & 31;
+                const int parts = (ov + 31) >> 5;
+                const int rem = (parts << 5) - ov;
+                int mism = 0;
+                for(int w = 0; w < parts && mism < maxErr; w++){
+                    uint32_t xh = Xh[w + ps];
+                    uint32_t xl = Xl[w + ps];
+                    if(bs){
+                        xh = (xh << bs) | (Xh[w + ps + 1] >> (32 - bs));
+                        xl = (xl << bs) | (Xl[w + ps + 1] >> (32 - bs));
+                    }
+                    uint32_t bits = (xh ^ Yh[w]) | (xl ^ Yl[w]);
+                    if(w == parts - 1 && rem) bits &= 0xFFFFFFFFu << rem;
+                    mism += __popc(bits);
+                }
+                if(mism < maxErr) score = mism + total - 2 * ov;
+            }
+        }
+        const uint64_t key = (uint64_t(uint32_t(score)) << 32) | uint32_t(t);
+        const uint64_t m = waveMinU64(key);
+        const int mscore = int(m >> 32);
+        if(mscore < best){
+            best = mscore;
+            bestIdx = int(uint32_t(m));
+        }
+    }
+    ShdResult r;
+    const int bestShift = bestIdx < 0 ? -Lc : (bestIdx < F ? bestIdx : -(bestIdx - F + 1));
+    r.valid = bestShift != -Lc;
+    const int cb = max(-bestShift, 0);
+    const int ce = min(Lc, La - bestShift);
+    r.overlap = ce - cb;
+    r.score = best;
+    r.shift = bestShift;
+    r.nOps = best - total + 2 * r.overlap;
+    return r;
+}
+
+// include/alignmentorientation.hpp:14-92 (error bound = estimatedErrorrate * 4.0f).
+__device__ int chooseOrientation(const ShdResult& f, const ShdResult& r, int La, const AlignParams& p){
+    const int ratioOverlap = int(La * p.minOverlapRatio);
+    const int minimumOverlap = ratioOverlap > p.minOverlap ? ratioOverlap : p.minOverlap;
+    const float maxErr = p.orientErrorRate;
+    int ret = 3;
+    if(f.valid && f.overlap >= minimumOverlap){
+        if(r.valid && r.overlap >= minimumOverlap){
+            const float ratio = (float)f.nOps / f.overlap;
+            const float rcratio = (float)r.nOps / r.overlap;
+            if(ratio < rcratio){
+                if(ratio < maxErr) ret = 1;
+            }else if(rcratio < ratio){
+                if(rcratio < maxErr) ret = 2;
+            }else{
+                if(ratio < maxErr) ret = f.overlap > r.overlap ? 1 : 2;
+            }
+        }else{
+            if((float)f.nOps / f.overlap < maxErr) ret = 1;
+        }
+    }else{
+        if(r.valid && r.overlap >= minimumOverlap){
+            if((float)r.nOps / r.overlap < maxErr) ret = 2;
+        }
+    }
+    return ret;
+}
+
+constexpr int kAlignWaves = kBlock / kWave;
+
+__global__ void __launch_bounds__(kBlock)
+k_align(AlignParams p){
+    __shared__ uint32_t planes[kAlignWaves][6][kMaxHiLoWords + 1];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint32_t c = blockIdx.x * kAlignWaves + wave;
+    const bool active = c < p.total;
+    uint32_t id = 0;
+    uint64_t a = 0;
+    if(active){
+        id = p.ids[c];
+        a = p.b0 + p.anchorOf[c];
+        const int W = p.W;
+        const uint32_t* ah = p.hilo + a * uint64_t(4 * W);
+        const uint32_t* ch = p.hilo + uint64_t(id) * uint64_t(4 * W);
+        for(int w = lane; w <= W; w += kWave){
+            const bool in = w < W;
+            planes[wave][0][w] = in ? ah[w] : 0u;
+            planes[wave][1][w] = in ? ah[W + w] : 0u;
+            planes[wave][2][w] = in ? ch[w] : 0u;
+            planes[wave][3][w] = in ? ch[W + w] : 0u;
+            planes[wave][4][w] = in ? ch[2 * W + w] : 0u;
+            planes[wave][5][w] = in ? ch[3 * W + w] : 0u;
+        }
+    }
+    __syncthreads();
+    if(!active) return;
+    const int La = p.len[a];
+    const int Lc = p.len[id];
+    const ShdResult f = alignOne(planes[wave][0], planes[wave][1], planes[wave][2], planes[wave][3], La, Lc, p, lane);
+    const ShdResult r = alignOne(planes[wave][0], planes[wave][1], planes[wave][4], planes[wave][5], La, Lc, p, lane);
+    const int flag = chooseOrientation(f, r, La, p);
+    if(lane == 0){
+        AlignBest b;
+        const ShdResult& s = flag == 2 ? r : f;
+        b.shift = s.shift;
+        b.overlap = s.overlap;
+        b.nOps = s.nOps;
+        b.orientation = flag;
+        p.best[c] = b;
+        if(p.dbg){
+            int* d = p.dbg + uint64_t(c) * 11;
+            d[0] = f.score; d[1] = f.overlap; d[2] = f.shift; d[3] = f.nOps; d[4] = f.valid;
+            d[5] = r.score; d[6] = r.overlap; d[7] = r.shift; d[8] = r.nOps; d[9] = r.valid;
+            d[10] = flag;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- MSA + decision
+
+struct MsaParams {
+    const uint32_t* enc; size_t pitch; const int* len; const uint8_t* qual; size_t qpitch; const uint8_t* ambig;
+    const float* qweight;
+    uint64_t b0;
+    const uint32_t* offsets; const uint32_t* ids; const AlignBest* best; const uint8_t* paired;
+    // per-candidate working lists (same indexing as ids)
+    uint32_t* kId; int* kShift; int* kLen; int* kNops; int* kOvl; float* kW; uint8_t* kOrient; uint8_t* kPaired;
+    int isPaired; int useQ;
+    float maxErrorRate; float ratio2, ratio3, ratio4; float goodCount; float pairedThreshold;
+    float avgThr, minThr, minCovThr; int pct; float pctHalf; int sigThr;
+    int maxCols; int maxLen;
+    uint8_t* slots; int slotBytes; uint32_t* recBytes;
+    int* info;                      // [B][kAnchorInfo] or null
+    uint8_t* msaDbg; int dbgStride; // per anchor: 1 int ncol + cols*(4 int + 4 float + int + float + 1 byte)
+    int* error;
+};
+
+struct MsaShared {
+    int tmp[kBlock / kWave + 1];
+    float ftmp[kBlock / kWave + 1];
+    int n;          // working candidate count
+    int col;
+    int sigBase;
+    int origBase;
+    int ncol;
+    int ab;
+};
+
+__device__ __forceinline__ int orientedBase(const MsaParams& p, uint32_t id, int lc, int orient, int pos){
+    const uint32_t* row = p.enc + p.pitch * id;
+    return orient == 1 ? baseAt(row, pos) : 3 - baseAt(row, lc - 1 - pos);
+}
+
+__device__ __forceinline__ float orientedQualityWeight(const MsaParams& p, uint32_t id, int orient, int pos){
+    const uint8_t* q = p.qual + p.qpitch * id;
+    // reverse-complement candidates read the pitch-wide quality row reversed
+    // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
+    return p.qweight[orient == 1 ? q[pos] : q[p.qpitch - 1 - pos]];
+}
+
+// Builds the MSA of the anchor and the first `nk` working candidates in LDS
+// (src/msa.cpp:9-173). Returns false on an impossible column count.
+__device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, const uint32_t* arow, int beg, int nk,
+                         int* cnt, float* wgt, int* cov, float* sup, uint8_t* cons, int* origCov){
+    int lo = 0, hi = La;
+    for(int j = threadIdx.x; j < nk; j += kBlock){
+        const int s = p.kShift[beg + j];
+        lo = min(lo, s);
+        hi = max(hi, s + p.kLen[beg + j]);
+    }
+    lo = blockReduce(lo, sh.tmp, OpMin(), INT_MAX);
+    hi = blockReduce(hi, sh.tmp, OpMax(), INT_MIN);
+    const int ncol = hi - lo;
+    const int ab = max(-lo, 0);
+    if(ncol > p.maxCols) return false;
+    if(threadIdx.x == 0){ sh.ncol = ncol; sh.ab = ab; }
+    const uint8_t* aq = p.useQ ? p.qual + p.qpitch * a : nullptr;
+    for(int c = threadIdx.x; c < ncol; c += kBlock){
+        int n0 = 0, n1 = 0, n2 = 0, n3 = 0, cv = 0;
+        float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;
+        const int pa = c - ab;
+        int abase = -1;
+        if(pa >= 0 && pa < La){
+            abase = baseAt(arow, pa);
+            const float w = 1.0f * (p.useQ ? p.qweight[aq[pa]] : 1.0f);
+            if(abase == 0){ n0++; w0 += w; } else if(abase == 1){ n1++; w1 += w; }
+            else if(abase == 2){ n2++; w2 += w; } else { n3++; w3 += w; }
+            cv++;
+        }
+        for(int j = 0; j < nk; j++){
+            const int s = p.kShift[beg + j];
+            const int lc = p.kLen[beg + j];
+            const int pos = pa - s;
+            if(pos >= 0 && pos < lc){
+                const uint32_t id = p.kId[beg + j];
+                const int orient = p.kOrient[beg + j];
+                const int b = orientedBase(p, id, lc, orient, pos);
+                const float w = p.kW[beg + j] * (p.useQ ? orientedQualityWeight(p, id, orient, pos) : 1.0f);
+                if(b == 0){ n0++; w0 += w; } else if(b == 1){ n1++; w1 += w; }
+                else if(b == 2){ n2++; w2 += w; } else { n3++; w3 += w; }
+                cv++;
+            }
+        }
+        cnt[4 * c + 0] = n0; cnt[4 * c + 1] = n1; cnt[4 * c + 2] = n2; cnt[4 * c + 3] = n3;
+        wgt[4 * c + 0] = w0; wgt[4 * c + 1] = w1; wgt[4 * c + 2] = w2; wgt[4 * c + 3] = w3;
+        cov[c] = cv;
+        // findConsensus (src/msa.cpp:135-157)
+        int cb = 0;
+        float cw = w0;
+        if(w1 > cw){ cb = 1; cw = w1; }
+        if(w2 > cw){ cb = 2; cw = w2; }
+        if(w3 > cw){ cb = 3; cw = w3; }
+        cons[c] = uint8_t(cb);
+        const float colW = w0 + w1 + w2 + w3;
+        sup[c] = cw / colW;
+        // findOrigWeightAndCoverage (src/msa.cpp:159-173)
+        if(abase >= 0) origCov[c] = abase == 0 ? n0 : abase == 1 ? n1 : abase == 2 ? n2 : n3;
+    }
+    __syncthreads();
+    return true;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_msa(MsaParams p){
+    extern __shared__ uint8_t smem[];
+    __shared__ MsaShared sh;
+    const int C = p.maxCols;
+    int* cnt = reinterpret_cast<int*>(smem);
+    float* wgt = reinterpret_cast<float*>(cnt + 4 * C);
+    int* cov = reinterpret_cast<int*>(wgt + 4 * C);
+    float* sup = reinterpret_cast<float*>(cov + C);
+    int* origCov = reinterpret_cast<int*>(sup + C);
+    uint8_t* cons = reinterpret_cast<uint8_t*>(origCov + C);
+    uint8_t* corr = cons + C;
+
+    const int i = blockIdx.x;
+    const uint64_t a = p.b0 + i;
+    const int tid = threadIdx.x;
+    const int La = p.len[a];
+    const uint32_t* arow = p.enc + p.pitch * a;
+    const int beg = int(p.offsets[i]);
+    const int n0 = int(p.offsets[i + 1]) - beg;
+    int* info = p.info ? p.info + size_t(i) * kAnchorInfo : nullptr;
+    if(tid == 0){
+        p.recBytes[i] = 0;
+        if(info) for(int q = 0; q < kAnchorInfo; q++) info[q] = 0;
+    }
+    if(n0 == 0) return;
+
+    // 1) drop candidates without an orientation (include/corrector.hpp:982-1050)
+    int n1 = 0;
+    for(int base = 0; base < n0; base += kBlock){
+        const int j = base + tid;
+        AlignBest b{};
+        bool keep = false;
+        if(j < n0){ b = p.best[beg + j]; keep = b.orientation != 3; }
+        int total;
+        const int pos = blockExclusiveSum(keep ? 1 : 0, sh.tmp, total);
+        if(keep){
+            const int d = beg + n1 + pos;
+            const uint32_t id = p.ids[beg + j];
+            p.kId[d] = id;
+            p.kShift[d] = b.shift;
+            p.kLen[d] = p.len[id];
+            p.kNops[d] = b.nOps;
+            p.kOvl[d] = b.overlap;
+            p.kOrient[d] = uint8_t(b.orientation);
+            p.kPaired[d] = p.paired ? p.paired[beg + j] : 0;
+        }
+        n1 += total;
+    }
+    if(n1 == 0) return;
+
+    // 2) mismatch-ratio filter (include/corrector.hpp:1053-1149; paired :1151-1211)
+    float threshold = -std::numeric_limits<float>::max();
+    if(!p.isPaired){
+        int c0 = 0, c1 = 0, c2 = 0;
+        for(int j = tid; j < n1; j += kBlock){
+            const float ratio = float(p.kNops[beg + j]) / float(p.kOvl[beg + j]);
+            c0 += ratio < p.ratio2;
+            c1 += ratio < p.ratio3;
+            c2 += ratio < p.ratio4;
+        }
+        c0 = blockReduce(c0, sh.tmp, OpSum(), 0);
+        c1 = blockReduce(c1, sh.tmp, OpSum(), 0);
+        c2 = blockReduce(c2, sh.tmp, OpSum(), 0);
+        if(c0 > 0 || c1 > 0 || c2 > 0){
+            if(c0 >= p.goodCount) threshold = p.ratio2;
+            else if(c1 >= p.goodCount) threshold = p.ratio3;
+            else if(c2 >= p.goodCount) threshold = p.ratio4;
+        }
+    }
+    int n2 = 0;
+    for(int base = 0; base < n1; base += kBlock){
+        const int j = base + tid;
+        bool keep = false;
+        uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0;
+        if(j < n1){
+            id = p.kId[beg + j]; s = p.kShift[beg + j]; l = p.kLen[beg + j]; no = p.kNops[beg + j];
+            ov = p.kOvl[beg + j]; o = p.kOrient[beg + j]; pr = p.kPaired[beg + j];
+            const float ratio = float(no) / float(ov);
+            keep = p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold);
+        }
+        int total;
+        const int pos = blockExclusiveSum(keep ? 1 : 0, sh.tmp, total);   // all reads of this chunk precede its writes
+        if(keep){
+            const int d = beg + n2 + pos;
+            p.kId[d] = id; p.kShift[d] = s; p.kLen[d] = l; p.kNops[d] = no; p.kOvl[d] = ov;
+            p.kOrient[d] = o; p.kPaired[d] = pr;
+            // overlap weight (include/hostdevicefunctions.cuh:93-97)
+            p.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
+        }
+        n2 += total;
+        __syncthreads();
+    }
+    if(info && tid == 0) info[0] = n2;
+    if(n2 == 0) return;
+
+    // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
+    int nk = n2;
+    if(!buildMsa(p, sh, a, La, arow, beg, nk, cnt, wgt, cov, sup, cons, origCov)){
+        if(tid == 0) atomicAdd(p.error, 1);
+        return;
+    }
+    int rounds = 0;
+    for(int it = 0; it < 5; it++){
+        const int ab = sh.ab;
+        int mis = 0, colCand = INT_MAX;
+        for(int q = tid; q < La; q += kBlock){
+            const int c = ab + q;
+            const int ci = cons[c];
+            if(ci != baseAt(arow, q)) mis = 1;
+            int sig = -1;
+            for(int b = 0; b < 4; b++){
+                if(b != ci && p.sigThr <= cnt[4 * c + b]) sig = b;
+            }
+            if(sig != -1) colCand = min(colCand, c);
+        }
+        mis = blockReduce(mis, sh.tmp, OpOr(), 0);
+        if(!mis) break;
+        const int col = blockReduce(colCand, sh.tmp, OpMin(), INT_MAX);
+        if(col == INT_MAX) break;
+        if(tid == 0){
+            const int ci = cons[col];
+            int sig = -1;
+            for(int b = 0; b < 4; b++){
+                if(b != ci && p.sigThr <= cnt[4 * col + b]) sig = b;
+            }
+            sh.sigBase = sig;
+            sh.origBase = baseAt(arow, col - ab);
+        }
+        __syncthreads();
+        const int sigBase = sh.sigBase;
+        const bool keepMatching = sh.origBase == sigBase;
+        int anyRemove = 0, veryGood = 0;
+        for(int j = tid; j < nk; j += kBlock){
+            const int rb = ab + p.kShift[beg + j];
+            const int re = rb + p.kLen[beg + j];
+            if(col >= rb && col < re){
+                const int b = orientedBase(p, p.kId[beg + j], p.kLen[beg + j], p.kOrient[beg + j], col - rb);
+                const bool rem = keepMatching ? (b != sigBase) : (b == sigBase);
+                if(rem){
+                    anyRemove = 1;
+                    if(p.kW[beg + j] >= 0.90f) veryGood = 1;
+                }
+            }
+        }
+        anyRemove = blockReduce(anyRemove, sh.tmp, OpOr(), 0);
+        veryGood = blockReduce(veryGood, sh.tmp, OpOr(), 0);
+        // a round that removes nothing leaves the MSA, and hence every later round, unchanged
+        if(!anyRemove || veryGood) break;
+        int nn = 0;
+        for(int base = 0; base < nk; base += kBlock){
+            const int j = base + tid;
+            bool keep = false;
+            uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0; float w = 0.0f;
+            if(j < nk){
+                id = p.kId[beg + j]; s = p.kShift[beg + j]; l = p.kLen[beg + j]; no = p.kNops[beg + j];
+                ov = p.kOvl[beg + j]; o = p.kOrient[beg + j]; pr = p.kPaired[beg + j]; w = p.kW[beg + j];
+                const int rb = ab + s;
+                keep = true;
+                if(col >= rb && col < rb + l){
+                    const int b = orientedBase(p, id, l, o, col - rb);
+                    keep = !(keepMatching ? (b != sigBase) : (b == sigBase));
+                }
+            }
+            int total;
+            const int pos = blockExclusiveSum(keep ? 1 : 0, sh.tmp, total);
+            if(keep){
+                const int d = beg + nn + pos;
+                p.kId[d] = id; p.kShift[d] = s; p.kLen[d] = l; p.kNops[d] = no; p.kOvl[d] = ov;
+                p.kOrient[d] = o; p.kPaired[d] = pr; p.kW[d] = w;
+            }
+            nn += total;
+            __syncthreads();
+        }
+        nk = nn;
+        rounds++;
+        if(!buildMsa(p, sh, a, La, arow, beg, nk, cnt, wgt, cov, sup, cons, origCov)){
+            if(tid == 0) atomicAdd(p.error, 1);
+            return;
+        }
+    }
+    const int ab = sh.ab;
+    const int ncol = sh.ncol;
+    if(p.msaDbg){
+        uint8_t* d = p.msaDbg + size_t(i) * p.dbgStride;
+        int* dn = reinterpret_cast<int*>(d);
+        int* dc = dn + 1;
+        float* dw = reinterpret_cast<float*>(dc + 4 * C);
+        int* dv = reinterpret_cast<int*>(dw + 4 * C);
+        float* ds = reinterpret_cast<float*>(dv + C);
+        uint8_t* dk = reinterpret_cast<uint8_t*>(ds + C);
+        if(tid == 0) dn[0] = ncol;
+        for(int c = tid; c < ncol; c += kBlock){
+            for(int b = 0; b < 4; b++){ dc[4 * c + b] = cnt[4 * c + b]; dw[4 * c + b] = wgt[4 * c + b]; }
+            dv[c] = cov[c]; ds[c] = sup[c]; dk[c] = "ACGT"[cons[c]];
+        }
+    }
+    if(info && tid == 0){ info[1] = nk; info[2] = ncol; info[3] = ab; info[8] = rounds; }
+    // msa.nCandidates == 0: cannot be corrected (src/msa.cpp:388-394)
+    if(nk == 0) return;
+
+    // 4) properties and decision (src/msa.cpp:304-453)
+    float ms = INFINITY;
+    int mincov = INT_MAX, maxcov = INT_MIN;
+    for(int q = tid; q < La; q += kBlock){
+        const int c = ab + q;
+        ms = sup[c] < ms ? sup[c] : ms;
+        mincov = min(mincov, cov[c]);
+        maxcov = max(maxcov, cov[c]);
+    }
+    const float minSupport = blockMinF(ms, sh.ftmp);
+    mincov = blockReduce(mincov, sh.tmp, OpMin(), INT_MAX);
+    maxcov = blockReduce(maxcov, sh.tmp, OpMax(), INT_MIN);
+    if(tid == 0){
+        float sum = 0.0f;                       // sequential, as std::accumulate
+        for(int q = 0; q < La; q++) sum += sup[ab + q];
+        sh.ftmp[0] = sum / La;
+    }
+    __syncthreads();
+    const float avgSupport = sh.ftmp[0];
+    __syncthreads();
+    const bool allGood = fgeq(avgSupport, p.avgThr) && fgeq(minSupport, p.minThr) && fgeq(float(mincov), p.minCovThr);
+    bool isHQ = false;
+    if(allGood){
+        int smallest = 100;
+        for(int percent = p.pct; percent >= 0; percent--){
+            const float factor = percent / 100.0f;
+            const float avgT = 1.0f - 1.0f * factor;
+            const float minT = 1.0f - 3.0f * factor;
+            if(fgeq(avgSupport, avgT) && fgeq(minSupport, minT)) smallest = percent;
+        }
+        isHQ = fgeq(float(mincov), p.minCovThr) && fleq(float(smallest), p.pctHalf);
+    }
+    int nEdits = 0;
+    for(int q = tid; q < La; q += kBlock){
+        const int c = ab + q;
+        const int ob = baseAt(arow, q);
+        int cb;
+        if(allGood) cb = cons[c];
+        else cb = (sup[c] > 0.90f && origCov[c] <= 2) ? cons[c] : ob;
+        corr[q] = uint8_t(cb);
+        nEdits += cb != ob;
+    }
+    nEdits = blockReduce(nEdits, sh.tmp, OpSum(), 0);
+    if(info && tid == 0){
+        info[4] = isHQ; info[5] = 1; info[6] = mincov; info[7] = maxcov;
+        info[9] = __float_as_int(avgSupport); info[10] = __float_as_int(minSupport);
+    }
+
+    // 5) record (include/corrector.hpp:1639-1761, include/correctedsequence.hpp:299-387);
+    // HQ records without edits are not stored (src/correct_cpu.cpp:263-273)
+    const bool ambiguous = p.ambig[a] != 0;
+    const bool useEdits = !ambiguous && nEdits <= La / 7;
+    if(isHQ && useEdits && nEdits == 0) return;
+    const uint32_t payload = useEdits ? uint32_t(4 + 5 * nEdits) : uint32_t(4 + La);
+    uint8_t* rec = p.slots + size_t(i) * p.slotBytes;
+    if(tid == 0){
+        const uint32_t rid = uint32_t(a);
+        const uint32_t flags = (uint32_t(isHQ) << 31) | (uint32_t(useEdits) << 30) | payload;
+        memcpy(rec, &rid, 4);
+        memcpy(rec + 4, &flags, 4);
+        const int first = useEdits ? nEdits : La;
+        memcpy(rec + 8, &first, 4);
+        p.recBytes[i] = 8 + payload;
+        if(info) info[11] = int(8 + payload);
+    }
+    if(useEdits){
+        int running = 0;
+        for(int base = 0; base < La; base += kBlock){
+            const int q = base + tid;
+            bool e = false;
+            int cb = 0;
+            if(q < La){ cb = corr[q]; e = cb != baseAt(arow, q); }
+            int total;
+            const int pos = blockExclusiveSum(e ? 1 : 0, sh.tmp, total);
+            if(e){
+                const int k = running + pos;
+                memcpy(rec + 12 + 4 * k, &q, 4);
+                rec[12 + 4 * nEdits + k] = uint8_t("ACGT"[cb]);
+            }
+            running += total;
+        }
+    }else{
+        for(int q = tid; q < La; q += kBlock) rec[12 + q] = uint8_t("ACGT"[corr[q]]);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_gather(const uint8_t* slots, int slotBytes, const uint32_t* recBytes, const uint64_t* outOff, uint8_t* out, int B){
+    const int i = blockIdx.x;
+    if(i >= B) return;
+    const uint32_t nb = recBytes[i];
+    const uint8_t* src = slots + size_t(i) * slotBytes;
+    uint8_t* dst = out + outOff[i];
+    for(uint32_t q = threadIdx.x; q < nb; q += kBlock) dst[q] = src[q];
+}
+
+__global__ void k_widen(const uint32_t* in, uint64_t* out, int n){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) out[i] = in[i];
+}
+
+// ---------------------------------------------------------------- host side
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t b){
+        if(b <= bytes) return hipSuccess;
+        if(p){ hipError_t e = hipFree(p); if(e != hipSuccess) return e; p = nullptr; bytes = 0; }
+        const size_t nb = std::max<size_t>(b, 256);
+        hipError_t e = hipMalloc(&p, nb);
+        if(e == hipSuccess) bytes = nb;
+        return e;
+    }
+    void release(){ if(p) (void)hipFree(p); p = nullptr; bytes = 0; }
+    template<class T> T* as() const { return static_cast<T*>(p); }
+};
+
+}  // namespace
+
+struct care_ctx {
+    care_opts o{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    uint64_t n = 0;
+    int maxLen = 0;
+    size_t pitch = 0;
+    size_t qpitch = 0;
+    int W = 0;
+    int k = 0;
+    int M = 0;
+    int thr = 0;
+    bool hasReads = false;
+    bool hasTables = false;
+
+    DBuf enc, len, qual, ambig, sig, hilo, qw;
+    std::vector<DBuf> slotBufs;
+    std::vector<uint32_t> caps;
+    DBuf slotPtrs, capsDev, values;
+
+    // batch working memory
+    DBuf raw, uniq, offsets, ids, anchorOf, best, dbg, paired;
+    DBuf kId, kShift, kLen, kNops, kOvl, kW, kOrient, kPaired;
+    DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp;
+    std::vector<uint8_t> records;
+    size_t numRecords = 0;
+
+    // last batch (for inspection)
+    uint64_t lastB0 = 0, lastB = 0, lastTotal = 0;
+    int maxCols = 0;
+    int dbgStride = 0;
+
+    care_timings t{};
+    hipEvent_t ev[8] = {};
+
+    int fail(int code, const char* fmt, ...){
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof(buf), fmt, ap);
+        va_end(ap);
+        err = buf;
+        return code;
+    }
+};
+
+#define HIPCHECK(expr) do { hipError_t e_ = (expr); if(e_ != hipSuccess){ \
+    return ctx->fail(CARE_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); } } while(0)
+
+namespace {
+
+float qualityWeight(int i){
+    // include/qualityscoreweights.hpp:19-28: std::pow(10.0f, -(i-33)/10.0f) in float
+    const float p = i < 33 ? 1.0f : std::pow(10.0f, -(i - 33) / 10.0f);
+    return std::max(0.001f, 1.0f - p);
+}
+
+int resultsPerMapThreshold(float coverage){
+    const int cov = int(coverage);
+    int result = int(cov * 2.5f);
+    result = std::min(result, int(std::numeric_limits<uint16_t>::max()));
+    return std::max(10, result);
+}
+
+int buildTables(care_ctx* ctx){
+    const uint64_t n = ctx->n;
+    const int M = ctx->M;
+    hipStream_t st = ctx->stream;
+    DBuf iota, keysOut, uniqKeys, runCounts, runOffsets, numRuns, kept, temp;
+    auto cleanup = [&](){ iota.release(); keysOut.release(); uniqKeys.release(); runCounts.release();
+                          runOffsets.release(); numRuns.release(); kept.release(); temp.release(); };
+    struct Guard { decltype(cleanup)& f; ~Guard(){ f(); } } guard{cleanup};
+
+    HIPCHECK(ctx->values.ensure(std::max<uint64_t>(1, n) * M * sizeof(uint32_t)));
+    HIPCHECK(iota.ensure(std::max<uint64_t>(1, n) * sizeof(uint32_t)));
+    HIPCHECK(keysOut.ensure(std::max<uint64_t>(1, n) * sizeof(uint64_t)));
+    HIPCHECK(uniqKeys.ensure(std::max<uint64_t>(1, n) * sizeof(uint64_t)));
+    HIPCHECK(runCounts.ensure(std::max<uint64_t>(1, n) * sizeof(uint32_t)));
+    HIPCHECK(runOffsets.ensure(std::max<uint64_t>(1, n) * sizeof(uint32_t)));
+    HIPCHECK(numRuns.ensure(sizeof(int)));
+    HIPCHECK(kept.ensure(sizeof(uint32_t)));
+    if(n) k_iota<<<dim3((n + 255) / 256), dim3(256), 0, st>>>(iota.as<uint32_t>(), n);
+    HIPCHECK(hipGetLastError());
+
+    size_t tb1 = 0, tb2 = 0, tb3 = 0;
+    const int endBit = 2 * ctx->k;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb1, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                (const uint32_t*)nullptr, (uint32_t*)nullptr, int(n), 0, endBit, st));
+    HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(nullptr, tb2, (const uint64_t*)nullptr, (uint64_t*)nullptr,
+                                                   (uint32_t*)nullptr, (int*)nullptr, int(n), st));
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb3, (const uint32_t*)nullptr, (uint32_t*)nullptr, int(n), st));
+    HIPCHECK(temp.ensure(std::max({tb1, tb2, tb3, size_t(256)})));
+
+    ctx->slotBufs.resize(M);
+    ctx->caps.assign(M, 0);
+    std::vector<Slot*> ptrs(M, nullptr);
+    for(int m = 0; m < M; m++){
+        const uint64_t* keysIn = ctx->sig.as<uint64_t>() + uint64_t(m) * n;
+        uint32_t* vals = ctx->values.as<uint32_t>() + uint64_t(m) * n;
+        size_t b = temp.bytes;
+        // stable LSD radix sort: equal keys keep ascending read ids
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(temp.p, b, keysIn, keysOut.as<uint64_t>(), iota.as<uint32_t>(),
+                                                    vals, int(n), 0, endBit, st));
+        b = temp.bytes;
+        HIPCHECK(hipcub::DeviceRunLengthEncode::Encode(temp.p, b, keysOut.as<uint64_t>(), uniqKeys.as<uint64_t>(),
+                                                       runCounts.as<uint32_t>(), numRuns.as<int>(), int(n), st));
+        b = temp.bytes;
+        HIPCHECK(hipcub::DeviceScan::ExclusiveSum(temp.p, b, runCounts.as<uint32_t>(), runOffsets.as<uint32_t>(),
+                                                  int(n), st));
+        HIPCHECK(hipMemsetAsync(kept.p, 0, sizeof(uint32_t), st));
+        const int grid = int((n + 255) / 256);
+        if(grid){
+            k_count_kept<<<grid, 256, 0, st>>>(runCounts.as<uint32_t>(), numRuns.as<int>(), ctx->thr, kept.as<uint32_t>());
+            HIPCHECK(hipGetLastError());
+        }
+        uint32_t keptHost = 0;
+        HIPCHECK(hipMemcpyAsync(&keptHost, kept.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        uint32_t cap = 64;
+        while(cap < 2u * keptHost) cap <<= 1;
+        ctx->caps[m] = cap;
+        HIPCHECK(ctx->slotBufs[m].ensure(size_t(cap) * sizeof(Slot)));
+        HIPCHECK(hipMemsetAsync(ctx->slotBufs[m].p, 0, size_t(cap) * sizeof(Slot), st));
+        if(grid){
+            k_insert<<<grid, 256, 0, st>>>(uniqKeys.as<uint64_t>(), runCounts.as<uint32_t>(), runOffsets.as<uint32_t>(),
+                                           numRuns.as<int>(), ctx->thr, ctx->slotBufs[m].as<Slot>(), cap - 1);
+            HIPCHECK(hipGetLastError());
+        }
+        ptrs[m] = ctx->slotBufs[m].as<Slot>();
+    }
+    HIPCHECK(ctx->slotPtrs.ensure(sizeof(Slot*) * kMaxMaps));
+    HIPCHECK(ctx->capsDev.ensure(sizeof(uint32_t) * kMaxMaps));
+    std::vector<Slot*> ptrs48(kMaxMaps, nullptr);
+    std::vector<uint32_t> caps48(kMaxMaps, 1);
+    for(int m = 0; m < M; m++){ ptrs48[m] = ptrs[m]; caps48[m] = ctx->caps[m]; }
+    HIPCHECK(hipMemcpyAsync(ctx->slotPtrs.p, ptrs48.data(), sizeof(Slot*) * kMaxMaps, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(ctx->capsDev.p, caps48.data(), sizeof(uint32_t) * kMaxMaps, hipMemcpyHostToDevice, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return CARE_OK;
+}
+
+size_t msaLdsBytes(int maxCols, int maxLen){
+    return size_t(maxCols) * (16 + 16 + 4 + 4 + 4 + 1) + size_t(maxLen) + 16;
+}
+
+int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_t& outRecords){
+    hipStream_t st = ctx->stream;
+    const int capPerAnchor = ctx->M * ctx->thr;
+    int P = 1;
+    while(P < capPerAnchor) P <<= 1;
+    // counters: [0] gathered bucket values, [1] error flag
+    HIPCHECK(ctx->counters.ensure(32));
+    HIPCHECK(ctx->raw.ensure(size_t(B) * capPerAnchor * sizeof(uint32_t)));
+    HIPCHECK(ctx->uniq.ensure(size_t(B + 1) * sizeof(uint32_t)));
+    HIPCHECK(ctx->offsets.ensure(size_t(B + 1) * sizeof(uint32_t)));
+    HIPCHECK(hipMemsetAsync(ctx->uniq.p, 0, size_t(B + 1) * sizeof(uint32_t), st));
+
+    HIPCHECK(hipEventRecord(ctx->ev[0], st));
+    CandParams cp;
+    cp.sig = ctx->sig.as<uint64_t>(); cp.n = ctx->n; cp.M = ctx->M; cp.k = ctx->k; cp.thr = ctx->thr;
+    cp.excludeAmbiguous = ctx->o.exclude_ambiguous; cp.len = ctx->len.as<int>(); cp.ambig = ctx->ambig.as<uint8_t>();
+    cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
+    cp.b0 = b0; cp.B = B; cp.capPerAnchor = capPerAnchor; cp.raw = ctx->raw.as<uint32_t>(); cp.uniq = ctx->uniq.as<uint32_t>();
+    cp.gathered = ctx->counters.as<unsigned long long>();
+    const size_t candLds = size_t(P) * sizeof(uint32_t);
+    if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
+    k_candidates<<<B, kBlock, candLds, st>>>(cp);
+    HIPCHECK(hipGetLastError());
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), B + 1, st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), B + 1, st));
+    uint32_t total = 0;
+    HIPCHECK(hipMemcpyAsync(&total, ctx->offsets.as<uint32_t>() + B, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    const size_t T = std::max<size_t>(total, 1);
+    HIPCHECK(ctx->ids.ensure(T * 4));
+    HIPCHECK(ctx->anchorOf.ensure(T * 4));
+    HIPCHECK(ctx->best.ensure(T * sizeof(AlignBest)));
+    HIPCHECK(ctx->paired.ensure(T));
+    HIPCHECK(ctx->kId.ensure(T * 4)); HIPCHECK(ctx->kShift.ensure(T * 4)); HIPCHECK(ctx->kLen.ensure(T * 4));
+    HIPCHECK(ctx->kNops.ensure(T * 4)); HIPCHECK(ctx->kOvl.ensure(T * 4)); HIPCHECK(ctx->kW.ensure(T * 4));
+    HIPCHECK(ctx->kOrient.ensure(T)); HIPCHECK(ctx->kPaired.ensure(T));
+    if(ctx->o.keep_intermediates) HIPCHECK(ctx->dbg.ensure(T * 11 * sizeof(int)));
+    k_compact<<<B, kBlock, 0, st>>>(ctx->raw.as<uint32_t>(), capPerAnchor, ctx->offsets.as<uint32_t>(),
+                                    ctx->ids.as<uint32_t>(), ctx->anchorOf.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    if(ctx->o.paired){
+        const int pairs = B / 2;
+        k_pair_flags<<<(pairs + 127) / 128, 128, 0, st>>>(ctx->offsets.as<uint32_t>(), ctx->ids.as<uint32_t>(),
+                                                          ctx->paired.as<uint8_t>(), pairs);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipEventRecord(ctx->ev[1], st));
+
+    AlignParams ap;
+    ap.hilo = ctx->hilo.as<uint32_t>(); ap.W = ctx->W; ap.len = ctx->len.as<int>();
+    ap.ids = ctx->ids.as<uint32_t>(); ap.anchorOf = ctx->anchorOf.as<uint32_t>(); ap.b0 = b0; ap.total = total;
+    ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
+    ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
+    ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
+    if(total){
+        k_align<<<(total + kAlignWaves - 1) / kAlignWaves, kBlock, 0, st>>>(ap);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipEventRecord(ctx->ev[2], st));
+
+    const int maxCols = ctx->maxCols;
+    const int slotBytes = ((8 + 4 + std::max(ctx->maxLen, 5 * (ctx->maxLen / 7 + 1))) + 15) / 16 * 16;
+    HIPCHECK(ctx->slots.ensure(size_t(B) * slotBytes));
+    HIPCHECK(ctx->recBytes.ensure(size_t(B + 1) * 4));
+    HIPCHECK(ctx->recOff64.ensure(size_t(B + 1) * 8));
+    HIPCHECK(hipMemsetAsync(ctx->recBytes.p, 0, size_t(B + 1) * 4, st));
+    if(ctx->o.keep_intermediates){
+        HIPCHECK(ctx->info.ensure(size_t(B) * kAnchorInfo * 4));
+        HIPCHECK(ctx->msaDbg.ensure(size_t(B) * ctx->dbgStride));
+    }
+    MsaParams mp;
+    mp.enc = ctx->enc.as<uint32_t>(); mp.pitch = ctx->pitch; mp.len = ctx->len.as<int>();
+    mp.qual = ctx->qual.as<uint8_t>(); mp.qpitch = ctx->qpitch; mp.ambig = ctx->ambig.as<uint8_t>();
+    mp.qweight = ctx->qw.as<float>(); mp.b0 = b0;
+    mp.offsets = ctx->offsets.as<uint32_t>(); mp.ids = ctx->ids.as<uint32_t>(); mp.best = ctx->best.as<AlignBest>();
+    mp.paired = ctx->o.paired ? ctx->paired.as<uint8_t>() : nullptr;
+    mp.kId = ctx->kId.as<uint32_t>(); mp.kShift = ctx->kShift.as<int>(); mp.kLen = ctx->kLen.as<int>();
+    mp.kNops = ctx->kNops.as<int>(); mp.kOvl = ctx->kOvl.as<int>(); mp.kW = ctx->kW.as<float>();
+    mp.kOrient = ctx->kOrient.as<uint8_t>(); mp.kPaired = ctx->kPaired.as<uint8_t>();
+    mp.isPaired = ctx->o.paired; mp.useQ = ctx->o.use_quality_scores;
+    mp.maxErrorRate = ctx->o.max_error_rate;
+    // host-side thresholds, written as in the reference (float arithmetic)
+    const float base = ctx->o.estimated_errorrate * 1.0f;
+    mp.ratio2 = 2 * base; mp.ratio3 = 3 * base; mp.ratio4 = 4 * base;
+    mp.goodCount = ctx->o.estimated_coverage * ctx->o.m_coverage;
+    mp.pairedThreshold = ctx->o.paired_filter_threshold;
+    const float e = ctx->o.estimated_errorrate;
+    mp.avgThr = 1.0f - 1.0f * e;
+    mp.minThr = 1.0f - 3.0f * e;
+    mp.minCovThr = ctx->o.m_coverage / 6.0f * ctx->o.estimated_coverage;
+    mp.pct = int(std::ceil(e * 100.0f));
+    mp.pctHalf = mp.pct * 0.5f;
+    mp.sigThr = int(int(ctx->o.estimated_coverage) * 0.3f);
+    mp.maxCols = maxCols; mp.maxLen = ctx->maxLen;
+    mp.slots = ctx->slots.as<uint8_t>(); mp.slotBytes = slotBytes; mp.recBytes = ctx->recBytes.as<uint32_t>();
+    mp.info = ctx->o.keep_intermediates ? ctx->info.as<int>() : nullptr;
+    mp.msaDbg = ctx->o.keep_intermediates ? ctx->msaDbg.as<uint8_t>() : nullptr;
+    mp.dbgStride = ctx->dbgStride;
+    mp.error = ctx->counters.as<int>() + 2;
+    const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen);
+    if(msaLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_msa, hipFuncAttributeMaxDynamicSharedMemorySize, int(msaLds)));
+    k_msa<<<B, kBlock, msaLds, st>>>(mp);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[3], st));
+
+    // record compaction
+    k_widen<<<(B + 1 + 255) / 256, 256, 0, st>>>(ctx->recBytes.as<uint32_t>(), ctx->recOff64.as<uint64_t>(), B + 1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(ctx->recOff.ensure(size_t(B + 1) * 8));
+    tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
+    uint64_t outBytes = 0;
+    int errFlag = 0;
+    HIPCHECK(hipMemcpyAsync(&outBytes, ctx->recOff.as<uint64_t>() + B, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&errFlag, ctx->counters.as<int>() + 2, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if(errFlag) return ctx->fail(CARE_EINVAL, "MSA column bound exceeded in batch at %llu", (unsigned long long)b0);
+    HIPCHECK(ctx->outBuf.ensure(std::max<uint64_t>(outBytes, 1)));
+    k_gather<<<B, kBlock, 0, st>>>(ctx->slots.as<uint8_t>(), slotBytes, ctx->recBytes.as<uint32_t>(),
+                                   ctx->recOff.as<uint64_t>(), ctx->outBuf.as<uint8_t>(), B);
+    HIPCHECK(hipGetLastError());
+    const size_t old = out.size();
+    out.resize(old + outBytes);
+    if(outBytes) HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->outBuf.p, outBytes, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipEventRecord(ctx->ev[4], st));
+    HIPCHECK(hipStreamSynchronize(st));
+
+    // count records of this batch
+    size_t q = old;
+    while(q < out.size()){
+        uint32_t flags;
+        memcpy(&flags, out.data() + q + 4, 4);
+        q += 8 + (flags & ((1u << 29) - 1));
+        outRecords++;
+    }
+    float ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1])); ctx->t.candidates_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2])); ctx->t.align_ms += ms; ctx->t.align_kernel_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3])); ctx->t.msa_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4])); ctx->t.output_ms += ms;
+    ctx->t.candidates += total;
+    ctx->lastB0 = b0; ctx->lastB = uint64_t(B); ctx->lastTotal = total;
+    return CARE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void care_hip_default_opts(care_opts* o){
+    std::memset(o, 0, sizeof(*o));
+    o->kmerlength = 20;
+    o->num_hash_maps = 48;
+    o->estimated_coverage = 30.0f;
+    o->min_overlap = 30;
+    o->max_error_rate = 0.2f;
+    o->min_overlap_ratio = 0.30f;
+    o->estimated_errorrate = 0.06f;
+    o->m_coverage = 0.6f;
+    o->paired_filter_threshold = 0.06f;
+}
+
+int care_hip_create(const care_opts* opts, int device, care_ctx** out){
+    if(!opts || !out) return CARE_EINVAL;
+    *out = nullptr;
+    if(opts->num_hash_maps < 1 || opts->num_hash_maps > kMaxMaps) return CARE_EINVAL;
+    if(opts->kmerlength < 0 || opts->kmerlength > 32) return CARE_EINVAL;
+    if(!(opts->estimated_coverage > 0.0f)) return CARE_EINVAL;
+    int count = 0;
+    if(hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return CARE_ENODEV;
+    auto* ctx = new (std::nothrow) care_ctx;
+    if(!ctx) return CARE_ENOMEM;
+    ctx->o = *opts;
+    ctx->device = device;
+    if(hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess){
+        delete ctx;
+        return CARE_EHIP;
+    }
+    for(auto& e : ctx->ev){
+        if(hipEventCreate(&e) != hipSuccess){ delete ctx; return CARE_EHIP; }
+    }
+    *out = ctx;
+    return CARE_OK;
+}
+
+void care_hip_destroy(care_ctx* ctx){
+    if(!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if(ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->qw, &ctx->slotPtrs,
+                   &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
+                   &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
+                   &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
+                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp}){
+        b->release();
+    }
+    for(auto& b : ctx->slotBufs) b.release();
+    for(auto& e : ctx->ev) if(e) (void)hipEventDestroy(e);
+    if(ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* care_hip_last_error(const care_ctx* ctx){
+    return ctx ? ctx->err.c_str() : "no context";
+}
+
+int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints, const int32_t* lengths,
+                       const uint8_t* quals, size_t qual_pitch, uint64_t num_reads, const uint8_t* ambiguous){
+    if(!ctx) return CARE_EINVAL;
+    if(!enc2bit || !lengths || !ambiguous || num_reads == 0) return ctx->fail(CARE_EINVAL, "empty or null read storage");
+    if(num_reads >= 0xFFFFFFFFull) return ctx->fail(CARE_EINVAL, "read ids are 32-bit");
+    if(ctx->o.paired && (num_reads & 1)) return ctx->fail(CARE_EINVAL, "paired input needs an even number of reads");
+    int maxLen = 0;
+    for(uint64_t i = 0; i < num_reads; i++){
+        if(lengths[i] < 0) return ctx->fail(CARE_EINVAL, "negative length at read %llu", (unsigned long long)i);
+        maxLen = std::max(maxLen, int(lengths[i]));
+    }
+    if(maxLen > 32 * kMaxHiLoWords) return ctx->fail(CARE_EINVAL, "reads longer than %d bases are not supported", 32 * kMaxHiLoWords);
+    if(pitch_ints * 16 < size_t(maxLen)) return ctx->fail(CARE_EINVAL, "pitch_ints too small for the longest read");
+    if(ctx->o.use_quality_scores){
+        if(!quals) return ctx->fail(CARE_EINVAL, "use_quality_scores set but no qualities given");
+        if(qual_pitch < size_t(maxLen)) return ctx->fail(CARE_EINVAL, "qual_pitch smaller than the longest read");
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    ctx->n = num_reads;
+    ctx->maxLen = maxLen;
+    ctx->pitch = pitch_ints;
+    ctx->W = std::max(1, (maxLen + 31) / 32);
+    ctx->k = ctx->o.kmerlength ? ctx->o.kmerlength : (maxLen < 160 ? 20 : 32);
+    ctx->M = ctx->o.num_hash_maps;
+    ctx->thr = resultsPerMapThreshold(ctx->o.estimated_coverage);
+    if(ctx->M * ctx->thr > kMaxSortEntries) return ctx->fail(CARE_EINVAL, "maps*threshold = %d exceeds %d", ctx->M * ctx->thr, kMaxSortEntries);
+    const int minoverlap = std::max(ctx->o.min_overlap, int(float(maxLen) * ctx->o.min_overlap_ratio));
+    ctx->maxCols = maxLen + 2 * std::max(0, maxLen - std::min(minoverlap, maxLen)) + 2;
+    ctx->maxCols = std::max(ctx->maxCols, 3 * maxLen + 2);
+    if(msaLdsBytes(ctx->maxCols, maxLen) > 150 * 1024) return ctx->fail(CARE_EINVAL, "reads too long for the LDS-resident MSA");
+    ctx->dbgStride = int((4 + size_t(ctx->maxCols) * (16 + 16 + 4 + 4 + 1) + 15) / 16 * 16);
+
+    HIPCHECK(ctx->enc.ensure(num_reads * pitch_ints * 4));
+    HIPCHECK(hipMemcpyAsync(ctx->enc.p, enc2bit, num_reads * pitch_ints * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->len.ensure(num_reads * 4));
+    HIPCHECK(hipMemcpyAsync(ctx->len.p, lengths, num_reads * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->ambig.ensure(num_reads));
+    HIPCHECK(hipMemcpyAsync(ctx->ambig.p, ambiguous, num_reads, hipMemcpyHostToDevice, st));
+    if(ctx->o.use_quality_scores){
+        ctx->qpitch = qual_pitch;
+        HIPCHECK(ctx->qual.ensure(num_reads * qual_pitch));
+        HIPCHECK(hipMemcpyAsync(ctx->qual.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
+    }
+    float qw[256];
+    for(int i = 0; i < 256; i++) qw[i] = qualityWeight(i);
+    HIPCHECK(ctx->qw.ensure(sizeof(qw)));
+    HIPCHECK(hipMemcpyAsync(ctx->qw.p, qw, sizeof(qw), hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4));
+    const uint64_t hiloThreads = num_reads * uint64_t(ctx->W);
+    k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), pitch_ints, ctx->len.as<int>(),
+                                                                  num_reads, ctx->W, ctx->hilo.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->hasReads = true;
+    ctx->hasTables = false;
+    return CARE_OK;
+}
+
+int care_hip_build_tables(care_ctx* ctx, int* maps_built){
+    if(!ctx) return CARE_EINVAL;
+    if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "set_reads first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    HIPCHECK(ctx->sig.ensure(ctx->n * ctx->M * 8));
+    k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
+                                                                   ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>());
+    HIPCHECK(hipGetLastError());
+    const int rc = buildTables(ctx);
+    if(rc != CARE_OK) return rc;
+    ctx->hasTables = true;
+    if(maps_built) *maps_built = ctx->M;
+    return CARE_OK;
+}
+
+int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_records, size_t* num_bytes){
+    if(!ctx) return CARE_EINVAL;
+    if(!ctx->hasTables) return ctx->fail(CARE_ESTATE, "build_tables first");
+    if(first > ctx->n || n > ctx->n - first) return ctx->fail(CARE_EINVAL, "anchor range out of bounds");
+    if(ctx->o.paired && ((first & 1) || (n & 1))) return ctx->fail(CARE_EINVAL, "paired input: first and n must be even");
+    HIPCHECK(hipSetDevice(ctx->device));
+    ctx->t = care_timings{};
+    ctx->records.clear();
+    ctx->numRecords = 0;
+    HIPCHECK(ctx->counters.ensure(32));
+    HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, 32, ctx->stream));
+    int B = ctx->o.batch_size > 0 ? ctx->o.batch_size : 0;
+    if(B <= 0){
+        // keep the per-anchor candidate staging area near 1 GiB
+        const size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + 512;
+        B = int(std::min<size_t>(size_t(1) << 18, std::max<size_t>(1024, (size_t(1) << 30) / perAnchor)));
+    }
+    if(ctx->o.paired && (B & 1)) B++;
+    const auto t0 = std::chrono::steady_clock::now();
+    for(uint64_t b0 = first; b0 < first + n; b0 += uint64_t(B)){
+        const int cur = int(std::min<uint64_t>(uint64_t(B), first + n - b0));
+        const int rc = runBatch(ctx, b0, cur, ctx->records, ctx->numRecords);
+        if(rc != CARE_OK) return rc;
+        ctx->t.batches++;
+    }
+    unsigned long long gathered = 0;
+    HIPCHECK(hipMemcpy(&gathered, ctx->counters.p, 8, hipMemcpyDeviceToHost));
+    const auto t1 = std::chrono::steady_clock::now();
+    ctx->t.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ctx->t.anchors = n;
+    ctx->t.records = ctx->numRecords;
+    ctx->t.record_bytes = ctx->records.size();
+    ctx->t.bucket_values = gathered;
+    if(num_records) *num_records = ctx->numRecords;
+    if(num_bytes) *num_bytes = ctx->records.size();
+    return CARE_OK;
+}
+
+int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap){
+    if(!ctx || (!dst && !ctx->records.empty())) return CARE_EINVAL;
+    if(cap < ctx->records.size()) return ctx->fail(CARE_EINVAL, "buffer too small");
+    if(!ctx->records.empty()) std::memcpy(dst, ctx->records.data(), ctx->records.size());
+    return CARE_OK;
+}
+
+int care_hip_get_timings(const care_ctx* ctx, care_timings* out){
+    if(!ctx || !out) return CARE_EINVAL;
+    *out = ctx->t;
+    return CARE_OK;
+}
+
+int care_hip_inspect_signatures(care_ctx* ctx, uint64_t first, uint64_t n, uint64_t* out){
+    if(!ctx || !out) return CARE_EINVAL;
+    if(!ctx->hasTables) return ctx->fail(CARE_ESTATE, "build_tables first");
+    if(first > ctx->n || n > ctx->n - first) return ctx->fail(CARE_EINVAL, "range out of bounds");
+    HIPCHECK(hipSetDevice(ctx->device));
+    std::vector<uint64_t> col(n);
+    for(int m = 0; m < ctx->M; m++){
+        HIPCHECK(hipMemcpy(col.data(), ctx->sig.as<uint64_t>() + uint64_t(m) * ctx->n + first, n * 8, hipMemcpyDeviceToHost));
+        for(uint64_t r = 0; r < n; r++) out[r * ctx->M + m] = col[r];
+    }
+    return CARE_OK;
+}
+
+int care_hip_inspect_batch(care_ctx* ctx, uint64_t* batch_first, uint64_t* batch_n, uint64_t* total,
+                           uint32_t* candidate_offsets, uint32_t* candidate_ids, int32_t* alignments,
+                           uint8_t* paired, int32_t* anchor_info){
+    if(!ctx) return CARE_EINVAL;
+    if(!ctx->o.keep_intermediates) return ctx->fail(CARE_ESTATE, "keep_intermediates is off");
+    if(ctx->lastB == 0) return ctx->fail(CARE_ESTATE, "no batch yet");
+    HIPCHECK(hipSetDevice(ctx->device));
+    if(batch_first) *batch_first = ctx->lastB0;
+    if(batch_n) *batch_n = ctx->lastB;
+    if(total) *total = ctx->lastTotal;
+    const size_t B = ctx->lastB, T = ctx->lastTotal;
+    if(candidate_offsets) HIPCHECK(hipMemcpy(candidate_offsets, ctx->offsets.p, (B + 1) * 4, hipMemcpyDeviceToHost));
+    if(candidate_ids && T) HIPCHECK(hipMemcpy(candidate_ids, ctx->ids.p, T * 4, hipMemcpyDeviceToHost));
+    if(alignments && T) HIPCHECK(hipMemcpy(alignments, ctx->dbg.p, T * 11 * 4, hipMemcpyDeviceToHost));
+    if(paired && T){
+        if(ctx->o.paired) HIPCHECK(hipMemcpy(paired, ctx->paired.p, T, hipMemcpyDeviceToHost));
+        else std::memset(paired, 0, T);
+    }
+    if(anchor_info) HIPCHECK(hipMemcpy(anchor_info, ctx->info.p, B * kAnchorInfo * 4, hipMemcpyDeviceToHost));
+    return CARE_OK;
+}
+
+int care_hip_inspect_msa(care_ctx* ctx, uint64_t index, int cap_columns, int* num_columns, int32_t* counts,
+                         float* weights, int32_t* coverage, uint8_t* consensus, float* support){
+    if(!ctx) return CARE_EINVAL;
+    if(!ctx->o.keep_intermediates) return ctx->fail(CARE_ESTATE, "keep_intermediates is off");
+    if(index >= ctx->lastB) return ctx->fail(CARE_EINVAL, "index out of range");
+    HIPCHECK(hipSetDevice(ctx->device));
+    std::vector<uint8_t> blob(ctx->dbgStride);
+    HIPCHECK(hipMemcpy(blob.data(), ctx->msaDbg.as<uint8_t>() + index * ctx->dbgStride, ctx->dbgStride, hipMemcpyDeviceToHost));
+    int info[kAnchorInfo];
+    HIPCHECK(hipMemcpy(info, ctx->info.as<int>() + index * kAnchorInfo, sizeof(info), hipMemcpyDeviceToHost));
+    if(info[1] == 0 && info[2] == 0){ if(num_columns) *num_columns = 0; return CARE_OK; }
+    const int C = ctx->maxCols;
+    int ncol;
+    memcpy(&ncol, blob.data(), 4);
+    if(num_columns) *num_columns = ncol;
+    const int* dc = reinterpret_cast<const int*>(blob.data()) + 1;
+    const float* dw = reinterpret_cast<const float*>(dc + 4 * C);
+    const int* dv = reinterpret_cast<const int*>(dw + 4 * C);
+    const float* ds = reinterpret_cast<const float*>(dv + C);
+    const uint8_t* dk = reinterpret_cast<const uint8_t*>(ds + C);
+    const int m = std::min(ncol, cap_columns);
+    for(int c = 0; c < m; c++){
+        for(int b = 0; b < 4; b++){
+            if(counts) counts[4 * c + b] = dc[4 * c + b];
+            if(weights) weights[4 * c + b] = dw[4 * c + b];
+        }
+        if(coverage) coverage[c] = dv[c];
+        if(support) support[c] = ds[c];
+        if(consensus) consensus[c] = dk[c];
+    }
+    return CARE_OK;
+}
+
+}  // extern "C"

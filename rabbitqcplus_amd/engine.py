@@ -1,0 +1,214 @@
+"""Host-side mirror of the reference correction entry point, over the HIP C-ABI.
+
+``correct_gpu(opts, storage)`` plays the role of the reference's
+``care::cpu::correct_cpu(programOptions, minhasher, readStorage)``
+(include/correct_cpu.hpp:17-21): given a read storage it builds the device-resident
+minhash tables and returns the serialized correction records in read-id order.
+
+The native library ``rabbitqcplus_amd/_lib/libcare_hip.so`` is required; there is no
+CPU fallback. A missing library or device raises ``CareError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import build as _build
+
+LIB_PATH = os.path.join(_build.LIBDIR, "libcare_hip.so")
+_LIB = None
+
+CARE_OK = 0
+
+
+class CareError(RuntimeError):
+    pass
+
+
+class CareOpts(ctypes.Structure):
+    _fields_ = [("kmerlength", ctypes.c_int), ("num_hash_maps", ctypes.c_int),
+                ("estimated_coverage", ctypes.c_float), ("use_quality_scores", ctypes.c_int),
+                ("exclude_ambiguous", ctypes.c_int), ("paired", ctypes.c_int),
+                ("min_overlap", ctypes.c_int), ("max_error_rate", ctypes.c_float),
+                ("min_overlap_ratio", ctypes.c_float), ("estimated_errorrate", ctypes.c_float),
+                ("m_coverage", ctypes.c_float), ("paired_filter_threshold", ctypes.c_float),
+                ("batch_size", ctypes.c_int), ("keep_intermediates", ctypes.c_int)]
+
+
+class CareTimings(ctypes.Structure):
+    _fields_ = [("total_ms", ctypes.c_double), ("candidates_ms", ctypes.c_double),
+                ("align_ms", ctypes.c_double), ("msa_ms", ctypes.c_double),
+                ("output_ms", ctypes.c_double), ("align_kernel_ms", ctypes.c_double),
+                ("anchors", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
+                ("records", ctypes.c_uint64), ("record_bytes", ctypes.c_uint64),
+                ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64)]
+
+
+EXPORTS = [
+    "care_hip_default_opts", "care_hip_create", "care_hip_destroy", "care_hip_last_error",
+    "care_hip_set_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
+    "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
+    "care_hip_inspect_msa",
+]
+
+
+def load_library(path: str = LIB_PATH):
+    """Loads the HIP library (no GPU call is made). Raises CareError if absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(path):
+        raise CareError(f"HIP library not built: {path} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    vp, sz, u64, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64, ctypes.c_int
+    lib.care_hip_default_opts.argtypes = [ctypes.POINTER(CareOpts)]
+    lib.care_hip_default_opts.restype = None
+    lib.care_hip_create.argtypes = [ctypes.POINTER(CareOpts), i32, ctypes.POINTER(vp)]
+    lib.care_hip_destroy.argtypes = [vp]
+    lib.care_hip_destroy.restype = None
+    lib.care_hip_last_error.argtypes = [vp]
+    lib.care_hip_last_error.restype = ctypes.c_char_p
+    lib.care_hip_set_reads.argtypes = [vp, vp, sz, vp, vp, sz, u64, vp]
+    lib.care_hip_build_tables.argtypes = [vp, ctypes.POINTER(i32)]
+    lib.care_hip_correct.argtypes = [vp, u64, u64, ctypes.POINTER(sz), ctypes.POINTER(sz)]
+    lib.care_hip_copy_records.argtypes = [vp, vp, sz]
+    lib.care_hip_get_timings.argtypes = [vp, ctypes.POINTER(CareTimings)]
+    lib.care_hip_inspect_signatures.argtypes = [vp, u64, u64, vp]
+    lib.care_hip_inspect_batch.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                           vp, vp, vp, vp, vp]
+    lib.care_hip_inspect_msa.argtypes = [vp, u64, i32, ctypes.POINTER(i32), vp, vp, vp, vp, vp]
+    for name in EXPORTS:
+        getattr(lib, name)
+    _LIB = lib
+    return lib
+
+
+def default_opts(**kw) -> CareOpts:
+    lib = load_library()
+    o = CareOpts()
+    lib.care_hip_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
+
+
+@dataclass
+class BatchInspection:
+    first: int
+    n: int
+    offsets: np.ndarray
+    candidate_ids: np.ndarray
+    alignments: np.ndarray
+    paired: np.ndarray
+    anchor_info: np.ndarray
+
+
+class CareEngine:
+    """One device context: read storage + tables resident on the GPU."""
+
+    def __init__(self, device: int = 0, **opts):
+        self.lib = load_library()
+        self.opts = default_opts(**opts)
+        h = ctypes.c_void_p()
+        rc = self.lib.care_hip_create(ctypes.byref(self.opts), device, ctypes.byref(h))
+        if rc != CARE_OK:
+            raise CareError(f"care_hip_create failed with code {rc}")
+        self.h = h
+        self.storage = None
+
+    def _check(self, rc, what):
+        if rc != CARE_OK:
+            msg = self.lib.care_hip_last_error(self.h).decode(errors="replace")
+            raise CareError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.care_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_reads(self, rs):
+        self.storage = rs
+        q = rs.quals if self.opts.use_quality_scores else None
+        if self.opts.use_quality_scores and q is None:
+            raise CareError("use_quality_scores set but the storage has no qualities")
+        rc = self.lib.care_hip_set_reads(self.h, rs.enc.ctypes.data, rs.pitch_ints, rs.lengths.ctypes.data,
+                                         None if q is None else q.ctypes.data, 0 if q is None else q.shape[1],
+                                         rs.n, rs.ambiguous.ctypes.data)
+        self._check(rc, "care_hip_set_reads")
+
+    def build_tables(self) -> int:
+        maps = ctypes.c_int(0)
+        self._check(self.lib.care_hip_build_tables(self.h, ctypes.byref(maps)), "care_hip_build_tables")
+        return maps.value
+
+    def correct(self, first: int = 0, n: int | None = None) -> tuple[int, bytes]:
+        n = self.storage.n - first if n is None else n
+        nrec, nbytes = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        self._check(self.lib.care_hip_correct(self.h, first, n, ctypes.byref(nrec), ctypes.byref(nbytes)),
+                    "care_hip_correct")
+        buf = ctypes.create_string_buffer(max(1, nbytes.value))
+        self._check(self.lib.care_hip_copy_records(self.h, buf, nbytes.value), "care_hip_copy_records")
+        return nrec.value, buf.raw[:nbytes.value]
+
+    def timings(self) -> dict:
+        t = CareTimings()
+        self._check(self.lib.care_hip_get_timings(self.h, ctypes.byref(t)), "care_hip_get_timings")
+        return {k: getattr(t, k) for k, _ in CareTimings._fields_}
+
+    def signatures(self, first: int = 0, n: int | None = None) -> np.ndarray:
+        n = self.storage.n - first if n is None else n
+        out = np.zeros((n, self.opts.num_hash_maps), np.uint64)
+        self._check(self.lib.care_hip_inspect_signatures(self.h, first, n, out.ctypes.data), "inspect_signatures")
+        return out
+
+    def inspect_batch(self) -> BatchInspection:
+        f, b, t = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self.lib.care_hip_inspect_batch(self.h, ctypes.byref(f), ctypes.byref(b), ctypes.byref(t),
+                                                    None, None, None, None, None), "inspect_batch")
+        B, T = b.value, t.value
+        offs = np.zeros(B + 1, np.uint32)
+        ids = np.zeros(max(T, 1), np.uint32)
+        al = np.zeros((max(T, 1), 11), np.int32)
+        pr = np.zeros(max(T, 1), np.uint8)
+        info = np.zeros((B, 12), np.int32)
+        self._check(self.lib.care_hip_inspect_batch(self.h, None, None, None, offs.ctypes.data, ids.ctypes.data,
+                                                    al.ctypes.data, pr.ctypes.data, info.ctypes.data), "inspect_batch")
+        return BatchInspection(f.value, B, offs, ids[:T], al[:T], pr[:T], info)
+
+    def inspect_msa(self, index: int, cap_columns: int = 4096) -> dict:
+        ncol = ctypes.c_int(0)
+        counts = np.zeros((cap_columns, 4), np.int32)
+        weights = np.zeros((cap_columns, 4), np.float32)
+        cov = np.zeros(cap_columns, np.int32)
+        cons = np.zeros(cap_columns, np.uint8)
+        sup = np.zeros(cap_columns, np.float32)
+        self._check(self.lib.care_hip_inspect_msa(self.h, index, cap_columns, ctypes.byref(ncol), counts.ctypes.data,
+                                                  weights.ctypes.data, cov.ctypes.data, cons.ctypes.data,
+                                                  sup.ctypes.data), "inspect_msa")
+        c = ncol.value
+        return dict(num_columns=c, counts=counts[:c], weights=weights[:c], coverage=cov[:c],
+                    consensus=cons[:c], support=sup[:c])
+
+
+def correct_gpu(storage, device: int = 0, **opts) -> tuple[int, bytes]:
+    """Reads -> tables -> records, the whole correction stage on one GPU."""
+    opts.setdefault("paired", int(storage.paired))
+    opts.setdefault("use_quality_scores", int(storage.quals is not None))
+    eng = CareEngine(device, **opts)
+    try:
+        eng.set_reads(storage)
+        eng.build_tables()
+        return eng.correct()
+    finally:
+        eng.close()

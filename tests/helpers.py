@@ -1,0 +1,97 @@
+"""Shared workload builders for the parity tests (seeded, small enough for the oracle)."""
+from __future__ import annotations
+
+import functools
+
+import numpy as np
+
+from rabbitqcplus_amd import reads, sim
+
+# name -> (simulator kwargs, engine/oracle option overrides)
+CASES = {
+    # plain SE at the 200 kbp / 150 bp / 30x size, qualities off (RabbitQCPlus default)
+    "se": (dict(genome_length=200_000, read_length_min=150, read_length_max=150, coverage=30.0, seed=11), {}),
+    # same reads, --useQualityScores
+    "se_q": (dict(genome_length=200_000, read_length_min=150, read_length_max=150, coverage=30.0, seed=11),
+             dict(use_quality_scores=1)),
+    # high error profile (Q24 -> Q16, ~1.5% errors) so that many reads are corrected
+    "se_higherr": (dict(genome_length=60_000, read_length_min=150, read_length_max=150, coverage=30.0,
+                        q_first=24.0, q_last=16.0, seed=12), {}),
+    "se_higherr_q": (dict(genome_length=60_000, read_length_min=150, read_length_max=150, coverage=30.0,
+                          q_first=24.0, q_last=16.0, seed=12), dict(use_quality_scores=1)),
+    # repeats and duplicated segments: heavy-tailed candidate counts, refinement rounds
+    "se_repeats": (dict(genome_length=120_000, repeat_fraction=0.2, repeat_unit_length=1000, repeat_copies=10,
+                        repeat_divergence=0.01, dup_fraction=0.05, dup_segment_length=800, dup_copies=2,
+                        dup_divergence=0.005, read_length_min=150, read_length_max=150, coverage=30.0,
+                        q_first=30.0, q_last=20.0, seed=13), {}),
+    # paired end, insert N(350,50)
+    "pe": (dict(genome_length=100_000, read_length_min=150, read_length_max=150, coverage=30.0, paired=1,
+                insert_mean=350.0, insert_sd=50.0, q_first=30.0, q_last=20.0, seed=14), {}),
+    # variable lengths 100..250 with qualities (k forced to 20)
+    "varlen_q": (dict(genome_length=80_000, read_length_min=100, read_length_max=250, coverage=30.0,
+                      q_first=30.0, q_last=20.0, seed=15), dict(use_quality_scores=1, kmerlength=20)),
+    # N bases -> ambiguous reads (edits disabled for them)
+    "nbases": (dict(genome_length=60_000, read_length_min=150, read_length_max=150, coverage=30.0,
+                    q_first=28.0, q_last=20.0, n_rate=0.002, seed=16), {}),
+    "nbases_exclude": (dict(genome_length=60_000, read_length_min=150, read_length_max=150, coverage=30.0,
+                            q_first=28.0, q_last=20.0, n_rate=0.002, seed=16), dict(exclude_ambiguous=1)),
+    # short reads below k and near the minimum overlap
+    "short": (dict(genome_length=20_000, read_length_min=15, read_length_max=60, coverage=40.0,
+                   q_first=30.0, q_last=24.0, seed=17), dict(kmerlength=20, estimated_coverage=40.0)),
+}
+
+
+@functools.lru_cache(maxsize=None)
+def case(name):
+    simkw, opts = CASES[name]
+    s = sim.simulate(**simkw)
+    use_q = bool(opts.get("use_quality_scores", 0))
+    rs = reads.storage_from_sim(s, use_qualities=use_q)
+    o = dict(opts)
+    o.setdefault("paired", int(rs.paired))
+    o.setdefault("use_quality_scores", int(use_q))
+    return s, rs, o
+
+
+def parse_records(data: bytes):
+    out = []
+    p = 0
+    while p < len(data):
+        rid = int.from_bytes(data[p:p + 4], "little")
+        flags = int.from_bytes(data[p + 4:p + 8], "little")
+        nb = flags & ((1 << 29) - 1)
+        out.append((rid, flags, data[p + 8:p + 8 + nb]))
+        p += 8 + nb
+    return out
+
+
+def apply_records(rs, data: bytes) -> list[bytes]:
+    """Corrected sequences of all reads (uncorrected where no record exists)."""
+    seqs = [rs.decoded(i) for i in range(rs.n)]
+    for rid, flags, payload in parse_records(data):
+        use_edits = (flags >> 30) & 1
+        if use_edits:
+            ne = int.from_bytes(payload[0:4], "little", signed=True)
+            pos = np.frombuffer(payload[4:4 + 4 * ne], np.int32)
+            bases = payload[4 + 4 * ne:4 + 5 * ne]
+            s = bytearray(seqs[rid])
+            for p, b in zip(pos, bases):
+                s[int(p)] = b
+            seqs[rid] = bytes(s)
+        else:
+            ln = int.from_bytes(payload[0:4], "little", signed=True)
+            seqs[rid] = bytes(payload[4:4 + ln])
+    return seqs
+
+
+def error_rate(sim_reads, seqs) -> float:
+    mism = 0
+    total = 0
+    for i, s in enumerate(seqs):
+        L = int(sim_reads.lengths[i])
+        t = sim_reads.truth[i, :L].tobytes()
+        a = np.frombuffer(s, np.uint8)
+        b = np.frombuffer(t, np.uint8)
+        mism += int((a != b).sum())
+        total += L
+    return mism / max(total, 1)

@@ -1,0 +1,128 @@
+"""GPU parity: every stage of the HIP path against the CPU oracle on the same seeded reads.
+
+Bar: bit-exact signatures, candidate ids, alignment (score, overlap, shift, nOps,
+validity, orientation), MSA counts/coverage/consensus and records; MSA float weights
+and supports within 1e-5 relative (observed: identical).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from rabbitqcplus_amd import engine
+from tests import helpers
+
+pytestmark = pytest.mark.gpu
+
+WEIGHT_RTOL = 1e-5
+
+
+def _engine(name, **extra):
+    s, rs, o = helpers.case(name)
+    o = dict(o)
+    o.update(extra)
+    eng = engine.CareEngine(0, **o)
+    eng.set_reads(rs)
+    eng.build_tables()
+    return s, rs, o, eng
+
+
+@pytest.fixture(scope="module")
+def se_gpu():
+    s, rs, o, eng = _engine("se", keep_intermediates=1, batch_size=4096)
+    cpu = oracle.CareCpu(rs, threads=8, **o)
+    yield s, rs, o, eng, cpu
+    eng.close()
+    cpu.close()
+
+
+def test_signatures_bit_exact(se_gpu):
+    s, rs, o, eng, cpu = se_gpu
+    g = eng.signatures()
+    c = cpu.signatures()
+    assert g.shape == c.shape
+    assert np.array_equal(g, c)
+
+
+def test_candidates_alignments_msa_first_batch(se_gpu):
+    s, rs, o, eng, cpu = se_gpu
+    n = 4096
+    eng.correct(0, n)
+    ins = eng.inspect_batch()
+    assert ins.first == 0 and ins.n == n
+    rng = np.random.default_rng(0)
+    sample = set(rng.choice(n, 300, replace=False).tolist()) | {0, n - 1}
+    for i in range(n):
+        g = ins.candidate_ids[ins.offsets[i]:ins.offsets[i + 1]]
+        c = cpu.candidates(i)
+        assert np.array_equal(g, c), f"candidates of anchor {i}"
+    for i in sorted(sample):
+        tr = cpu.trace(i)
+        b, e = ins.offsets[i], ins.offsets[i + 1]
+        al = ins.alignments[b:e]
+        assert np.array_equal(al[:, 0:5], tr["fwd_alignments"]), f"fwd alignments of anchor {i}"
+        assert np.array_equal(al[:, 5:10], tr["rc_alignments"]), f"rc alignments of anchor {i}"
+        assert np.array_equal(al[:, 10], tr["orientation"].astype(np.int32)), f"orientation of anchor {i}"
+        info = ins.anchor_info[i]
+        assert info[0] == tr["num_filtered"], f"filtered count of anchor {i}"
+        assert info[1] == tr["num_final"], f"final count of anchor {i}"
+        if tr["num_final"] == 0:
+            continue
+        assert info[2] == tr["num_columns"] and info[3] == tr["anchor_column_begin"]
+        assert info[5] == tr["is_corrected"] and info[4] == tr["is_hq"]
+        assert info[6] == tr["min_coverage"] and info[7] == tr["max_coverage"]
+        assert np.float32(info[9:10].view(np.float32)[0]) == np.float32(tr["avg_support"])
+        assert np.float32(info[10:11].view(np.float32)[0]) == np.float32(tr["min_support"])
+        m = eng.inspect_msa(i)
+        assert m["num_columns"] == tr["num_columns"]
+        assert np.array_equal(m["counts"], tr["counts"])
+        assert np.array_equal(m["coverage"], tr["coverage"])
+        assert np.array_equal(m["consensus"], tr["consensus"])
+        np.testing.assert_allclose(m["weights"], tr["weights"], rtol=WEIGHT_RTOL, atol=0)
+        np.testing.assert_allclose(m["support"], tr["support"], rtol=WEIGHT_RTOL, atol=0)
+
+
+@pytest.mark.parametrize("name", list(helpers.CASES))
+def test_records_match_oracle(name):
+    s, rs, o = helpers.case(name)
+    print(f"[case {name}] reads={rs.n} opts={o}", flush=True)
+    n_gpu, d_gpu = engine.correct_gpu(rs, **o)
+    cpu = oracle.CareCpu(rs, threads=8, **o)
+    n_cpu, d_cpu = cpu.correct(threads=8)
+    cpu.close()
+    assert n_gpu == n_cpu
+    if d_gpu != d_cpu:
+        rg = {r[0]: r for r in helpers.parse_records(d_gpu)}
+        rc = {r[0]: r for r in helpers.parse_records(d_cpu)}
+        diff = sorted(k for k in set(rg) | set(rc) if rg.get(k) != rc.get(k))
+        pytest.fail(f"{len(diff)} differing records, first ids {diff[:10]}")
+    if oracle.ref_available():
+        ref = oracle.CareRef(rs, threads=8, **o)
+        n_ref, d_ref = ref.correct(threads=8)
+        ref.close()
+        assert d_gpu == d_ref
+
+
+def test_batching_invariance():
+    """Record bytes do not depend on the batch size or on the anchor range split."""
+    s, rs, o = helpers.case("se_higherr")
+    outs = []
+    for bs in (0, 1000, 4097):
+        eng = engine.CareEngine(0, batch_size=bs, **o)
+        eng.set_reads(rs)
+        eng.build_tables()
+        outs.append(eng.correct()[1])
+        if bs == 1000:
+            a = eng.correct(0, 1234)[1]
+            b = eng.correct(1234, rs.n - 1234)[1]
+            assert a + b == outs[-1]
+        eng.close()
+    assert outs[0] == outs[1] == outs[2]
+
+
+def test_correction_reduces_error_rate():
+    s, rs, o = helpers.case("se_higherr")
+    _, data = engine.correct_gpu(rs, **o)
+    before = helpers.error_rate(s, [rs.decoded(i) for i in range(rs.n)])
+    after = helpers.error_rate(s, helpers.apply_records(rs, data))
+    print(f"error rate {before:.5f} -> {after:.5f}")
+    assert after < 0.25 * before
