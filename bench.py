@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark: corrected reads/s of the CARE correction stage on one MI355X.
+
+A step is one pass of the correction stage over every read of the workload
+(candidate lookup -> alignment -> MSA/refinement/decision -> serialized records on
+the host), with the read storage and the minhash tables already resident in HBM.
+Table construction is timed once and reported separately.
+
+Prints one JSON line (see the keys below). Options:
+  --gpus N        only 1 is supported (one GPU per box); N > 1 exits with a message
+  --steps K       timed passes (default 3)
+  --warmup W      untimed passes (default 1)
+  --workload      simulator workload name (default se_5mbp_150bp_30x_dup)
+  --no-quality    correct without quality scores
+  --cpu-sample S  anchors for the CPU baseline leg (default 300000; 0 disables)
+  --cpu-threads T threads for the CPU baseline (default min(16, cores))
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from rabbitqcplus_amd import engine, reads, sim  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def decode_all(rs) -> np.ndarray:
+    shifts = (30 - 2 * np.arange(16, dtype=np.uint32)).astype(np.uint32)
+    codes = ((rs.enc[:, :, None] >> shifts[None, None, :]) & 3).reshape(rs.n, -1).astype(np.uint8)
+    return np.frombuffer(b"ACGT", np.uint8)[codes]
+
+
+def error_rates(s, rs, data: bytes):
+    seqs = decode_all(rs)
+    L = seqs.shape[1]
+    maxlen = s.truth.shape[1]
+    w = min(L, maxlen)
+    valid = np.arange(w)[None, :] < rs.lengths[:, None]
+    before = int(((seqs[:, :w] != s.truth[:, :w]) & valid).sum())
+    p = 0
+    mv = memoryview(data)
+    while p < len(data):
+        rid = int.from_bytes(mv[p:p + 4], "little")
+        flags = int.from_bytes(mv[p + 4:p + 8], "little")
+        nb = flags & ((1 << 29) - 1)
+        pl = mv[p + 8:p + 8 + nb]
+        if (flags >> 30) & 1:
+            ne = int.from_bytes(pl[0:4], "little")
+            pos = np.frombuffer(pl[4:4 + 4 * ne], np.int32)
+            seqs[rid, pos] = np.frombuffer(pl[4 + 4 * ne:4 + 5 * ne], np.uint8)
+        else:
+            ln = int.from_bytes(pl[0:4], "little")
+            seqs[rid, :ln] = np.frombuffer(pl[4:4 + ln], np.uint8)
+        p += 8 + nb
+    after = int(((seqs[:, :w] != s.truth[:, :w]) & valid).sum())
+    bases = int(rs.lengths.sum())
+    return before / bases, after / bases
+
+
+def algorithmic_bytes(t: dict, rs, W: int, M: int):
+    """Bytes each kernel must move at minimum, per launch (one launch per batch)."""
+    batches = max(1, t["batches"])
+    A, C, G = t["anchors"], t["candidates"], t["bucket_values"]
+    pitch_b = 4 * rs.pitch_ints
+    # k_candidates: length+ambiguity flag, M signatures (8 B) and M table slots (16 B) per
+    # anchor, every gathered bucket value (4 B), unique candidate ids written (4 B)
+    cand = A * (4 + 1 + 24 * M) + 4 * G + 4 * C
+    # k_align per (anchor, candidate): ids (8 B), both lengths (8 B), anchor fwd HiLo planes
+    # (8W B), candidate fwd+rc planes (16W B), chosen alignment written (16 B)
+    align = C * (32 + 24 * W)
+    # k_msa per candidate: chosen alignment + id (20 B), its 2-bit row once (pitch) and
+    # qualities once when enabled; per anchor: its row, offsets, record slot bytes
+    qbytes = rs.quals.shape[1] if rs.quals is not None else 0
+    msa = C * (20 + pitch_b + qbytes) + A * (pitch_b + 8 + qbytes) + t["record_bytes"]
+    return {"k_candidates": cand / batches, "k_align": align / batches, "k_msa": msa / batches}
+
+
+def cpu_baseline(rs, opts, sample, threads):
+    import oracle
+    n = min(sample, rs.n)
+    if opts.get("paired"):
+        n -= n % 2
+    if oracle.ref_available():
+        ref = oracle.CareRef(rs, threads=threads, **opts)
+        secs, recs = ref.correct_timed(threads, n)
+        ref.close()
+        kind = "reference"
+    else:
+        cpu = oracle.CareCpu(rs, threads=threads, **opts)
+        cpu.correct(0, n, threads=threads)
+        secs = cpu.last_seconds()
+        cpu.close()
+        kind = "port"
+    return {"value": n / secs, "unit": "reads/s", "cores": threads, "kind": kind,
+            "sample": f"first {n} anchors of the same workload (tables over all {rs.n} reads), "
+                      f"{secs:.2f} s wall on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="se_5mbp_150bp_30x_dup")
+    ap.add_argument("--seed", type=int, default=12345)
+    ap.add_argument("--no-quality", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=300000)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-size", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the correction stage shards "
+              f"by anchor range across GPUs, which is not run here.", file=sys.stderr)
+        sys.exit(2)
+
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.time()
+    s = sim.workload(args.workload, seed=args.seed)
+    use_q = not args.no_quality
+    rs = reads.storage_from_sim(s, use_qualities=use_q)
+    opts = dict(use_quality_scores=int(use_q), paired=int(rs.paired), batch_size=args.batch_size)
+    if rs.max_length >= 160:
+        opts["kmerlength"] = 20
+    log(f"workload {args.workload}: {rs.n} reads, max length {rs.max_length}, simulated in {time.time() - t0:.1f} s")
+
+    eng = engine.CareEngine(0, **opts)
+    t1 = time.time()
+    eng.set_reads(rs)
+    t2 = time.time()
+    maps = eng.build_tables()
+    t3 = time.time()
+    log(f"reads to device {1e3 * (t2 - t1):.1f} ms, tables ({maps} maps) {1e3 * (t3 - t2):.1f} ms")
+
+    for w in range(args.warmup):
+        eng.correct()
+        log(f"warmup {w + 1}/{args.warmup} done")
+    step_ms = []
+    data = b""
+    nrec = 0
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        nrec, data = eng.correct()
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
+    t = eng.timings()
+    ms_per_step = float(np.mean(step_ms))
+    value = rs.n / (ms_per_step / 1e3)
+
+    W = max(1, (rs.max_length + 31) // 32)
+    per_launch = algorithmic_bytes(t, rs, W, eng.opts.num_hash_maps)
+    batches = max(1, t["batches"])
+    kernel_ms = {"k_candidates": t["cand_kernel_ms"] / batches, "k_align": t["align_kernel_ms"] / batches,
+                 "k_msa": t["msa_kernel_ms"] / batches}
+    dominant = max(kernel_ms, key=kernel_ms.get)
+    achieved = per_launch[dominant] / (kernel_ms[dominant] / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
+                "avg_launch_ms": round(kernel_ms[dominant], 4), "algorithmic_bytes_per_launch": int(per_launch[dominant]),
+                "launches_per_step": int(batches)}
+    before, after = error_rates(s, rs, data)
+    eng.close()
+
+    cpu = None
+    if args.cpu_sample > 0:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"CPU baseline on {threads} threads, {args.cpu_sample} anchors")
+        copts = {k: v for k, v in opts.items() if k != "batch_size"}
+        cpu = cpu_baseline(rs, copts, args.cpu_sample, threads)
+
+    line = {
+        "metric": "corrected reads/s on 1 MI355X (plain path); post-correction error rate",
+        "value": round(value, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+fp32",
+        "data": "synthetic (seeded simulator)",
+        "config": {"workload": args.workload, "reads": rs.n, "read_length": [int(rs.lengths.min()), rs.max_length],
+                   "coverage": 30, "k": int(eng.opts.kmerlength) or (20 if rs.max_length < 160 else 32),
+                   "maps": int(eng.opts.num_hash_maps), "quality_scores": use_q, "paired": bool(rs.paired),
+                   "path": "plain"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "error_rate_before": round(before, 6),
+        "error_rate_after": round(after, 6),
+        "records": nrec,
+        "table_build_ms": round(1e3 * (t3 - t2), 2),
+        "stage_ms": {"candidates": round(t["candidates_ms"], 3), "align": round(t["align_ms"], 3),
+                     "msa": round(t["msa_ms"], 3), "output": round(t["output_ms"], 3)},
+        "kernel_avg_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
+        "pairs_aligned": int(t["candidates"]),
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -67,7 +67,9 @@ typedef struct care_timings {
     double align_ms;          /* shifted-Hamming alignment, both orientations */
     double msa_ms;            /* filters + MSA + refinement + decision + record encoding */
     double output_ms;         /* record compaction + device-to-host copy */
-    double align_kernel_ms;   /* sum of alignment kernel durations */
+    double align_kernel_ms;   /* sum of k_align durations (one launch per batch) */
+    double cand_kernel_ms;    /* sum of k_candidates durations */
+    double msa_kernel_ms;     /* sum of k_msa durations */
     uint64_t anchors;
     uint64_t candidates;      /* anchor/candidate pairs aligned */
     uint64_t records;

@@ -45,6 +45,8 @@ typedef struct care_sim_config {
     int q_min;
     int q_max;
     double n_rate;              /* probability of replacing a base by 'N' */
+    double tandem_fraction;     /* genome fraction overwritten by tandem repeats */
+    int tandem_period_max;      /* repeat unit length 1..tandem_period_max */
 } care_sim_config;
 
 typedef struct care_sim_result care_sim_result;

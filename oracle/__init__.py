@@ -130,7 +130,7 @@ def ref_lib():
         lib.ref_candidates.argtypes = [vp, ctypes.c_uint, vp, i32]
         lib.ref_align.argtypes = [vp, i32, vp, i32, i32, ctypes.c_float, ctypes.c_float, vp]
         lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
-        lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 9
+        lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 8
         _ref = lib
     return _ref
 

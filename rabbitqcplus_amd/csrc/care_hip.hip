@@ -1183,6 +1183,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
     k_candidates<<<B, kBlock, candLds, st>>>(cp);
     HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[5], st));
     size_t tb = 0;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), B + 1, st));
     HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
@@ -1303,8 +1304,9 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     }
     float ms;
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1])); ctx->t.candidates_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5])); ctx->t.cand_kernel_ms += ms;
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2])); ctx->t.align_ms += ms; ctx->t.align_kernel_ms += ms;
-    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3])); ctx->t.msa_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3])); ctx->t.msa_ms += ms; ctx->t.msa_kernel_ms += ms;
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4])); ctx->t.output_ms += ms;
     ctx->t.candidates += total;
     ctx->lastB0 = b0; ctx->lastB = uint64_t(B); ctx->lastTotal = total;

@@ -127,6 +127,23 @@ std::string makeGenome(const care_sim_config& cfg, Rng& rng){
             }
         }
     }
+    // Tandem repeats (microsatellites): units of 1..tandem_period_max bases repeated
+    // over 40..400 bases, about tandem_fraction of the genome. They make low-complexity
+    // reads whose alignments have many equally scored shifts.
+    if(cfg.tandem_fraction > 0 && cfg.tandem_period_max > 0){
+        double covered = 0.0;
+        const double target = cfg.tandem_fraction * double(G);
+        while(covered < target){
+            const int period = 1 + int(rng.below(std::uint64_t(cfg.tandem_period_max)));
+            const std::size_t len = 40 + std::size_t(rng.below(361));
+            if(len >= G) break;
+            std::string unit(period, 'A');
+            for(auto& c : unit) c = randomBase(rng, cfg.gc_content);
+            const std::size_t dst = rng.below(G - len);
+            for(std::size_t i = 0; i < len; i++) genome[dst + i] = unit[i % period];
+            covered += double(len);
+        }
+    }
     return genome;
 }
 
@@ -162,8 +179,12 @@ void finishRead(const care_sim_config& cfg, Rng& rng, ReadOut& r){
     r.qual.resize(L);
     for(int j = 0; j < L; j++){
         r.qual[j] = drawQuality(cfg, rng, j, L);
-        const int Q = r.qual[j] - 33;
-        const double perr = std::pow(10.0, -double(Q) / 10.0);
+        static const std::vector<double> kErr = [](){
+            std::vector<double> t(128);
+            for(int q = 0; q < 128; q++) t[q] = std::pow(10.0, -double(q) / 10.0);
+            return t;
+        }();
+        const double perr = kErr[(r.qual[j] - 33) & 127];
         if(rng.uniform() < perr) r.seq[j] = otherBase(rng, r.seq[j]);
         if(cfg.n_rate > 0 && rng.uniform() < cfg.n_rate) r.seq[j] = 'N';
     }

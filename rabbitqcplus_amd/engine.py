@@ -42,6 +42,7 @@ class CareTimings(ctypes.Structure):
     _fields_ = [("total_ms", ctypes.c_double), ("candidates_ms", ctypes.c_double),
                 ("align_ms", ctypes.c_double), ("msa_ms", ctypes.c_double),
                 ("output_ms", ctypes.c_double), ("align_kernel_ms", ctypes.c_double),
+                ("cand_kernel_ms", ctypes.c_double), ("msa_kernel_ms", ctypes.c_double),
                 ("anchors", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("records", ctypes.c_uint64), ("record_bytes", ctypes.c_uint64),
                 ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64)]

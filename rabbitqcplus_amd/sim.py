@@ -43,6 +43,8 @@ class SimConfig(ctypes.Structure):
         ("q_min", ctypes.c_int),
         ("q_max", ctypes.c_int),
         ("n_rate", ctypes.c_double),
+        ("tandem_fraction", ctypes.c_double),
+        ("tandem_period_max", ctypes.c_int),
     ]
 
 

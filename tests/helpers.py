@@ -35,6 +35,9 @@ CASES = {
                     q_first=28.0, q_last=20.0, n_rate=0.002, seed=16), {}),
     "nbases_exclude": (dict(genome_length=60_000, read_length_min=150, read_length_max=150, coverage=30.0,
                             q_first=28.0, q_last=20.0, n_rate=0.002, seed=16), dict(exclude_ambiguous=1)),
+    # tandem repeats: low-complexity reads with many equally scored alignment shifts
+    "se_tandem": (dict(genome_length=60_000, tandem_fraction=0.3, tandem_period_max=6, read_length_min=150,
+                       read_length_max=150, coverage=30.0, q_first=30.0, q_last=20.0, seed=18), {}),
     # short reads below k and near the minimum overlap
     "short": (dict(genome_length=20_000, read_length_min=15, read_length_max=60, coverage=40.0,
                    q_first=30.0, q_last=24.0, seed=17), dict(kmerlength=20, estimated_coverage=40.0)),

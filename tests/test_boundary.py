@@ -1,0 +1,119 @@
+"""The C-ABI boundary: library loads and exports every declared symbol (no GPU call),
+read-storage encoding matches the reference's 2-bit format, host logic errors are
+reported loudly."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from rabbitqcplus_amd import build, engine, reads
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return sorted(set(re.findall(r"\b(care_(?:hip|sim)_[a-z_]+)\s*\(", text)))
+
+
+def test_hip_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(engine.LIB_PATH)
+    names = declared("care_hip.h")
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(names) == set(engine.EXPORTS)
+
+
+def test_sim_library_exports_every_declared_symbol():
+    path = os.path.join(build.LIBDIR, "libcaresim.so")
+    if not os.path.exists(path):
+        build.build_sim()
+    lib = ctypes.CDLL(path)
+    for n in declared("care_sim.h"):
+        assert hasattr(lib, n), n
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    saved = engine._LIB
+    engine._LIB = None
+    try:
+        with pytest.raises(engine.CareError):
+            engine.load_library(str(tmp_path / "nope.so"))
+    finally:
+        engine._LIB = saved
+
+
+def test_product_never_imports_the_oracle():
+    pkg = os.path.join(ROOT, "rabbitqcplus_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            # build.py only invokes oracle/Makefile; it does not load the oracle
+            if f.endswith((".py", ".hip", ".cpp", ".h")) and f != "build.py":
+                text = open(os.path.join(dirpath, f)).read()
+                assert "import oracle" not in text and "carecpu" not in text and "careref" not in text, f
+
+
+def ref_encode(seq: bytes) -> list[int]:
+    """encodeSequence2Bit (reference include/sequencehelpers.hpp:154-187), written per base."""
+    n = (len(seq) + 15) // 16
+    out = [0] * n
+    enc = {ord("A"): 0, ord("C"): 1, ord("G"): 2, ord("T"): 3}
+    for i, ch in enumerate(seq):
+        out[i // 16] = ((out[i // 16] << 2) | enc.get(ch, 0)) & 0xFFFFFFFF
+    left = 2 * (n * 16 - len(seq))
+    if left:
+        out[n - 1] = (out[n - 1] << left) & 0xFFFFFFFF
+    return out
+
+
+def test_encoding_matches_reference_layout():
+    rng = np.random.default_rng(0)
+    lens = rng.integers(1, 70, size=50).astype(np.int32)
+    seqs = np.zeros((50, 70), np.uint8)
+    for i, L in enumerate(lens):
+        seqs[i, :L] = rng.choice(np.frombuffer(b"ACGT", np.uint8), L)
+    rs = reads.make_storage(seqs, lens)
+    for i, L in enumerate(lens):
+        want = ref_encode(seqs[i, :L].tobytes())
+        assert rs.enc[i, :len(want)].tolist() == want
+        assert rs.decoded(i) == seqs[i, :L].tobytes()
+
+
+def test_ambiguous_bases_replaced_cyclically_per_parser_batch(monkeypatch):
+    """Non-ACGT -> A,C,G,T,A,... with the counter restarting every parser batch
+    (reference include/chunkedreadstorageconstruction.hpp:67-92,106)."""
+    monkeypatch.setattr(reads, "PARSER_BATCH", 3)
+    rows = [b"ANNA", b"aCgT", b"NNNN", b"GNx", b"NTTN", b"N"]
+    L = np.array([len(r) for r in rows], np.int32)
+    seqs = np.zeros((len(rows), 4), np.uint8)
+    for i, r in enumerate(rows):
+        seqs[i, :len(r)] = np.frombuffer(r, np.uint8)
+    rs = reads.make_storage(seqs, L)
+    expect = []
+    counter = 0
+    for i, r in enumerate(rows):
+        if i % 3 == 0:
+            counter = 0
+        s = bytearray()
+        for ch in r:
+            c = chr(ch).upper()
+            if c in "ACGT":
+                s.append(ord(c))
+            else:
+                s.append(b"ACGT"[counter])
+                counter = (counter + 1) % 4
+        expect.append(bytes(s))
+    assert [rs.decoded(i) for i in range(len(rows))] == expect
+    assert rs.ambiguous.tolist() == [1, 0, 1, 1, 1, 1]
+
+
+def test_storage_rejects_nothing_silently():
+    # qualities are clipped to each read's length and zero-padded (the pitch is the max length)
+    seqs = np.frombuffer(b"ACGTACGTAC" + b"GGGG\0\0\0\0\0\0", np.uint8).reshape(2, 10)
+    quals = np.full((2, 10), ord("I"), np.uint8)
+    rs = reads.make_storage(seqs, np.array([10, 4], np.int32), quals)
+    assert rs.quals.shape == (2, 10)
+    assert rs.quals[1, 4:].tolist() == [0] * 6
