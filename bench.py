@@ -84,6 +84,23 @@ def algorithmic_bytes(t: dict, rs, W: int, M: int):
     return {"k_candidates": cand / batches, "k_align": align / batches, "k_msa": msa / batches}
 
 
+def calibration_copy(nbytes: int):
+    """One plain device-to-device copy of known size (the runtime's copy kernel), so
+    that profiler byte counters can be scaled against it."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for b in (src, dst):
+        if hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:   # hipMemcpyDeviceToDevice
+        raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
 def cpu_baseline(rs, opts, sample, threads):
     import oracle
     n = min(sample, rs.n)
@@ -116,6 +133,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=300000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-size", type=int, default=0)
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="run one device-to-device copy of this size first (profiler byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the correction stage shards "
@@ -135,6 +154,9 @@ def main():
     log(f"workload {args.workload}: {rs.n} reads, max length {rs.max_length}, simulated in {time.time() - t0:.1f} s")
 
     eng = engine.CareEngine(0, **opts)
+    if args.calib_copy_mb > 0:
+        calibration_copy(args.calib_copy_mb << 20)
+        log(f"calibration copy of {args.calib_copy_mb} MiB done")
     t1 = time.time()
     eng.set_reads(rs)
     t2 = time.time()

@@ -274,7 +274,7 @@ struct CandParams {
     const uint64_t* sig; uint64_t n; int M; int k; int thr; int excludeAmbiguous;
     const int* len; const uint8_t* ambig;
     const Slot* const* slots; const uint32_t* caps; const uint32_t* values;
-    uint64_t b0; int B; int capPerAnchor;
+    uint64_t b0; int B; int capPerAnchor; int hashSlots;
     uint32_t* raw; uint32_t* uniq; unsigned long long* gathered;
 };
 
@@ -290,12 +290,15 @@ __device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint
 
 // Workgroup per anchor: candidate read ids, ascending, unique, self removed
 // (include/corrector.hpp:571-650; include/ordinaryminhasher.hpp:135-221,437-449).
+// The gathered bucket values are deduplicated on insertion into an LDS hash set (the
+// first inserter of a value appends it to the anchor's output area), then only the
+// unique ids are sorted (bitonic, in LDS).
 __global__ void __launch_bounds__(kBlock)
 k_candidates(CandParams p){
-    extern __shared__ uint32_t buf[];
+    extern __shared__ uint32_t hset[];       // hashSlots entries; reused for the sort
     __shared__ uint32_t sOff[kMaxMaps];
     __shared__ uint32_t sPre[kMaxMaps + 1];
-    __shared__ int tmp[kBlock / kWave + 1];
+    __shared__ int sUnique;
     const int i = blockIdx.x;
     const uint64_t a = p.b0 + i;
     const int tid = threadIdx.x;
@@ -310,6 +313,7 @@ k_candidates(CandParams p){
         sOff[tid] = off;
         sPre[tid + 1] = cnt;
     }
+    if(tid == 0) sUnique = 0;
     __syncthreads();
     if(tid == 0){
         sPre[0] = 0;
@@ -317,52 +321,54 @@ k_candidates(CandParams p){
     }
     __syncthreads();
     const int T = int(sPre[kMaxMaps]);
-    int P = 1;
-    while(P < T) P <<= 1;
-    for(int idx = tid; idx < P; idx += kBlock){
-        uint32_t v = 0xFFFFFFFFu;
-        if(idx < T){
-            int lo = 0, hi = kMaxMaps - 1;       // map m with sPre[m] <= idx < sPre[m+1]
-            while(lo < hi){
-                const int mid = (lo + hi + 1) >> 1;
-                if(int(sPre[mid]) <= idx) lo = mid; else hi = mid - 1;
-            }
-            v = p.values[uint64_t(lo) * p.n + sOff[lo] + (idx - sPre[lo])];
+    // table of at least 2T slots, strictly more than T (hashSlots > maps*threshold >= T)
+    int H = 64;
+    while(H < 2 * T && H < p.hashSlots) H <<= 1;
+    for(int q = tid; q < H; q += kBlock) hset[q] = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t* out = p.raw + uint64_t(i) * p.capPerAnchor;
+    const int hbits = 31 - __clz(H);
+    for(int idx = tid; idx < T; idx += kBlock){
+        int lo = 0, hi = kMaxMaps - 1;       // map m with sPre[m] <= idx < sPre[m+1]
+        while(lo < hi){
+            const int mid = (lo + hi + 1) >> 1;
+            if(int(sPre[mid]) <= idx) lo = mid; else hi = mid - 1;
         }
-        buf[idx] = v;
+        const uint32_t v = p.values[uint64_t(lo) * p.n + sOff[lo] + (idx - sPre[lo])];
+        if(uint64_t(v) == a || (p.excludeAmbiguous && p.ambig[v])) continue;
+        uint32_t h = (v * 2654435761u) >> (32 - hbits);
+        for(;;){
+            const uint32_t prev = atomicCAS(&hset[h], 0xFFFFFFFFu, v);
+            if(prev == 0xFFFFFFFFu){
+                out[atomicAdd(&sUnique, 1)] = v;
+                break;
+            }
+            if(prev == v) break;
+            h = (h + 1) & uint32_t(H - 1);
+        }
     }
+    __syncthreads();
+    const int U = sUnique;
+    int P = 1;
+    while(P < U) P <<= 1;
+    for(int q = tid; q < P; q += kBlock) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
     __syncthreads();
     for(int kk = 2; kk <= P; kk <<= 1){
         for(int j = kk >> 1; j > 0; j >>= 1){
             for(int idx = tid; idx < P; idx += kBlock){
                 const int ixj = idx ^ j;
                 if(ixj > idx){
-                    const uint32_t x = buf[idx], y = buf[ixj];
+                    const uint32_t x = hset[idx], y = hset[ixj];
                     const bool up = (idx & kk) == 0;
-                    if((x > y) == up){ buf[idx] = y; buf[ixj] = x; }
+                    if((x > y) == up){ hset[idx] = y; hset[ixj] = x; }
                 }
             }
             __syncthreads();
         }
     }
-    uint32_t* out = p.raw + uint64_t(i) * p.capPerAnchor;
-    int running = 0;
-    for(int base = 0; base < T; base += kBlock){
-        const int idx = base + tid;
-        bool keep = false;
-        uint32_t v = 0;
-        if(idx < T){
-            v = buf[idx];
-            keep = (idx == 0 || buf[idx - 1] != v) && uint64_t(v) != a
-                   && !(p.excludeAmbiguous && p.ambig[v]);
-        }
-        int total;
-        const int pos = blockExclusiveSum(keep ? 1 : 0, tmp, total);
-        if(keep) out[running + pos] = v;
-        running += total;
-    }
+    for(int q = tid; q < U; q += kBlock) out[q] = hset[q];
     if(tid == 0){
-        p.uniq[i] = uint32_t(running);
+        p.uniq[i] = uint32_t(U);
         atomicAdd(p.gathered, (unsigned long long)T);
     }
 }
@@ -440,73 +446,152 @@ struct AlignParams {
 
 struct ShdResult { int score, overlap, shift, nOps, valid; };
 
-// Best shift of one orientation (src/cpu_alignment.cpp:32-497). Lanes take shifts in
-// scan order (0..La-minoverlap, then -1..-(Lc-minoverlap)) 64 at a time. A shift is
-// admissible iff mismatches < int(float(overlap)*maxErrorRate); its score is
-// mismatches + La + Lc - 2*overlap; the reference's strict-'<' scan keeps the first
-// admissible shift of minimum score, and its early break / pruning never discards a
-// shift that could beat the running best, so argmin over (score, scan index) of the
-// admissible shifts gives the same (score, shift). The running best of earlier
-// chunks bounds each lane's mismatch count.
-__device__ ShdResult alignOne(const uint32_t* Ahi, const uint32_t* Alo, const uint32_t* Chi, const uint32_t* Clo,
-                              int La, int Lc, const AlignParams& p, int lane){
-    const int total = La + Lc;
-    const int minoverlap = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
-    const int F = max(La - minoverlap + 1, 0);
-    const int Bn = max(Lc - minoverlap, 0);
-    const int S = F + Bn;
-    int best = total;
-    int bestIdx = -1;
-    for(int base = 0; base < S; base += kWave){
-        const int t = base + lane;
-        int score = INT_MAX;
-        if(t < S){
-            const int s = t < F ? t : -(t - F + 1);
-            const int ov = s >= 0 ? min(La - s, Lc) : min(La, Lc + s);
-            const int maxErr = min(int(float(ov) * p.maxErrorRate), best - total + 2 * ov);
-            if(maxErr > 0 && ov > 0){
-                const uint32_t* Xh = s >= 0 ? Ahi : Chi;
-                const uint32_t* Xl = s >= 0 ? Alo : Clo;
-                const uint32_t* Yh = s >= 0 ? Chi : Ahi;
-                const uint32_t* Yl = s >= 0 ? Clo : Alo;
-                const int sh = s >= 0 ? s : -s;
-                const int ps = sh >> 5;
-                const int bs = sh & 31;
-                const int parts = (ov + 31) >> 5;
-                const int rem = (parts << 5) - ov;
-                int mism = 0;
-                for(int w = 0; w < parts && mism < maxErr; w++){
-                    uint32_t xh = Xh[w + ps];
-                    uint32_t xl = Xl[w + ps];
-                    if(bs){
-                        xh = (xh << bs) | (Xh[w + ps + 1] >> (32 - bs));
-                        xl = (xl << bs) | (Xl[w + ps + 1] >> (32 - bs));
-                    }
-                    uint32_t bits = (xh ^ Yh[w]) | (xl ^ Yl[w]);
-                    if(w == parts - 1 && rem) bits &= 0xFFFFFFFFu << rem;
-                    mism += __popc(bits);
-                }
-                if(mism < maxErr) score = mism + total - 2 * ov;
-            }
-        }
-        const uint64_t key = (uint64_t(uint32_t(score)) << 32) | uint32_t(t);
-        const uint64_t m = waveMinU64(key);
-        const int mscore = int(m >> 32);
-        if(mscore < best){
-            best = mscore;
-            bestIdx = int(uint32_t(m));
-        }
-    }
+// Minimum over the 64 lanes with DPP row shifts/broadcasts (no LDS round trips);
+// the result is read from lane 63 and is wave-uniform.
+__device__ __forceinline__ uint32_t waveMinU32(uint32_t v){
+    constexpr int kId = -1;   // 0xFFFFFFFF, identity of unsigned min
+    uint32_t t;
+    t = uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x111, 0xf, 0xf, false)); v = t < v ? t : v;  // row_shr:1
+    t = uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x112, 0xf, 0xf, false)); v = t < v ? t : v;  // row_shr:2
+    t = uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x114, 0xf, 0xf, false)); v = t < v ? t : v;  // row_shr:4
+    t = uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x118, 0xf, 0xf, false)); v = t < v ? t : v;  // row_shr:8
+    t = uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x142, 0xa, 0xf, false)); v = t < v ? t : v;  // row_bcast:15
+    t = uint32_t(__builtin_amdgcn_update_dpp(kId, int(v), 0x143, 0xc, 0xf, false)); v = t < v ? t : v;  // row_bcast:31
+    return uint32_t(__builtin_amdgcn_readlane(int(v), 63));
+}
+
+// Best shifts of both orientations (src/cpu_alignment.cpp:32-497, called for the
+// forward and the reverse-complement candidate by include/corrector.hpp:925-979).
+//
+// The reference scans shift 0..La-minoverlap (anchor shifted), then -1..-(Lc-minoverlap)
+// (candidate shifted); shift t of that scan order is admissible iff its mismatches are
+// < int(float(overlap)*maxErrorRate), scores mismatches + La + Lc - 2*overlap, and the
+// strict-'<' scan keeps the first admissible shift of minimum score. Its early break and
+// its running-best bound only skip shifts that cannot beat the running best (the bound
+// best-total+2*overlap never grows along a scan direction), so the result is
+// argmin over admissible t of (score, t), or (La+Lc, -Lc) when none is admissible.
+//
+// Lanes take 64 consecutive shifts of one direction at a time, nearest shift 0 first in
+// both directions (largest overlaps, where the best usually is), so later chunks are
+// skipped wave-uniformly once the running bests rule them out. Because chunks are not
+// visited in scan order, a lane may still win a tie when its scan index precedes the
+// running best's, which the bound allows (+1). Keys (score << 16 | t) make the wave
+// minimum the lexicographic (score, t) minimum. Both orientations share the candidate
+// length, hence every per-shift quantity except the running best, and the anchor's
+// planes; they are evaluated in the same pass.
+//
+// Planes are LDS pairs (word i+1, word i): one 64-bit read gives (word i << 32 | word i+1)
+// for the funnel shift, and .y is word i itself.
+struct OrientState {
+    uint32_t key = 0xFFFFFFFFu;
+    int best = 0;
+    int bestT = INT_MAX;
+};
+
+__device__ __forceinline__ uint32_t shiftedWord(const uint2* X, int idx, int rs){
+    const uint2 v = X[idx];
+    return uint32_t(((uint64_t(v.y) << 32) | v.x) >> rs);
+}
+
+__device__ __forceinline__ ShdResult finishOrientation(const OrientState& st, int La, int Lc, int F){
     ShdResult r;
-    const int bestShift = bestIdx < 0 ? -Lc : (bestIdx < F ? bestIdx : -(bestIdx - F + 1));
+    const int total = La + Lc;
+    const int bestShift = st.bestT == INT_MAX ? -Lc : (st.bestT < F ? st.bestT : -(st.bestT - F + 1));
     r.valid = bestShift != -Lc;
     const int cb = max(-bestShift, 0);
     const int ce = min(Lc, La - bestShift);
     r.overlap = ce - cb;
-    r.score = best;
+    r.score = st.best;
     r.shift = bestShift;
-    r.nOps = best - total + 2 * r.overlap;
+    r.nOps = st.best - total + 2 * r.overlap;
     return r;
+}
+
+__device__ void alignBoth(const uint2* Ahi, const uint2* Alo, const uint2* Fhi, const uint2* Flo,
+                          const uint2* Rhi, const uint2* Rlo, int La, int Lc, const AlignParams& p, int lane,
+                          ShdResult& outF, ShdResult& outR){
+    const int total = La + Lc;
+    const int minoverlap = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
+    const int F = max(La - minoverlap + 1, 0);
+    const int Bn = max(Lc - minoverlap, 0);
+    const int nfc = (F + kWave - 1) / kWave;
+    const int nbc = (Bn + kWave - 1) / kWave;
+    OrientState sf, sr;
+    sf.best = total;
+    sr.best = total;
+    bool fwdOpen = nfc > 0, bwdOpen = nbc > 0;
+    for(int c = 0; fwdOpen || bwdOpen; c++){
+        for(int dir = 0; dir < 2; dir++){
+            bool& open = dir == 0 ? fwdOpen : bwdOpen;
+            if(!open) continue;
+            if(c >= (dir == 0 ? nfc : nbc)){ open = false; continue; }
+            const int first = c * kWave;
+            const int region = dir == 0 ? F : Bn;
+            const int tBase = dir == 0 ? 0 : F;
+            // chunk-level bounds at the chunk's largest overlap (its first shift)
+            const int s0 = dir == 0 ? first : -(first + 1);
+            const int ov0 = dir == 0 ? min(La - s0, Lc) : min(La, Lc + s0);
+            const int e0 = int(float(ov0) * p.maxErrorRate);
+            const int b0f = min(e0, sf.best - total + 2 * ov0 + ((tBase + first) < sf.bestT ? 1 : 0));
+            const int b0r = min(e0, sr.best - total + 2 * ov0 + ((tBase + first) < sr.bestT ? 1 : 0));
+            if(b0f <= 0 && b0r <= 0){ open = false; continue; }   // bounds only shrink along a direction
+            const int idx = first + lane;
+            uint32_t keyF = 0xFFFFFFFFu, keyR = 0xFFFFFFFFu;
+            if(idx < region){
+                const int s = dir == 0 ? idx : -(idx + 1);
+                const int t = tBase + idx;
+                const int ov = dir == 0 ? min(La - s, Lc) : min(La, Lc + s);
+                const int e = int(float(ov) * p.maxErrorRate);
+                const int mF = min(e, sf.best - total + 2 * ov + (t < sf.bestT ? 1 : 0));
+                const int mR = min(e, sr.best - total + 2 * ov + (t < sr.bestT ? 1 : 0));
+                if((mF > 0 || mR > 0) && ov > 0){
+                    const int sh = dir == 0 ? s : -s;
+                    const int ps = sh >> 5;
+                    const int rs = 32 - (sh & 31);          // 64-bit funnel: 32 means no shift
+                    const int parts = (ov + 31) >> 5;
+                    const int rem = (parts << 5) - ov;
+                    int misF = 0, misR = 0;
+                    bool actF = mF > 0, actR = mR > 0;
+                    for(int w = 0; w < parts && (actF || actR); w++){
+                        const uint32_t mask = (w == parts - 1 && rem) ? (0xFFFFFFFFu << rem) : 0xFFFFFFFFu;
+                        if(dir == 0){
+                            // anchor shifted (shared), candidate in place
+                            const uint32_t xh = shiftedWord(Ahi, w + ps, rs);
+                            const uint32_t xl = shiftedWord(Alo, w + ps, rs);
+                            if(actF){
+                                misF += __popc(((xh ^ Fhi[w].y) | (xl ^ Flo[w].y)) & mask);
+                                actF = misF < mF;
+                            }
+                            if(actR){
+                                misR += __popc(((xh ^ Rhi[w].y) | (xl ^ Rlo[w].y)) & mask);
+                                actR = misR < mR;
+                            }
+                        }else{
+                            // candidate shifted, anchor in place (shared)
+                            const uint32_t yh = Ahi[w].y;
+                            const uint32_t yl = Alo[w].y;
+                            if(actF){
+                                misF += __popc(((shiftedWord(Fhi, w + ps, rs) ^ yh) | (shiftedWord(Flo, w + ps, rs) ^ yl)) & mask);
+                                actF = misF < mF;
+                            }
+                            if(actR){
+                                misR += __popc(((shiftedWord(Rhi, w + ps, rs) ^ yh) | (shiftedWord(Rlo, w + ps, rs) ^ yl)) & mask);
+                                actR = misR < mR;
+                            }
+                        }
+                    }
+                    if(mF > 0 && misF < mF) keyF = (uint32_t(misF + total - 2 * ov) << 16) | uint32_t(t);
+                    if(mR > 0 && misR < mR) keyR = (uint32_t(misR + total - 2 * ov) << 16) | uint32_t(t);
+                }
+            }
+            const uint32_t wf = waveMinU32(keyF);
+            const uint32_t wr = waveMinU32(keyR);
+            if(wf < sf.key){ sf.key = wf; sf.best = int(wf >> 16); sf.bestT = int(wf & 0xFFFFu); }
+            if(wr < sr.key){ sr.key = wr; sr.best = int(wr >> 16); sr.bestT = int(wr & 0xFFFFu); }
+        }
+    }
+    outF = finishOrientation(sf, La, Lc, F);
+    outR = finishOrientation(sr, La, Lc, F);
 }
 
 // include/alignmentorientation.hpp:14-92 (error bound = estimatedErrorrate * 4.0f).
@@ -541,7 +626,9 @@ constexpr int kAlignWaves = kBlock / kWave;
 
 __global__ void __launch_bounds__(kBlock)
 k_align(AlignParams p){
-    __shared__ uint32_t planes[kAlignWaves][6][kMaxHiLoWords + 1];
+    // per wave: 6 planes (anchor hi/lo, candidate fwd hi/lo, candidate rc hi/lo) stored as
+    // (word i+1, word i) pairs so one 64-bit LDS read feeds a funnel shift
+    __shared__ uint2 planes[kAlignWaves][6][kMaxHiLoWords + 1];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const uint32_t c = blockIdx.x * kAlignWaves + wave;
@@ -554,22 +641,22 @@ k_align(AlignParams p){
         const int W = p.W;
         const uint32_t* ah = p.hilo + a * uint64_t(4 * W);
         const uint32_t* ch = p.hilo + uint64_t(id) * uint64_t(4 * W);
-        for(int w = lane; w <= W; w += kWave){
-            const bool in = w < W;
-            planes[wave][0][w] = in ? ah[w] : 0u;
-            planes[wave][1][w] = in ? ah[W + w] : 0u;
-            planes[wave][2][w] = in ? ch[w] : 0u;
-            planes[wave][3][w] = in ? ch[W + w] : 0u;
-            planes[wave][4][w] = in ? ch[2 * W + w] : 0u;
-            planes[wave][5][w] = in ? ch[3 * W + w] : 0u;
+        const uint32_t* src[6] = {ah, ah + W, ch, ch + W, ch + 2 * W, ch + 3 * W};
+        for(int q = lane; q < 6 * (W + 1); q += kWave){
+            const int pl = q / (W + 1);
+            const int w = q - pl * (W + 1);
+            const uint32_t x0 = w < W ? src[pl][w] : 0u;
+            const uint32_t x1 = w + 1 < W ? src[pl][w + 1] : 0u;
+            planes[wave][pl][w] = make_uint2(x1, x0);
         }
     }
     __syncthreads();
     if(!active) return;
     const int La = p.len[a];
     const int Lc = p.len[id];
-    const ShdResult f = alignOne(planes[wave][0], planes[wave][1], planes[wave][2], planes[wave][3], La, Lc, p, lane);
-    const ShdResult r = alignOne(planes[wave][0], planes[wave][1], planes[wave][4], planes[wave][5], La, Lc, p, lane);
+    ShdResult f, r;
+    alignBoth(planes[wave][0], planes[wave][1], planes[wave][2], planes[wave][3], planes[wave][4], planes[wave][5],
+              La, Lc, p, lane, f, r);
     const int flag = chooseOrientation(f, r, La, p);
     if(lane == 0){
         AlignBest b;
@@ -601,6 +688,7 @@ struct MsaParams {
     float maxErrorRate; float ratio2, ratio3, ratio4; float goodCount; float pairedThreshold;
     float avgThr, minThr, minCovThr; int pct; float pctHalf; int sigThr;
     int maxCols; int maxLen;
+    const uint32_t* hilo; int W; int tile;
     uint8_t* slots; int slotBytes; uint32_t* recBytes;
     int* info;                      // [B][kAnchorInfo] or null
     uint8_t* msaDbg; int dbgStride; // per anchor: 1 int ncol + cols*(4 int + 4 float + int + float + 1 byte)
@@ -630,10 +718,25 @@ __device__ __forceinline__ float orientedQualityWeight(const MsaParams& p, uint3
     return p.qweight[orient == 1 ? q[pos] : q[p.qpitch - 1 - pos]];
 }
 
+// LDS staging area for a tile of candidates (order kept): shift, length, overlap weight,
+// oriented HiLo planes (W words each) and oriented qualities (maxLen bytes each).
+struct MsaTile {
+    int* shift;
+    int* len;
+    float* w;
+    uint32_t* hi;
+    uint32_t* lo;
+    uint8_t* q;
+    const float* qw;     // quality-weight table in LDS
+};
+
 // Builds the MSA of the anchor and the first `nk` working candidates in LDS
-// (src/msa.cpp:9-173). Returns false on an impossible column count.
+// (src/msa.cpp:9-173). Candidates are staged tile by tile (HiLo planes and qualities of
+// the chosen orientation); each thread owns whole columns and adds the anchor, then the
+// candidates in list order, so every column sum is formed in the reference's order.
+// Returns false on an impossible column count.
 __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, const uint32_t* arow, int beg, int nk,
-                         int* cnt, float* wgt, int* cov, float* sup, uint8_t* cons, int* origCov){
+                         int* cnt, float* wgt, int* cov, float* sup, uint8_t* cons, int* origCov, const MsaTile& t){
     int lo = 0, hi = La;
     for(int j = threadIdx.x; j < nk; j += kBlock){
         const int s = p.kShift[beg + j];
@@ -647,35 +750,80 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
     if(ncol > p.maxCols) return false;
     if(threadIdx.x == 0){ sh.ncol = ncol; sh.ab = ab; }
     const uint8_t* aq = p.useQ ? p.qual + p.qpitch * a : nullptr;
+    // anchor first (weight 1.0f * quality weight)
     for(int c = threadIdx.x; c < ncol; c += kBlock){
-        int n0 = 0, n1 = 0, n2 = 0, n3 = 0, cv = 0;
-        float w0 = 0.0f, w1 = 0.0f, w2 = 0.0f, w3 = 0.0f;
+        int n[4] = {0, 0, 0, 0};
+        float w[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        int cv = 0;
         const int pa = c - ab;
-        int abase = -1;
         if(pa >= 0 && pa < La){
-            abase = baseAt(arow, pa);
-            const float w = 1.0f * (p.useQ ? p.qweight[aq[pa]] : 1.0f);
-            if(abase == 0){ n0++; w0 += w; } else if(abase == 1){ n1++; w1 += w; }
-            else if(abase == 2){ n2++; w2 += w; } else { n3++; w3 += w; }
-            cv++;
+            const int b = baseAt(arow, pa);
+            const float x = 1.0f * (p.useQ ? t.qw[aq[pa]] : 1.0f);
+            #pragma unroll
+            for(int q = 0; q < 4; q++){ if(q == b){ n[q]++; w[q] += x; } }
+            cv = 1;
         }
-        for(int j = 0; j < nk; j++){
-            const int s = p.kShift[beg + j];
-            const int lc = p.kLen[beg + j];
-            const int pos = pa - s;
-            if(pos >= 0 && pos < lc){
-                const uint32_t id = p.kId[beg + j];
-                const int orient = p.kOrient[beg + j];
-                const int b = orientedBase(p, id, lc, orient, pos);
-                const float w = p.kW[beg + j] * (p.useQ ? orientedQualityWeight(p, id, orient, pos) : 1.0f);
-                if(b == 0){ n0++; w0 += w; } else if(b == 1){ n1++; w1 += w; }
-                else if(b == 2){ n2++; w2 += w; } else { n3++; w3 += w; }
-                cv++;
+        #pragma unroll
+        for(int q = 0; q < 4; q++){ cnt[4 * c + q] = n[q]; wgt[4 * c + q] = w[q]; }
+        cov[c] = cv;
+    }
+    const int W = p.W;
+    const int T = p.tile;
+    for(int tb = 0; tb < nk; tb += T){
+        const int tn = min(T, nk - tb);
+        __syncthreads();   // previous tile fully consumed
+        for(int q = threadIdx.x; q < tn; q += kBlock){
+            t.shift[q] = p.kShift[beg + tb + q];
+            t.len[q] = p.kLen[beg + tb + q];
+            t.w[q] = p.kW[beg + tb + q];
+        }
+        for(int q = threadIdx.x; q < tn * W; q += kBlock){
+            const int j = q / W;
+            const int wd = q - j * W;
+            const uint32_t id = p.kId[beg + tb + j];
+            const int off = p.kOrient[beg + tb + j] == 1 ? 0 : 2 * W;
+            const uint32_t* src = p.hilo + uint64_t(id) * uint64_t(4 * W) + off;
+            t.hi[q] = src[wd];
+            t.lo[q] = src[W + wd];
+        }
+        if(p.useQ){
+            const int L = p.maxLen;
+            for(int q = threadIdx.x; q < tn * L; q += kBlock){
+                const int j = q / L;
+                const int pos = q - j * L;
+                const uint32_t id = p.kId[beg + tb + j];
+                const uint8_t* qr = p.qual + p.qpitch * id;
+                // reverse-complement candidates read the pitch-wide quality row reversed
+                // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
+                t.q[q] = p.kOrient[beg + tb + j] == 1 ? qr[pos] : qr[p.qpitch - 1 - pos];
             }
         }
-        cnt[4 * c + 0] = n0; cnt[4 * c + 1] = n1; cnt[4 * c + 2] = n2; cnt[4 * c + 3] = n3;
-        wgt[4 * c + 0] = w0; wgt[4 * c + 1] = w1; wgt[4 * c + 2] = w2; wgt[4 * c + 3] = w3;
-        cov[c] = cv;
+        __syncthreads();
+        for(int c = threadIdx.x; c < ncol; c += kBlock){
+            int n0 = cnt[4 * c + 0], n1 = cnt[4 * c + 1], n2 = cnt[4 * c + 2], n3 = cnt[4 * c + 3];
+            float w0 = wgt[4 * c + 0], w1 = wgt[4 * c + 1], w2 = wgt[4 * c + 2], w3 = wgt[4 * c + 3];
+            int cv = cov[c];
+            const int pa = c - ab;
+            for(int j = 0; j < tn; j++){
+                const int pos = pa - t.shift[j];
+                if(pos >= 0 && pos < t.len[j]){
+                    const int sh5 = 31 - (pos & 31);
+                    const int wd = j * W + (pos >> 5);
+                    const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
+                    const float x = t.w[j] * (p.useQ ? t.qw[t.q[j * p.maxLen + pos]] : 1.0f);
+                    if(b == 0){ n0++; w0 += x; } else if(b == 1){ n1++; w1 += x; }
+                    else if(b == 2){ n2++; w2 += x; } else { n3++; w3 += x; }
+                    cv++;
+                }
+            }
+            cnt[4 * c + 0] = n0; cnt[4 * c + 1] = n1; cnt[4 * c + 2] = n2; cnt[4 * c + 3] = n3;
+            wgt[4 * c + 0] = w0; wgt[4 * c + 1] = w1; wgt[4 * c + 2] = w2; wgt[4 * c + 3] = w3;
+            cov[c] = cv;
+        }
+    }
+    __syncthreads();
+    for(int c = threadIdx.x; c < ncol; c += kBlock){
+        const float w0 = wgt[4 * c + 0], w1 = wgt[4 * c + 1], w2 = wgt[4 * c + 2], w3 = wgt[4 * c + 3];
         // findConsensus (src/msa.cpp:135-157)
         int cb = 0;
         float cw = w0;
@@ -686,7 +834,8 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
         const float colW = w0 + w1 + w2 + w3;
         sup[c] = cw / colW;
         // findOrigWeightAndCoverage (src/msa.cpp:159-173)
-        if(abase >= 0) origCov[c] = abase == 0 ? n0 : abase == 1 ? n1 : abase == 2 ? n2 : n3;
+        const int pa = c - ab;
+        if(pa >= 0 && pa < La) origCov[c] = cnt[4 * c + baseAt(arow, pa)];
     }
     __syncthreads();
     return true;
@@ -703,7 +852,22 @@ k_msa(MsaParams p){
     float* sup = reinterpret_cast<float*>(cov + C);
     int* origCov = reinterpret_cast<int*>(sup + C);
     uint8_t* cons = reinterpret_cast<uint8_t*>(origCov + C);
-    uint8_t* corr = cons + C;
+    // 16-byte aligned sub-areas (same layout as msaLdsBytes)
+    uint8_t* corr = smem + (size_t(45) * C + 15) / 16 * 16;
+    __shared__ float sQw[256];
+    MsaTile tile;
+    {
+        uint8_t* tp = corr + ((p.maxLen + 15) / 16) * 16;
+        const int T = p.tile;
+        tile.shift = reinterpret_cast<int*>(tp);
+        tile.len = tile.shift + T;
+        tile.w = reinterpret_cast<float*>(tile.len + T);
+        tile.hi = reinterpret_cast<uint32_t*>(tile.w + T);
+        tile.lo = tile.hi + T * p.W;
+        tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
+        tile.qw = sQw;
+        if(p.useQ) for(int q = threadIdx.x; q < 256; q += kBlock) sQw[q] = p.qweight[q];
+    }
 
     const int i = blockIdx.x;
     const uint64_t a = p.b0 + i;
@@ -790,7 +954,7 @@ k_msa(MsaParams p){
 
     // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
     int nk = n2;
-    if(!buildMsa(p, sh, a, La, arow, beg, nk, cnt, wgt, cov, sup, cons, origCov)){
+    if(!buildMsa(p, sh, a, La, arow, beg, nk, cnt, wgt, cov, sup, cons, origCov, tile)){
         if(tid == 0) atomicAdd(p.error, 1);
         return;
     }
@@ -868,7 +1032,7 @@ k_msa(MsaParams p){
         }
         nk = nn;
         rounds++;
-        if(!buildMsa(p, sh, a, La, arow, beg, nk, cnt, wgt, cov, sup, cons, origCov)){
+        if(!buildMsa(p, sh, a, La, arow, beg, nk, cnt, wgt, cov, sup, cons, origCov, tile)){
             if(tid == 0) atomicAdd(p.error, 1);
             return;
         }
@@ -1156,15 +1320,28 @@ int buildTables(care_ctx* ctx){
     return CARE_OK;
 }
 
-size_t msaLdsBytes(int maxCols, int maxLen){
-    return size_t(maxCols) * (16 + 16 + 4 + 4 + 4 + 1) + size_t(maxLen) + 16;
+// dynamic LDS of k_msa: column arrays, corrected anchor, candidate tile
+size_t msaLdsBytes(int maxCols, int maxLen, int W, int tile, bool useQ){
+    size_t b = size_t(maxCols) * (16 + 16 + 4 + 4 + 4 + 1);
+    b = (b + 15) / 16 * 16 + (size_t(maxLen) + 15) / 16 * 16;
+    b += size_t(tile) * (12 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0));
+    return b + 16;
+}
+
+// Candidates per LDS tile: as many as fit next to the column arrays in ~40 KB, 16..64.
+int msaTile(int maxCols, int maxLen, int W, bool useQ){
+    const size_t base = msaLdsBytes(maxCols, maxLen, W, 0, useQ);
+    const size_t per = 12 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0);
+    const size_t budget = 40 * 1024;
+    size_t t = base < budget ? (budget - base) / per : 16;
+    return int(std::min<size_t>(64, std::max<size_t>(16, t)));
 }
 
 int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     const int capPerAnchor = ctx->M * ctx->thr;
-    int P = 1;
-    while(P < capPerAnchor) P <<= 1;
+    int P = 64;
+    while(P <= capPerAnchor) P <<= 1;      // hash slots > the most values an anchor can gather
     // counters: [0] gathered bucket values, [1] error flag
     HIPCHECK(ctx->counters.ensure(32));
     HIPCHECK(ctx->raw.ensure(size_t(B) * capPerAnchor * sizeof(uint32_t)));
@@ -1177,7 +1354,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     cp.sig = ctx->sig.as<uint64_t>(); cp.n = ctx->n; cp.M = ctx->M; cp.k = ctx->k; cp.thr = ctx->thr;
     cp.excludeAmbiguous = ctx->o.exclude_ambiguous; cp.len = ctx->len.as<int>(); cp.ambig = ctx->ambig.as<uint8_t>();
     cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
-    cp.b0 = b0; cp.B = B; cp.capPerAnchor = capPerAnchor; cp.raw = ctx->raw.as<uint32_t>(); cp.uniq = ctx->uniq.as<uint32_t>();
+    cp.b0 = b0; cp.B = B; cp.capPerAnchor = capPerAnchor; cp.hashSlots = P; cp.raw = ctx->raw.as<uint32_t>(); cp.uniq = ctx->uniq.as<uint32_t>();
     cp.gathered = ctx->counters.as<unsigned long long>();
     const size_t candLds = size_t(P) * sizeof(uint32_t);
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
@@ -1258,12 +1435,14 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     mp.pctHalf = mp.pct * 0.5f;
     mp.sigThr = int(int(ctx->o.estimated_coverage) * 0.3f);
     mp.maxCols = maxCols; mp.maxLen = ctx->maxLen;
+    mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W;
+    mp.tile = msaTile(maxCols, ctx->maxLen, ctx->W, ctx->o.use_quality_scores != 0);
     mp.slots = ctx->slots.as<uint8_t>(); mp.slotBytes = slotBytes; mp.recBytes = ctx->recBytes.as<uint32_t>();
     mp.info = ctx->o.keep_intermediates ? ctx->info.as<int>() : nullptr;
     mp.msaDbg = ctx->o.keep_intermediates ? ctx->msaDbg.as<uint8_t>() : nullptr;
     mp.dbgStride = ctx->dbgStride;
     mp.error = ctx->counters.as<int>() + 2;
-    const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen);
+    const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen, ctx->W, mp.tile, ctx->o.use_quality_scores != 0);
     if(msaLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_msa, hipFuncAttributeMaxDynamicSharedMemorySize, int(msaLds)));
     k_msa<<<B, kBlock, msaLds, st>>>(mp);
     HIPCHECK(hipGetLastError());
@@ -1400,11 +1579,15 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
     ctx->k = ctx->o.kmerlength ? ctx->o.kmerlength : (maxLen < 160 ? 20 : 32);
     ctx->M = ctx->o.num_hash_maps;
     ctx->thr = resultsPerMapThreshold(ctx->o.estimated_coverage);
-    if(ctx->M * ctx->thr > kMaxSortEntries) return ctx->fail(CARE_EINVAL, "maps*threshold = %d exceeds %d", ctx->M * ctx->thr, kMaxSortEntries);
+    if(ctx->M * ctx->thr >= kMaxSortEntries) return ctx->fail(CARE_EINVAL, "maps*threshold = %d must be below %d", ctx->M * ctx->thr, kMaxSortEntries);
     const int minoverlap = std::max(ctx->o.min_overlap, int(float(maxLen) * ctx->o.min_overlap_ratio));
     ctx->maxCols = maxLen + 2 * std::max(0, maxLen - std::min(minoverlap, maxLen)) + 2;
     ctx->maxCols = std::max(ctx->maxCols, 3 * maxLen + 2);
-    if(msaLdsBytes(ctx->maxCols, maxLen) > 150 * 1024) return ctx->fail(CARE_EINVAL, "reads too long for the LDS-resident MSA");
+    {
+        const bool q = ctx->o.use_quality_scores != 0;
+        if(msaLdsBytes(ctx->maxCols, maxLen, ctx->W, msaTile(ctx->maxCols, maxLen, ctx->W, q), q) > 150 * 1024)
+            return ctx->fail(CARE_EINVAL, "reads too long for the LDS-resident MSA");
+    }
     ctx->dbgStride = int((4 + size_t(ctx->maxCols) * (16 + 16 + 4 + 4 + 1) + 15) / 16 * 16);
 
     HIPCHECK(ctx->enc.ensure(num_reads * pitch_ints * 4));
