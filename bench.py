@@ -101,6 +101,31 @@ def calibration_copy(nbytes: int):
     hip.hipFree(dst)
 
 
+def cpu_baseline_child(args):
+    """CPU-baseline leg in a child process (regenerates the same seeded workload).
+
+    The reference's table construction runs on its own thread pool, whose scheduler can
+    miss a wake-up and stall (include/parallel/parallel_task_queue.h:306-313 waits on a
+    condition variable that notify_task_complete() signals without holding its mutex), so
+    the leg runs under a time limit and is retried once."""
+    import subprocess
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", args.workload,
+           "--seed", str(args.seed), "--cpu-sample", str(args.cpu_sample), "--cpu-threads", str(args.cpu_threads)]
+    if args.no_quality:
+        cmd.append("--no-quality")
+    for attempt in range(2):
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=150)
+        except subprocess.TimeoutExpired:
+            print(f"[bench] CPU baseline attempt {attempt + 1} timed out", file=sys.stderr, flush=True)
+            continue
+        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+        if r.returncode == 0 and lines:
+            return json.loads(lines[-1])
+        print(f"[bench] CPU baseline attempt {attempt + 1} failed: {r.stderr[-500:]}", file=sys.stderr, flush=True)
+    return None
+
+
 def cpu_baseline(rs, opts, sample, threads):
     import oracle
     n = min(sample, rs.n)
@@ -135,7 +160,18 @@ def main():
     ap.add_argument("--batch-size", type=int, default=0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (profiler byte-counter calibration)")
+    ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_baseline_only:
+        s = sim.workload(args.workload, seed=args.seed)
+        use_q = not args.no_quality
+        rs = reads.storage_from_sim(s, use_qualities=use_q)
+        copts = dict(use_quality_scores=int(use_q), paired=int(rs.paired))
+        if rs.max_length >= 160:
+            copts["kmerlength"] = 20
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(json.dumps(cpu_baseline(rs, copts, args.cpu_sample, threads)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the correction stage shards "
               f"by anchor range across GPUs, which is not run here.", file=sys.stderr)
@@ -192,13 +228,13 @@ def main():
                 "launches_per_step": int(batches)}
     before, after = error_rates(s, rs, data)
     eng.close()
+    log(f"GPU: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
+        f"stages {json.dumps({k: round(t[k], 2) for k in ('candidates_ms', 'align_ms', 'msa_ms', 'output_ms')})}")
 
     cpu = None
     if args.cpu_sample > 0:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        log(f"CPU baseline on {threads} threads, {args.cpu_sample} anchors")
-        copts = {k: v for k, v in opts.items() if k != "batch_size"}
-        cpu = cpu_baseline(rs, copts, args.cpu_sample, threads)
+        log(f"CPU baseline, {args.cpu_sample} anchors")
+        cpu = cpu_baseline_child(args)
 
     line = {
         "metric": "corrected reads/s on 1 MI355X (plain path); post-correction error rate",

@@ -724,11 +724,50 @@ struct MsaTile {
     int* shift;
     int* len;
     float* w;
+    uint32_t* id;
+    int* orient;
     uint32_t* hi;
     uint32_t* lo;
     uint8_t* q;
     const float* qw;     // quality-weight table in LDS
 };
+
+// Stages candidates [tb, tb+tn) of the working list into the LDS tile: one thread per
+// candidate for the metadata and its oriented HiLo planes; qualities with threads over
+// positions (coalesced), candidates unrolled so several rows are in flight.
+// Caller synchronizes before (tile free) and after (tile complete).
+__device__ __forceinline__ void stageTile(const MsaParams& p, const MsaTile& t, int beg, int tb, int tn,
+                                          int tid, int nthreads){
+    const int W = p.W;
+    for(int q = tid; q < tn; q += nthreads){
+        const int d = beg + tb + q;
+        const uint32_t id = p.kId[d];
+        const int orient = p.kOrient[d];
+        t.shift[q] = p.kShift[d];
+        t.len[q] = p.kLen[d];
+        t.w[q] = p.kW[d];
+        t.id[q] = id;
+        t.orient[q] = orient;
+        const uint32_t* src = p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 1 ? 0 : 2 * W);
+        for(int wd = 0; wd < W; wd++){
+            t.hi[q * W + wd] = src[wd];
+            t.lo[q * W + wd] = src[W + wd];
+        }
+    }
+    if(!p.useQ) return;
+    __syncthreads();
+    const int L = p.maxLen;
+    #pragma unroll 4
+    for(int j = 0; j < tn; j++){
+        const uint8_t* qr = p.qual + p.qpitch * t.id[j];
+        const bool fwd = t.orient[j] == 1;
+        for(int pos = tid; pos < L; pos += nthreads){
+            // reverse-complement candidates read the pitch-wide quality row reversed
+            // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
+            t.q[j * L + pos] = fwd ? qr[pos] : qr[p.qpitch - 1 - pos];
+        }
+    }
+}
 
 // Builds the MSA of the anchor and the first `nk` working candidates in LDS
 // (src/msa.cpp:9-173). Candidates are staged tile by tile (HiLo planes and qualities of
@@ -772,32 +811,7 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();   // previous tile fully consumed
-        for(int q = threadIdx.x; q < tn; q += kBlock){
-            t.shift[q] = p.kShift[beg + tb + q];
-            t.len[q] = p.kLen[beg + tb + q];
-            t.w[q] = p.kW[beg + tb + q];
-        }
-        for(int q = threadIdx.x; q < tn * W; q += kBlock){
-            const int j = q / W;
-            const int wd = q - j * W;
-            const uint32_t id = p.kId[beg + tb + j];
-            const int off = p.kOrient[beg + tb + j] == 1 ? 0 : 2 * W;
-            const uint32_t* src = p.hilo + uint64_t(id) * uint64_t(4 * W) + off;
-            t.hi[q] = src[wd];
-            t.lo[q] = src[W + wd];
-        }
-        if(p.useQ){
-            const int L = p.maxLen;
-            for(int q = threadIdx.x; q < tn * L; q += kBlock){
-                const int j = q / L;
-                const int pos = q - j * L;
-                const uint32_t id = p.kId[beg + tb + j];
-                const uint8_t* qr = p.qual + p.qpitch * id;
-                // reverse-complement candidates read the pitch-wide quality row reversed
-                // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
-                t.q[q] = p.kOrient[beg + tb + j] == 1 ? qr[pos] : qr[p.qpitch - 1 - pos];
-            }
-        }
+        stageTile(p, t, beg, tb, tn, threadIdx.x, kBlock);
         __syncthreads();
         for(int c = threadIdx.x; c < ncol; c += kBlock){
             int n0 = cnt[4 * c + 0], n1 = cnt[4 * c + 1], n2 = cnt[4 * c + 2], n3 = cnt[4 * c + 3];
@@ -862,7 +876,9 @@ k_msa(MsaParams p){
         tile.shift = reinterpret_cast<int*>(tp);
         tile.len = tile.shift + T;
         tile.w = reinterpret_cast<float*>(tile.len + T);
-        tile.hi = reinterpret_cast<uint32_t*>(tile.w + T);
+        tile.id = reinterpret_cast<uint32_t*>(tile.w + T);
+        tile.orient = reinterpret_cast<int*>(tile.id + T);
+        tile.hi = reinterpret_cast<uint32_t*>(tile.orient + T);
         tile.lo = tile.hi + T * p.W;
         tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
         tile.qw = sQw;
@@ -1143,6 +1159,444 @@ k_msa(MsaParams p){
     }
 }
 
+// ---------------------------------------------------------------- MSA, one wavefront per anchor
+
+// Inclusive Hillis-Steele pattern over the wave with DPP (row shifts 1,2,4,8, then the
+// row broadcasts of lanes 15 and 31); lane 63 ends with the reduction of all lanes.
+template<class Op>
+__device__ __forceinline__ int waveReduce(int v, Op op, int identity){
+    int t;
+    t = __builtin_amdgcn_update_dpp(identity, v, 0x111, 0xf, 0xf, false); v = op(v, t);
+    t = __builtin_amdgcn_update_dpp(identity, v, 0x112, 0xf, 0xf, false); v = op(v, t);
+    t = __builtin_amdgcn_update_dpp(identity, v, 0x114, 0xf, 0xf, false); v = op(v, t);
+    t = __builtin_amdgcn_update_dpp(identity, v, 0x118, 0xf, 0xf, false); v = op(v, t);
+    t = __builtin_amdgcn_update_dpp(identity, v, 0x142, 0xa, 0xf, false); v = op(v, t);
+    t = __builtin_amdgcn_update_dpp(identity, v, 0x143, 0xc, 0xf, false); v = op(v, t);
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+__device__ __forceinline__ uint64_t lanesBelow(int lane){ return lane ? (~0ull >> (64 - lane)) : 0ull; }
+
+// Per-lane MSA columns: lane owns columns lane + 64*k, k < KC.
+template<int KC>
+struct WaveMsa {
+    float w[KC][4];
+    uint32_t c01[KC];   // count of A (low 16 bits) and C (high 16 bits)
+    uint32_t c23[KC];   // count of G and T
+    float sup[KC];
+    uint32_t cons;      // 2 bits per slot
+    uint32_t abase;     // 2 bits per slot: anchor base of anchor columns
+    uint32_t isAnchor;  // 1 bit per slot
+    int ncol, ab;
+};
+
+template<int KC>
+__device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
+    const uint32_t v = b < 2 ? m.c01[k] : m.c23[k];
+    return int((b & 1) ? (v >> 16) : (v & 0xFFFFu));
+}
+
+// MSA of the anchor and the first nk working candidates (src/msa.cpp:9-173), one wave:
+// every lane adds the anchor then the candidates, in list order, to the columns it owns.
+template<int KC>
+__device__ bool buildMsaWave(const MsaParams& p, uint64_t a, int La, const uint32_t* arow, int beg, int nk,
+                             WaveMsa<KC>& m, const MsaTile& t, int lane){
+    int lo = 0, hi = La;
+    for(int j = lane; j < nk; j += kWave){
+        const int s = p.kShift[beg + j];
+        lo = min(lo, s);
+        hi = max(hi, s + p.kLen[beg + j]);
+    }
+    lo = waveReduce(lo, OpMin(), INT_MAX);
+    hi = waveReduce(hi, OpMax(), INT_MIN);
+    const int ncol = hi - lo;
+    const int ab = max(-lo, 0);
+    if(ncol > KC * kWave || ncol > p.maxCols) return false;
+    m.ncol = ncol;
+    m.ab = ab;
+    m.abase = 0;
+    m.isAnchor = 0;
+    const uint8_t* aq = p.useQ ? p.qual + p.qpitch * a : nullptr;
+    #pragma unroll
+    for(int k = 0; k < KC; k++){
+        #pragma unroll
+        for(int b = 0; b < 4; b++) m.w[k][b] = 0.0f;
+        m.c01[k] = 0;
+        m.c23[k] = 0;
+        const int pa = lane + kWave * k - ab;
+        if(pa >= 0 && pa < La){
+            const int b = baseAt(arow, pa);
+            const float x = 1.0f * (p.useQ ? t.qw[aq[pa]] : 1.0f);
+            m.w[k][0] = b == 0 ? x : 0.0f;
+            m.w[k][1] = b == 1 ? x : 0.0f;
+            m.w[k][2] = b == 2 ? x : 0.0f;
+            m.w[k][3] = b == 3 ? x : 0.0f;
+            m.c01[k] = b == 0 ? 1u : (b == 1 ? 0x10000u : 0u);
+            m.c23[k] = b == 2 ? 1u : (b == 3 ? 0x10000u : 0u);
+            m.abase |= uint32_t(b) << (2 * k);
+            m.isAnchor |= 1u << k;
+        }
+    }
+    const int W = p.W;
+    const int T = p.tile;
+    for(int tb = 0; tb < nk; tb += T){
+        const int tn = min(T, nk - tb);
+        __syncthreads();
+        stageTile(p, t, beg, tb, tn, lane, kWave);
+        __syncthreads();
+        for(int j = 0; j < tn; j++){
+            const int cs = ab + t.shift[j];          // first column of candidate j (>= 0)
+            const int ln = t.len[j];
+            const float wj = t.w[j];
+            const int k0 = cs >> 6;
+            const int k1 = (cs + ln - 1) >> 6;
+            #pragma unroll
+            for(int k = 0; k < KC; k++){
+                if(k < k0 || k > k1) continue;        // wave-uniform
+                const int pos = lane + kWave * k - cs;
+                if(pos >= 0 && pos < ln){
+                    const int sh5 = 31 - (pos & 31);
+                    const int wd = j * W + (pos >> 5);
+                    const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
+                    const float x = wj * (p.useQ ? t.qw[t.q[j * p.maxLen + pos]] : 1.0f);
+                    if(b == 0){ m.w[k][0] += x; m.c01[k] += 1u; }
+                    else if(b == 1){ m.w[k][1] += x; m.c01[k] += 0x10000u; }
+                    else if(b == 2){ m.w[k][2] += x; m.c23[k] += 1u; }
+                    else { m.w[k][3] += x; m.c23[k] += 0x10000u; }
+                }
+            }
+        }
+    }
+    // consensus and support (src/msa.cpp:135-157)
+    m.cons = 0;
+    #pragma unroll
+    for(int k = 0; k < KC; k++){
+        int cb = 0;
+        float cw = m.w[k][0];
+        if(m.w[k][1] > cw){ cb = 1; cw = m.w[k][1]; }
+        if(m.w[k][2] > cw){ cb = 2; cw = m.w[k][2]; }
+        if(m.w[k][3] > cw){ cb = 3; cw = m.w[k][3]; }
+        m.cons |= uint32_t(cb) << (2 * k);
+        const float colW = m.w[k][0] + m.w[k][1] + m.w[k][2] + m.w[k][3];
+        m.sup[k] = (lane + kWave * k < ncol) ? cw / colW : 0.0f;
+    }
+    return true;
+}
+
+// Filters, MSA, refinement, decision and record of one anchor per wavefront
+// (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734).
+template<int KC>
+__global__ void __launch_bounds__(kWave)
+k_msa_wave(MsaParams p){
+    extern __shared__ uint8_t smem[];
+    __shared__ float sQw[256];
+    const int lane = threadIdx.x;
+    MsaTile tile;
+    float* sSup;
+    {
+        uint8_t* tp = smem;
+        const int T = p.tile;
+        tile.shift = reinterpret_cast<int*>(tp);
+        tile.len = tile.shift + T;
+        tile.w = reinterpret_cast<float*>(tile.len + T);
+        tile.id = reinterpret_cast<uint32_t*>(tile.w + T);
+        tile.orient = reinterpret_cast<int*>(tile.id + T);
+        tile.hi = reinterpret_cast<uint32_t*>(tile.orient + T);
+        tile.lo = tile.hi + T * p.W;
+        tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
+        tile.qw = sQw;
+        sSup = reinterpret_cast<float*>(tp + (size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * p.maxLen : 0) + 15) / 16 * 16);
+        if(p.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = p.qweight[q];
+    }
+    const int i = blockIdx.x;
+    const uint64_t a = p.b0 + i;
+    const int La = p.len[a];
+    const uint32_t* arow = p.enc + p.pitch * a;
+    const int beg = int(p.offsets[i]);
+    const int n0 = int(p.offsets[i + 1]) - beg;
+    int* info = p.info ? p.info + size_t(i) * kAnchorInfo : nullptr;
+    if(lane == 0) p.recBytes[i] = 0;
+    if(info && lane < kAnchorInfo) info[lane] = 0;
+    if(n0 == 0) return;
+    const uint64_t below = lanesBelow(lane);
+
+    // 1) drop candidates without an orientation (include/corrector.hpp:982-1050)
+    int n1 = 0;
+    for(int base = 0; base < n0; base += kWave){
+        const int j = base + lane;
+        AlignBest b{};
+        bool keep = false;
+        if(j < n0){ b = p.best[beg + j]; keep = b.orientation != 3; }
+        const uint64_t mask = __ballot(keep);
+        if(keep){
+            const int d = beg + n1 + __popcll(mask & below);
+            const uint32_t id = p.ids[beg + j];
+            p.kId[d] = id;
+            p.kShift[d] = b.shift;
+            p.kLen[d] = p.len[id];
+            p.kNops[d] = b.nOps;
+            p.kOvl[d] = b.overlap;
+            p.kOrient[d] = uint8_t(b.orientation);
+            p.kPaired[d] = p.paired ? p.paired[beg + j] : 0;
+        }
+        n1 += __popcll(mask);
+    }
+    if(n1 == 0) return;
+
+    // 2) mismatch-ratio filter (include/corrector.hpp:1053-1149; paired :1151-1211)
+    float threshold = -std::numeric_limits<float>::max();
+    if(!p.isPaired){
+        int c0 = 0, c1 = 0, c2 = 0;
+        for(int base = 0; base < n1; base += kWave){
+            const int j = base + lane;
+            float ratio = 1e30f;
+            if(j < n1) ratio = float(p.kNops[beg + j]) / float(p.kOvl[beg + j]);
+            c0 += __popcll(__ballot(j < n1 && ratio < p.ratio2));
+            c1 += __popcll(__ballot(j < n1 && ratio < p.ratio3));
+            c2 += __popcll(__ballot(j < n1 && ratio < p.ratio4));
+        }
+        if(c0 > 0 || c1 > 0 || c2 > 0){
+            if(c0 >= p.goodCount) threshold = p.ratio2;
+            else if(c1 >= p.goodCount) threshold = p.ratio3;
+            else if(c2 >= p.goodCount) threshold = p.ratio4;
+        }
+    }
+    int n2 = 0;
+    for(int base = 0; base < n1; base += kWave){
+        const int j = base + lane;
+        bool keep = false;
+        uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0;
+        if(j < n1){
+            id = p.kId[beg + j]; s = p.kShift[beg + j]; l = p.kLen[beg + j]; no = p.kNops[beg + j];
+            ov = p.kOvl[beg + j]; o = p.kOrient[beg + j]; pr = p.kPaired[beg + j];
+            const float ratio = float(no) / float(ov);
+            keep = p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold);
+        }
+        const uint64_t mask = __ballot(keep);
+        if(keep){
+            const int d = beg + n2 + __popcll(mask & below);
+            p.kId[d] = id; p.kShift[d] = s; p.kLen[d] = l; p.kNops[d] = no; p.kOvl[d] = ov;
+            p.kOrient[d] = o; p.kPaired[d] = pr;
+            // overlap weight (include/hostdevicefunctions.cuh:93-97)
+            p.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
+        }
+        n2 += __popcll(mask);
+    }
+    if(info && lane == 0) info[0] = n2;
+    if(n2 == 0) return;
+
+    // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
+    WaveMsa<KC> m;
+    int nk = n2;
+    if(!buildMsaWave<KC>(p, a, La, arow, beg, nk, m, tile, lane)){
+        if(lane == 0) atomicAdd(p.error, 1);
+        return;
+    }
+    int rounds = 0;
+    for(int it = 0; it < 5; it++){
+        bool mis = false;
+        int colCand = INT_MAX;
+        #pragma unroll
+        for(int k = 0; k < KC; k++){
+            if(!((m.isAnchor >> k) & 1u)) continue;
+            const int ci = int((m.cons >> (2 * k)) & 3u);
+            if(ci != int((m.abase >> (2 * k)) & 3u)) mis = true;
+            int sig = -1;
+            for(int b = 0; b < 4; b++){
+                if(b != ci && p.sigThr <= countOf(m, k, b)) sig = b;
+            }
+            if(sig != -1) colCand = min(colCand, lane + kWave * k);
+        }
+        if(__ballot(mis) == 0) break;
+        const int col = waveReduce(colCand, OpMin(), INT_MAX);
+        if(col == INT_MAX) break;
+        int mySig = -1;
+        #pragma unroll
+        for(int k = 0; k < KC; k++){
+            if(k == (col >> 6)){
+                const int ci = int((m.cons >> (2 * k)) & 3u);
+                for(int b = 0; b < 4; b++){
+                    if(b != ci && p.sigThr <= countOf(m, k, b)) mySig = b;
+                }
+            }
+        }
+        const int sigBase = __builtin_amdgcn_readlane(mySig, col & 63);
+        const int origBase = baseAt(arow, col - m.ab);
+        const bool keepMatching = origBase == sigBase;
+        bool anyRemove = false, veryGood = false;
+        for(int base = 0; base < nk; base += kWave){
+            const int j = base + lane;
+            bool rem = false;
+            if(j < nk){
+                const int rb = m.ab + p.kShift[beg + j];
+                const int l = p.kLen[beg + j];
+                if(col >= rb && col < rb + l){
+                    const int b = orientedBase(p, p.kId[beg + j], l, p.kOrient[beg + j], col - rb);
+                    rem = keepMatching ? (b != sigBase) : (b == sigBase);
+                }
+            }
+            anyRemove |= __ballot(rem) != 0;
+            veryGood |= __ballot(rem && p.kW[beg + j] >= 0.90f) != 0;
+        }
+        // a round that removes nothing leaves the MSA, and hence every later round, unchanged
+        if(!anyRemove || veryGood) break;
+        int nn = 0;
+        for(int base = 0; base < nk; base += kWave){
+            const int j = base + lane;
+            bool keep = false;
+            uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0; float w = 0.0f;
+            if(j < nk){
+                id = p.kId[beg + j]; s = p.kShift[beg + j]; l = p.kLen[beg + j]; no = p.kNops[beg + j];
+                ov = p.kOvl[beg + j]; o = p.kOrient[beg + j]; pr = p.kPaired[beg + j]; w = p.kW[beg + j];
+                const int rb = m.ab + s;
+                keep = true;
+                if(col >= rb && col < rb + l){
+                    const int b = orientedBase(p, id, l, o, col - rb);
+                    keep = !(keepMatching ? (b != sigBase) : (b == sigBase));
+                }
+            }
+            const uint64_t mask = __ballot(keep);
+            if(keep){
+                const int d = beg + nn + __popcll(mask & below);
+                p.kId[d] = id; p.kShift[d] = s; p.kLen[d] = l; p.kNops[d] = no; p.kOvl[d] = ov;
+                p.kOrient[d] = o; p.kPaired[d] = pr; p.kW[d] = w;
+            }
+            nn += __popcll(mask);
+        }
+        nk = nn;
+        rounds++;
+        if(!buildMsaWave<KC>(p, a, La, arow, beg, nk, m, tile, lane)){
+            if(lane == 0) atomicAdd(p.error, 1);
+            return;
+        }
+    }
+    const int ab = m.ab;
+    const int ncol = m.ncol;
+    if(p.msaDbg){
+        uint8_t* d = p.msaDbg + size_t(i) * p.dbgStride;
+        const int C = p.maxCols;
+        int* dn = reinterpret_cast<int*>(d);
+        int* dc = dn + 1;
+        float* dw = reinterpret_cast<float*>(dc + 4 * C);
+        int* dv = reinterpret_cast<int*>(dw + 4 * C);
+        float* ds = reinterpret_cast<float*>(dv + C);
+        uint8_t* dk = reinterpret_cast<uint8_t*>(ds + C);
+        if(lane == 0) dn[0] = ncol;
+        #pragma unroll
+        for(int k = 0; k < KC; k++){
+            const int c = lane + kWave * k;
+            if(c < ncol){
+                int cv = 0;
+                for(int b = 0; b < 4; b++){ dc[4 * c + b] = countOf(m, k, b); dw[4 * c + b] = m.w[k][b]; cv += countOf(m, k, b); }
+                dv[c] = cv; ds[c] = m.sup[k]; dk[c] = "ACGT"[(m.cons >> (2 * k)) & 3u];
+            }
+        }
+    }
+    if(info && lane == 0){ info[1] = nk; info[2] = ncol; info[3] = ab; info[8] = rounds; }
+    // msa.nCandidates == 0: cannot be corrected (src/msa.cpp:388-394)
+    if(nk == 0) return;
+
+    // 4) properties and decision (src/msa.cpp:304-453)
+    int minSupBits = INT_MAX;
+    int mincov = INT_MAX, maxcov = INT_MIN;
+    #pragma unroll
+    for(int k = 0; k < KC; k++){
+        const int c = lane + kWave * k;
+        if((m.isAnchor >> k) & 1u){
+            // supports are non-negative, so their bit patterns order like the floats
+            minSupBits = min(minSupBits, __float_as_int(m.sup[k]));
+            const int cv = countOf(m, k, 0) + countOf(m, k, 1) + countOf(m, k, 2) + countOf(m, k, 3);
+            mincov = min(mincov, cv);
+            maxcov = max(maxcov, cv);
+            sSup[c - ab] = m.sup[k];
+        }
+    }
+    const float minSupport = __int_as_float(waveReduce(minSupBits, OpMin(), INT_MAX));
+    mincov = waveReduce(mincov, OpMin(), INT_MAX);
+    maxcov = waveReduce(maxcov, OpMax(), INT_MIN);
+    __syncthreads();
+    float avgSupport = 0.0f;
+    if(lane == 0){
+        float sum = 0.0f;                       // sequential, as std::accumulate
+        for(int q = 0; q < La; q++) sum += sSup[q];
+        avgSupport = sum / La;
+    }
+    avgSupport = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(avgSupport)));
+    const bool allGood = fgeq(avgSupport, p.avgThr) && fgeq(minSupport, p.minThr) && fgeq(float(mincov), p.minCovThr);
+    bool isHQ = false;
+    if(allGood){
+        int smallest = 100;
+        for(int percent = p.pct; percent >= 0; percent--){
+            const float factor = percent / 100.0f;
+            const float avgT = 1.0f - 1.0f * factor;
+            const float minT = 1.0f - 3.0f * factor;
+            if(fgeq(avgSupport, avgT) && fgeq(minSupport, minT)) smallest = percent;
+        }
+        isHQ = fgeq(float(mincov), p.minCovThr) && fleq(float(smallest), p.pctHalf);
+    }
+    uint32_t corrBases = 0;     // 2 bits per slot
+    uint32_t editBits = 0;      // 1 bit per slot
+    int nEdits = 0;
+    #pragma unroll
+    for(int k = 0; k < KC; k++){
+        const bool anc = (m.isAnchor >> k) & 1u;
+        int cb = 0;
+        bool e = false;
+        if(anc){
+            const int ob = int((m.abase >> (2 * k)) & 3u);
+            const int cn = int((m.cons >> (2 * k)) & 3u);
+            const int oc = countOf(m, k, ob);
+            if(allGood) cb = cn;
+            else cb = (m.sup[k] > 0.90f && oc <= 2) ? cn : ob;
+            e = cb != ob;
+        }
+        corrBases |= uint32_t(cb) << (2 * k);
+        editBits |= uint32_t(e) << k;
+        nEdits += __popcll(__ballot(e));
+    }
+    if(info && lane == 0){
+        info[4] = isHQ; info[5] = 1; info[6] = mincov; info[7] = maxcov;
+        info[9] = __float_as_int(avgSupport); info[10] = __float_as_int(minSupport);
+    }
+
+    // 5) record (include/corrector.hpp:1639-1761, include/correctedsequence.hpp:299-387);
+    // HQ records without edits are not stored (src/correct_cpu.cpp:263-273)
+    const bool ambiguous = p.ambig[a] != 0;
+    const bool useEdits = !ambiguous && nEdits <= La / 7;
+    if(isHQ && useEdits && nEdits == 0) return;
+    const uint32_t payload = useEdits ? uint32_t(4 + 5 * nEdits) : uint32_t(4 + La);
+    uint8_t* rec = p.slots + size_t(i) * p.slotBytes;
+    if(lane == 0){
+        const uint32_t rid = uint32_t(a);
+        const uint32_t flags = (uint32_t(isHQ) << 31) | (uint32_t(useEdits) << 30) | payload;
+        memcpy(rec, &rid, 4);
+        memcpy(rec + 4, &flags, 4);
+        const int first = useEdits ? nEdits : La;
+        memcpy(rec + 8, &first, 4);
+        p.recBytes[i] = 8 + payload;
+        if(info) info[11] = int(8 + payload);
+    }
+    int running = 0;
+    #pragma unroll
+    for(int k = 0; k < KC; k++){
+        const bool anc = (m.isAnchor >> k) & 1u;
+        const int q = lane + kWave * k - ab;
+        const int cb = int((corrBases >> (2 * k)) & 3u);
+        if(useEdits){
+            const bool e = (editBits >> k) & 1u;
+            const uint64_t mask = __ballot(e);
+            if(e){
+                const int r = running + __popcll(mask & below);
+                memcpy(rec + 12 + 4 * r, &q, 4);
+                rec[12 + 4 * nEdits + r] = uint8_t("ACGT"[cb]);
+            }
+            running += __popcll(mask);
+        }else if(anc){
+            rec[12 + q] = uint8_t("ACGT"[cb]);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_gather(const uint8_t* slots, int slotBytes, const uint32_t* recBytes, const uint64_t* outOff, uint8_t* out, int B){
     const int i = blockIdx.x;
@@ -1324,14 +1778,31 @@ int buildTables(care_ctx* ctx){
 size_t msaLdsBytes(int maxCols, int maxLen, int W, int tile, bool useQ){
     size_t b = size_t(maxCols) * (16 + 16 + 4 + 4 + 4 + 1);
     b = (b + 15) / 16 * 16 + (size_t(maxLen) + 15) / 16 * 16;
-    b += size_t(tile) * (12 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0));
+    b += size_t(tile) * (20 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0));
     return b + 16;
+}
+
+// Register columns per lane of k_msa_wave for an MSA of at most maxCols columns
+// (0: too wide, use the workgroup kernel k_msa).
+int waveMsaColumnsPerLane(int maxCols){
+    const int need = (maxCols + kWave - 1) / kWave;
+    if(need <= 4) return 4;
+    if(need <= 8) return 8;
+    if(need <= 12) return 12;
+    if(need <= 16) return 16;
+    return 0;
+}
+
+// dynamic LDS of k_msa_wave: candidate tile + anchor-column supports
+size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
+    size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * maxLen : 0);
+    return (b + 15) / 16 * 16 + size_t(maxLen) * 4 + 16;
 }
 
 // Candidates per LDS tile: as many as fit next to the column arrays in ~40 KB, 16..64.
 int msaTile(int maxCols, int maxLen, int W, bool useQ){
     const size_t base = msaLdsBytes(maxCols, maxLen, W, 0, useQ);
-    const size_t per = 12 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0);
+    const size_t per = 20 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0);
     const size_t budget = 40 * 1024;
     size_t t = base < budget ? (budget - base) / per : 16;
     return int(std::min<size_t>(64, std::max<size_t>(16, t)));
@@ -1442,9 +1913,23 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     mp.msaDbg = ctx->o.keep_intermediates ? ctx->msaDbg.as<uint8_t>() : nullptr;
     mp.dbgStride = ctx->dbgStride;
     mp.error = ctx->counters.as<int>() + 2;
-    const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen, ctx->W, mp.tile, ctx->o.use_quality_scores != 0);
-    if(msaLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_msa, hipFuncAttributeMaxDynamicSharedMemorySize, int(msaLds)));
-    k_msa<<<B, kBlock, msaLds, st>>>(mp);
+    const int kc = waveMsaColumnsPerLane(maxCols);
+    if(kc > 0){
+        // one wavefront per anchor, MSA columns in registers
+        const bool q = ctx->o.use_quality_scores != 0;
+        mp.tile = q ? 32 : 64;
+        const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q);
+        switch(kc){
+            case 4: k_msa_wave<4><<<B, kWave, lds, st>>>(mp); break;
+            case 8: k_msa_wave<8><<<B, kWave, lds, st>>>(mp); break;
+            case 12: k_msa_wave<12><<<B, kWave, lds, st>>>(mp); break;
+            default: k_msa_wave<16><<<B, kWave, lds, st>>>(mp); break;
+        }
+    }else{
+        const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen, ctx->W, mp.tile, ctx->o.use_quality_scores != 0);
+        if(msaLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_msa, hipFuncAttributeMaxDynamicSharedMemorySize, int(msaLds)));
+        k_msa<<<B, kBlock, msaLds, st>>>(mp);
+    }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
 

@@ -38,6 +38,13 @@ CASES = {
     # tandem repeats: low-complexity reads with many equally scored alignment shifts
     "se_tandem": (dict(genome_length=60_000, tandem_fraction=0.3, tandem_period_max=6, read_length_min=150,
                        read_length_max=150, coverage=30.0, q_first=30.0, q_last=20.0, seed=18), {}),
+    # 280..330 bp (16 register columns per lane in the wave MSA kernel), k = 32 as the
+    # reference picks for reads of 160 bp and more
+    "len300_q": (dict(genome_length=60_000, read_length_min=280, read_length_max=330, coverage=30.0,
+                      q_first=30.0, q_last=20.0, seed=19), dict(use_quality_scores=1, kmerlength=32)),
+    # 360..420 bp: MSA wider than the register kernel, workgroup kernel path
+    "len400": (dict(genome_length=60_000, read_length_min=360, read_length_max=420, coverage=30.0,
+                    q_first=30.0, q_last=20.0, seed=20), dict(kmerlength=32)),
     # short reads below k and near the minimum overlap
     "short": (dict(genome_length=20_000, read_length_min=15, read_length_max=60, coverage=40.0,
                    q_first=30.0, q_last=24.0, seed=17), dict(kmerlength=20, estimated_coverage=40.0)),
