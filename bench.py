@@ -81,7 +81,14 @@ def algorithmic_bytes(t: dict, rs, W: int, M: int):
     # qualities once when enabled; per anchor: its row, offsets, record slot bytes
     qbytes = rs.quals.shape[1] if rs.quals is not None else 0
     msa = C * (20 + pitch_b + qbytes) + A * (pitch_b + 8 + qbytes) + t["record_bytes"]
-    return {"k_candidates": cand / batches, "k_align": align / batches, "k_msa": msa / batches}
+    # k_fused (one launch per pass) moves the same inputs once, with the candidate lists and
+    # alignments kept in wave scratch: per anchor length+flag, signatures, table slots, its
+    # planes (8W), row, qualities; every gathered value; per candidate its length (4 B),
+    # fwd+rc planes (16W), 2-bit row and qualities; the record bytes
+    fused = (A * (4 + 1 + 24 * M + 8 * W + pitch_b + qbytes) + 4 * G
+             + C * (4 + 16 * W + pitch_b + qbytes) + t["record_bytes"])
+    return {"k_candidates": cand / batches, "k_align": align / batches, "k_msa": msa / batches,
+            "k_fused": fused / batches}
 
 
 def calibration_copy(nbytes: int):
@@ -158,6 +165,9 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=300000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-size", type=int, default=0)
+    ap.add_argument("--engine", choices=["auto", "staged", "fused"], default="auto",
+                    help="auto = fused persistent kernel where the reads fit it, else staged kernels")
+    ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="run one device-to-device copy of this size first (profiler byte-counter calibration)")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
@@ -184,7 +194,8 @@ def main():
     s = sim.workload(args.workload, seed=args.seed)
     use_q = not args.no_quality
     rs = reads.storage_from_sim(s, use_qualities=use_q)
-    opts = dict(use_quality_scores=int(use_q), paired=int(rs.paired), batch_size=args.batch_size)
+    opts = dict(use_quality_scores=int(use_q), paired=int(rs.paired), batch_size=args.batch_size,
+                engine={"auto": 0, "staged": 1, "fused": 2}[args.engine], waves_per_cu=args.waves_per_cu)
     if rs.max_length >= 160:
         opts["kmerlength"] = 20
     log(f"workload {args.workload}: {rs.n} reads, max length {rs.max_length}, simulated in {time.time() - t0:.1f} s")
@@ -218,8 +229,11 @@ def main():
     W = max(1, (rs.max_length + 31) // 32)
     per_launch = algorithmic_bytes(t, rs, W, eng.opts.num_hash_maps)
     batches = max(1, t["batches"])
-    kernel_ms = {"k_candidates": t["cand_kernel_ms"] / batches, "k_align": t["align_kernel_ms"] / batches,
-                 "k_msa": t["msa_kernel_ms"] / batches}
+    if t["fused_kernel_ms"] > 0:
+        kernel_ms = {"k_fused": t["fused_kernel_ms"] / batches}
+    else:
+        kernel_ms = {"k_candidates": t["cand_kernel_ms"] / batches, "k_align": t["align_kernel_ms"] / batches,
+                     "k_msa": t["msa_kernel_ms"] / batches}
     dominant = max(kernel_ms, key=kernel_ms.get)
     achieved = per_launch[dominant] / (kernel_ms[dominant] / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -252,7 +266,7 @@ def main():
         "config": {"workload": args.workload, "reads": rs.n, "read_length": [int(rs.lengths.min()), rs.max_length],
                    "coverage": 30, "k": int(eng.opts.kmerlength) or (20 if rs.max_length < 160 else 32),
                    "maps": int(eng.opts.num_hash_maps), "quality_scores": use_q, "paired": bool(rs.paired),
-                   "path": "plain"},
+                   "path": "plain", "engine": "fused" if "k_fused" in kernel_ms else "staged"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "error_rate_before": round(before, 6),

@@ -54,8 +54,10 @@ typedef struct care_opts {
     float estimated_errorrate;      /* 0.06 */
     float m_coverage;               /* 0.6 */
     float paired_filter_threshold;  /* 0.06 */
-    int batch_size;                 /* anchors per device batch, 0 = automatic */
-    int keep_intermediates;         /* 1: keep per-batch stage outputs for care_hip_inspect_* */
+    int batch_size;                 /* staged engine: anchors per device batch, 0 = automatic */
+    int keep_intermediates;         /* 1: keep per-batch stage outputs for care_hip_inspect_* (staged engine) */
+    int engine;                     /* 0 = automatic (staged), 1 = staged per-stage kernels, 2 = fused persistent kernel */
+    int waves_per_cu;               /* fused engine: resident waves per compute unit, 0 = default */
 } care_opts;
 
 typedef struct care_ctx care_ctx;
@@ -70,6 +72,7 @@ typedef struct care_timings {
     double align_kernel_ms;   /* sum of k_align durations (one launch per batch) */
     double cand_kernel_ms;    /* sum of k_candidates durations */
     double msa_kernel_ms;     /* sum of k_msa durations */
+    double fused_kernel_ms;   /* fused engine: duration of the persistent kernel */
     uint64_t anchors;
     uint64_t candidates;      /* anchor/candidate pairs aligned */
     uint64_t records;

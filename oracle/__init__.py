@@ -98,6 +98,7 @@ def cpu_lib():
         lib.carecpu_records_data.argtypes = [vp]
         lib.carecpu_records_data.restype = vp
         lib.carecpu_last_seconds.argtypes = [vp]
+        lib.carecpu_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
         lib.carecpu_last_seconds.restype = ctypes.c_double
         _cpu = lib
     return _cpu
@@ -129,6 +130,7 @@ def ref_lib():
         lib.ref_hash.argtypes = [vp, i32, i32, i32, vp]
         lib.ref_candidates.argtypes = [vp, ctypes.c_uint, vp, i32]
         lib.ref_align.argtypes = [vp, i32, vp, i32, i32, ctypes.c_float, ctypes.c_float, vp]
+        lib.ref_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
         lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
         lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 8
         _ref = lib
@@ -308,6 +310,25 @@ def cpu_align(a, la, c, lc, min_overlap=30, max_error_rate=0.2, ratio=0.3):
     c = np.ascontiguousarray(c, np.uint32)
     cpu_lib().carecpu_align(a.ctypes.data, la, c.ctypes.data, lc, min_overlap, max_error_rate, ratio, out.ctypes.data)
     return out
+
+
+def _flag_pairs(fn, c1, c2):
+    c1 = np.ascontiguousarray(c1, dtype=np.uint32)
+    c2 = np.ascontiguousarray(c2, dtype=np.uint32)
+    f1 = np.zeros(len(c1), np.uint8)
+    f2 = np.zeros(len(c2), np.uint8)
+    fn(_p(c1), len(c1), _p(c2), len(c2), _p(f1), _p(f2))
+    return f1.astype(bool), f2.astype(bool)
+
+
+def cpu_flag_pairs(c1, c2):
+    """Paired-candidate flags of two mates' sorted candidate lists (oracle restatement)."""
+    return _flag_pairs(cpu_lib().carecpu_flag_pairs, c1, c2)
+
+
+def ref_flag_pairs(c1, c2):
+    """The same flags from the reference's findPositionsOfPairedReadIds."""
+    return _flag_pairs(ref_lib().ref_flag_pairs, c1, c2)
 
 
 def parse_records(data: bytes):

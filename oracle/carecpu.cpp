@@ -827,6 +827,14 @@ int carecpu_trace_anchor(carecpu_ctx* ctx, uint32_t read_id, carecpu_trace* trac
     return 0;
 }
 
+void carecpu_flag_pairs(const uint32_t* c1, int n1, const uint32_t* c2, int n2, uint8_t* f1, uint8_t* f2){
+    std::vector<uint32_t> a(c1, c1 + n1), b(c2, c2 + n2);
+    std::vector<uint8_t> fa, fb;
+    flagPairs(a, b, fa, fb);
+    std::copy(fa.begin(), fa.end(), f1);
+    std::copy(fb.begin(), fb.end(), f2);
+}
+
 size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads){
     if(threads > 0) omp_set_num_threads(threads);
     const bool paired = ctx->o.paired != 0;

@@ -90,6 +90,9 @@ void carecpu_signatures(carecpu_ctx* ctx, uint64_t first, uint64_t n, uint64_t* 
 int carecpu_bucket(carecpu_ctx* ctx, int map, uint64_t key, const uint32_t** values);
 
 int carecpu_candidates(carecpu_ctx* ctx, uint32_t read_id, uint32_t* out, int cap);
+/* Paired-candidate flags of two mates' sorted, unique candidate lists
+   (include/util.hpp:1153-1200 over :1016-1141). */
+void carecpu_flag_pairs(const uint32_t* c1, int n1, const uint32_t* c2, int n2, uint8_t* f1, uint8_t* f2);
 
 void carecpu_align(const uint32_t* anchor, int la, const uint32_t* cand, int lc,
                    int min_overlap, float max_error_rate, float min_overlap_ratio, int* out5);

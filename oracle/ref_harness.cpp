@@ -338,6 +338,18 @@ int ref_candidates(ref_ctx* c, unsigned int readId, unsigned int* out, int cap){
 
 // One shifted-Hamming alignment (reference routine) of a 2-bit anchor and a 2-bit
 // candidate; out = {score, overlap, shift, nOps, isValid}.
+// include/util.hpp:1153 as called by include/corrector.hpp:785-801
+void ref_flag_pairs(const unsigned int* c1, int n1, const unsigned int* c2, int n2, unsigned char* f1, unsigned char* f2){
+    std::vector<int> p1(std::min(n1, n2)), p2(std::min(n1, n2));
+    auto ends = findPositionsOfPairedReadIds(c1, c1 + n1, c2, c2 + n2, p1.begin(), p2.begin());
+    p1.erase(ends.first, p1.end());
+    p2.erase(ends.second, p2.end());
+    std::fill(f1, f1 + n1, 0);
+    std::fill(f2, f2 + n2, 0);
+    for(int i : p1) f1[i] = 1;
+    for(int i : p2) f2[i] = 1;
+}
+
 void ref_align(const unsigned int* a, int la, const unsigned int* cnd, int lc, int min_overlap,
                float maxErrorRate, float min_overlap_ratio, int* out){
     care::cpu::shd::CpuAlignmentHandle handle;

@@ -507,7 +507,7 @@ __device__ __forceinline__ ShdResult finishOrientation(const OrientState& st, in
     return r;
 }
 
-__device__ void alignBoth(const uint2* Ahi, const uint2* Alo, const uint2* Fhi, const uint2* Flo,
+__device__ __forceinline__ void alignBoth(const uint2* Ahi, const uint2* Alo, const uint2* Fhi, const uint2* Flo,
                           const uint2* Rhi, const uint2* Rlo, int La, int Lc, const AlignParams& p, int lane,
                           ShdResult& outF, ShdResult& outR){
     const int total = La + Lc;
@@ -595,7 +595,7 @@ __device__ void alignBoth(const uint2* Ahi, const uint2* Alo, const uint2* Fhi, 
 }
 
 // include/alignmentorientation.hpp:14-92 (error bound = estimatedErrorrate * 4.0f).
-__device__ int chooseOrientation(const ShdResult& f, const ShdResult& r, int La, const AlignParams& p){
+__device__ __forceinline__ int chooseOrientation(const ShdResult& f, const ShdResult& r, int La, const AlignParams& p){
     const int ratioOverlap = int(La * p.minOverlapRatio);
     const int minimumOverlap = ratioOverlap > p.minOverlap ? ratioOverlap : p.minOverlap;
     const float maxErr = p.orientErrorRate;
@@ -1199,7 +1199,7 @@ __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
 // MSA of the anchor and the first nk working candidates (src/msa.cpp:9-173), one wave:
 // every lane adds the anchor then the candidates, in list order, to the columns it owns.
 template<int KC>
-__device__ bool buildMsaWave(const MsaParams& p, uint64_t a, int La, const uint32_t* arow, int beg, int nk,
+__device__ __forceinline__ bool buildMsaWave(const MsaParams& p, uint64_t a, int La, const uint32_t* arow, int beg, int nk,
                              WaveMsa<KC>& m, const MsaTile& t, int lane){
     int lo = 0, hi = La;
     for(int j = lane; j < nk; j += kWave){
@@ -1285,35 +1285,30 @@ __device__ bool buildMsaWave(const MsaParams& p, uint64_t a, int La, const uint3
 
 // Filters, MSA, refinement, decision and record of one anchor per wavefront
 // (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734).
+// Carves the k_msa_wave / k_fused candidate tile and the anchor-support array out of `tp`
+// (layout sized by waveMsaLdsBytes).
+__device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, const float* qw, MsaTile& tile, float*& sSup){
+    const int T = p.tile;
+    tile.shift = reinterpret_cast<int*>(tp);
+    tile.len = tile.shift + T;
+    tile.w = reinterpret_cast<float*>(tile.len + T);
+    tile.id = reinterpret_cast<uint32_t*>(tile.w + T);
+    tile.orient = reinterpret_cast<int*>(tile.id + T);
+    tile.hi = reinterpret_cast<uint32_t*>(tile.orient + T);
+    tile.lo = tile.hi + T * p.W;
+    tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
+    tile.qw = qw;
+    sSup = reinterpret_cast<float*>(tp + (size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * p.maxLen : 0) + 15) / 16 * 16);
+}
+
+// Filters, MSA, refinement, decision and record of anchor `a` by one wavefront
+// (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734). Its candidates
+// are p.ids/p.best/p.paired[beg, beg+n0); the record goes to slot `i`.
 template<int KC>
-__global__ void __launch_bounds__(kWave)
-k_msa_wave(MsaParams p){
-    extern __shared__ uint8_t smem[];
-    __shared__ float sQw[256];
-    const int lane = threadIdx.x;
-    MsaTile tile;
-    float* sSup;
-    {
-        uint8_t* tp = smem;
-        const int T = p.tile;
-        tile.shift = reinterpret_cast<int*>(tp);
-        tile.len = tile.shift + T;
-        tile.w = reinterpret_cast<float*>(tile.len + T);
-        tile.id = reinterpret_cast<uint32_t*>(tile.w + T);
-        tile.orient = reinterpret_cast<int*>(tile.id + T);
-        tile.hi = reinterpret_cast<uint32_t*>(tile.orient + T);
-        tile.lo = tile.hi + T * p.W;
-        tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
-        tile.qw = sQw;
-        sSup = reinterpret_cast<float*>(tp + (size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * p.maxLen : 0) + 15) / 16 * 16);
-        if(p.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = p.qweight[q];
-    }
-    const int i = blockIdx.x;
-    const uint64_t a = p.b0 + i;
+__device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, int i, int beg, int n0, const MsaTile& tile,
+                              float* sSup, int lane){
     const int La = p.len[a];
     const uint32_t* arow = p.enc + p.pitch * a;
-    const int beg = int(p.offsets[i]);
-    const int n0 = int(p.offsets[i + 1]) - beg;
     int* info = p.info ? p.info + size_t(i) * kAnchorInfo : nullptr;
     if(lane == 0) p.recBytes[i] = 0;
     if(info && lane < kAnchorInfo) info[lane] = 0;
@@ -1597,6 +1592,280 @@ k_msa_wave(MsaParams p){
     }
 }
 
+template<int KC>
+__global__ void __launch_bounds__(kWave)
+k_msa_wave(MsaParams p){
+    extern __shared__ uint8_t smem[];
+    __shared__ float sQw[256];
+    const int lane = threadIdx.x;
+    MsaTile tile;
+    float* sSup;
+    carveWaveTile(p, smem, sQw, tile, sSup);
+    if(p.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = p.qweight[q];
+    __syncthreads();
+    const int i = blockIdx.x;
+    const int beg = int(p.offsets[i]);
+    const int n0 = int(p.offsets[i + 1]) - beg;
+    msaAnchorWave<KC>(p, p.b0 + i, i, beg, n0, tile, sSup, lane);
+}
+
+// ---------------------------------------------------------------- fused persistent kernel
+
+// Inclusive prefix sum over the wave (DPP row shifts, then row broadcasts).
+__device__ __forceinline__ int waveInclusiveSum(int v){
+    int t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); v += t;
+    return v;
+}
+
+struct FusedParams {
+    MsaParams m;             // storage, thresholds and record slots (per-wave lists set in the kernel)
+    CandParams c;            // tables
+    AlignParams al;          // alignment thresholds
+    uint32_t* counter;       // next work unit
+    unsigned long long* aligned;   // candidates aligned (sum of per-anchor candidate counts)
+    uint64_t first;          // first anchor
+    uint32_t units;          // anchors (SE) or mate pairs (PE)
+    uint8_t* scratch;        // wave-private scratch, scratchStride bytes per wave
+    size_t scratchStride;
+    int cap;                 // maps * threshold: most candidates an anchor can have
+};
+
+// Per-wave scratch layout (cap entries each).
+struct WaveScratch {
+    uint32_t* idsA; uint32_t* idsB; AlignBest* best; uint8_t* pairedA; uint8_t* pairedB;
+    uint32_t* kId; int* kShift; int* kLen; int* kNops; int* kOvl; float* kW; uint8_t* kOrient; uint8_t* kPaired;
+};
+
+__host__ __device__ inline size_t waveScratchBytes(int cap){
+    return size_t(cap) * (4 + 4 + 16 + 1 + 1 + 4 + 4 * 4 + 4 + 1 + 1) + 256;
+}
+
+__device__ inline WaveScratch carveScratch(uint8_t* base, int cap){
+    WaveScratch w;
+    uint8_t* q = base;
+    w.best = reinterpret_cast<AlignBest*>(q); q += size_t(cap) * 16;
+    w.idsA = reinterpret_cast<uint32_t*>(q); q += size_t(cap) * 4;
+    w.idsB = reinterpret_cast<uint32_t*>(q); q += size_t(cap) * 4;
+    w.kId = reinterpret_cast<uint32_t*>(q); q += size_t(cap) * 4;
+    w.kShift = reinterpret_cast<int*>(q); q += size_t(cap) * 4;
+    w.kLen = reinterpret_cast<int*>(q); q += size_t(cap) * 4;
+    w.kNops = reinterpret_cast<int*>(q); q += size_t(cap) * 4;
+    w.kOvl = reinterpret_cast<int*>(q); q += size_t(cap) * 4;
+    w.kW = reinterpret_cast<float*>(q); q += size_t(cap) * 4;
+    w.pairedA = q; q += cap;
+    w.pairedB = q; q += cap;
+    w.kOrient = q; q += cap;
+    w.kPaired = q;
+    return w;
+}
+
+// Candidate ids of anchor a by one wave (include/corrector.hpp:571-650): the same
+// lookup / LDS hash-set dedup / sort as k_candidates. Returns the count; ids in out[].
+__device__ __forceinline__ int candidatesWave(const CandParams& p, uint64_t a, uint32_t* hset, uint32_t* sOff, uint32_t* sPre,
+                              int* sUnique, uint32_t* out, int lane){
+    const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
+    uint32_t cnt = 0, off = 0;
+    if(valid && lane < p.M){
+        const uint64_t key = p.sig[uint64_t(lane) * p.n + a];
+        cnt = lookup(p.slots[lane], p.caps[lane], key, &off);
+        if(cnt > uint32_t(p.thr)) cnt = 0;
+    }
+    const int incl = waveInclusiveSum(int(cnt));
+    if(lane < kMaxMaps){ sOff[lane] = off; sPre[lane + 1] = uint32_t(incl); }
+    if(lane == 0){ sPre[0] = 0; *sUnique = 0; }
+    const int T = __builtin_amdgcn_readlane(incl, 63);
+    if(lane == 0 && T) atomicAdd(p.gathered, (unsigned long long)T);
+    int H = 64;
+    while(H < 2 * T && H < p.hashSlots) H <<= 1;
+    __syncthreads();
+    for(int q = lane; q < H; q += kWave) hset[q] = 0xFFFFFFFFu;
+    __syncthreads();
+    const int hbits = 31 - __clz(H);
+    for(int idx = lane; idx < T; idx += kWave){
+        int lo = 0, hi = kMaxMaps - 1;
+        while(lo < hi){
+            const int mid = (lo + hi + 1) >> 1;
+            if(int(sPre[mid]) <= idx) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t v = p.values[uint64_t(lo) * p.n + sOff[lo] + (idx - sPre[lo])];
+        if(uint64_t(v) == a || (p.excludeAmbiguous && p.ambig[v])) continue;
+        uint32_t h = (v * 2654435761u) >> (32 - hbits);
+        for(;;){
+            const uint32_t prev = atomicCAS(&hset[h], 0xFFFFFFFFu, v);
+            if(prev == 0xFFFFFFFFu){ out[atomicAdd(sUnique, 1)] = v; break; }
+            if(prev == v) break;
+            h = (h + 1) & uint32_t(H - 1);
+        }
+    }
+    __syncthreads();
+    const int U = *sUnique;
+    int P = 1;
+    while(P < U) P <<= 1;
+    for(int q = lane; q < P; q += kWave) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
+    __syncthreads();
+    for(int kk = 2; kk <= P; kk <<= 1){
+        for(int j = kk >> 1; j > 0; j >>= 1){
+            for(int idx = lane; idx < P; idx += kWave){
+                const int ixj = idx ^ j;
+                if(ixj > idx){
+                    const uint32_t x = hset[idx], y = hset[ixj];
+                    const bool up = (idx & kk) == 0;
+                    if((x > y) == up){ hset[idx] = y; hset[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for(int q = lane; q < U; q += kWave) out[q] = hset[q];
+    __syncthreads();
+    return U;
+}
+
+__device__ __forceinline__ bool sortedContains(const uint32_t* v, int n, uint32_t x){
+    int lo = 0, hi = n;
+    while(lo < hi){
+        const int mid = (lo + hi) >> 1;
+        if(v[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo < n && v[lo] == x;
+}
+
+// Alignments of all candidates of one anchor (both orientations + orientation choice),
+// candidates in order; the next candidate's planes are loaded while the current one is
+// aligned (double-buffered LDS).
+__device__ __forceinline__ void alignAllWave(const AlignParams& p, uint64_t a, const uint32_t* ids, int U, AlignBest* best,
+                             uint2* planeA, uint2* planeC, int lane){
+    const int W = p.W;
+    const int stride = W + 1;
+    const uint32_t* ah = p.hilo + a * uint64_t(4 * W);
+    for(int q = lane; q < 2 * stride; q += kWave){
+        const int pl = q / stride, w = q - pl * stride;
+        const uint32_t x0 = w < W ? ah[pl * W + w] : 0u;
+        const uint32_t x1 = w + 1 < W ? ah[pl * W + w + 1] : 0u;
+        planeA[q] = make_uint2(x1, x0);
+    }
+    const int La = p.len[a];
+    // prefetch registers: up to 3 pair entries per lane (4 planes of at most 33 entries)
+    uint2 pre[3];
+    int nextLen = 0;
+    auto fetch = [&](int c){
+        const uint32_t id = ids[c];
+        const uint32_t* ch = p.hilo + uint64_t(id) * uint64_t(4 * W);
+        #pragma unroll
+        for(int r = 0; r < 3; r++){
+            const int q = lane + kWave * r;
+            uint2 v = make_uint2(0u, 0u);
+            if(q < 4 * stride){
+                const int pl = q / stride, w = q - pl * stride;
+                const uint32_t x0 = w < W ? ch[pl * W + w] : 0u;
+                const uint32_t x1 = w + 1 < W ? ch[pl * W + w + 1] : 0u;
+                v = make_uint2(x1, x0);
+            }
+            pre[r] = v;
+        }
+        nextLen = p.len[id];
+    };
+    if(U > 0) fetch(0);
+    for(int c = 0; c < U; c++){
+        uint2* buf = planeC + (c & 1) * 4 * stride;
+        #pragma unroll
+        for(int r = 0; r < 3; r++){
+            const int q = lane + kWave * r;
+            if(q < 4 * stride) buf[q] = pre[r];
+        }
+        const int Lc = nextLen;
+        if(c + 1 < U) fetch(c + 1);
+        __syncthreads();
+        ShdResult f, r;
+        alignBoth(planeA, planeA + stride, buf, buf + stride, buf + 2 * stride, buf + 3 * stride, La, Lc, p, lane, f, r);
+        const int flag = chooseOrientation(f, r, La, p);
+        if(lane == 0){
+            AlignBest b;
+            const ShdResult& sres = flag == 2 ? r : f;
+            b.shift = sres.shift; b.overlap = sres.overlap; b.nOps = sres.nOps; b.orientation = flag;
+            best[c] = b;
+        }
+    }
+    __syncthreads();
+}
+
+// Persistent fused pass: each wave takes work units (an anchor, or a mate pair) from a
+// counter and runs candidates -> pair flags -> alignments -> filters/MSA/refinement/
+// decision -> record for them, keeping per-anchor lists in its private scratch.
+template<int KC>
+__global__ void __launch_bounds__(kWave)
+k_fused(FusedParams fp){
+    extern __shared__ uint8_t smem[];
+    __shared__ float sQw[256];
+    __shared__ uint32_t sOff[kMaxMaps];
+    __shared__ uint32_t sPre[kMaxMaps + 1];
+    __shared__ int sUnique;
+    __shared__ uint32_t sUnit;
+    const int lane = threadIdx.x;
+    MsaParams mp = fp.m;
+    const WaveScratch ws = carveScratch(fp.scratch + size_t(blockIdx.x) * fp.scratchStride, fp.cap);
+    mp.kId = ws.kId; mp.kShift = ws.kShift; mp.kLen = ws.kLen; mp.kNops = ws.kNops; mp.kOvl = ws.kOvl;
+    mp.kW = ws.kW; mp.kOrient = ws.kOrient; mp.kPaired = ws.kPaired; mp.best = ws.best;
+    MsaTile tile;
+    float* sSup;
+    carveWaveTile(mp, smem, sQw, tile, sSup);
+    uint32_t* hset = reinterpret_cast<uint32_t*>(smem);
+    const int stride = fp.al.W + 1;
+    uint2* planeA = reinterpret_cast<uint2*>(smem);
+    uint2* planeC = planeA + 2 * stride;
+    if(mp.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = mp.qweight[q];
+    __syncthreads();
+    const bool paired = mp.isPaired != 0;
+    for(;;){
+        if(lane == 0) sUnit = atomicAdd(fp.counter, 1u);
+        __syncthreads();
+        const uint32_t unit = sUnit;
+        __syncthreads();
+        if(unit >= fp.units) break;
+        if(!paired){
+            const uint64_t a = fp.first + unit;
+            const int U = candidatesWave(fp.c, a, hset, sOff, sPre, &sUnique, ws.idsA, lane);
+            if(lane == 0 && U) atomicAdd(fp.aligned, (unsigned long long)U);
+            alignAllWave(fp.al, a, ws.idsA, U, ws.best, planeA, planeC, lane);
+            mp.ids = ws.idsA;
+            mp.paired = nullptr;
+            msaAnchorWave<KC>(mp, a, int(unit), 0, U, tile, sSup, lane);
+        }else{
+            const uint64_t a0 = fp.first + 2 * uint64_t(unit);
+            const int U0 = candidatesWave(fp.c, a0, hset, sOff, sPre, &sUnique, ws.idsA, lane);
+            const int U1 = candidatesWave(fp.c, a0 + 1, hset, sOff, sPre, &sUnique, ws.idsB, lane);
+            if(lane == 0 && U0 + U1) atomicAdd(fp.aligned, (unsigned long long)(U0 + U1));
+            // pair flags: the reference's merge (include/util.hpp:1016-1200) flags x in one
+            // mate's list exactly when x^1 (the other read of x's pair) is in the other list
+            for(int j = lane; j < U0; j += kWave) ws.pairedA[j] = sortedContains(ws.idsB, U1, ws.idsA[j] ^ 1u);
+            for(int j = lane; j < U1; j += kWave) ws.pairedB[j] = sortedContains(ws.idsA, U0, ws.idsB[j] ^ 1u);
+            __syncthreads();
+            for(int mate = 0; mate < 2; mate++){
+                const uint64_t a = a0 + mate;
+                const uint32_t* ids = mate == 0 ? ws.idsA : ws.idsB;
+                const int U = mate == 0 ? U0 : U1;
+                alignAllWave(fp.al, a, ids, U, ws.best, planeA, planeC, lane);
+                mp.ids = ids;
+                mp.paired = mate == 0 ? ws.pairedA : ws.pairedB;
+                msaAnchorWave<KC>(mp, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_zero_u32(uint32_t* v, int n){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < n) v[i] = 0u;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_gather(const uint8_t* slots, int slotBytes, const uint32_t* recBytes, const uint64_t* outOff, uint8_t* out, int B){
     const int i = blockIdx.x;
@@ -1656,7 +1925,7 @@ struct care_ctx {
     // batch working memory
     DBuf raw, uniq, offsets, ids, anchorOf, best, dbg, paired;
     DBuf kId, kShift, kLen, kNops, kOvl, kW, kOrient, kPaired;
-    DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp;
+    DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp, scratch;
     std::vector<uint8_t> records;
     size_t numRecords = 0;
 
@@ -1806,6 +2075,157 @@ int msaTile(int maxCols, int maxLen, int W, bool useQ){
     const size_t budget = 40 * 1024;
     size_t t = base < budget ? (budget - base) / per : 16;
     return int(std::min<size_t>(64, std::max<size_t>(16, t)));
+}
+
+// Storage pointers and thresholds shared by the staged and the fused paths. The float
+// thresholds are computed here, on the host, with the reference's expressions.
+void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
+    mp.enc = ctx->enc.as<uint32_t>(); mp.pitch = ctx->pitch; mp.len = ctx->len.as<int>();
+    mp.qual = ctx->qual.as<uint8_t>(); mp.qpitch = ctx->qpitch; mp.ambig = ctx->ambig.as<uint8_t>();
+    mp.qweight = ctx->qw.as<float>();
+    mp.isPaired = ctx->o.paired; mp.useQ = ctx->o.use_quality_scores;
+    mp.maxErrorRate = ctx->o.max_error_rate;
+    const float base = ctx->o.estimated_errorrate * 1.0f;
+    mp.ratio2 = 2 * base; mp.ratio3 = 3 * base; mp.ratio4 = 4 * base;
+    mp.goodCount = ctx->o.estimated_coverage * ctx->o.m_coverage;
+    mp.pairedThreshold = ctx->o.paired_filter_threshold;
+    const float e = ctx->o.estimated_errorrate;
+    mp.avgThr = 1.0f - 1.0f * e;
+    mp.minThr = 1.0f - 3.0f * e;
+    mp.minCovThr = ctx->o.m_coverage / 6.0f * ctx->o.estimated_coverage;
+    mp.pct = int(std::ceil(e * 100.0f));
+    mp.pctHalf = mp.pct * 0.5f;
+    mp.sigThr = int(int(ctx->o.estimated_coverage) * 0.3f);
+    mp.maxCols = ctx->maxCols; mp.maxLen = ctx->maxLen;
+    mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W;
+    mp.info = nullptr; mp.msaDbg = nullptr; mp.dbgStride = ctx->dbgStride;
+}
+
+void fillCandCommon(const care_ctx* ctx, CandParams& cp){
+    cp.sig = ctx->sig.as<uint64_t>(); cp.n = ctx->n; cp.M = ctx->M; cp.k = ctx->k; cp.thr = ctx->thr;
+    cp.excludeAmbiguous = ctx->o.exclude_ambiguous; cp.len = ctx->len.as<int>(); cp.ambig = ctx->ambig.as<uint8_t>();
+    cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
+    cp.capPerAnchor = ctx->M * ctx->thr;
+    int P = 64;
+    while(P <= cp.capPerAnchor) P <<= 1;      // hash slots > the most values an anchor can gather
+    cp.hashSlots = P;
+}
+
+void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
+    ap.hilo = ctx->hilo.as<uint32_t>(); ap.W = ctx->W; ap.len = ctx->len.as<int>();
+    ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
+    ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
+}
+
+int recordSlotBytes(int maxLen){
+    return ((8 + 4 + std::max(maxLen, 5 * (maxLen / 7 + 1))) + 15) / 16 * 16;
+}
+
+// Compacts the per-anchor record slots [0, B) into one buffer and appends it to `out`.
+int gatherRecords(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out, size_t& outRecords){
+    hipStream_t st = ctx->stream;
+    k_widen<<<(B + 1 + 255) / 256, 256, 0, st>>>(ctx->recBytes.as<uint32_t>(), ctx->recOff64.as<uint64_t>(), B + 1);
+    HIPCHECK(hipGetLastError());
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
+    uint64_t outBytes = 0;
+    int errFlag = 0;
+    HIPCHECK(hipMemcpyAsync(&outBytes, ctx->recOff.as<uint64_t>() + B, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&errFlag, ctx->counters.as<int>() + 2, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if(errFlag) return ctx->fail(CARE_EINVAL, "MSA column bound exceeded");
+    HIPCHECK(ctx->outBuf.ensure(std::max<uint64_t>(outBytes, 1)));
+    k_gather<<<B, kBlock, 0, st>>>(ctx->slots.as<uint8_t>(), slotBytes, ctx->recBytes.as<uint32_t>(),
+                                   ctx->recOff.as<uint64_t>(), ctx->outBuf.as<uint8_t>(), B);
+    HIPCHECK(hipGetLastError());
+    const size_t old = out.size();
+    out.resize(old + outBytes);
+    if(outBytes) HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->outBuf.p, outBytes, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    size_t q = old;
+    while(q < out.size()){
+        uint32_t flags;
+        memcpy(&flags, out.data() + q + 4, 4);
+        q += 8 + (flags & ((1u << 29) - 1));
+        outRecords++;
+    }
+    return CARE_OK;
+}
+
+// Fused pass over anchors [first, first+n): one persistent kernel, then record compaction.
+int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& out, size_t& outRecords){
+    hipStream_t st = ctx->stream;
+    const bool paired = ctx->o.paired != 0;
+    const uint64_t units = paired ? n / 2 : n;
+    const int kc = waveMsaColumnsPerLane(ctx->maxCols);
+    if(n == 0) return CARE_OK;
+    if(n > uint64_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "too many anchors for one fused pass");
+    const int B = int(n);
+    const int slotBytes = recordSlotBytes(ctx->maxLen);
+    FusedParams fp;
+    fillMsaCommon(ctx, fp.m);
+    fillCandCommon(ctx, fp.c);
+    fillAlignCommon(ctx, fp.al);
+    const bool q = ctx->o.use_quality_scores != 0;
+    fp.m.tile = q ? 32 : 64;
+    fp.cap = fp.c.capPerAnchor;
+    int cus = 0;
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int wpc = ctx->o.waves_per_cu > 0 ? ctx->o.waves_per_cu : 8;
+    uint64_t G = uint64_t(std::max(1, cus)) * uint64_t(wpc);
+    G = std::max<uint64_t>(1, std::min<uint64_t>(G, units));
+    fp.scratchStride = (waveScratchBytes(fp.cap) + 255) / 256 * 256;
+    HIPCHECK(ctx->scratch.ensure(G * fp.scratchStride));
+    HIPCHECK(ctx->slots.ensure(size_t(B) * slotBytes));
+    HIPCHECK(ctx->recBytes.ensure(size_t(B + 1) * 4));
+    HIPCHECK(ctx->recOff64.ensure(size_t(B + 1) * 8));
+    HIPCHECK(ctx->recOff.ensure(size_t(B + 1) * 8));
+    HIPCHECK(ctx->counters.ensure(32));
+    fp.m.b0 = first; fp.m.offsets = nullptr; fp.m.ids = nullptr; fp.m.best = nullptr; fp.m.paired = nullptr;
+    fp.m.slots = ctx->slots.as<uint8_t>(); fp.m.slotBytes = slotBytes; fp.m.recBytes = ctx->recBytes.as<uint32_t>();
+    fp.m.error = ctx->counters.as<int>() + 2;
+    fp.c.b0 = first; fp.c.B = B; fp.c.raw = nullptr; fp.c.uniq = nullptr; fp.c.gathered = ctx->counters.as<unsigned long long>();
+    // counters: [0,8) gathered bucket values, [8,12) error flag, [16,20) work counter,
+    // [24,32) candidates aligned
+    fp.counter = ctx->counters.as<uint32_t>() + 4;
+    fp.aligned = ctx->counters.as<unsigned long long>() + 3;
+    fp.first = first;
+    fp.units = uint32_t(units);
+    fp.scratch = ctx->scratch.as<uint8_t>();
+    const size_t lds = std::max({size_t(fp.c.hashSlots) * 4, size_t(10) * (ctx->W + 1) * 8,
+                                 waveMsaLdsBytes(fp.m.tile, ctx->W, ctx->maxLen, q)});
+    // recBytes[B] must read as zero for the exclusive scan; the kernel writes [0, B)
+    k_zero_u32<<<1, 64, 0, st>>>(ctx->recBytes.as<uint32_t>() + B, 1);
+    k_zero_u32<<<1, 64, 0, st>>>(ctx->counters.as<uint32_t>() + 4, 1);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[0], st));
+    switch(kc){
+        case 4:
+            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            k_fused<4><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
+        case 8:
+            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<8>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            k_fused<8><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
+        case 12:
+            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<12>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            k_fused<12><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
+        default:
+            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<16>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+            k_fused<16><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[1], st));
+    const int rc = gatherRecords(ctx, B, slotBytes, out, outRecords);
+    if(rc != CARE_OK) return rc;
+    float ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+    ctx->t.fused_kernel_ms += ms;
+    ctx->t.batches += 1;
+    ctx->lastB = 0;
+    return CARE_OK;
 }
 
 int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_t& outRecords){
@@ -2025,7 +2445,7 @@ void care_hip_destroy(care_ctx* ctx){
                    &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
-                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp}){
+                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp, &ctx->scratch}){
         b->release();
     }
     for(auto& b : ctx->slotBufs) b.release();
@@ -2135,15 +2555,27 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
         B = int(std::min<size_t>(size_t(1) << 18, std::max<size_t>(1024, (size_t(1) << 30) / perAnchor)));
     }
     if(ctx->o.paired && (B & 1)) B++;
+    // the fused kernel keeps MSA columns in registers (<= 1024 columns) and prefetches a
+    // candidate's four planes into 3 uint2 per lane (4 * (W + 1) <= 192)
+    const bool fusedFits = waveMsaColumnsPerLane(ctx->maxCols) > 0 && 4 * (ctx->W + 1) <= 3 * kWave;
+    if(ctx->o.engine == 2 && !fusedFits) return ctx->fail(CARE_EINVAL, "reads too long for the fused engine");
+    const bool fused = !ctx->o.keep_intermediates && ctx->o.engine == 2 && fusedFits;
     const auto t0 = std::chrono::steady_clock::now();
-    for(uint64_t b0 = first; b0 < first + n; b0 += uint64_t(B)){
-        const int cur = int(std::min<uint64_t>(uint64_t(B), first + n - b0));
-        const int rc = runBatch(ctx, b0, cur, ctx->records, ctx->numRecords);
+    if(fused){
+        const int rc = runFused(ctx, first, n, ctx->records, ctx->numRecords);
         if(rc != CARE_OK) return rc;
-        ctx->t.batches++;
+    }else{
+        for(uint64_t b0 = first; b0 < first + n; b0 += uint64_t(B)){
+            const int cur = int(std::min<uint64_t>(uint64_t(B), first + n - b0));
+            const int rc = runBatch(ctx, b0, cur, ctx->records, ctx->numRecords);
+            if(rc != CARE_OK) return rc;
+            ctx->t.batches++;
+        }
     }
-    unsigned long long gathered = 0;
-    HIPCHECK(hipMemcpy(&gathered, ctx->counters.p, 8, hipMemcpyDeviceToHost));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    HIPCHECK(hipMemcpy(cnt, ctx->counters.p, 32, hipMemcpyDeviceToHost));
+    const unsigned long long gathered = cnt[0];
+    if(fused) ctx->t.candidates = cnt[3];
     const auto t1 = std::chrono::steady_clock::now();
     ctx->t.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     ctx->t.anchors = n;

@@ -35,7 +35,8 @@ class CareOpts(ctypes.Structure):
                 ("min_overlap", ctypes.c_int), ("max_error_rate", ctypes.c_float),
                 ("min_overlap_ratio", ctypes.c_float), ("estimated_errorrate", ctypes.c_float),
                 ("m_coverage", ctypes.c_float), ("paired_filter_threshold", ctypes.c_float),
-                ("batch_size", ctypes.c_int), ("keep_intermediates", ctypes.c_int)]
+                ("batch_size", ctypes.c_int), ("keep_intermediates", ctypes.c_int),
+                ("engine", ctypes.c_int), ("waves_per_cu", ctypes.c_int)]
 
 
 class CareTimings(ctypes.Structure):
@@ -43,6 +44,7 @@ class CareTimings(ctypes.Structure):
                 ("align_ms", ctypes.c_double), ("msa_ms", ctypes.c_double),
                 ("output_ms", ctypes.c_double), ("align_kernel_ms", ctypes.c_double),
                 ("cand_kernel_ms", ctypes.c_double), ("msa_kernel_ms", ctypes.c_double),
+                ("fused_kernel_ms", ctypes.c_double),
                 ("anchors", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("records", ctypes.c_uint64), ("record_bytes", ctypes.c_uint64),
                 ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64)]

@@ -102,12 +102,59 @@ def test_records_match_oracle(name):
         assert d_gpu == d_ref
 
 
+STAGED_CASES = ["se_q", "pe", "varlen_q", "nbases_exclude", "se_tandem", "len300_q"]
+
+
+@pytest.mark.parametrize("name", STAGED_CASES)
+def test_staged_engine_matches_fused_and_oracle(name):
+    """The staged per-stage kernels (engine=1) and the persistent fused kernel (engine=2)
+    give the same record bytes, equal to the oracle's."""
+    s, rs, o = helpers.case(name)
+    print(f"[case {name}] reads={rs.n}", flush=True)
+    _, d_staged = engine.correct_gpu(rs, engine=1, **o)
+    _, d_fused = engine.correct_gpu(rs, engine=2, **o)
+    cpu = oracle.CareCpu(rs, threads=8, **o)
+    _, d_cpu = cpu.correct(threads=8)
+    cpu.close()
+    assert d_staged == d_cpu
+    assert d_fused == d_cpu
+
+
+def test_fused_engine_rejects_wide_reads():
+    s, rs, o = helpers.case("len400")
+    eng = engine.CareEngine(0, engine=2, **o)
+    eng.set_reads(rs)
+    eng.build_tables()
+    with pytest.raises(engine.CareError, match="too long for the fused engine"):
+        eng.correct()
+    eng.close()
+
+
+@pytest.mark.parametrize("paired", [False, True])
+def test_fused_range_split_invariance(paired):
+    """Fused records of [0, n) equal those of [0, m) followed by [m, n), also with
+    fewer resident waves than work units."""
+    s, rs, o = helpers.case("pe" if paired else "se_higherr")
+    eng = engine.CareEngine(0, engine=2, **o)
+    eng.set_reads(rs)
+    eng.build_tables()
+    whole = eng.correct()[1]
+    m = 1234
+    assert eng.correct(0, m)[1] + eng.correct(m, rs.n - m)[1] == whole
+    eng.close()
+    eng = engine.CareEngine(0, engine=2, waves_per_cu=1, **o)
+    eng.set_reads(rs)
+    eng.build_tables()
+    assert eng.correct()[1] == whole
+    eng.close()
+
+
 def test_batching_invariance():
-    """Record bytes do not depend on the batch size or on the anchor range split."""
+    """Staged record bytes do not depend on the batch size or on the anchor range split."""
     s, rs, o = helpers.case("se_higherr")
     outs = []
     for bs in (0, 1000, 4097):
-        eng = engine.CareEngine(0, batch_size=bs, **o)
+        eng = engine.CareEngine(0, batch_size=bs, engine=1, **o)
         eng.set_reads(rs)
         eng.build_tables()
         outs.append(eng.correct()[1])

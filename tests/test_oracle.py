@@ -283,3 +283,38 @@ def test_oracle_vs_reference_fresh_seeds(seed, paired, q):
     n2, d2 = ref.correct(threads=8)
     ref.close()
     assert n1 == n2 and d1 == d2
+
+
+def _pair_lists(rng, universe, n1, n2):
+    c1 = np.unique(rng.choice(universe, n1)).astype(np.uint32)
+    c2 = np.unique(rng.choice(universe, n2)).astype(np.uint32)
+    return c1, c2
+
+
+def test_pair_flags_equal_membership_rule():
+    """The GPU sets paired flags by membership: x in one mate's list is flagged exactly
+    when x ^ 1 (the other read of x's pair) is in the other mate's list. The reference's
+    merge state machine (include/util.hpp:1016-1141) must give the same flags on dense,
+    sparse, empty and overlapping sorted lists."""
+    rng = np.random.default_rng(5)
+    cases = [(np.array([], np.uint32), np.array([], np.uint32)),
+             (np.array([4, 5], np.uint32), np.array([], np.uint32)),
+             (np.array([4, 5], np.uint32), np.array([4, 5], np.uint32)),
+             (np.array([4], np.uint32), np.array([4], np.uint32)),
+             (np.array([4, 5], np.uint32), np.array([4], np.uint32)),
+             (np.array([4, 5], np.uint32), np.array([5], np.uint32)),
+             (np.array([5], np.uint32), np.array([4, 5, 6], np.uint32))]
+    for universe, n in [(16, 10), (64, 30), (400, 120), (4000, 300), (1 << 20, 500)]:
+        for _ in range(60):
+            cases.append(_pair_lists(rng, universe, int(rng.integers(0, n)), int(rng.integers(0, n))))
+    flagged = 0
+    for c1, c2 in cases:
+        m1 = np.isin(c1 ^ np.uint32(1), c2)
+        m2 = np.isin(c2 ^ np.uint32(1), c1)
+        f1, f2 = oracle.cpu_flag_pairs(c1, c2)
+        assert np.array_equal(f1, m1) and np.array_equal(f2, m2), (c1, c2)
+        if oracle.ref_available():
+            r1, r2 = oracle.ref_flag_pairs(c1, c2)
+            assert np.array_equal(r1, m1) and np.array_equal(r2, m2), (c1, c2)
+        flagged += int(m1.sum())
+    assert flagged > 500
