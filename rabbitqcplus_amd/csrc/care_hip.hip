@@ -584,10 +584,15 @@ __device__ __forceinline__ void alignBoth(const uint2* Ahi, const uint2* Alo, co
                     if(mR > 0 && misR < mR) keyR = (uint32_t(misR + total - 2 * ov) << 16) | uint32_t(t);
                 }
             }
-            const uint32_t wf = waveMinU32(keyF);
-            const uint32_t wr = waveMinU32(keyR);
-            if(wf < sf.key){ sf.key = wf; sf.best = int(wf >> 16); sf.bestT = int(wf & 0xFFFFu); }
-            if(wr < sr.key){ sr.key = wr; sr.best = int(wr >> 16); sr.bestT = int(wr & 0xFFFFu); }
+            // reduce only where some lane holds an admissible shift (a wave-uniform branch)
+            if(__builtin_amdgcn_ballot_w64(keyF != 0xFFFFFFFFu)){
+                const uint32_t wf = waveMinU32(keyF);
+                if(wf < sf.key){ sf.key = wf; sf.best = int(wf >> 16); sf.bestT = int(wf & 0xFFFFu); }
+            }
+            if(__builtin_amdgcn_ballot_w64(keyR != 0xFFFFFFFFu)){
+                const uint32_t wr = waveMinU32(keyR);
+                if(wr < sr.key){ sr.key = wr; sr.best = int(wr >> 16); sr.bestT = int(wr & 0xFFFFu); }
+            }
         }
     }
     outF = finishOrientation(sf, La, Lc, F);
@@ -1593,7 +1598,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
 }
 
 template<int KC>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 4 : 2)))
 k_msa_wave(MsaParams p){
     extern __shared__ uint8_t smem[];
     __shared__ float sQw[256];
