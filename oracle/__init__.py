@@ -99,6 +99,14 @@ def cpu_lib():
         lib.carecpu_records_data.restype = vp
         lib.carecpu_last_seconds.argtypes = [vp]
         lib.carecpu_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
+        lib.carecpu_set_classifier.argtypes = [vp, vp, i32]
+        lib.carecpu_set_classifier.restype = i32
+        lib.carecpu_mlp_num_params.restype = i32
+        lib.carecpu_features.argtypes = [vp, u64, u64, i32, vp, vp, sz]
+        lib.carecpu_features.restype = sz
+        lib.carecpu_mlp_logits_bf16.argtypes = [vp, vp, u64, vp]
+        lib.carecpu_mlp_loss_grad_f64.argtypes = [vp, vp, vp, u64, vp, vp]
+        lib.carecpu_mlp_loss_grad_f64.restype = ctypes.c_double
         lib.carecpu_last_seconds.restype = ctypes.c_double
         _cpu = lib
     return _cpu
@@ -131,6 +139,8 @@ def ref_lib():
         lib.ref_candidates.argtypes = [vp, ctypes.c_uint, vp, i32]
         lib.ref_align.argtypes = [vp, i32, vp, i32, i32, ctypes.c_float, ctypes.c_float, vp]
         lib.ref_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
+        lib.ref_set_anchor_forest.argtypes = [vp, ctypes.c_char_p, ctypes.c_float]
+        lib.ref_set_anchor_print.argtypes = [vp, ctypes.c_char_p]
         lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
         lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 8
         _ref = lib
@@ -234,6 +244,24 @@ class CareCpu:
     def last_seconds(self):
         return self.lib.carecpu_last_seconds(self.h)
 
+    def set_classifier(self, params):
+        """MLP parameters (float32, MLP_PARAMS values) for the learned seam; None = classic."""
+        if params is None:
+            self.lib.carecpu_set_classifier(self.h, None, 0)
+            return
+        self._params = np.ascontiguousarray(params, np.float32)
+        if self.lib.carecpu_set_classifier(self.h, self._params.ctypes.data, self._params.size) != 0:
+            raise ValueError("wrong number of classifier parameters")
+
+    def features(self, first=0, n=None, threads=0):
+        """Decision columns of anchors [first, first+n): (features [m,16] f32, meta [m,4] u32)."""
+        n = self.rs.n - first if n is None else n
+        total = self.lib.carecpu_features(self.h, first, n, threads, None, None, 0)
+        feats = np.zeros((max(total, 1), 16), np.float32)
+        meta = np.zeros((max(total, 1), 4), np.uint32)
+        self.lib.carecpu_features(self.h, first, n, threads, feats.ctypes.data, meta.ctypes.data, total)
+        return feats[:total], meta[:total]
+
 
 class CareRef:
     """The reference engine (compiled from source) over the same ReadStorage."""
@@ -280,6 +308,65 @@ class CareRef:
         buf = np.zeros(cap, np.uint32)
         n = self.lib.ref_candidates(self.h, read_id, buf.ctypes.data, cap)
         return buf[:n].copy()
+
+    def print_anchor_features(self, path):
+        """Runs the reference's feature print mode over all anchors (one thread) and returns
+        (read ids, positions, consensus chars, features [m,13]) parsed from `path`."""
+        self.lib.ref_set_anchor_print(self.h, os.fsencode(path))
+        self.lib.ref_correct(self.h, 1, 0)
+        self.lib.ref_set_anchor_forest(self.h, None, 0.5)
+        rows = []
+        with open(path) as fh:
+            fh.readline()                        # extractor descriptor
+            for line in fh:
+                parts = line.split()
+                if parts:
+                    rows.append(parts)
+        rid = np.array([int(r[0]) for r in rows], np.int64)
+        pos = np.array([int(r[1]) for r in rows], np.int64)
+        cons = np.array([r[2] for r in rows])
+        feats = np.array([[float(v) for v in r[3:]] for r in rows], np.float64).reshape(len(rows), -1)
+        return rid, pos, cons, feats
+
+    def set_anchor_forest(self, path, threshold=0.5):
+        """Forest-classifier anchor correction with the forest file at `path`; None = classic."""
+        self.lib.ref_set_anchor_forest(self.h, None if path is None else os.fsencode(path), threshold)
+
+
+def write_constant_forest(path, prob):
+    """A forest file of one single-leaf tree that answers `prob` for every column, in the
+    reference's format (include/forest.hpp:141-162 and include/deserialize.hpp:31-36):
+    u64 descriptor length, descriptor, u32 tree count, per tree u32 node count and the
+    nodes in preorder (u8 feature, f64 split, u8 leaf flags, f32 leaf probabilities)."""
+    import struct
+    desc = b"13 extract_anchor_v3"
+    blob = struct.pack("<Q", len(desc)) + desc + struct.pack("<I", 1) + struct.pack("<I", 1)
+    blob += struct.pack("<B", 0) + struct.pack("<d", 0.0) + struct.pack("<B", 3) + struct.pack("<ff", prob, prob)
+    with open(path, "wb") as fh:
+        fh.write(blob)
+
+
+MLP_PARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1
+
+
+def mlp_logits_bf16(params, X):
+    """Logits of the learned seam's inference arithmetic (bfloat16 weights/activations)."""
+    params = np.ascontiguousarray(params, np.float32)
+    X = np.ascontiguousarray(X, np.float32)
+    z = np.zeros(len(X), np.float32)
+    cpu_lib().carecpu_mlp_logits_bf16(params.ctypes.data, X.ctypes.data, len(X), z.ctypes.data)
+    return z
+
+
+def mlp_loss_grad_f64(params, X, y=None, grad=True):
+    """(mean BCE loss, gradient or None, logits) in double precision."""
+    params = np.ascontiguousarray(params, np.float64)
+    X = np.ascontiguousarray(X, np.float32)
+    yy = None if y is None else np.ascontiguousarray(y, np.float32)
+    G = np.zeros(MLP_PARAMS, np.float64) if grad else None
+    z = np.zeros(len(X), np.float64)
+    loss = cpu_lib().carecpu_mlp_loss_grad_f64(params.ctypes.data, X.ctypes.data, _p(yy), len(X), _p(G), z.ctypes.data)
+    return loss, G, z
 
 
 def ref_hash(row, length, k=20, nmaps=48):

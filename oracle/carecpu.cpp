@@ -379,6 +379,94 @@ Properties msaProperties(const Msa& msa, float e, float cov, float m_cov){
     return p;
 }
 
+// ---------------------------------------------------------------- learned seam
+//
+// The classifier seam of correctAnchorClf (include/corrector.hpp:1448-1472): HQ anchors
+// take the consensus; in the other anchors every anchor column whose original base
+// differs from the consensus is decided by a classifier. Here the classifier is an MLP
+// 16 -> 64 -> 64 -> 1 (ReLU, ReLU, logit) in place of the forest (include/forest.hpp),
+// and a logit inside (-ln 9, ln 9) - confidence below 0.9 - falls back to the classic
+// column decision of getCorrectedAnchor (src/msa.cpp:379-453). This part has no
+// counterpart in the reference beyond the seam itself.
+
+constexpr int kFeat = 16, kHid = 64;
+constexpr int kOffW1 = 0, kOffB1 = kOffW1 + kHid * kFeat, kOffW2 = kOffB1 + kHid, kOffB2 = kOffW2 + kHid * kHid,
+              kOffW3 = kOffB2 + kHid, kOffB3 = kOffW3 + kHid, kMlpParams = kOffB3 + 1;
+constexpr float kConfLogit = 2.1972245773f;   // ln 9: sigmoid(z) >= 0.9  <=>  z >= ln 9
+
+// float -> bfloat16 -> float, round to nearest even (finite inputs)
+inline float bf16r(float x){
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    u &= 0xFFFF0000u;
+    float r;
+    std::memcpy(&r, &u, 4);
+    return r;
+}
+
+// Inference as the device runs it: weights and activations rounded to bfloat16, float
+// accumulation in a fixed order (bias first, then inputs in index order), biases float.
+float mlpLogitBf16(const float* P, const float* f){
+    float x[kFeat], h1[kHid], h2[kHid];
+    for(int k = 0; k < kFeat; k++) x[k] = bf16r(f[k]);
+    for(int j = 0; j < kHid; j++){
+        float a = P[kOffB1 + j];
+        for(int k = 0; k < kFeat; k++) a = a + bf16r(P[kOffW1 + j * kFeat + k]) * x[k];
+        h1[j] = bf16r(a > 0.0f ? a : 0.0f);
+    }
+    for(int j = 0; j < kHid; j++){
+        float a = P[kOffB2 + j];
+        for(int k = 0; k < kHid; k++) a = a + bf16r(P[kOffW2 + j * kHid + k]) * h1[k];
+        h2[j] = bf16r(a > 0.0f ? a : 0.0f);
+    }
+    float z = P[kOffB3];
+    for(int j = 0; j < kHid; j++) z = z + bf16r(P[kOffW3 + j]) * h2[j];
+    return z;
+}
+
+struct Properties;
+
+// The 16 features of anchor column i: the 13 of extract_anchor_v3
+// (include/classification.hpp:748-797), then the anchor base's quality weight (1 without
+// qualities), the relative position i/La and the smaller support of the neighbouring
+// anchor columns (1 past either end).
+void columnFeatures(const Msa& msa, float avgSupport, float minSupport, int minCov, int maxCov, int i, int La,
+                    int ob, float qw, float estCov, float* f){
+    const int col = msa.ab + i;
+    const float* w = msa.weights.data() + size_t(col) * 4;
+    const int* c = msa.counts.data() + size_t(col) * 4;
+    const float wsum = w[0] + w[1] + w[2] + w[3];
+    const int cn = baseIndexOfChar(msa.consensus[col]);
+    const int cov = msa.coverage[col];
+    const int origCov = c[ob];
+    const float origW = w[ob];
+    f[0] = float(origCov) / float(cov);
+    f[1] = origW / wsum;
+    f[2] = origW / float(origCov);
+    f[3] = float(c[cn]) / float(cov);
+    f[4] = msa.support[col];
+    f[5] = w[cn] / float(c[cn]);
+    f[6] = wsum / float(cov);
+    f[7] = float(cov) / estCov;
+    f[8] = wsum / estCov;
+    f[9] = avgSupport;
+    f[10] = minSupport;
+    f[11] = float(maxCov) / estCov;
+    f[12] = float(minCov) / estCov;
+    f[13] = qw;
+    f[14] = float(i) / float(La);
+    const float left = i > 0 ? msa.support[col - 1] : 1.0f;
+    const float right = i + 1 < La ? msa.support[col + 1] : 1.0f;
+    f[15] = left < right ? left : right;
+}
+
+// Decision columns collected for training and parity checks.
+struct FeatureSink {
+    std::vector<float> feats;       // 16 per column
+    std::vector<uint32_t> meta;     // read id, position, orig | cons << 2 | classic << 4 | allGood << 6, 0
+};
+
 struct Table {
     std::vector<uint64_t> keys;      // sorted, only buckets that survive the 2..threshold rule
     std::vector<uint32_t> offsets;   // keys.size()+1
@@ -453,6 +541,7 @@ struct carecpu_ctx {
     std::vector<uint64_t> sigs;     // [read][map]
     std::vector<Table> tables;
     std::vector<uint8_t> records;
+    std::vector<float> mlp;         // classifier parameters (empty: classic decision)
     double seconds = 0.0;
 
     const uint32_t* row(uint32_t id) const { return enc.data() + pitch * id; }
@@ -504,11 +593,11 @@ struct carecpu_ctx {
     // One anchor through CpuErrorCorrector::process (include/corrector.hpp:145-312),
     // given its candidate ids and (PE) pair flags. Appends its record to `out`.
     void processAnchor(uint32_t a, const std::vector<uint32_t>& cids, const std::vector<uint8_t>& pairFlags,
-                       std::vector<uint8_t>& out, carecpu_trace* tr) const;
+                       std::vector<uint8_t>& out, carecpu_trace* tr, FeatureSink* sink = nullptr) const;
 };
 
 void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, const std::vector<uint8_t>& pairFlags,
-                                std::vector<uint8_t>& out, carecpu_trace* tr) const {
+                                std::vector<uint8_t>& out, carecpu_trace* tr, FeatureSink* sink) const {
     const int La = lens[a];
     const std::vector<uint8_t> anchor = decodeBases(a);
     const bool useQ = o.use_quality_scores != 0;
@@ -663,6 +752,29 @@ void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
             const int col = msa.ab + i;
             if(msa.support[col] > 0.90f && msa.origCoverages[col] <= 2) corrected[i] = msa.consensus[col];
             else corrected[i] = kDecode[anchor[i]];
+        }
+    }
+    if((!mlp.empty() || sink) && !p.isHQ){
+        for(int i = 0; i < La; i++){
+            const int col = msa.ab + i;
+            const int ob = anchor[i];
+            const int cn = baseIndexOfChar(msa.consensus[col]);
+            if(ob == cn) continue;
+            const int classic = baseIndexOfChar(corrected[i]);
+            const float qw = useQ ? qualityWeights().w[uint8_t(aq[i])] : 1.0f;
+            float f[kFeat];
+            columnFeatures(msa, p.avg_support, p.min_support, p.min_coverage, p.max_coverage, i, La, ob, qw,
+                           o.estimated_coverage, f);
+            if(sink){
+                sink->feats.insert(sink->feats.end(), f, f + kFeat);
+                const uint32_t m[4] = {a, uint32_t(i), uint32_t(ob | (cn << 2) | (classic << 4) | (int(p.allGood) << 6)), 0u};
+                sink->meta.insert(sink->meta.end(), m, m + 4);
+            }
+            if(!mlp.empty()){
+                const float z = mlpLogitBf16(mlp.data(), f);
+                const int d = z >= kConfLogit ? cn : (z <= -kConfLogit ? ob : classic);
+                corrected[i] = kDecode[d];
+            }
         }
     }
     if(tr){
@@ -876,6 +988,102 @@ size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads
         ctx->records.insert(ctx->records.end(), v.begin(), v.end());
     }
     return count;
+}
+
+int carecpu_set_classifier(carecpu_ctx* ctx, const float* params, int count){
+    if(!params){ ctx->mlp.clear(); return 0; }
+    if(count != kMlpParams) return -1;
+    ctx->mlp.assign(params, params + count);
+    return 0;
+}
+
+size_t carecpu_features(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads, float* feats, uint32_t* meta,
+                        size_t cap){
+    if(threads > 0) omp_set_num_threads(threads);
+    const bool paired = ctx->o.paired != 0;
+    if(first + n > ctx->n) n = ctx->n > first ? ctx->n - first : 0;
+    if(paired && ((first & 1) || (n & 1))) return 0;
+    const long long units = paired ? (long long)(n / 2) : (long long)n;
+    std::vector<FeatureSink> sinks(units);
+    #pragma omp parallel for schedule(dynamic, 16)
+    for(long long u = 0; u < units; u++){
+        std::vector<uint8_t> rec;
+        if(paired){
+            const uint32_t a0 = uint32_t(first + 2 * u);
+            const auto c0 = ctx->candidates(a0);
+            const auto c1 = ctx->candidates(a0 + 1);
+            std::vector<uint8_t> f0, f1;
+            flagPairs(c0, c1, f0, f1);
+            ctx->processAnchor(a0, c0, f0, rec, nullptr, &sinks[u]);
+            ctx->processAnchor(a0 + 1, c1, f1, rec, nullptr, &sinks[u]);
+        }else{
+            const uint32_t a = uint32_t(first + u);
+            ctx->processAnchor(a, ctx->candidates(a), std::vector<uint8_t>{}, rec, nullptr, &sinks[u]);
+        }
+    }
+    size_t total = 0;
+    for(const auto& sk : sinks){
+        const size_t m = sk.meta.size() / 4;
+        for(size_t q = 0; q < m; q++, total++){
+            if(total >= cap) continue;
+            std::memcpy(feats + total * kFeat, sk.feats.data() + q * kFeat, kFeat * sizeof(float));
+            std::memcpy(meta + total * 4, sk.meta.data() + q * 4, 4 * sizeof(uint32_t));
+        }
+    }
+    return total;
+}
+
+int carecpu_mlp_num_params(void){ return kMlpParams; }
+
+void carecpu_mlp_logits_bf16(const float* params, const float* X, uint64_t n, float* z){
+    for(uint64_t r = 0; r < n; r++) z[r] = mlpLogitBf16(params, X + r * kFeat);
+}
+
+// Double-precision forward pass and mean binary cross-entropy on logits, with its exact
+// gradient by backpropagation (chain rule through the two ReLU layers).
+double carecpu_mlp_loss_grad_f64(const double* P, const float* X, const float* Y, uint64_t n, double* G, double* z){
+    if(G) std::fill(G, G + kMlpParams, 0.0);
+    double loss = 0.0;
+    double h1p[kHid], h1[kHid], h2p[kHid], h2[kHid], dh2[kHid], dh1[kHid];
+    for(uint64_t r = 0; r < n; r++){
+        const float* x = X + r * kFeat;
+        for(int j = 0; j < kHid; j++){
+            double a = P[kOffB1 + j];
+            for(int k = 0; k < kFeat; k++) a += P[kOffW1 + j * kFeat + k] * double(x[k]);
+            h1p[j] = a; h1[j] = a > 0.0 ? a : 0.0;
+        }
+        for(int j = 0; j < kHid; j++){
+            double a = P[kOffB2 + j];
+            for(int k = 0; k < kHid; k++) a += P[kOffW2 + j * kHid + k] * h1[k];
+            h2p[j] = a; h2[j] = a > 0.0 ? a : 0.0;
+        }
+        double zz = P[kOffB3];
+        for(int j = 0; j < kHid; j++) zz += P[kOffW3 + j] * h2[j];
+        if(z) z[r] = zz;
+        const double y = Y ? double(Y[r]) : 0.0;
+        loss += std::max(zz, 0.0) - y * zz + std::log1p(std::exp(-std::fabs(zz)));
+        if(!G) continue;
+        const double dz = (1.0 / (1.0 + std::exp(-zz)) - y) / double(n);
+        G[kOffB3] += dz;
+        for(int j = 0; j < kHid; j++){
+            G[kOffW3 + j] += dz * h2[j];
+            dh2[j] = h2p[j] > 0.0 ? dz * P[kOffW3 + j] : 0.0;
+        }
+        for(int k = 0; k < kHid; k++) dh1[k] = 0.0;
+        for(int j = 0; j < kHid; j++){
+            G[kOffB2 + j] += dh2[j];
+            for(int k = 0; k < kHid; k++){
+                G[kOffW2 + j * kHid + k] += dh2[j] * h1[k];
+                dh1[k] += dh2[j] * P[kOffW2 + j * kHid + k];
+            }
+        }
+        for(int k = 0; k < kHid; k++){
+            if(h1p[k] <= 0.0) continue;
+            G[kOffB1 + k] += dh1[k];
+            for(int m = 0; m < kFeat; m++) G[kOffW1 + k * kFeat + m] += dh1[k] * double(x[m]);
+        }
+    }
+    return n ? loss / double(n) : 0.0;
 }
 
 size_t carecpu_records_bytes(carecpu_ctx* ctx){ return ctx->records.size(); }

@@ -109,6 +109,25 @@ size_t carecpu_records_bytes(carecpu_ctx* ctx);
 const uint8_t* carecpu_records_data(carecpu_ctx* ctx);
 double carecpu_last_seconds(carecpu_ctx* ctx);
 
+/* Learned seam (correctAnchorClf, include/corrector.hpp:1448-1472, with an MLP
+   16->64->64->1 as the classifier and the classic column rule below confidence 0.9).
+   Parameter layout (float): W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3.
+   params == NULL restores the classic decision. Returns 0, or -1 for a wrong count. */
+int carecpu_set_classifier(carecpu_ctx* ctx, const float* params, int count);
+int carecpu_mlp_num_params(void);
+/* Decision columns (non-HQ anchors, original base != consensus) of anchors
+   [first, first+n), in anchor order: 16 features and 4 meta words each (read id,
+   position, orig | cons<<2 | classic<<4 | allGood<<6, 0). Returns the total count; at
+   most `cap` are written. */
+size_t carecpu_features(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads, float* feats, uint32_t* meta,
+                        size_t cap);
+/* Logits as the device computes them (bfloat16 weights/activations, float sums). */
+void carecpu_mlp_logits_bf16(const float* params, const float* X, uint64_t n, float* z);
+/* Mean BCE-with-logits loss over n rows in double precision; G (may be NULL) receives
+   the gradient, z (may be NULL) the logits. */
+double carecpu_mlp_loss_grad_f64(const double* params, const float* X, const float* Y, uint64_t n, double* G,
+                                 double* z);
+
 #ifdef __cplusplus
 }
 #endif

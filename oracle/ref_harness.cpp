@@ -194,6 +194,28 @@ ref_ctx* ref_create(const ref_opts* o, std::size_t n, const unsigned int* enc, s
 
 void ref_destroy(ref_ctx* c){ delete c; }
 
+// Switches the anchor correction to the feature print mode (include/corrector.hpp:1474-1493,
+// include/classification.hpp:50-57): the features of every decision column go to `path`
+// (sample rate 1); no anchor is corrected. Run with one thread for a whole-line file.
+void ref_set_anchor_print(ref_ctx* c, const char* path){
+    c->po.correctionType = care::CorrectionType::Print;
+    c->po.mlForestfilePrintAnchor = path;
+    c->po.sampleRateAnchor = 1.0f;
+}
+
+// Switches the anchor correction to the classifier seam (include/corrector.hpp:1448-1472)
+// with the forest file at `path` (format include/forest.hpp:141-162); null restores Classic.
+void ref_set_anchor_forest(ref_ctx* c, const char* path, float threshold){
+    if(path){
+        c->po.correctionType = care::CorrectionType::Forest;
+        c->po.mlForestfileAnchor = path;
+        c->po.thresholdAnchor = threshold;
+    }else{
+        c->po.correctionType = care::CorrectionType::Classic;
+        c->po.mlForestfileAnchor = "";
+    }
+}
+
 int ref_num_maps(ref_ctx* c){ return c->mh->getNumberOfMaps(); }
 
 // End to end through care::cpu::correct_cpu; records are then ordered by read id
