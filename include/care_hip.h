@@ -58,6 +58,10 @@ typedef struct care_opts {
     int keep_intermediates;         /* 1: keep per-batch stage outputs for care_hip_inspect_* (staged engine) */
     int engine;                     /* 0 = automatic (staged), 1 = staged per-stage kernels, 2 = fused persistent kernel */
     int waves_per_cu;               /* fused engine: resident waves per compute unit, 0 = default */
+    int correction_type;            /* 0 = classic decision (reference default), 1 = learned: the
+                                       classifier seam of correctAnchorClf (include/corrector.hpp:
+                                       1448-1472) with the MLP of care_hip_set_classifier, and the
+                                       classic column rule where its confidence is below 0.9 */
 } care_opts;
 
 typedef struct care_ctx care_ctx;
@@ -111,6 +115,55 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
 int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap);
 
 int care_hip_get_timings(const care_ctx* ctx, care_timings* out);
+
+/* ---- learned seam ---- */
+
+/* Number of classifier parameters: MLP 16 -> 64 -> 64 -> 1. */
+#define CARE_MLP_PARAMS 5313
+
+/* Loads the classifier used by correction_type = 1 (replaces the forest file of
+ * ForestClf, include/forest.hpp:130-170, and the clf_agent's decide_anchor,
+ * include/classification.hpp:74-83). params: float W1[64][16], b1[64], W2[64][64],
+ * b2[64], W3[64], b3 (ReLU after the two hidden layers, output = logit). On the device
+ * weights are bfloat16 (round to nearest even), sums float. NULL unloads. A column is
+ * set to the consensus when the logit is >= ln 9, kept when <= -ln 9, and decided by
+ * the classic rule (getCorrectedAnchor, src/msa.cpp:379-453) otherwise. */
+int care_hip_set_classifier(care_ctx* ctx, const float* params, size_t count);
+
+/* Runs care_hip_correct over [first, first+n) and returns its decision columns (anchor
+ * columns of non-HQ anchors whose base differs from the consensus; the columns of the
+ * reference's feature print mode, include/corrector.hpp:1474-1493) ordered by (read id,
+ * position): features [m][16] (the 13 of extract_anchor_v3,
+ * include/classification.hpp:748-797, then the anchor base's quality weight, i / length,
+ * the smaller neighbouring support) and meta [m][4] (read id, position,
+ * orig | cons << 2 | classic decision << 4 | allGood << 6, 0). *total receives the
+ * number of columns; at most cap are written. Needs reads of at most 340 bases. */
+int care_hip_extract_features(care_ctx* ctx, uint64_t first, uint64_t n, float* features, uint32_t* meta,
+                              size_t cap, size_t* total);
+
+/* ---- classifier training (no counterpart in the reference: its forests are trained
+ * offline). One handle per device; parameters, gradients, Adam moments and training data
+ * are device-resident. The forward/backward Function runs as kernels on the handle's
+ * stream; every call below returns when its results are on the host. ---- */
+
+typedef struct care_mlp care_mlp;
+
+int care_mlp_create(int device, care_mlp** out);       /* parameters and Adam moments start at 0 */
+void care_mlp_destroy(care_mlp* h);
+const char* care_mlp_last_error(const care_mlp* h);
+int care_mlp_set_params(care_mlp* h, const float* params, size_t count);   /* CARE_MLP_PARAMS floats */
+int care_mlp_get_params(care_mlp* h, float* params, size_t count);
+int care_mlp_reset_optimizer(care_mlp* h);              /* Adam moments and step count to 0 */
+/* Logits z[n] of rows X[n][16] (float). */
+int care_mlp_forward(care_mlp* h, const float* X, size_t n, float* z);
+/* Mean binary cross-entropy of sigmoid(z) against y[n] (0/1) and its gradient grad[CARE_MLP_PARAMS]. */
+int care_mlp_loss_grad(care_mlp* h, const float* X, const float* y, size_t n, float* loss, float* grad);
+/* Training rows, copied to the device. */
+int care_mlp_set_data(care_mlp* h, const float* X, const float* y, size_t n);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) over `epochs` passes of shuffled batches of
+ * `batch` rows (the last batch of a pass may be smaller); epoch_loss[epochs] (may be
+ * NULL) receives each pass's mean batch loss. */
+int care_mlp_train(care_mlp* h, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss);
 
 /* ---- stage inspection (parity tests) ---- */
 

@@ -36,7 +36,7 @@ HIP_FLAGS = [
     f"-I{INCLUDE}",
 ]
 
-HIP_SOURCES = ["care_hip.hip"]
+HIP_SOURCES = ["care_hip.hip", "mlp.hip"]
 
 
 def _run(cmd, cwd=None):

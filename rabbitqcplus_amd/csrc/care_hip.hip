@@ -41,6 +41,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <numeric>
 #include <vector>
 
 namespace {
@@ -51,6 +52,14 @@ constexpr int kWave = 64;
 constexpr int kMaxHiLoWords = 32;    // reads up to 1024 bases
 constexpr int kMaxSortEntries = 32768;  // per-anchor gathered bucket values held in LDS
 constexpr int kAnchorInfo = 12;
+// learned seam: MLP 16 -> 64 -> 64 -> 1 over the features of one decision column
+constexpr int kFeat = 16;
+constexpr int kHid = 64;
+constexpr int kMlpParams = kHid * kFeat + kHid + kHid * kHid + kHid + kHid + 1;
+constexpr float kConfLogit = 2.1972245773f;      // ln 9: sigmoid(z) >= 0.9 <=> z >= ln 9
+// LDS of the seam: 64 feature rows, both hidden layers, 64 logits
+constexpr size_t kSeamLdsBytes = size_t(kWave) * kFeat * 4 + 2 * kHid * 4 + kWave * 4;
+constexpr size_t kSeamWeightBytes = size_t(kFeat * kHid + kHid * kHid + kHid) * 2;   // 10368, 16-byte aligned
 
 struct Slot {            // one open-addressing slot of a per-map table (16 bytes)
     uint64_t key;
@@ -698,6 +707,12 @@ struct MsaParams {
     int* info;                      // [B][kAnchorInfo] or null
     uint8_t* msaDbg; int dbgStride; // per anchor: 1 int ncol + cols*(4 int + 4 float + int + float + 1 byte)
     int* error;
+    // learned seam (classifier decision of non-HQ anchor columns)
+    int seamMode;                   // 0 classic decision, 1 MLP with classic fallback
+    const uint16_t* seamW;          // bfloat16: W1^T [16][64], W2^T [64][64], W3 [64]
+    const float* seamB;             // b1 [64], b2 [64], b3
+    float estCov;
+    float* featOut; uint32_t* metaOut; unsigned long long* featCount; unsigned long long featCap;
 };
 
 struct MsaShared {
@@ -1195,6 +1210,13 @@ struct WaveMsa {
     int ncol, ab;
 };
 
+// weight of base b (0..3) in slot k, by selects: a dynamic index would move the
+// register-resident columns to scratch memory
+template<int KC>
+__device__ __forceinline__ float weightOf(const WaveMsa<KC>& m, int k, int b){
+    return b == 0 ? m.w[k][0] : (b == 1 ? m.w[k][1] : (b == 2 ? m.w[k][2] : m.w[k][3]));
+}
+
 template<int KC>
 __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
     const uint32_t v = b < 2 ? m.c01[k] : m.c23[k];
@@ -1288,6 +1310,38 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, uint64_t a, int
     return true;
 }
 
+// float -> bfloat16 -> float, round to nearest even (finite inputs)
+__device__ __forceinline__ float bf16r(float x){
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+__device__ __forceinline__ float bf16f(uint16_t h){ return __uint_as_float(uint32_t(h) << 16); }
+
+// Logit of the feature row sx (already rounded to bfloat16) by the whole wave: lane j
+// computes hidden unit j of both layers, each sum starting at the bias and adding the
+// inputs in index order; the output sum runs over the hidden units in order on every
+// lane. Weights are bfloat16, sums float. Returns the wave-uniform logit.
+__device__ __forceinline__ float seamLogit(const MsaParams& p, const float* sx, float* sH1, float* sH2, int lane){
+    const uint16_t* w1t = p.seamW;
+    const uint16_t* w2t = p.seamW + kFeat * kHid;
+    const uint16_t* w3 = w2t + kHid * kHid;
+    float a = p.seamB[lane];
+    #pragma unroll
+    for(int k = 0; k < kFeat; k++) a = a + bf16f(w1t[k * kHid + lane]) * sx[k];
+    sH1[lane] = bf16r(a > 0.0f ? a : 0.0f);
+    __syncthreads();
+    a = p.seamB[kHid + lane];
+    #pragma unroll 16
+    for(int k = 0; k < kHid; k++) a = a + bf16f(w2t[k * kHid + lane]) * sH1[k];
+    sH2[lane] = bf16r(a > 0.0f ? a : 0.0f);
+    __syncthreads();
+    float z = p.seamB[2 * kHid];
+    #pragma unroll 16
+    for(int j = 0; j < kHid; j++) z = z + bf16f(w3[j]) * sH2[j];
+    return z;
+}
+
 // Filters, MSA, refinement, decision and record of one anchor per wavefront
 // (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734).
 // Carves the k_msa_wave / k_fused candidate tile and the anchor-support array out of `tp`
@@ -1303,7 +1357,8 @@ __device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, c
     tile.lo = tile.hi + T * p.W;
     tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
     tile.qw = qw;
-    sSup = reinterpret_cast<float*>(tp + (size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * p.maxLen : 0) + 15) / 16 * 16);
+    const size_t tileBytes = size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * p.maxLen : 0);
+    sSup = reinterpret_cast<float*>(tp + (max(tileBytes, kSeamLdsBytes) + 15) / 16 * 16);
 }
 
 // Filters, MSA, refinement, decision and record of anchor `a` by one wavefront
@@ -1534,6 +1589,87 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
         }
         isHQ = fgeq(float(mincov), p.minCovThr) && fleq(float(smallest), p.pctHalf);
     }
+    // learned seam (include/corrector.hpp:1448-1472): in a non-HQ anchor, every anchor
+    // column whose base differs from the consensus is a decision column; a logit >= ln 9
+    // takes the consensus, <= -ln 9 keeps the base, anything between falls back to the
+    // classic rule below. Decision columns can also be written out with their features.
+    uint32_t seamTake = 0, seamKeep = 0;    // 1 bit per slot
+    if((p.seamMode != 0 || p.featOut) && !isHQ){
+        float* sX = reinterpret_cast<float*>(tile.shift);       // the tile region is free now
+        float* sH1 = sX + kWave * kFeat;
+        float* sH2 = sH1 + kHid;
+        float* sZ = sH2 + kHid;
+        #pragma unroll
+        for(int k = 0; k < KC; k++){
+            const bool anc = (m.isAnchor >> k) & 1u;
+            const int ob = int((m.abase >> (2 * k)) & 3u);
+            const int cn = int((m.cons >> (2 * k)) & 3u);
+            const bool dec = anc && ob != cn;
+            const uint64_t mask = __ballot(dec);
+            if(mask == 0) continue;
+            const int n = __popcll(mask);
+            const int e = __popcll(mask & below);
+            const int i = lane + kWave * k - ab;
+            unsigned long long fbase = 0;
+            if(p.featOut){
+                if(lane == 0) fbase = atomicAdd(p.featCount, (unsigned long long)n);
+                fbase = __shfl(fbase, 0);
+            }
+            if(dec){
+                // features (include/classification.hpp:748-797, then quality weight,
+                // relative position, smaller neighbour support)
+                const int c0 = countOf(m, k, 0), c1 = countOf(m, k, 1), c2 = countOf(m, k, 2), c3 = countOf(m, k, 3);
+                const int cov = c0 + c1 + c2 + c3;
+                const float wsum = m.w[k][0] + m.w[k][1] + m.w[k][2] + m.w[k][3];
+                const int origCov = countOf(m, k, ob);
+                const float origW = weightOf(m, k, ob);
+                const int consCount = countOf(m, k, cn);
+                float f[kFeat];
+                f[0] = float(origCov) / float(cov);
+                f[1] = origW / wsum;
+                f[2] = origW / float(origCov);
+                f[3] = float(consCount) / float(cov);
+                f[4] = m.sup[k];
+                f[5] = weightOf(m, k, cn) / float(consCount);
+                f[6] = wsum / float(cov);
+                f[7] = float(cov) / p.estCov;
+                f[8] = wsum / p.estCov;
+                f[9] = avgSupport;
+                f[10] = minSupport;
+                f[11] = float(maxcov) / p.estCov;
+                f[12] = float(mincov) / p.estCov;
+                f[13] = p.useQ ? tile.qw[p.qual[p.qpitch * a + i]] : 1.0f;
+                f[14] = float(i) / float(La);
+                const float left = i > 0 ? sSup[i - 1] : 1.0f;
+                const float right = i + 1 < La ? sSup[i + 1] : 1.0f;
+                f[15] = left < right ? left : right;
+                #pragma unroll
+                for(int q = 0; q < kFeat; q++) sX[e * kFeat + q] = bf16r(f[q]);
+                if(p.featOut && fbase + e < p.featCap){
+                    const int classic = allGood ? cn : ((m.sup[k] > 0.90f && origCov <= 2) ? cn : ob);
+                    float* fo = p.featOut + (fbase + e) * kFeat;
+                    #pragma unroll
+                    for(int q = 0; q < kFeat; q++) fo[q] = f[q];
+                    uint32_t* mo = p.metaOut + (fbase + e) * 4;
+                    mo[0] = uint32_t(a); mo[1] = uint32_t(i);
+                    mo[2] = uint32_t(ob | (cn << 2) | (classic << 4) | (int(allGood) << 6)); mo[3] = 0u;
+                }
+            }
+            if(p.seamMode == 0) continue;
+            __syncthreads();
+            for(int r = 0; r < n; r++){
+                const float z = seamLogit(p, sX + r * kFeat, sH1, sH2, lane);
+                if(lane == 0) sZ[r] = z;
+            }
+            __syncthreads();
+            if(dec){
+                const float z = sZ[e];
+                seamTake |= uint32_t(z >= kConfLogit) << k;
+                seamKeep |= uint32_t(z <= -kConfLogit) << k;
+            }
+            __syncthreads();
+        }
+    }
     uint32_t corrBases = 0;     // 2 bits per slot
     uint32_t editBits = 0;      // 1 bit per slot
     int nEdits = 0;
@@ -1548,6 +1684,8 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
             const int oc = countOf(m, k, ob);
             if(allGood) cb = cn;
             else cb = (m.sup[k] > 0.90f && oc <= 2) ? cn : ob;
+            if((seamTake >> k) & 1u) cb = cn;
+            else if((seamKeep >> k) & 1u) cb = ob;
             e = cb != ob;
         }
         corrBases |= uint32_t(cb) << (2 * k);
@@ -1598,7 +1736,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
 }
 
 template<int KC>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 4 : 2)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 4 : (KC <= 12 ? 3 : 2))))
 k_msa_wave(MsaParams p){
     extern __shared__ uint8_t smem[];
     __shared__ float sQw[256];
@@ -1931,6 +2069,11 @@ struct care_ctx {
     DBuf raw, uniq, offsets, ids, anchorOf, best, dbg, paired;
     DBuf kId, kShift, kLen, kNops, kOvl, kW, kOrient, kPaired;
     DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp, scratch;
+    // learned seam: classifier (bfloat16 weights then float biases) and feature output
+    DBuf seam, featBuf, metaBuf;
+    bool hasSeam = false;
+    bool featActive = false;
+    unsigned long long featCap = 0;
     std::vector<uint8_t> records;
     size_t numRecords = 0;
 
@@ -2070,6 +2213,7 @@ int waveMsaColumnsPerLane(int maxCols){
 // dynamic LDS of k_msa_wave: candidate tile + anchor-column supports
 size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
     size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * maxLen : 0);
+    b = std::max(b, kSeamLdsBytes);      // the seam reuses the tile region
     return (b + 15) / 16 * 16 + size_t(maxLen) * 4 + 16;
 }
 
@@ -2104,6 +2248,14 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.maxCols = ctx->maxCols; mp.maxLen = ctx->maxLen;
     mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W;
     mp.info = nullptr; mp.msaDbg = nullptr; mp.dbgStride = ctx->dbgStride;
+    mp.seamMode = ctx->o.correction_type == 1 ? 1 : 0;
+    mp.seamW = ctx->seam.as<uint16_t>();
+    mp.seamB = reinterpret_cast<const float*>(ctx->seam.as<uint8_t>() + kSeamWeightBytes);
+    mp.estCov = ctx->o.estimated_coverage;
+    mp.featOut = ctx->featActive ? ctx->featBuf.as<float>() : nullptr;
+    mp.metaOut = ctx->featActive ? ctx->metaBuf.as<uint32_t>() : nullptr;
+    mp.featCount = ctx->counters.as<unsigned long long>() + 4;
+    mp.featCap = ctx->featCap;
 }
 
 void fillCandCommon(const care_ctx* ctx, CandParams& cp){
@@ -2188,13 +2340,13 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     HIPCHECK(ctx->recBytes.ensure(size_t(B + 1) * 4));
     HIPCHECK(ctx->recOff64.ensure(size_t(B + 1) * 8));
     HIPCHECK(ctx->recOff.ensure(size_t(B + 1) * 8));
-    HIPCHECK(ctx->counters.ensure(32));
+    HIPCHECK(ctx->counters.ensure(64));
     fp.m.b0 = first; fp.m.offsets = nullptr; fp.m.ids = nullptr; fp.m.best = nullptr; fp.m.paired = nullptr;
     fp.m.slots = ctx->slots.as<uint8_t>(); fp.m.slotBytes = slotBytes; fp.m.recBytes = ctx->recBytes.as<uint32_t>();
     fp.m.error = ctx->counters.as<int>() + 2;
     fp.c.b0 = first; fp.c.B = B; fp.c.raw = nullptr; fp.c.uniq = nullptr; fp.c.gathered = ctx->counters.as<unsigned long long>();
     // counters: [0,8) gathered bucket values, [8,12) error flag, [16,20) work counter,
-    // [24,32) candidates aligned
+    // [24,32) candidates aligned, [32,40) decision columns written out
     fp.counter = ctx->counters.as<uint32_t>() + 4;
     fp.aligned = ctx->counters.as<unsigned long long>() + 3;
     fp.first = first;
@@ -2239,7 +2391,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     int P = 64;
     while(P <= capPerAnchor) P <<= 1;      // hash slots > the most values an anchor can gather
     // counters: [0] gathered bucket values, [1] error flag
-    HIPCHECK(ctx->counters.ensure(32));
+    HIPCHECK(ctx->counters.ensure(64));
     HIPCHECK(ctx->raw.ensure(size_t(B) * capPerAnchor * sizeof(uint32_t)));
     HIPCHECK(ctx->uniq.ensure(size_t(B + 1) * sizeof(uint32_t)));
     HIPCHECK(ctx->offsets.ensure(size_t(B + 1) * sizeof(uint32_t)));
@@ -2308,30 +2460,14 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
         HIPCHECK(ctx->msaDbg.ensure(size_t(B) * ctx->dbgStride));
     }
     MsaParams mp;
-    mp.enc = ctx->enc.as<uint32_t>(); mp.pitch = ctx->pitch; mp.len = ctx->len.as<int>();
-    mp.qual = ctx->qual.as<uint8_t>(); mp.qpitch = ctx->qpitch; mp.ambig = ctx->ambig.as<uint8_t>();
-    mp.qweight = ctx->qw.as<float>(); mp.b0 = b0;
+    fillMsaCommon(ctx, mp);
+    mp.b0 = b0;
     mp.offsets = ctx->offsets.as<uint32_t>(); mp.ids = ctx->ids.as<uint32_t>(); mp.best = ctx->best.as<AlignBest>();
     mp.paired = ctx->o.paired ? ctx->paired.as<uint8_t>() : nullptr;
     mp.kId = ctx->kId.as<uint32_t>(); mp.kShift = ctx->kShift.as<int>(); mp.kLen = ctx->kLen.as<int>();
     mp.kNops = ctx->kNops.as<int>(); mp.kOvl = ctx->kOvl.as<int>(); mp.kW = ctx->kW.as<float>();
     mp.kOrient = ctx->kOrient.as<uint8_t>(); mp.kPaired = ctx->kPaired.as<uint8_t>();
-    mp.isPaired = ctx->o.paired; mp.useQ = ctx->o.use_quality_scores;
-    mp.maxErrorRate = ctx->o.max_error_rate;
-    // host-side thresholds, written as in the reference (float arithmetic)
-    const float base = ctx->o.estimated_errorrate * 1.0f;
-    mp.ratio2 = 2 * base; mp.ratio3 = 3 * base; mp.ratio4 = 4 * base;
-    mp.goodCount = ctx->o.estimated_coverage * ctx->o.m_coverage;
-    mp.pairedThreshold = ctx->o.paired_filter_threshold;
-    const float e = ctx->o.estimated_errorrate;
-    mp.avgThr = 1.0f - 1.0f * e;
-    mp.minThr = 1.0f - 3.0f * e;
-    mp.minCovThr = ctx->o.m_coverage / 6.0f * ctx->o.estimated_coverage;
-    mp.pct = int(std::ceil(e * 100.0f));
-    mp.pctHalf = mp.pct * 0.5f;
-    mp.sigThr = int(int(ctx->o.estimated_coverage) * 0.3f);
-    mp.maxCols = maxCols; mp.maxLen = ctx->maxLen;
-    mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W;
+    mp.maxCols = maxCols;
     mp.tile = msaTile(maxCols, ctx->maxLen, ctx->W, ctx->o.use_quality_scores != 0);
     mp.slots = ctx->slots.as<uint8_t>(); mp.slotBytes = slotBytes; mp.recBytes = ctx->recBytes.as<uint32_t>();
     mp.info = ctx->o.keep_intermediates ? ctx->info.as<int>() : nullptr;
@@ -2551,8 +2687,8 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
     ctx->t = care_timings{};
     ctx->records.clear();
     ctx->numRecords = 0;
-    HIPCHECK(ctx->counters.ensure(32));
-    HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, 32, ctx->stream));
+    HIPCHECK(ctx->counters.ensure(64));
+    HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, 64, ctx->stream));
     int B = ctx->o.batch_size > 0 ? ctx->o.batch_size : 0;
     if(B <= 0){
         // keep the per-anchor candidate staging area near 1 GiB
@@ -2565,6 +2701,9 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
     const bool fusedFits = waveMsaColumnsPerLane(ctx->maxCols) > 0 && 4 * (ctx->W + 1) <= 3 * kWave;
     if(ctx->o.engine == 2 && !fusedFits) return ctx->fail(CARE_EINVAL, "reads too long for the fused engine");
     const bool fused = !ctx->o.keep_intermediates && ctx->o.engine == 2 && fusedFits;
+    if(ctx->o.correction_type == 1 && !ctx->hasSeam) return ctx->fail(CARE_ESTATE, "learned correction: set_classifier first");
+    if((ctx->o.correction_type == 1 || ctx->featActive) && waveMsaColumnsPerLane(ctx->maxCols) == 0)
+        return ctx->fail(CARE_EINVAL, "learned correction and feature output need reads of at most 340 bases");
     const auto t0 = std::chrono::steady_clock::now();
     if(fused){
         const int rc = runFused(ctx, first, n, ctx->records, ctx->numRecords);
@@ -2596,6 +2735,79 @@ int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap){
     if(!ctx || (!dst && !ctx->records.empty())) return CARE_EINVAL;
     if(cap < ctx->records.size()) return ctx->fail(CARE_EINVAL, "buffer too small");
     if(!ctx->records.empty()) std::memcpy(dst, ctx->records.data(), ctx->records.size());
+    return CARE_OK;
+}
+
+int care_hip_set_classifier(care_ctx* ctx, const float* params, size_t count){
+    if(!ctx) return CARE_EINVAL;
+    if(!params){ ctx->hasSeam = false; return CARE_OK; }
+    if(count != size_t(kMlpParams)) return ctx->fail(CARE_EINVAL, "classifier needs %d parameters, got %zu", kMlpParams, count);
+    for(size_t i = 0; i < count; i++)
+        if(!std::isfinite(params[i])) return ctx->fail(CARE_EINVAL, "classifier parameter %zu is not finite", i);
+    // device layout: W1^T [16][64], W2^T [64][64], W3 [64] as bfloat16 (round to nearest
+    // even), then b1, b2, b3 as float; the host layout is W1[64][16], b1, W2[64][64], b2, W3, b3
+    auto bf = [](float x){
+        uint32_t u;
+        std::memcpy(&u, &x, 4);
+        u += 0x7FFFu + ((u >> 16) & 1u);
+        return uint16_t(u >> 16);
+    };
+    const float* W1 = params;
+    const float* b1 = W1 + kHid * kFeat;
+    const float* W2 = b1 + kHid;
+    const float* b2 = W2 + kHid * kHid;
+    const float* W3 = b2 + kHid;
+    const float* b3 = W3 + kHid;
+    std::vector<uint8_t> blob(kSeamWeightBytes + (2 * kHid + 1) * 4);
+    uint16_t* w = reinterpret_cast<uint16_t*>(blob.data());
+    for(int k = 0; k < kFeat; k++)
+        for(int j = 0; j < kHid; j++) w[k * kHid + j] = bf(W1[j * kFeat + k]);
+    uint16_t* w2 = w + kFeat * kHid;
+    for(int k = 0; k < kHid; k++)
+        for(int j = 0; j < kHid; j++) w2[k * kHid + j] = bf(W2[j * kHid + k]);
+    uint16_t* w3 = w2 + kHid * kHid;
+    for(int j = 0; j < kHid; j++) w3[j] = bf(W3[j]);
+    float* b = reinterpret_cast<float*>(blob.data() + kSeamWeightBytes);
+    std::memcpy(b, b1, kHid * 4);
+    std::memcpy(b + kHid, b2, kHid * 4);
+    b[2 * kHid] = b3[0];
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(ctx->seam.ensure(blob.size()));
+    HIPCHECK(hipMemcpy(ctx->seam.p, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    ctx->hasSeam = true;
+    return CARE_OK;
+}
+
+int care_hip_extract_features(care_ctx* ctx, uint64_t first, uint64_t n, float* features, uint32_t* meta,
+                              size_t cap, size_t* total){
+    if(!ctx || !total || (cap && (!features || !meta))) return CARE_EINVAL;
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(ctx->featBuf.ensure(std::max<size_t>(cap, 1) * kFeat * 4));
+    HIPCHECK(ctx->metaBuf.ensure(std::max<size_t>(cap, 1) * 16));
+    ctx->featCap = cap;
+    ctx->featActive = true;
+    const int rc = care_hip_correct(ctx, first, n, nullptr, nullptr);
+    ctx->featActive = false;
+    if(rc != CARE_OK) return rc;
+    unsigned long long count = 0;
+    HIPCHECK(hipMemcpy(&count, ctx->counters.as<unsigned long long>() + 4, 8, hipMemcpyDeviceToHost));
+    *total = size_t(count);
+    const size_t m = std::min<size_t>(size_t(count), cap);
+    if(m == 0) return CARE_OK;
+    std::vector<float> f(m * kFeat);
+    std::vector<uint32_t> q(m * 4);
+    HIPCHECK(hipMemcpy(f.data(), ctx->featBuf.p, m * kFeat * 4, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(q.data(), ctx->metaBuf.p, m * 16, hipMemcpyDeviceToHost));
+    // waves append in completion order: return the columns ordered by (read id, position)
+    std::vector<size_t> order(m);
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::sort(order.begin(), order.end(), [&](size_t x, size_t y){
+        return q[4 * x] != q[4 * y] ? q[4 * x] < q[4 * y] : q[4 * x + 1] < q[4 * y + 1];
+    });
+    for(size_t r = 0; r < m; r++){
+        std::memcpy(features + r * kFeat, f.data() + order[r] * kFeat, kFeat * 4);
+        std::memcpy(meta + r * 4, q.data() + order[r] * 4, 16);
+    }
     return CARE_OK;
 }
 

@@ -36,7 +36,8 @@ class CareOpts(ctypes.Structure):
                 ("min_overlap_ratio", ctypes.c_float), ("estimated_errorrate", ctypes.c_float),
                 ("m_coverage", ctypes.c_float), ("paired_filter_threshold", ctypes.c_float),
                 ("batch_size", ctypes.c_int), ("keep_intermediates", ctypes.c_int),
-                ("engine", ctypes.c_int), ("waves_per_cu", ctypes.c_int)]
+                ("engine", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
+                ("correction_type", ctypes.c_int)]
 
 
 class CareTimings(ctypes.Structure):
@@ -54,8 +55,12 @@ EXPORTS = [
     "care_hip_default_opts", "care_hip_create", "care_hip_destroy", "care_hip_last_error",
     "care_hip_set_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
-    "care_hip_inspect_msa",
+    "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features",
+    "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
+    "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
 ]
+
+MLP_PARAMS = 5313   # CARE_MLP_PARAMS: W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3
 
 
 def load_library(path: str = LIB_PATH):
@@ -83,6 +88,20 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_inspect_batch.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
                                            vp, vp, vp, vp, vp]
     lib.care_hip_inspect_msa.argtypes = [vp, u64, i32, ctypes.POINTER(i32), vp, vp, vp, vp, vp]
+    lib.care_hip_set_classifier.argtypes = [vp, vp, sz]
+    lib.care_hip_extract_features.argtypes = [vp, u64, u64, vp, vp, sz, ctypes.POINTER(sz)]
+    lib.care_mlp_create.argtypes = [i32, ctypes.POINTER(vp)]
+    lib.care_mlp_destroy.argtypes = [vp]
+    lib.care_mlp_destroy.restype = None
+    lib.care_mlp_last_error.argtypes = [vp]
+    lib.care_mlp_last_error.restype = ctypes.c_char_p
+    lib.care_mlp_set_params.argtypes = [vp, vp, sz]
+    lib.care_mlp_get_params.argtypes = [vp, vp, sz]
+    lib.care_mlp_reset_optimizer.argtypes = [vp]
+    lib.care_mlp_forward.argtypes = [vp, vp, sz, vp]
+    lib.care_mlp_loss_grad.argtypes = [vp, vp, vp, sz, vp, vp]
+    lib.care_mlp_set_data.argtypes = [vp, vp, vp, sz]
+    lib.care_mlp_train.argtypes = [vp, i32, i32, ctypes.c_float, u64, vp]
     for name in EXPORTS:
         getattr(lib, name)
     _LIB = lib
@@ -164,6 +183,30 @@ class CareEngine:
         self._check(self.lib.care_hip_copy_records(self.h, buf, nbytes.value), "care_hip_copy_records")
         return nrec.value, buf.raw[:nbytes.value]
 
+    def set_classifier(self, params) -> None:
+        """MLP parameters (MLP_PARAMS float32 values) for correction_type=1; None unloads."""
+        if params is None:
+            self._check(self.lib.care_hip_set_classifier(self.h, None, 0), "care_hip_set_classifier")
+            return
+        p = np.ascontiguousarray(params, np.float32)
+        self._check(self.lib.care_hip_set_classifier(self.h, p.ctypes.data, p.size), "care_hip_set_classifier")
+
+    def extract_features(self, first: int = 0, n: int | None = None, cap: int | None = None):
+        """Decision columns of anchors [first, first+n): (features [m,16] f32, meta [m,4] u32),
+        ordered by (read id, position)."""
+        n = self.storage.n - first if n is None else n
+        total = ctypes.c_size_t(0)
+        if cap is None:
+            self._check(self.lib.care_hip_extract_features(self.h, first, n, None, None, 0, ctypes.byref(total)),
+                        "care_hip_extract_features")
+            cap = total.value
+        feats = np.zeros((max(cap, 1), 16), np.float32)
+        meta = np.zeros((max(cap, 1), 4), np.uint32)
+        self._check(self.lib.care_hip_extract_features(self.h, first, n, feats.ctypes.data, meta.ctypes.data, cap,
+                                                       ctypes.byref(total)), "care_hip_extract_features")
+        m = min(total.value, cap)
+        return feats[:m], meta[:m]
+
     def timings(self) -> dict:
         t = CareTimings()
         self._check(self.lib.care_hip_get_timings(self.h, ctypes.byref(t)), "care_hip_get_timings")
@@ -204,14 +247,19 @@ class CareEngine:
                     consensus=cons[:c], support=sup[:c])
 
 
-def correct_gpu(storage, device: int = 0, **opts) -> tuple[int, bytes]:
-    """Reads -> tables -> records, the whole correction stage on one GPU."""
+def correct_gpu(storage, device: int = 0, classifier=None, **opts) -> tuple[int, bytes]:
+    """Reads -> tables -> records, the whole correction stage on one GPU. With a
+    classifier (MLP parameters) the learned seam is used (correction_type=1)."""
     opts.setdefault("paired", int(storage.paired))
     opts.setdefault("use_quality_scores", int(storage.quals is not None))
+    if classifier is not None:
+        opts.setdefault("correction_type", 1)
     eng = CareEngine(device, **opts)
     try:
         eng.set_reads(storage)
         eng.build_tables()
+        if classifier is not None:
+            eng.set_classifier(classifier)
         return eng.correct()
     finally:
         eng.close()

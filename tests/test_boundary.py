@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared(header):
     text = open(os.path.join(ROOT, "include", header)).read()
-    return sorted(set(re.findall(r"\b(care_(?:hip|sim)_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(care_(?:hip|sim|mlp)_[a-z_]+)\s*\(", text)))
 
 
 def test_hip_library_exports_every_declared_symbol():
