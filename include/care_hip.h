@@ -62,6 +62,8 @@ typedef struct care_opts {
                                        classifier seam of correctAnchorClf (include/corrector.hpp:
                                        1448-1472) with the MLP of care_hip_set_classifier, and the
                                        classic column rule where its confidence is below 0.9 */
+    int align_variant;              /* staged engine: 0 = one lane per candidate where reads have at
+                                       most 352 bases (else 1), 1 = one wavefront per pair */
 } care_opts;
 
 typedef struct care_ctx care_ctx;

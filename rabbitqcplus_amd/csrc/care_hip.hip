@@ -449,6 +449,8 @@ __global__ void k_pair_flags(const uint32_t* offsets, const uint32_t* ids, uint8
 struct AlignParams {
     const uint32_t* hilo; int W; const int* len;
     const uint32_t* ids; const uint32_t* anchorOf; uint64_t b0; uint32_t total;
+    const uint32_t* offsets; int B;     // lane kernel: candidates of anchor b0+i are [offsets[i], offsets[i+1])
+    int maxLen;
     int minOverlap; float maxErrorRate; float minOverlapRatio; float orientErrorRate;
     AlignBest* best; int* dbg;   // dbg: [total][11] or null
 };
@@ -689,6 +691,172 @@ k_align(AlignParams p){
     }
 }
 
+// ---------------------------------------------------------------- alignment, lane per candidate
+//
+// The reference's scan (src/cpu_alignment.cpp:32-497) with one lane per candidate. All
+// lanes of a wave share the anchor, so at each step of the scan the anchor's shifted
+// words are wave-uniform (scalar registers, shifted on the scalar unit), and each lane
+// compares them with its candidate's planes, both orientations, held in vector
+// registers. A lane follows the reference step for step: shifts 0..La-mo with the anchor
+// shifted, then -1..-(Lc-mo) with the candidate shifted (computed here as the anchor
+// shifted the other way, in candidate coordinates); max_errors = min(int(float(ov) *
+// maxErrorRate), best - total + 2*ov), the direction ends at the first shift where it is
+// <= 0, and a shift with fewer mismatches than max_errors becomes the best (it always
+// scores below the best then). The word-offset loops are unrolled so every register
+// index is a constant.
+//
+// Anchor words are kept with one zero word in front: AX[i + 1] = word i, AX[0] = 0, and
+// zeros past the read.
+template<int WM>
+__device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, const uint32_t* ids, int n,
+                                           AlignBest* best, int* dbg, int lane){
+    const int W = p.W;
+    const int La = p.len[a];
+    const uint32_t* arow = p.hilo + a * uint64_t(4 * W);
+    uint32_t AH[2 * WM + 2], AL[2 * WM + 2];
+    #pragma unroll
+    for(int i = 0; i < 2 * WM + 2; i++){
+        const bool in = i >= 1 && i <= W;
+        AH[i] = in ? arow[i - 1] : 0u;
+        AL[i] = in ? arow[W + i - 1] : 0u;
+    }
+    const int mo = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
+    const int lastF = La - mo;
+    for(int c0 = 0; c0 < n; c0 += kWave){
+        const int c = c0 + lane;
+        const bool valid = c < n;
+        uint32_t id = 0;
+        int Lc = 0;
+        uint32_t FH[WM], FL[WM], RH[WM], RL[WM];
+        if(valid){
+            id = ids[c];
+            Lc = p.len[id];
+        }
+        const uint32_t* crow = p.hilo + uint64_t(id) * uint64_t(4 * W);
+        #pragma unroll
+        for(int w = 0; w < WM; w++){
+            const bool in = valid && w < W;
+            FH[w] = in ? crow[w] : 0u;
+            FL[w] = in ? crow[W + w] : 0u;
+            RH[w] = in ? crow[2 * W + w] : 0u;
+            RL[w] = in ? crow[3 * W + w] : 0u;
+        }
+        const int total = La + Lc;
+        int bF = total, sF = -Lc, bR = total, sR = -Lc;
+        // anchor shifted: shifts 0..lastF
+        bool oF = valid, oR = valid;
+        #pragma unroll
+        for(int ps = 0; ps <= WM; ps++){
+            if(32 * ps > lastF || __ballot(oF || oR) == 0) break;
+            const int rEnd = min(31, lastF - 32 * ps);
+            for(int r = 0; r <= rEnd; r++){
+                const int sft = 32 * ps + r;
+                const int ov = min(La - sft, Lc);
+                const int e = int(float(ov) * p.maxErrorRate);
+                const int mF = min(e, bF - total + 2 * ov);
+                const int mR = min(e, bR - total + 2 * ov);
+                oF = oF && mF > 0;
+                oR = oR && mR > 0;
+                if(__ballot(oF || oR) == 0) break;
+                const int nw = (ov + 31) >> 5;
+                const uint32_t lastMask = 0xFFFFFFFFu << ((nw << 5) - ov);
+                int dF = 0, dR = 0;
+                #pragma unroll
+                for(int w = 0; w < WM; w++){
+                    const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
+                    const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
+                    const uint32_t m = w < nw ? (w == nw - 1 ? lastMask : 0xFFFFFFFFu) : 0u;
+                    dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
+                    dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
+                }
+                if(oF && dF < mF){ bF = dF + total - 2 * ov; sF = sft; }
+                if(oR && dR < mR){ bR = dR + total - 2 * ov; sR = sft; }
+            }
+        }
+        // candidate shifted: shifts -1..-(Lc-mo), here t = 1..Lc-mo with the anchor
+        // shifted right by t against the candidate's positions [t, t+ov)
+        const int lastB = Lc - mo;
+        const int lastBw = p.maxLen - mo;
+        oF = valid; oR = valid;
+        #pragma unroll
+        for(int pt = 0; pt <= WM; pt++){
+            if(32 * pt > lastBw || __ballot(oF || oR) == 0) break;
+            const int rEnd = min(31, lastBw - 32 * pt);
+            for(int r = (pt == 0 ? 1 : 0); r <= rEnd; r++){
+                const int t = 32 * pt + r;
+                const int ov = min(La, Lc - t);
+                const int e = int(float(ov) * p.maxErrorRate);
+                const int mF = min(e, bF - total + 2 * ov);
+                const int mR = min(e, bR - total + 2 * ov);
+                const bool inRange = t <= lastB;
+                oF = oF && inRange && mF > 0;
+                oR = oR && inRange && mR > 0;
+                if(__ballot(oF || oR) == 0) break;
+                const int w1 = (t + ov - 1) >> 5;
+                const uint32_t lastMask = 0xFFFFFFFFu << (31 - ((t + ov - 1) & 31));
+                const uint32_t firstMask = 0xFFFFFFFFu >> r;
+                int dF = 0, dR = 0;
+                #pragma unroll
+                for(int w = pt; w < WM; w++){
+                    const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
+                    const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
+                    uint32_t m = w <= w1 ? 0xFFFFFFFFu : 0u;
+                    if(w == pt) m &= firstMask;
+                    if(w == w1) m &= lastMask;
+                    dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
+                    dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
+                }
+                if(oF && dF < mF){ bF = dF + total - 2 * ov; sF = -t; }
+                if(oR && dR < mR){ bR = dR + total - 2 * ov; sR = -t; }
+            }
+        }
+        if(!valid) continue;
+        ShdResult f, rr;
+        {
+            f.score = bF; f.shift = sF; f.valid = sF != -Lc;
+            f.overlap = min(Lc, La - sF) - max(-sF, 0);
+            f.nOps = bF - total + 2 * f.overlap;
+            rr.score = bR; rr.shift = sR; rr.valid = sR != -Lc;
+            rr.overlap = min(Lc, La - sR) - max(-sR, 0);
+            rr.nOps = bR - total + 2 * rr.overlap;
+        }
+        const int flag = chooseOrientation(f, rr, La, p);
+        AlignBest b;
+        const ShdResult& sres = flag == 2 ? rr : f;
+        b.shift = sres.shift; b.overlap = sres.overlap; b.nOps = sres.nOps; b.orientation = flag;
+        best[c] = b;
+        if(dbg){
+            int* d = dbg + uint64_t(c) * 11;
+            d[0] = f.score; d[1] = f.overlap; d[2] = f.shift; d[3] = f.nOps; d[4] = f.valid;
+            d[5] = rr.score; d[6] = rr.overlap; d[7] = rr.shift; d[8] = rr.nOps; d[9] = rr.valid;
+            d[10] = flag;
+        }
+    }
+}
+
+// One wavefront per anchor of the batch, its candidates in chunks of 64 lanes.
+template<int WM>
+__global__ void __launch_bounds__(kWave)
+k_align_lanes(AlignParams p){
+    const int i = blockIdx.x;
+    if(i >= p.B) return;
+    const uint32_t beg = p.offsets[i];
+    const int n = int(p.offsets[i + 1] - beg);
+    if(n == 0) return;
+    alignLanes<WM>(p, p.b0 + i, p.ids + beg, n, p.best + beg, p.dbg ? p.dbg + uint64_t(beg) * 11 : nullptr,
+                   threadIdx.x);
+}
+
+// Words per plane handled by the lane kernel for reads of W words (0: too long).
+int alignLaneWords(int W){
+    if(W <= 2) return 2;
+    if(W <= 4) return 4;
+    if(W <= 5) return 5;
+    if(W <= 8) return 8;
+    if(W <= 11) return 11;
+    return 0;
+}
+
 // ---------------------------------------------------------------- MSA + decision
 
 struct MsaParams {
@@ -714,6 +882,17 @@ struct MsaParams {
     float estCov;
     float* featOut; uint32_t* metaOut; unsigned long long* featCount; unsigned long long featCap;
 };
+
+// Per-anchor candidate lists of the MSA stage: the staged kernels index batch-wide
+// arrays, the fused kernel a wave's private scratch.
+struct AnchorLists {
+    const uint32_t* ids; const AlignBest* best; const uint8_t* paired;
+    uint32_t* kId; int* kShift; int* kLen; int* kNops; int* kOvl; float* kW; uint8_t* kOrient; uint8_t* kPaired;
+};
+
+__device__ __forceinline__ AnchorLists listsOf(const MsaParams& p){
+    return AnchorLists{p.ids, p.best, p.paired, p.kId, p.kShift, p.kLen, p.kNops, p.kOvl, p.kW, p.kOrient, p.kPaired};
+}
 
 struct MsaShared {
     int tmp[kBlock / kWave + 1];
@@ -756,16 +935,17 @@ struct MsaTile {
 // candidate for the metadata and its oriented HiLo planes; qualities with threads over
 // positions (coalesced), candidates unrolled so several rows are in flight.
 // Caller synchronizes before (tile free) and after (tile complete).
-__device__ __forceinline__ void stageTile(const MsaParams& p, const MsaTile& t, int beg, int tb, int tn,
+__device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists& lst, const MsaTile& t, int beg,
+                                          int tb, int tn,
                                           int tid, int nthreads){
     const int W = p.W;
     for(int q = tid; q < tn; q += nthreads){
         const int d = beg + tb + q;
-        const uint32_t id = p.kId[d];
-        const int orient = p.kOrient[d];
-        t.shift[q] = p.kShift[d];
-        t.len[q] = p.kLen[d];
-        t.w[q] = p.kW[d];
+        const uint32_t id = lst.kId[d];
+        const int orient = lst.kOrient[d];
+        t.shift[q] = lst.kShift[d];
+        t.len[q] = lst.kLen[d];
+        t.w[q] = lst.kW[d];
         t.id[q] = id;
         t.orient[q] = orient;
         const uint32_t* src = p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 1 ? 0 : 2 * W);
@@ -831,7 +1011,7 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();   // previous tile fully consumed
-        stageTile(p, t, beg, tb, tn, threadIdx.x, kBlock);
+        stageTile(p, listsOf(p), t, beg, tb, tn, threadIdx.x, kBlock);
         __syncthreads();
         for(int c = threadIdx.x; c < ncol; c += kBlock){
             int n0 = cnt[4 * c + 0], n1 = cnt[4 * c + 1], n2 = cnt[4 * c + 2], n3 = cnt[4 * c + 3];
@@ -1226,13 +1406,14 @@ __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
 // MSA of the anchor and the first nk working candidates (src/msa.cpp:9-173), one wave:
 // every lane adds the anchor then the candidates, in list order, to the columns it owns.
 template<int KC>
-__device__ __forceinline__ bool buildMsaWave(const MsaParams& p, uint64_t a, int La, const uint32_t* arow, int beg, int nk,
+__device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int La,
+                                             const uint32_t* arow, int beg, int nk,
                              WaveMsa<KC>& m, const MsaTile& t, int lane){
     int lo = 0, hi = La;
     for(int j = lane; j < nk; j += kWave){
-        const int s = p.kShift[beg + j];
+        const int s = lst.kShift[beg + j];
         lo = min(lo, s);
-        hi = max(hi, s + p.kLen[beg + j]);
+        hi = max(hi, s + lst.kLen[beg + j]);
     }
     lo = waveReduce(lo, OpMin(), INT_MAX);
     hi = waveReduce(hi, OpMax(), INT_MIN);
@@ -1269,7 +1450,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, uint64_t a, int
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();
-        stageTile(p, t, beg, tb, tn, lane, kWave);
+        stageTile(p, lst, t, beg, tb, tn, lane, kWave);
         __syncthreads();
         for(int j = 0; j < tn; j++){
             const int cs = ab + t.shift[j];          // first column of candidate j (>= 0)
@@ -1363,9 +1544,10 @@ __device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, c
 
 // Filters, MSA, refinement, decision and record of anchor `a` by one wavefront
 // (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734). Its candidates
-// are p.ids/p.best/p.paired[beg, beg+n0); the record goes to slot `i`.
+// are lst.ids/lst.best/lst.paired[beg, beg+n0); the record goes to slot `i`.
 template<int KC>
-__device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, int i, int beg, int n0, const MsaTile& tile,
+__device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int i, int beg,
+                                              int n0, const MsaTile& tile,
                               float* sSup, int lane){
     const int La = p.len[a];
     const uint32_t* arow = p.enc + p.pitch * a;
@@ -1381,18 +1563,18 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
         const int j = base + lane;
         AlignBest b{};
         bool keep = false;
-        if(j < n0){ b = p.best[beg + j]; keep = b.orientation != 3; }
+        if(j < n0){ b = lst.best[beg + j]; keep = b.orientation != 3; }
         const uint64_t mask = __ballot(keep);
         if(keep){
             const int d = beg + n1 + __popcll(mask & below);
-            const uint32_t id = p.ids[beg + j];
-            p.kId[d] = id;
-            p.kShift[d] = b.shift;
-            p.kLen[d] = p.len[id];
-            p.kNops[d] = b.nOps;
-            p.kOvl[d] = b.overlap;
-            p.kOrient[d] = uint8_t(b.orientation);
-            p.kPaired[d] = p.paired ? p.paired[beg + j] : 0;
+            const uint32_t id = lst.ids[beg + j];
+            lst.kId[d] = id;
+            lst.kShift[d] = b.shift;
+            lst.kLen[d] = p.len[id];
+            lst.kNops[d] = b.nOps;
+            lst.kOvl[d] = b.overlap;
+            lst.kOrient[d] = uint8_t(b.orientation);
+            lst.kPaired[d] = lst.paired ? lst.paired[beg + j] : 0;
         }
         n1 += __popcll(mask);
     }
@@ -1405,7 +1587,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
         for(int base = 0; base < n1; base += kWave){
             const int j = base + lane;
             float ratio = 1e30f;
-            if(j < n1) ratio = float(p.kNops[beg + j]) / float(p.kOvl[beg + j]);
+            if(j < n1) ratio = float(lst.kNops[beg + j]) / float(lst.kOvl[beg + j]);
             c0 += __popcll(__ballot(j < n1 && ratio < p.ratio2));
             c1 += __popcll(__ballot(j < n1 && ratio < p.ratio3));
             c2 += __popcll(__ballot(j < n1 && ratio < p.ratio4));
@@ -1422,18 +1604,18 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
         bool keep = false;
         uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0;
         if(j < n1){
-            id = p.kId[beg + j]; s = p.kShift[beg + j]; l = p.kLen[beg + j]; no = p.kNops[beg + j];
-            ov = p.kOvl[beg + j]; o = p.kOrient[beg + j]; pr = p.kPaired[beg + j];
+            id = lst.kId[beg + j]; s = lst.kShift[beg + j]; l = lst.kLen[beg + j]; no = lst.kNops[beg + j];
+            ov = lst.kOvl[beg + j]; o = lst.kOrient[beg + j]; pr = lst.kPaired[beg + j];
             const float ratio = float(no) / float(ov);
             keep = p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold);
         }
         const uint64_t mask = __ballot(keep);
         if(keep){
             const int d = beg + n2 + __popcll(mask & below);
-            p.kId[d] = id; p.kShift[d] = s; p.kLen[d] = l; p.kNops[d] = no; p.kOvl[d] = ov;
-            p.kOrient[d] = o; p.kPaired[d] = pr;
+            lst.kId[d] = id; lst.kShift[d] = s; lst.kLen[d] = l; lst.kNops[d] = no; lst.kOvl[d] = ov;
+            lst.kOrient[d] = o; lst.kPaired[d] = pr;
             // overlap weight (include/hostdevicefunctions.cuh:93-97)
-            p.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
+            lst.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
         }
         n2 += __popcll(mask);
     }
@@ -1443,7 +1625,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
     // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
     WaveMsa<KC> m;
     int nk = n2;
-    if(!buildMsaWave<KC>(p, a, La, arow, beg, nk, m, tile, lane)){
+    if(!buildMsaWave<KC>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
         if(lane == 0) atomicAdd(p.error, 1);
         return;
     }
@@ -1483,15 +1665,15 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
             const int j = base + lane;
             bool rem = false;
             if(j < nk){
-                const int rb = m.ab + p.kShift[beg + j];
-                const int l = p.kLen[beg + j];
+                const int rb = m.ab + lst.kShift[beg + j];
+                const int l = lst.kLen[beg + j];
                 if(col >= rb && col < rb + l){
-                    const int b = orientedBase(p, p.kId[beg + j], l, p.kOrient[beg + j], col - rb);
+                    const int b = orientedBase(p, lst.kId[beg + j], l, lst.kOrient[beg + j], col - rb);
                     rem = keepMatching ? (b != sigBase) : (b == sigBase);
                 }
             }
             anyRemove |= __ballot(rem) != 0;
-            veryGood |= __ballot(rem && p.kW[beg + j] >= 0.90f) != 0;
+            veryGood |= __ballot(rem && lst.kW[beg + j] >= 0.90f) != 0;
         }
         // a round that removes nothing leaves the MSA, and hence every later round, unchanged
         if(!anyRemove || veryGood) break;
@@ -1501,8 +1683,8 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
             bool keep = false;
             uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0; float w = 0.0f;
             if(j < nk){
-                id = p.kId[beg + j]; s = p.kShift[beg + j]; l = p.kLen[beg + j]; no = p.kNops[beg + j];
-                ov = p.kOvl[beg + j]; o = p.kOrient[beg + j]; pr = p.kPaired[beg + j]; w = p.kW[beg + j];
+                id = lst.kId[beg + j]; s = lst.kShift[beg + j]; l = lst.kLen[beg + j]; no = lst.kNops[beg + j];
+                ov = lst.kOvl[beg + j]; o = lst.kOrient[beg + j]; pr = lst.kPaired[beg + j]; w = lst.kW[beg + j];
                 const int rb = m.ab + s;
                 keep = true;
                 if(col >= rb && col < rb + l){
@@ -1513,14 +1695,14 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, uint64_t a, in
             const uint64_t mask = __ballot(keep);
             if(keep){
                 const int d = beg + nn + __popcll(mask & below);
-                p.kId[d] = id; p.kShift[d] = s; p.kLen[d] = l; p.kNops[d] = no; p.kOvl[d] = ov;
-                p.kOrient[d] = o; p.kPaired[d] = pr; p.kW[d] = w;
+                lst.kId[d] = id; lst.kShift[d] = s; lst.kLen[d] = l; lst.kNops[d] = no; lst.kOvl[d] = ov;
+                lst.kOrient[d] = o; lst.kPaired[d] = pr; lst.kW[d] = w;
             }
             nn += __popcll(mask);
         }
         nk = nn;
         rounds++;
-        if(!buildMsaWave<KC>(p, a, La, arow, beg, nk, m, tile, lane)){
+        if(!buildMsaWave<KC>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
             if(lane == 0) atomicAdd(p.error, 1);
             return;
         }
@@ -1749,7 +1931,7 @@ k_msa_wave(MsaParams p){
     const int i = blockIdx.x;
     const int beg = int(p.offsets[i]);
     const int n0 = int(p.offsets[i + 1]) - beg;
-    msaAnchorWave<KC>(p, p.b0 + i, i, beg, n0, tile, sSup, lane);
+    msaAnchorWave<KC>(p, listsOf(p), p.b0 + i, i, beg, n0, tile, sSup, lane);
 }
 
 // ---------------------------------------------------------------- fused persistent kernel
@@ -1941,8 +2123,8 @@ __device__ __forceinline__ void alignAllWave(const AlignParams& p, uint64_t a, c
 // Persistent fused pass: each wave takes work units (an anchor, or a mate pair) from a
 // counter and runs candidates -> pair flags -> alignments -> filters/MSA/refinement/
 // decision -> record for them, keeping per-anchor lists in its private scratch.
-template<int KC>
-__global__ void __launch_bounds__(kWave)
+template<int KC, bool PAIRED>
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 3 : 2)))
 k_fused(FusedParams fp){
     extern __shared__ uint8_t smem[];
     __shared__ float sQw[256];
@@ -1951,34 +2133,30 @@ k_fused(FusedParams fp){
     __shared__ int sUnique;
     __shared__ uint32_t sUnit;
     const int lane = threadIdx.x;
-    MsaParams mp = fp.m;
     const WaveScratch ws = carveScratch(fp.scratch + size_t(blockIdx.x) * fp.scratchStride, fp.cap);
-    mp.kId = ws.kId; mp.kShift = ws.kShift; mp.kLen = ws.kLen; mp.kNops = ws.kNops; mp.kOvl = ws.kOvl;
-    mp.kW = ws.kW; mp.kOrient = ws.kOrient; mp.kPaired = ws.kPaired; mp.best = ws.best;
     MsaTile tile;
     float* sSup;
-    carveWaveTile(mp, smem, sQw, tile, sSup);
+    carveWaveTile(fp.m, smem, sQw, tile, sSup);
     uint32_t* hset = reinterpret_cast<uint32_t*>(smem);
     const int stride = fp.al.W + 1;
     uint2* planeA = reinterpret_cast<uint2*>(smem);
     uint2* planeC = planeA + 2 * stride;
-    if(mp.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = mp.qweight[q];
+    if(fp.m.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = fp.m.qweight[q];
     __syncthreads();
-    const bool paired = mp.isPaired != 0;
+    AnchorLists lst{ws.idsA, ws.best, nullptr, ws.kId, ws.kShift, ws.kLen, ws.kNops, ws.kOvl, ws.kW, ws.kOrient,
+                    ws.kPaired};
     for(;;){
         if(lane == 0) sUnit = atomicAdd(fp.counter, 1u);
         __syncthreads();
         const uint32_t unit = sUnit;
         __syncthreads();
         if(unit >= fp.units) break;
-        if(!paired){
+        if(!PAIRED){
             const uint64_t a = fp.first + unit;
             const int U = candidatesWave(fp.c, a, hset, sOff, sPre, &sUnique, ws.idsA, lane);
             if(lane == 0 && U) atomicAdd(fp.aligned, (unsigned long long)U);
             alignAllWave(fp.al, a, ws.idsA, U, ws.best, planeA, planeC, lane);
-            mp.ids = ws.idsA;
-            mp.paired = nullptr;
-            msaAnchorWave<KC>(mp, a, int(unit), 0, U, tile, sSup, lane);
+            msaAnchorWave<KC>(fp.m, lst, a, int(unit), 0, U, tile, sSup, lane);
         }else{
             const uint64_t a0 = fp.first + 2 * uint64_t(unit);
             const int U0 = candidatesWave(fp.c, a0, hset, sOff, sPre, &sUnique, ws.idsA, lane);
@@ -1991,12 +2169,11 @@ k_fused(FusedParams fp){
             __syncthreads();
             for(int mate = 0; mate < 2; mate++){
                 const uint64_t a = a0 + mate;
-                const uint32_t* ids = mate == 0 ? ws.idsA : ws.idsB;
+                lst.ids = mate == 0 ? ws.idsA : ws.idsB;
+                lst.paired = mate == 0 ? ws.pairedA : ws.pairedB;
                 const int U = mate == 0 ? U0 : U1;
-                alignAllWave(fp.al, a, ids, U, ws.best, planeA, planeC, lane);
-                mp.ids = ids;
-                mp.paired = mate == 0 ? ws.pairedA : ws.pairedB;
-                msaAnchorWave<KC>(mp, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
+                alignAllWave(fp.al, a, lst.ids, U, ws.best, planeA, planeC, lane);
+                msaAnchorWave<KC>(fp.m, lst, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
                 __syncthreads();
             }
         }
@@ -2272,6 +2449,7 @@ void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
     ap.hilo = ctx->hilo.as<uint32_t>(); ap.W = ctx->W; ap.len = ctx->len.as<int>();
     ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
     ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
+    ap.maxLen = ctx->maxLen;
 }
 
 int recordSlotBytes(int maxLen){
@@ -2359,20 +2537,22 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     k_zero_u32<<<1, 64, 0, st>>>(ctx->counters.as<uint32_t>() + 4, 1);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[0], st));
+    auto launch = [&](auto kern) -> hipError_t {
+        if(lds > 65536){
+            const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
+            if(e != hipSuccess) return e;
+        }
+        kern<<<dim3(uint32_t(G)), kWave, lds, st>>>(fp);
+        return hipGetLastError();
+    };
+    hipError_t le;
     switch(kc){
-        case 4:
-            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<4>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            k_fused<4><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
-        case 8:
-            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<8>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            k_fused<8><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
-        case 12:
-            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<12>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            k_fused<12><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
-        default:
-            if(lds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_fused<16>, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
-            k_fused<16><<<dim3(uint32_t(G)), kWave, lds, st>>>(fp); break;
+        case 4: le = paired ? launch(k_fused<4, true>) : launch(k_fused<4, false>); break;
+        case 8: le = paired ? launch(k_fused<8, true>) : launch(k_fused<8, false>); break;
+        case 12: le = paired ? launch(k_fused<12, true>) : launch(k_fused<12, false>); break;
+        default: le = paired ? launch(k_fused<16, true>) : launch(k_fused<16, false>); break;
     }
+    HIPCHECK(le);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[1], st));
     const int rc = gatherRecords(ctx, B, slotBytes, out, outRecords);
@@ -2443,8 +2623,17 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
     ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
     ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
+    ap.offsets = ctx->offsets.as<uint32_t>(); ap.B = B; ap.maxLen = ctx->maxLen;
+    const int wm = ctx->o.align_variant == 1 ? 0 : alignLaneWords(ctx->W);
     if(total){
-        k_align<<<(total + kAlignWaves - 1) / kAlignWaves, kBlock, 0, st>>>(ap);
+        switch(wm){
+            case 2: k_align_lanes<2><<<B, kWave, 0, st>>>(ap); break;
+            case 4: k_align_lanes<4><<<B, kWave, 0, st>>>(ap); break;
+            case 5: k_align_lanes<5><<<B, kWave, 0, st>>>(ap); break;
+            case 8: k_align_lanes<8><<<B, kWave, 0, st>>>(ap); break;
+            case 11: k_align_lanes<11><<<B, kWave, 0, st>>>(ap); break;
+            default: k_align<<<(total + kAlignWaves - 1) / kAlignWaves, kBlock, 0, st>>>(ap); break;
+        }
         HIPCHECK(hipGetLastError());
     }
     HIPCHECK(hipEventRecord(ctx->ev[2], st));
@@ -2691,9 +2880,14 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
     HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, 64, ctx->stream));
     int B = ctx->o.batch_size > 0 ? ctx->o.batch_size : 0;
     if(B <= 0){
-        // keep the per-anchor candidate staging area near 1 GiB
-        const size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + 512;
-        B = int(std::min<size_t>(size_t(1) << 18, std::max<size_t>(1024, (size_t(1) << 30) / perAnchor)));
+        // one batch per pass where HBM allows: the per-anchor candidate staging area
+        // (maps * threshold ids) and record slot may take up to 16 GiB or a quarter of
+        // the free device memory; fewer batches mean fewer launch tails and host syncs
+        size_t freeB = 0, totalB = 0;
+        HIPCHECK(hipMemGetInfo(&freeB, &totalB));
+        const size_t budget = std::min<size_t>(size_t(16) << 30, freeB / 4);
+        const size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + size_t(recordSlotBytes(ctx->maxLen)) + 512;
+        B = int(std::min<size_t>(size_t(1) << 22, std::max<size_t>(1024, budget / perAnchor)));
     }
     if(ctx->o.paired && (B & 1)) B++;
     // the fused kernel keeps MSA columns in registers (<= 1024 columns) and prefetches a

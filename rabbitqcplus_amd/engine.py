@@ -37,7 +37,7 @@ class CareOpts(ctypes.Structure):
                 ("m_coverage", ctypes.c_float), ("paired_filter_threshold", ctypes.c_float),
                 ("batch_size", ctypes.c_int), ("keep_intermediates", ctypes.c_int),
                 ("engine", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("correction_type", ctypes.c_int)]
+                ("correction_type", ctypes.c_int), ("align_variant", ctypes.c_int)]
 
 
 class CareTimings(ctypes.Structure):
