@@ -707,6 +707,12 @@ k_align(AlignParams p){
 //
 // Anchor words are kept with one zero word in front: AX[i + 1] = word i, AX[0] = 0, and
 // zeros past the read.
+// Mask of the first min(32, max(0, k)) positions (most significant bits) of a word.
+__device__ __forceinline__ uint32_t overlapMask(int k){
+    const int c = min(max(k, 0), 32);
+    return uint32_t(0xFFFFFFFF00000000ull >> c);
+}
+
 template<int WM>
 __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, const uint32_t* ids, int n,
                                            AlignBest* best, int* dbg, int lane){
@@ -757,15 +763,17 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                 const int mR = min(e, bR - total + 2 * ov);
                 oF = oF && mF > 0;
                 oR = oR && mR > 0;
-                if(__ballot(oF || oR) == 0) break;
-                const int nw = (ov + 31) >> 5;
-                const uint32_t lastMask = 0xFFFFFFFFu << ((nw << 5) - ov);
+                if(__builtin_amdgcn_ballot_w64(oF || oR) == 0) break;
                 int dF = 0, dR = 0;
                 #pragma unroll
                 for(int w = 0; w < WM; w++){
+                    // once no lane can still stay below its bound, the remaining words
+                    // cannot make any shift admissible (mismatch counts only grow)
+                    if(w == 2 && __builtin_amdgcn_ballot_w64((oF && dF < mF) || (oR && dR < mR)) == 0) break;
                     const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
                     const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
-                    const uint32_t m = w < nw ? (w == nw - 1 ? lastMask : 0xFFFFFFFFu) : 0u;
+                    // top min(32, max(0, ov - 32w)) bits: positions [32w, 32w+32) inside the overlap
+                    const uint32_t m = overlapMask(ov - 32 * w);
                     dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                     dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
                 }
@@ -791,18 +799,16 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                 const bool inRange = t <= lastB;
                 oF = oF && inRange && mF > 0;
                 oR = oR && inRange && mR > 0;
-                if(__ballot(oF || oR) == 0) break;
-                const int w1 = (t + ov - 1) >> 5;
-                const uint32_t lastMask = 0xFFFFFFFFu << (31 - ((t + ov - 1) & 31));
-                const uint32_t firstMask = 0xFFFFFFFFu >> r;
+                if(__builtin_amdgcn_ballot_w64(oF || oR) == 0) break;
+                const int end = t + ov;       // window [t, end) in candidate positions
                 int dF = 0, dR = 0;
                 #pragma unroll
                 for(int w = pt; w < WM; w++){
+                    if(w == pt + 2 && __builtin_amdgcn_ballot_w64((oF && dF < mF) || (oR && dR < mR)) == 0) break;
                     const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
                     const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
-                    uint32_t m = w <= w1 ? 0xFFFFFFFFu : 0u;
-                    if(w == pt) m &= firstMask;
-                    if(w == w1) m &= lastMask;
+                    uint32_t m = overlapMask(end - 32 * w);
+                    if(w == pt) m &= 0xFFFFFFFFu >> r;
                     dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                     dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
                 }
@@ -861,6 +867,7 @@ int alignLaneWords(int W){
 
 struct MsaParams {
     const uint32_t* enc; size_t pitch; const int* len; const uint8_t* qual; size_t qpitch; const uint8_t* ambig;
+    size_t qdpitch;                 // device quality row pitch (qpitch rounded up to 16 bytes)
     const float* qweight;
     uint64_t b0;
     const uint32_t* offsets; const uint32_t* ids; const AlignBest* best; const uint8_t* paired;
@@ -911,7 +918,7 @@ __device__ __forceinline__ int orientedBase(const MsaParams& p, uint32_t id, int
 }
 
 __device__ __forceinline__ float orientedQualityWeight(const MsaParams& p, uint32_t id, int orient, int pos){
-    const uint8_t* q = p.qual + p.qpitch * id;
+    const uint8_t* q = p.qual + p.qdpitch * id;
     // reverse-complement candidates read the pitch-wide quality row reversed
     // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
     return p.qweight[orient == 1 ? q[pos] : q[p.qpitch - 1 - pos]];
@@ -931,13 +938,16 @@ struct MsaTile {
     const float* qw;     // quality-weight table in LDS
 };
 
+// Row stride of the tile's quality rows: 16-byte aligned, so that forward rows are
+// written with one 16-byte store per chunk.
+__host__ __device__ inline int qStride(int maxLen){ return (maxLen + 15) & ~15; }
+
 // Stages candidates [tb, tb+tn) of the working list into the LDS tile: one thread per
-// candidate for the metadata and its oriented HiLo planes; qualities with threads over
-// positions (coalesced), candidates unrolled so several rows are in flight.
-// Caller synchronizes before (tile free) and after (tile complete).
+// candidate for the metadata and its oriented HiLo planes (8-byte loads); qualities as
+// 16-byte chunks of the device rows, spread over the threads, reverse-complement rows
+// reversed byte by byte. Caller synchronizes before (tile free) and after (tile complete).
 __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists& lst, const MsaTile& t, int beg,
-                                          int tb, int tn,
-                                          int tid, int nthreads){
+                                          int tb, int tn, int tid, int nthreads){
     const int W = p.W;
     for(int q = tid; q < tn; q += nthreads){
         const int d = beg + tb + q;
@@ -948,23 +958,39 @@ __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists&
         t.w[q] = lst.kW[d];
         t.id[q] = id;
         t.orient[q] = orient;
-        const uint32_t* src = p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 1 ? 0 : 2 * W);
-        for(int wd = 0; wd < W; wd++){
-            t.hi[q * W + wd] = src[wd];
-            t.lo[q * W + wd] = src[W + wd];
+        // planes [hi W words | lo W words] of the orientation: W pairs, 8-byte aligned (rows
+        // are 16W bytes, the reverse-complement planes start 8W bytes in)
+        const uint2* src = reinterpret_cast<const uint2*>(p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 1 ? 0 : 2 * W));
+        for(int k2 = 0; k2 < W; k2++){
+            const uint2 v = src[k2];
+            const int w0 = 2 * k2, w1 = 2 * k2 + 1;
+            if(w0 < W) t.hi[q * W + w0] = v.x; else t.lo[q * W + w0 - W] = v.x;
+            if(w1 < W) t.hi[q * W + w1] = v.y; else t.lo[q * W + w1 - W] = v.y;
         }
     }
     if(!p.useQ) return;
     __syncthreads();
+    const int qs = qStride(p.maxLen);
     const int L = p.maxLen;
-    #pragma unroll 4
-    for(int j = 0; j < tn; j++){
-        const uint8_t* qr = p.qual + p.qpitch * t.id[j];
-        const bool fwd = t.orient[j] == 1;
-        for(int pos = tid; pos < L; pos += nthreads){
+    const int qp = int(p.qpitch);
+    const int nc = (qp + 15) >> 4;       // chunks per row; device rows are qdpitch >= 16*nc bytes
+    for(int u = tid; u < tn * nc; u += nthreads){
+        const int j = u / nc;
+        const int c = u - j * nc;
+        const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * t.id[j] + 16 * c);
+        uint8_t* dst = t.q + j * qs;
+        if(t.orient[j] == 1){
+            // positions past maxLen are never read; the chunk fits the row when 16c < qs
+            if(16 * c < qs) *reinterpret_cast<uint4*>(dst + 16 * c) = v;
+        }else{
             // reverse-complement candidates read the pitch-wide quality row reversed
             // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
-            t.q[j * L + pos] = fwd ? qr[pos] : qr[p.qpitch - 1 - pos];
+            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
+            #pragma unroll
+            for(int k = 0; k < 16; k++){
+                const int pos = qp - 1 - (16 * c + k);
+                if(pos >= 0 && pos < L) dst[pos] = uint8_t(wv[k >> 2] >> (8 * (k & 3)));
+            }
         }
     }
 }
@@ -988,7 +1014,7 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
     const int ab = max(-lo, 0);
     if(ncol > p.maxCols) return false;
     if(threadIdx.x == 0){ sh.ncol = ncol; sh.ab = ab; }
-    const uint8_t* aq = p.useQ ? p.qual + p.qpitch * a : nullptr;
+    const uint8_t* aq = p.useQ ? p.qual + p.qdpitch * a : nullptr;
     // anchor first (weight 1.0f * quality weight)
     for(int c = threadIdx.x; c < ncol; c += kBlock){
         int n[4] = {0, 0, 0, 0};
@@ -1024,7 +1050,7 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
                     const int sh5 = 31 - (pos & 31);
                     const int wd = j * W + (pos >> 5);
                     const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
-                    const float x = t.w[j] * (p.useQ ? t.qw[t.q[j * p.maxLen + pos]] : 1.0f);
+                    const float x = t.w[j] * (p.useQ ? t.qw[t.q[j * qStride(p.maxLen) + pos]] : 1.0f);
                     if(b == 0){ n0++; w0 += x; } else if(b == 1){ n1++; w1 += x; }
                     else if(b == 2){ n2++; w2 += x; } else { n3++; w3 += x; }
                     cv++;
@@ -1424,7 +1450,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     m.ab = ab;
     m.abase = 0;
     m.isAnchor = 0;
-    const uint8_t* aq = p.useQ ? p.qual + p.qpitch * a : nullptr;
+    const uint8_t* aq = p.useQ ? p.qual + p.qdpitch * a : nullptr;
     #pragma unroll
     for(int k = 0; k < KC; k++){
         #pragma unroll
@@ -1466,7 +1492,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
                     const int sh5 = 31 - (pos & 31);
                     const int wd = j * W + (pos >> 5);
                     const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
-                    const float x = wj * (p.useQ ? t.qw[t.q[j * p.maxLen + pos]] : 1.0f);
+                    const float x = wj * (p.useQ ? t.qw[t.q[j * qStride(p.maxLen) + pos]] : 1.0f);
                     if(b == 0){ m.w[k][0] += x; m.c01[k] += 1u; }
                     else if(b == 1){ m.w[k][1] += x; m.c01[k] += 0x10000u; }
                     else if(b == 2){ m.w[k][2] += x; m.c23[k] += 1u; }
@@ -1538,7 +1564,7 @@ __device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, c
     tile.lo = tile.hi + T * p.W;
     tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
     tile.qw = qw;
-    const size_t tileBytes = size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * p.maxLen : 0);
+    const size_t tileBytes = size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * qStride(p.maxLen) : 0);
     sSup = reinterpret_cast<float*>(tp + (max(tileBytes, kSeamLdsBytes) + 15) / 16 * 16);
 }
 
@@ -1820,7 +1846,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
                 f[10] = minSupport;
                 f[11] = float(maxcov) / p.estCov;
                 f[12] = float(mincov) / p.estCov;
-                f[13] = p.useQ ? tile.qw[p.qual[p.qpitch * a + i]] : 1.0f;
+                f[13] = p.useQ ? tile.qw[p.qual[p.qdpitch * a + i]] : 1.0f;
                 f[14] = float(i) / float(La);
                 const float left = i > 0 ? sSup[i - 1] : 1.0f;
                 const float right = i + 1 < La ? sSup[i + 1] : 1.0f;
@@ -2230,6 +2256,7 @@ struct care_ctx {
     int maxLen = 0;
     size_t pitch = 0;
     size_t qpitch = 0;
+    size_t qdpitch = 0;     // device quality row pitch: qpitch rounded up to 16 bytes
     int W = 0;
     int k = 0;
     int M = 0;
@@ -2372,7 +2399,7 @@ int buildTables(care_ctx* ctx){
 size_t msaLdsBytes(int maxCols, int maxLen, int W, int tile, bool useQ){
     size_t b = size_t(maxCols) * (16 + 16 + 4 + 4 + 4 + 1);
     b = (b + 15) / 16 * 16 + (size_t(maxLen) + 15) / 16 * 16;
-    b += size_t(tile) * (20 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0));
+    b += size_t(tile) * (20 + 8 * size_t(W) + (useQ ? size_t(qStride(maxLen)) : 0));
     return b + 16;
 }
 
@@ -2389,7 +2416,7 @@ int waveMsaColumnsPerLane(int maxCols){
 
 // dynamic LDS of k_msa_wave: candidate tile + anchor-column supports
 size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
-    size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * maxLen : 0);
+    size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * qStride(maxLen) : 0);
     b = std::max(b, kSeamLdsBytes);      // the seam reuses the tile region
     return (b + 15) / 16 * 16 + size_t(maxLen) * 4 + 16;
 }
@@ -2397,10 +2424,11 @@ size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
 // Candidates per LDS tile: as many as fit next to the column arrays in ~40 KB, 16..64.
 int msaTile(int maxCols, int maxLen, int W, bool useQ){
     const size_t base = msaLdsBytes(maxCols, maxLen, W, 0, useQ);
-    const size_t per = 20 + 8 * size_t(W) + (useQ ? size_t(maxLen) : 0);
+    const size_t per = 20 + 8 * size_t(W) + (useQ ? size_t(qStride(maxLen)) : 0);
     const size_t budget = 40 * 1024;
     size_t t = base < budget ? (budget - base) / per : 16;
-    return int(std::min<size_t>(64, std::max<size_t>(16, t)));
+    // a multiple of 4 keeps the quality rows 16-byte aligned
+    return int(std::min<size_t>(64, std::max<size_t>(16, t))) & ~3;
 }
 
 // Storage pointers and thresholds shared by the staged and the fused paths. The float
@@ -2408,6 +2436,7 @@ int msaTile(int maxCols, int maxLen, int W, bool useQ){
 void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.enc = ctx->enc.as<uint32_t>(); mp.pitch = ctx->pitch; mp.len = ctx->len.as<int>();
     mp.qual = ctx->qual.as<uint8_t>(); mp.qpitch = ctx->qpitch; mp.ambig = ctx->ambig.as<uint8_t>();
+    mp.qdpitch = ctx->qdpitch;
     mp.qweight = ctx->qw.as<float>();
     mp.isPaired = ctx->o.paired; mp.useQ = ctx->o.use_quality_scores;
     mp.maxErrorRate = ctx->o.max_error_rate;
@@ -2833,14 +2862,17 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
     HIPCHECK(hipMemcpyAsync(ctx->ambig.p, ambiguous, num_reads, hipMemcpyHostToDevice, st));
     if(ctx->o.use_quality_scores){
         ctx->qpitch = qual_pitch;
-        HIPCHECK(ctx->qual.ensure(num_reads * qual_pitch));
-        HIPCHECK(hipMemcpyAsync(ctx->qual.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
+        ctx->qdpitch = (qual_pitch + 15) / 16 * 16;
+        HIPCHECK(ctx->qual.ensure(num_reads * ctx->qdpitch));
+        HIPCHECK(hipMemsetAsync(ctx->qual.p, 0, num_reads * ctx->qdpitch, st));
+        HIPCHECK(hipMemcpy2DAsync(ctx->qual.p, ctx->qdpitch, quals, qual_pitch, qual_pitch, num_reads,
+                                  hipMemcpyHostToDevice, st));
     }
     float qw[256];
     for(int i = 0; i < 256; i++) qw[i] = qualityWeight(i);
     HIPCHECK(ctx->qw.ensure(sizeof(qw)));
     HIPCHECK(hipMemcpyAsync(ctx->qw.p, qw, sizeof(qw), hipMemcpyHostToDevice, st));
-    HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4));
+    HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4 + 64));
     const uint64_t hiloThreads = num_reads * uint64_t(ctx->W);
     k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), pitch_ints, ctx->len.as<int>(),
                                                                   num_reads, ctx->W, ctx->hilo.as<uint32_t>());
