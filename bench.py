@@ -14,6 +14,7 @@ Prints one JSON line (see the keys below). Options:
   --no-quality    correct without quality scores
   --cpu-sample S  anchors for the CPU baseline leg (default 300000; 0 disables)
   --cpu-threads T threads for the CPU baseline (default min(16, cores))
+  --no-learned    skip the learned leg (the classifier seam with the shipped classifier)
 """
 from __future__ import annotations
 
@@ -28,42 +29,9 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from rabbitqcplus_amd import engine, reads, sim  # noqa: E402
+from rabbitqcplus_amd import engine, evaluate, learned, reads, sim  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-
-
-def decode_all(rs) -> np.ndarray:
-    shifts = (30 - 2 * np.arange(16, dtype=np.uint32)).astype(np.uint32)
-    codes = ((rs.enc[:, :, None] >> shifts[None, None, :]) & 3).reshape(rs.n, -1).astype(np.uint8)
-    return np.frombuffer(b"ACGT", np.uint8)[codes]
-
-
-def error_rates(s, rs, data: bytes):
-    seqs = decode_all(rs)
-    L = seqs.shape[1]
-    maxlen = s.truth.shape[1]
-    w = min(L, maxlen)
-    valid = np.arange(w)[None, :] < rs.lengths[:, None]
-    before = int(((seqs[:, :w] != s.truth[:, :w]) & valid).sum())
-    p = 0
-    mv = memoryview(data)
-    while p < len(data):
-        rid = int.from_bytes(mv[p:p + 4], "little")
-        flags = int.from_bytes(mv[p + 4:p + 8], "little")
-        nb = flags & ((1 << 29) - 1)
-        pl = mv[p + 8:p + 8 + nb]
-        if (flags >> 30) & 1:
-            ne = int.from_bytes(pl[0:4], "little")
-            pos = np.frombuffer(pl[4:4 + 4 * ne], np.int32)
-            seqs[rid, pos] = np.frombuffer(pl[4 + 4 * ne:4 + 5 * ne], np.uint8)
-        else:
-            ln = int.from_bytes(pl[0:4], "little")
-            seqs[rid, :ln] = np.frombuffer(pl[4:4 + ln], np.uint8)
-        p += 8 + nb
-    after = int(((seqs[:, :w] != s.truth[:, :w]) & valid).sum())
-    bases = int(rs.lengths.sum())
-    return before / bases, after / bases
 
 
 def algorithmic_bytes(t: dict, rs, W: int, M: int):
@@ -74,9 +42,10 @@ def algorithmic_bytes(t: dict, rs, W: int, M: int):
     # k_candidates: length+ambiguity flag, M signatures (8 B) and M table slots (16 B) per
     # anchor, every gathered bucket value (4 B), unique candidate ids written (4 B)
     cand = A * (4 + 1 + 24 * M) + 4 * G + 4 * C
-    # k_align per (anchor, candidate): ids (8 B), both lengths (8 B), anchor fwd HiLo planes
-    # (8W B), candidate fwd+rc planes (16W B), chosen alignment written (16 B)
-    align = C * (32 + 24 * W)
+    # k_align (lane per candidate): per candidate its id and length (8 B), fwd+rc HiLo planes
+    # (16W B) and the chosen alignment written (16 B); per anchor its fwd planes (8W B), its
+    # length and candidate offsets (12 B)
+    align = C * (24 + 16 * W) + A * (8 * W + 12)
     # k_msa per candidate: chosen alignment + id (20 B), its 2-bit row once (pitch) and
     # qualities once when enabled; per anchor: its row, offsets, record slot bytes
     qbytes = rs.quals.shape[1] if rs.quals is not None else 0
@@ -154,6 +123,32 @@ def cpu_baseline(rs, opts, sample, threads):
                       f"{secs:.2f} s wall on {threads} threads"}
 
 
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the profiler byte counters
+    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a plain
+    copy of known size), as recorded in profiles/traffic.json; None when not recorded."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        rec = json.load(fh).get(workload, {}).get(kernel)
+    return None if rec is None else int(rec["bytes_per_launch"])
+
+
+def timed_passes(eng, steps, warmup, log, tag):
+    for w in range(warmup):
+        eng.correct()
+        log(f"{tag} warmup {w + 1}/{warmup} done")
+    step_ms = []
+    nrec, data = 0, b""
+    for k in range(steps):
+        ts = time.perf_counter()
+        nrec, data = eng.correct()
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        log(f"{tag} step {k + 1}/{steps}: {step_ms[-1]:.1f} ms")
+    return float(np.mean(step_ms)), nrec, data, eng.timings()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -166,12 +161,14 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batch-size", type=int, default=0)
     ap.add_argument("--engine", choices=["auto", "staged", "fused"], default="auto",
-                    help="auto = fused persistent kernel where the reads fit it, else staged kernels")
+                    help="auto = staged kernels; fused = one persistent kernel")
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="run one device-to-device copy of this size first (profiler byte-counter calibration)")
+                    help="also run one device-to-device copy of this size (profiler byte-counter calibration)")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+
     if args.cpu_baseline_only:
         s = sim.workload(args.workload, seed=args.seed)
         use_q = not args.no_quality
@@ -210,20 +207,8 @@ def main():
     maps = eng.build_tables()
     t3 = time.time()
     log(f"reads to device {1e3 * (t2 - t1):.1f} ms, tables ({maps} maps) {1e3 * (t3 - t2):.1f} ms")
-
-    for w in range(args.warmup):
-        eng.correct()
-        log(f"warmup {w + 1}/{args.warmup} done")
-    step_ms = []
-    data = b""
-    nrec = 0
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        nrec, data = eng.correct()
-        step_ms.append(1e3 * (time.perf_counter() - ts))
-        log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
-    t = eng.timings()
-    ms_per_step = float(np.mean(step_ms))
+    ms_per_step, nrec, data, t = timed_passes(eng, args.steps, args.warmup, log, "plain")
+    eng.close()
     value = rs.n / (ms_per_step / 1e3)
 
     W = max(1, (rs.max_length + 31) // 32)
@@ -237,13 +222,30 @@ def main():
     dominant = max(kernel_ms, key=kernel_ms.get)
     achieved = per_launch[dominant] / (kernel_ms[dominant] / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dominant,
-                "avg_launch_ms": round(kernel_ms[dominant], 4), "algorithmic_bytes_per_launch": int(per_launch[dominant]),
-                "launches_per_step": int(batches)}
-    before, after = error_rates(s, rs, data)
-    eng.close()
-    log(f"GPU: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
-        f"stages {json.dumps({k: round(t[k], 2) for k in ('candidates_ms', 'align_ms', 'msa_ms', 'output_ms')})}")
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant),
+                "kernel": dominant, "avg_launch_ms": round(kernel_ms[dominant], 4),
+                "algorithmic_bytes_per_launch": int(per_launch[dominant]), "launches_per_step": int(batches)}
+    plain_err = evaluate.error_counts(s, rs, data)
+    log(f"GPU plain: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
+        f"errors {plain_err}")
+
+    learned_line = None
+    if not args.no_learned and os.path.exists(learned.DEFAULT_CLASSIFIER) and rs.max_length <= 340:
+        params, _ = learned.load_classifier()
+        eng = engine.CareEngine(0, correction_type=1, **opts)
+        eng.set_reads(rs)
+        eng.build_tables()
+        eng.set_classifier(params)
+        lms, lrec, ldata, lt = timed_passes(eng, args.steps, args.warmup, log, "learned")
+        eng.close()
+        lerr = evaluate.error_counts(s, rs, ldata)
+        lvalue = rs.n / (lms / 1e3)
+        learned_line = {"value": round(lvalue, 1), "unit": "reads/s", "ms_per_step": round(lms, 3),
+                        "vs_plain": round(lvalue / value, 4), "records": lrec,
+                        "error_rate_after": lerr["after"] / lerr["bases"], "errors_after": lerr["after"],
+                        "fixed": lerr["fixed"], "miscorrected": lerr["miscorrected"],
+                        "classifier": os.path.relpath(learned.DEFAULT_CLASSIFIER, ROOT)}
+        log(f"GPU learned: {lvalue:.0f} reads/s, {lms:.1f} ms/pass; errors {lerr}")
 
     cpu = None
     if args.cpu_sample > 0:
@@ -251,7 +253,7 @@ def main():
         cpu = cpu_baseline_child(args)
 
     line = {
-        "metric": "corrected reads/s on 1 MI355X (plain path); post-correction error rate",
+        "metric": "corrected reads/s on 1 MI355X, plain vs learned; post-correction error rate",
         "value": round(value, 1),
         "unit": "reads/s",
         "n_gpus": 1,
@@ -269,8 +271,10 @@ def main():
                    "path": "plain", "engine": "fused" if "k_fused" in kernel_ms else "staged"},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "error_rate_before": round(before, 6),
-        "error_rate_after": round(after, 6),
+        "error_rate_before": plain_err["before"] / plain_err["bases"],
+        "error_rate_after": plain_err["after"] / plain_err["bases"],
+        "errors": plain_err,
+        "learned": learned_line,
         "records": nrec,
         "table_build_ms": round(1e3 * (t3 - t2), 2),
         "stage_ms": {"candidates": round(t["candidates_ms"], 3), "align": round(t["align_ms"], 3),

@@ -245,6 +245,16 @@ k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ l
     out[3 * W + w] = rl;
 }
 
+// Rows of `sp` bytes to rows of `dp` >= sp bytes, zero padded.
+__global__ void k_repack_rows(const uint8_t* __restrict__ src, size_t sp, uint8_t* __restrict__ dst, size_t dp,
+                              uint64_t total){
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(t >= total) return;
+    const uint64_t r = t / dp;
+    const uint64_t c = t - r * dp;
+    dst[t] = c < sp ? src[r * sp + c] : uint8_t(0);
+}
+
 __global__ void k_iota(uint32_t* v, uint64_t n){
     const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(i < n) v[i] = uint32_t(i);
@@ -297,65 +307,100 @@ __device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint
     }
 }
 
+// Inclusive prefix sum over the wave (DPP row shifts, then row broadcasts).
+__device__ __forceinline__ int waveInclusiveSum(int v){
+    int t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); v += t;
+    return v;
+}
+
+// Dynamic LDS of the candidate gather: hash set (hashSlots entries) then the table that
+// maps each gathered position to its map (one byte per value, capPerAnchor entries).
+__host__ __device__ inline size_t candLdsBytes(int hashSlots, int capPerAnchor){
+    return size_t(hashSlots) * 4 + (size_t(capPerAnchor) + 15) / 16 * 16;
+}
+
+// Inserts v into the LDS hash set of H slots (power of two, more slots than distinct
+// values); the first inserter appends v to out.
+__device__ __forceinline__ void hashInsert(uint32_t* hset, int H, int hbits, uint32_t v, uint32_t* out, int* sUnique){
+    uint32_t h = (v * 2654435761u) >> (32 - hbits);
+    for(;;){
+        const uint32_t prev = atomicCAS(&hset[h], 0xFFFFFFFFu, v);
+        if(prev == 0xFFFFFFFFu){ out[atomicAdd(sUnique, 1)] = v; return; }
+        if(prev == v) return;
+        h = (h + 1) & uint32_t(H - 1);
+    }
+}
+
+// Gathers the T bucket values of the anchor (positions [sPre[m], sPre[m+1]) belong to
+// map m, read from values[m][sOff[m] + ...]) into the hash set: loads in batches of
+// kGatherBatch per thread, all issued before their inserts.
+constexpr int kGatherBatch = 8;
+__device__ __forceinline__ void gatherInsert(const CandParams& p, uint64_t a, int T, const uint8_t* mapOf,
+                                             const uint32_t* sOff, const uint32_t* sPre, uint32_t* hset, int H,
+                                             uint32_t* out, int* sUnique, int tid, int nthreads){
+    const int hbits = 31 - __clz(H);
+    for(int base = 0; base < T; base += nthreads * kGatherBatch){
+        uint32_t v[kGatherBatch];
+        #pragma unroll
+        for(int u = 0; u < kGatherBatch; u++){
+            const int idx = base + u * nthreads + tid;
+            v[u] = 0xFFFFFFFFu;
+            if(idx < T){
+                const int m = mapOf[idx];
+                v[u] = p.values[uint64_t(m) * p.n + sOff[m] + uint32_t(idx - int(sPre[m]))];
+            }
+        }
+        #pragma unroll
+        for(int u = 0; u < kGatherBatch; u++){
+            const uint32_t x = v[u];
+            if(x == 0xFFFFFFFFu || uint64_t(x) == a || (p.excludeAmbiguous && p.ambig[x])) continue;
+            hashInsert(hset, H, hbits, x, out, sUnique);
+        }
+    }
+}
+
 // Workgroup per anchor: candidate read ids, ascending, unique, self removed
 // (include/corrector.hpp:571-650; include/ordinaryminhasher.hpp:135-221,437-449).
-// The gathered bucket values are deduplicated on insertion into an LDS hash set (the
-// first inserter of a value appends it to the anchor's output area), then only the
-// unique ids are sorted (bitonic, in LDS).
+// Wave 0 looks the 48 signatures up; the gathered bucket values are deduplicated on
+// insertion into an LDS hash set (the first inserter of a value appends it to the
+// anchor's output area), then only the unique ids are sorted (bitonic, in LDS).
 __global__ void __launch_bounds__(kBlock)
 k_candidates(CandParams p){
-    extern __shared__ uint32_t hset[];       // hashSlots entries; reused for the sort
+    extern __shared__ uint32_t hset[];       // hashSlots entries (reused for the sort), then mapOf
+    uint8_t* mapOf = reinterpret_cast<uint8_t*>(hset + p.hashSlots);
     __shared__ uint32_t sOff[kMaxMaps];
     __shared__ uint32_t sPre[kMaxMaps + 1];
     __shared__ int sUnique;
     const int i = blockIdx.x;
     const uint64_t a = p.b0 + i;
     const int tid = threadIdx.x;
-    const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
-    if(tid < kMaxMaps){
+    if(tid < kWave){
+        const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
         uint32_t cnt = 0, off = 0;
         if(valid && tid < p.M){
             const uint64_t key = p.sig[uint64_t(tid) * p.n + a];
             cnt = lookup(p.slots[tid], p.caps[tid], key, &off);
             if(cnt > uint32_t(p.thr)) cnt = 0;
         }
-        sOff[tid] = off;
-        sPre[tid + 1] = cnt;
-    }
-    if(tid == 0) sUnique = 0;
-    __syncthreads();
-    if(tid == 0){
-        sPre[0] = 0;
-        for(int m = 0; m < kMaxMaps; m++) sPre[m + 1] += sPre[m];
+        const int incl = waveInclusiveSum(int(cnt));
+        if(tid < kMaxMaps){ sOff[tid] = off; sPre[tid + 1] = uint32_t(incl); }
+        if(tid == 0){ sPre[0] = 0; sUnique = 0; }
+        for(int q = incl - int(cnt); q < incl; q++) mapOf[q] = uint8_t(tid);
     }
     __syncthreads();
     const int T = int(sPre[kMaxMaps]);
-    // table of at least 2T slots, strictly more than T (hashSlots > maps*threshold >= T)
     int H = 64;
-    while(H < 2 * T && H < p.hashSlots) H <<= 1;
+    while(H <= T) H <<= 1;                   // more slots than values; H <= hashSlots
     for(int q = tid; q < H; q += kBlock) hset[q] = 0xFFFFFFFFu;
     __syncthreads();
     uint32_t* out = p.raw + uint64_t(i) * p.capPerAnchor;
-    const int hbits = 31 - __clz(H);
-    for(int idx = tid; idx < T; idx += kBlock){
-        int lo = 0, hi = kMaxMaps - 1;       // map m with sPre[m] <= idx < sPre[m+1]
-        while(lo < hi){
-            const int mid = (lo + hi + 1) >> 1;
-            if(int(sPre[mid]) <= idx) lo = mid; else hi = mid - 1;
-        }
-        const uint32_t v = p.values[uint64_t(lo) * p.n + sOff[lo] + (idx - sPre[lo])];
-        if(uint64_t(v) == a || (p.excludeAmbiguous && p.ambig[v])) continue;
-        uint32_t h = (v * 2654435761u) >> (32 - hbits);
-        for(;;){
-            const uint32_t prev = atomicCAS(&hset[h], 0xFFFFFFFFu, v);
-            if(prev == 0xFFFFFFFFu){
-                out[atomicAdd(&sUnique, 1)] = v;
-                break;
-            }
-            if(prev == v) break;
-            h = (h + 1) & uint32_t(H - 1);
-        }
-    }
+    gatherInsert(p, a, T, mapOf, sOff, sPre, hset, H, out, &sUnique, tid, kBlock);
     __syncthreads();
     const int U = sUnique;
     int P = 1;
@@ -1493,10 +1538,15 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
                     const int wd = j * W + (pos >> 5);
                     const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
                     const float x = wj * (p.useQ ? t.qw[t.q[j * qStride(p.maxLen) + pos]] : 1.0f);
-                    if(b == 0){ m.w[k][0] += x; m.c01[k] += 1u; }
-                    else if(b == 1){ m.w[k][1] += x; m.c01[k] += 0x10000u; }
-                    else if(b == 2){ m.w[k][2] += x; m.c23[k] += 1u; }
-                    else { m.w[k][3] += x; m.c23[k] += 0x10000u; }
+                    // selects instead of branches: the other weights add +0.0f, which leaves
+                    // them bit-identical (weights start at +0.0f and only grow)
+                    m.w[k][0] += b == 0 ? x : 0.0f;
+                    m.w[k][1] += b == 1 ? x : 0.0f;
+                    m.w[k][2] += b == 2 ? x : 0.0f;
+                    m.w[k][3] += b == 3 ? x : 0.0f;
+                    const uint32_t inc = 1u << (16 * (b & 1));
+                    m.c01[k] += b < 2 ? inc : 0u;
+                    m.c23[k] += b < 2 ? 0u : inc;
                 }
             }
         }
@@ -1962,18 +2012,6 @@ k_msa_wave(MsaParams p){
 
 // ---------------------------------------------------------------- fused persistent kernel
 
-// Inclusive prefix sum over the wave (DPP row shifts, then row broadcasts).
-__device__ __forceinline__ int waveInclusiveSum(int v){
-    int t;
-    t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); v += t;
-    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); v += t;
-    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); v += t;
-    t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); v += t;
-    t = __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false); v += t;
-    t = __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false); v += t;
-    return v;
-}
-
 struct FusedParams {
     MsaParams m;             // storage, thresholds and record slots (per-wave lists set in the kernel)
     CandParams c;            // tables
@@ -2032,28 +2070,14 @@ __device__ __forceinline__ int candidatesWave(const CandParams& p, uint64_t a, u
     if(lane == 0){ sPre[0] = 0; *sUnique = 0; }
     const int T = __builtin_amdgcn_readlane(incl, 63);
     if(lane == 0 && T) atomicAdd(p.gathered, (unsigned long long)T);
+    uint8_t* mapOf = reinterpret_cast<uint8_t*>(hset + p.hashSlots);
+    for(int q = incl - int(cnt); q < incl; q++) mapOf[q] = uint8_t(lane);
     int H = 64;
-    while(H < 2 * T && H < p.hashSlots) H <<= 1;
+    while(H <= T) H <<= 1;                   // more slots than values; H <= hashSlots
     __syncthreads();
     for(int q = lane; q < H; q += kWave) hset[q] = 0xFFFFFFFFu;
     __syncthreads();
-    const int hbits = 31 - __clz(H);
-    for(int idx = lane; idx < T; idx += kWave){
-        int lo = 0, hi = kMaxMaps - 1;
-        while(lo < hi){
-            const int mid = (lo + hi + 1) >> 1;
-            if(int(sPre[mid]) <= idx) lo = mid; else hi = mid - 1;
-        }
-        const uint32_t v = p.values[uint64_t(lo) * p.n + sOff[lo] + (idx - sPre[lo])];
-        if(uint64_t(v) == a || (p.excludeAmbiguous && p.ambig[v])) continue;
-        uint32_t h = (v * 2654435761u) >> (32 - hbits);
-        for(;;){
-            const uint32_t prev = atomicCAS(&hset[h], 0xFFFFFFFFu, v);
-            if(prev == 0xFFFFFFFFu){ out[atomicAdd(sUnique, 1)] = v; break; }
-            if(prev == v) break;
-            h = (h + 1) & uint32_t(H - 1);
-        }
-    }
+    gatherInsert(p, a, T, mapOf, sOff, sPre, hset, H, out, sUnique, lane, kWave);
     __syncthreads();
     const int U = *sUnique;
     int P = 1;
@@ -2559,7 +2583,7 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     fp.first = first;
     fp.units = uint32_t(units);
     fp.scratch = ctx->scratch.as<uint8_t>();
-    const size_t lds = std::max({size_t(fp.c.hashSlots) * 4, size_t(10) * (ctx->W + 1) * 8,
+    const size_t lds = std::max({candLdsBytes(fp.c.hashSlots, fp.c.capPerAnchor), size_t(10) * (ctx->W + 1) * 8,
                                  waveMsaLdsBytes(fp.m.tile, ctx->W, ctx->maxLen, q)});
     // recBytes[B] must read as zero for the exclusive scan; the kernel writes [0, B)
     k_zero_u32<<<1, 64, 0, st>>>(ctx->recBytes.as<uint32_t>() + B, 1);
@@ -2613,7 +2637,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
     cp.b0 = b0; cp.B = B; cp.capPerAnchor = capPerAnchor; cp.hashSlots = P; cp.raw = ctx->raw.as<uint32_t>(); cp.uniq = ctx->uniq.as<uint32_t>();
     cp.gathered = ctx->counters.as<unsigned long long>();
-    const size_t candLds = size_t(P) * sizeof(uint32_t);
+    const size_t candLds = candLdsBytes(P, capPerAnchor);
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
     k_candidates<<<B, kBlock, candLds, st>>>(cp);
     HIPCHECK(hipGetLastError());
@@ -2864,9 +2888,20 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
         ctx->qpitch = qual_pitch;
         ctx->qdpitch = (qual_pitch + 15) / 16 * 16;
         HIPCHECK(ctx->qual.ensure(num_reads * ctx->qdpitch));
-        HIPCHECK(hipMemsetAsync(ctx->qual.p, 0, num_reads * ctx->qdpitch, st));
-        HIPCHECK(hipMemcpy2DAsync(ctx->qual.p, ctx->qdpitch, quals, qual_pitch, qual_pitch, num_reads,
-                                  hipMemcpyHostToDevice, st));
+        if(ctx->qdpitch == qual_pitch){
+            HIPCHECK(hipMemcpyAsync(ctx->qual.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
+        }else{
+            // one linear copy, then rows repacked to the 16-byte pitch on the device
+            struct Staging { DBuf b; ~Staging(){ b.release(); } } hold;
+            DBuf& staging = hold.b;
+            HIPCHECK(staging.ensure(num_reads * qual_pitch));
+            HIPCHECK(hipMemcpyAsync(staging.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
+            const uint64_t total = num_reads * ctx->qdpitch;
+            k_repack_rows<<<dim3(uint32_t((total + 255) / 256)), 256, 0, st>>>(staging.as<uint8_t>(), qual_pitch,
+                                                                              ctx->qual.as<uint8_t>(), ctx->qdpitch, total);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipStreamSynchronize(st));
+        }
     }
     float qw[256];
     for(int i = 0; i < 256; i++) qw[i] = qualityWeight(i);
