@@ -22,6 +22,8 @@
 #include <cpureadstorage.hpp>
 #include <cpuminhasher.hpp>
 #include <cpuminhasherconstruction.hpp>
+#include <ordinaryminhasher.hpp>
+#include <minhasherlimit.hpp>
 #include <correct_cpu.hpp>
 #include <corrector.hpp>
 #include <cpusequencehasher.hpp>
@@ -38,6 +40,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <stdexcept>
 #include <vector>
 
 #include <omp.h>
@@ -104,6 +107,46 @@ struct ArrayReadStorage : public care::CpuReadStorage {
     bool isPairedEnd() const override { return paired; }
     int getQualityBits() const override { return 8; }
 };
+
+// The steps of constructCpuMinhasherFromReadStorage (src/cpuminhasherconstruction.cpp:25-164,
+// minhasher created as at :176-183) with the minhasher's own insert/compact code, but with
+// no thread pool: the reference's loops then run sequentially (ForLoopExecutor,
+// include/threadpool.hpp:389-420). Its pool can miss a wake-up and stall
+// (include/parallel/parallel_task_queue.h:306-313); the tables are the same either way.
+std::unique_ptr<care::CpuMinhasher> buildMinhasherSequential(const care::ProgramOptions& po, const ArrayReadStorage& rs){
+    auto mh = std::make_unique<care::OrdinaryCpuMinhasher>(rs.getNumberOfReads(),
+                                                           calculateResultsPerMapThreshold(po.estimatedCoverage),
+                                                           po.kmerlength, po.hashtableLoadfactor);
+    const care::read_number numReads = rs.getNumberOfReads();
+    const std::size_t pitch = SequenceHelpers::getEncodedNumInts2Bit(rs.getSequenceLengthUpperBound());
+    mh->setHostMemoryLimitForConstruction(po.memoryForHashtables);
+    mh->setDeviceMemoryLimitsForConstruction({0});
+    mh->setThreadPool(nullptr);
+    int remaining = po.numHashFunctions;
+    constexpr care::read_number batchsize = 1000000;
+    std::vector<care::read_number> ids(batchsize);
+    std::vector<unsigned int> seq(batchsize * pitch);
+    std::vector<int> lens(batchsize);
+    while(remaining > 0){
+        const int existing = po.numHashFunctions - remaining;
+        std::vector<int> fnums(remaining);
+        std::iota(fnums.begin(), fnums.end(), existing);
+        const int added = mh->addHashTables(remaining, fnums.data());
+        if(added == 0) break;
+        for(care::read_number b0 = 0; b0 < numReads; b0 += batchsize){
+            const care::read_number cnt = std::min<care::read_number>(batchsize, numReads - b0);
+            std::iota(ids.begin(), ids.begin() + cnt, b0);
+            rs.gatherSequences(seq.data(), pitch, ids.data(), cnt);
+            rs.gatherSequenceLengths(lens.data(), ids.data(), cnt);
+            mh->insert(seq.data(), cnt, lens.data(), pitch, ids.data(), existing, added, fnums.data());
+            if(mh->checkInsertionErrors(existing, added) > 0) throw std::runtime_error("table construction error");
+        }
+        mh->compact();
+        remaining -= added;
+    }
+    mh->constructionIsFinished();
+    return mh;
+}
 
 }  // namespace
 
@@ -187,8 +230,7 @@ ref_ctx* ref_create(const ref_opts* o, std::size_t n, const unsigned int* enc, s
         }
     }
     fill_program_options(c->po, o, c->rs.maxLen);
-    auto pair = care::constructCpuMinhasherFromCpuReadStorage(c->po, c->rs, care::CpuMinhasherType::Ordinary);
-    c->mh = std::move(pair.first);
+    c->mh = buildMinhasherSequential(c->po, c->rs);
     return c;
 }
 
