@@ -78,12 +78,8 @@ def calibration_copy(nbytes: int):
 
 
 def cpu_baseline_child(args):
-    """CPU-baseline leg in a child process (regenerates the same seeded workload).
-
-    The reference's table construction runs on its own thread pool, whose scheduler can
-    miss a wake-up and stall (include/parallel/parallel_task_queue.h:306-313 waits on a
-    condition variable that notify_task_complete() signals without holding its mutex), so
-    the leg runs under a time limit and is retried once."""
+    """CPU-baseline leg in a child process (regenerates the same seeded workload), under
+    a time limit and retried once, so that a stalled CPU leg cannot hold the GPU line."""
     import subprocess
     cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", args.workload,
            "--seed", str(args.seed), "--cpu-sample", str(args.cpu_sample), "--cpu-threads", str(args.cpu_threads)]
