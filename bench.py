@@ -15,6 +15,7 @@ Prints one JSON line (see the keys below). Options:
   --cpu-sample S  anchors for the CPU baseline leg (default 300000; 0 disables)
   --cpu-threads T threads for the CPU baseline (default min(16, cores))
   --no-learned    skip the learned leg (the classifier seam with the shipped classifier)
+  --no-autotune   skip the autotuner (which times the schedules on 65,536 anchors first)
 """
 from __future__ import annotations
 
@@ -29,7 +30,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from rabbitqcplus_amd import engine, evaluate, learned, reads, sim  # noqa: E402
+from rabbitqcplus_amd import autotune, engine, evaluate, learned, reads, sim  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 
@@ -160,6 +161,8 @@ def main():
                     help="auto = staged kernels; fused = one persistent kernel")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --engine/--waves-per-cu as given instead of the autotuner's choice")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (profiler byte-counter calibration)")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
@@ -192,6 +195,12 @@ def main():
     if rs.max_length >= 160:
         opts["kmerlength"] = 20
     log(f"workload {args.workload}: {rs.n} reads, max length {rs.max_length}, simulated in {time.time() - t0:.1f} s")
+    tuned = None
+    if not args.no_autotune and args.engine == "auto":
+        base = {k: v for k, v in opts.items() if k not in ("engine", "waves_per_cu")}
+        tuned = autotune.tune(rs, log=log, cache=None, **base)
+        opts.update(tuned["choice"])
+        log(f"autotune choice {tuned['choice']}")
 
     eng = engine.CareEngine(0, **opts)
     if args.calib_copy_mb > 0:
@@ -264,7 +273,9 @@ def main():
         "config": {"workload": args.workload, "reads": rs.n, "read_length": [int(rs.lengths.min()), rs.max_length],
                    "coverage": 30, "k": int(eng.opts.kmerlength) or (20 if rs.max_length < 160 else 32),
                    "maps": int(eng.opts.num_hash_maps), "quality_scores": use_q, "paired": bool(rs.paired),
-                   "path": "plain", "engine": "fused" if "k_fused" in kernel_ms else "staged"},
+                   "path": "plain", "engine": "fused" if "k_fused" in kernel_ms else "staged",
+                   "schedule": {k: opts[k] for k in ("engine", "waves_per_cu", "align_variant", "use_graphs") if k in opts}},
+        "autotune": tuned["timings_ms"] if tuned else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "error_rate_before": plain_err["before"] / plain_err["bases"],

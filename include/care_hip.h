@@ -64,6 +64,8 @@ typedef struct care_opts {
                                        classic column rule where its confidence is below 0.9 */
     int align_variant;              /* staged engine: 0 = one lane per candidate where reads have at
                                        most 352 bases (else 1), 1 = one wavefront per pair */
+    int use_graphs;                 /* fused engine: 1 = capture a pass's device work into a HIP graph
+                                       and replay it while the pass's arguments and buffers hold */
 } care_opts;
 
 typedef struct care_ctx care_ctx;
@@ -85,9 +87,13 @@ typedef struct care_timings {
     uint64_t record_bytes;
     uint64_t bucket_values;   /* values gathered from the tables */
     uint64_t batches;
+    uint64_t graphs_built;    /* HIP graphs captured and instantiated by this context */
 } care_timings;
 
 void care_hip_default_opts(care_opts* o);
+
+/* Device name and architecture ("<name> <gcnArch>") and compute-unit count. */
+int care_hip_device_info(int device, char* name, size_t cap, int* compute_units);
 
 int care_hip_create(const care_opts* opts, int device, care_ctx** out);
 void care_hip_destroy(care_ctx* ctx);

@@ -2299,6 +2299,12 @@ struct care_ctx {
     DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp, scratch;
     // learned seam: classifier (bfloat16 weights then float biases) and feature output
     DBuf seam, featBuf, metaBuf;
+    // pinned host words: record bytes (u64) and the error flag (i32) of the last pass
+    void* pinned = nullptr;
+    // fused engine with use_graphs: the instantiated pass and the parameter bytes it was captured with
+    hipGraphExec_t fusedExec = nullptr;
+    std::vector<uint8_t> fusedKey;
+    uint64_t graphsBuilt = 0;
     bool hasSeam = false;
     bool featActive = false;
     unsigned long long featCap = 0;
@@ -2509,21 +2515,38 @@ int recordSlotBytes(int maxLen){
     return ((8 + 4 + std::max(maxLen, 5 * (maxLen / 7 + 1))) + 15) / 16 * 16;
 }
 
-// Compacts the per-anchor record slots [0, B) into one buffer and appends it to `out`.
-int gatherRecords(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out, size_t& outRecords){
+// Record compaction, device part: record sizes widened and scanned into byte offsets,
+// then the total and the error flag copied into the context's pinned host words (no host
+// synchronization; capturable). Uses the preallocated scan workspace only.
+int recordsDevicePart(care_ctx* ctx, int B){
     hipStream_t st = ctx->stream;
     k_widen<<<(B + 1 + 255) / 256, 256, 0, st>>>(ctx->recBytes.as<uint32_t>(), ctx->recOff64.as<uint64_t>(), B + 1);
     HIPCHECK(hipGetLastError());
-    size_t tb = 0;
-    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
-    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
-    tb = ctx->scanTmp.bytes;
+    size_t tb = ctx->scanTmp.bytes;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
+    HIPCHECK(hipMemcpyAsync(ctx->pinned, ctx->recOff.as<uint64_t>() + B, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(reinterpret_cast<uint8_t*>(ctx->pinned) + 8, ctx->counters.as<int>() + 2, 4,
+                            hipMemcpyDeviceToHost, st));
+    return CARE_OK;
+}
+
+// Scan workspace for B + 1 record sizes (allocated before any capture).
+int ensureRecordScan(care_ctx* ctx, int B){
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1,
+                                              ctx->stream));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    return CARE_OK;
+}
+
+// Record compaction, host part (after recordsDevicePart and a stream synchronization):
+// checks the error flag, gathers the records into one buffer and appends it to `out`.
+int recordsFinish(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out, size_t& outRecords){
+    hipStream_t st = ctx->stream;
     uint64_t outBytes = 0;
     int errFlag = 0;
-    HIPCHECK(hipMemcpyAsync(&outBytes, ctx->recOff.as<uint64_t>() + B, 8, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipMemcpyAsync(&errFlag, ctx->counters.as<int>() + 2, 4, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    std::memcpy(&outBytes, ctx->pinned, 8);
+    std::memcpy(&errFlag, reinterpret_cast<uint8_t*>(ctx->pinned) + 8, 4);
     if(errFlag) return ctx->fail(CARE_EINVAL, "MSA column bound exceeded");
     HIPCHECK(ctx->outBuf.ensure(std::max<uint64_t>(outBytes, 1)));
     k_gather<<<B, kBlock, 0, st>>>(ctx->slots.as<uint8_t>(), slotBytes, ctx->recBytes.as<uint32_t>(),
@@ -2543,6 +2566,16 @@ int gatherRecords(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out
     return CARE_OK;
 }
 
+// Compacts the per-anchor record slots [0, B) into one buffer and appends it to `out`.
+int gatherRecords(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out, size_t& outRecords){
+    int rc = ensureRecordScan(ctx, B);
+    if(rc != CARE_OK) return rc;
+    rc = recordsDevicePart(ctx, B);
+    if(rc != CARE_OK) return rc;
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return recordsFinish(ctx, B, slotBytes, out, outRecords);
+}
+
 // Fused pass over anchors [first, first+n): one persistent kernel, then record compaction.
 int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
@@ -2554,6 +2587,7 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     const int B = int(n);
     const int slotBytes = recordSlotBytes(ctx->maxLen);
     FusedParams fp;
+    std::memset(&fp, 0, sizeof(fp));     // padding included: the bytes key the pass's graph
     fillMsaCommon(ctx, fp.m);
     fillCandCommon(ctx, fp.c);
     fillAlignCommon(ctx, fp.al);
@@ -2585,30 +2619,65 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     fp.scratch = ctx->scratch.as<uint8_t>();
     const size_t lds = std::max({candLdsBytes(fp.c.hashSlots, fp.c.capPerAnchor), size_t(10) * (ctx->W + 1) * 8,
                                  waveMsaLdsBytes(fp.m.tile, ctx->W, ctx->maxLen, q)});
-    // recBytes[B] must read as zero for the exclusive scan; the kernel writes [0, B)
-    k_zero_u32<<<1, 64, 0, st>>>(ctx->recBytes.as<uint32_t>() + B, 1);
-    k_zero_u32<<<1, 64, 0, st>>>(ctx->counters.as<uint32_t>() + 4, 1);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ctx->ev[0], st));
-    auto launch = [&](auto kern) -> hipError_t {
-        if(lds > 65536){
-            const hipError_t e = hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, int(lds));
-            if(e != hipSuccess) return e;
+    auto kernelFor = [&]() -> const void* {
+        switch(kc){
+            case 4: return paired ? (const void*)k_fused<4, true> : (const void*)k_fused<4, false>;
+            case 8: return paired ? (const void*)k_fused<8, true> : (const void*)k_fused<8, false>;
+            case 12: return paired ? (const void*)k_fused<12, true> : (const void*)k_fused<12, false>;
+            default: return paired ? (const void*)k_fused<16, true> : (const void*)k_fused<16, false>;
         }
-        kern<<<dim3(uint32_t(G)), kWave, lds, st>>>(fp);
-        return hipGetLastError();
     };
-    hipError_t le;
-    switch(kc){
-        case 4: le = paired ? launch(k_fused<4, true>) : launch(k_fused<4, false>); break;
-        case 8: le = paired ? launch(k_fused<8, true>) : launch(k_fused<8, false>); break;
-        case 12: le = paired ? launch(k_fused<12, true>) : launch(k_fused<12, false>); break;
-        default: le = paired ? launch(k_fused<16, true>) : launch(k_fused<16, false>); break;
+    if(lds > 65536) HIPCHECK(hipFuncSetAttribute(kernelFor(), hipFuncAttributeMaxDynamicSharedMemorySize, int(lds)));
+    {
+        const int rc = ensureRecordScan(ctx, B);
+        if(rc != CARE_OK) return rc;
     }
-    HIPCHECK(le);
-    HIPCHECK(hipGetLastError());
-    HIPCHECK(hipEventRecord(ctx->ev[1], st));
-    const int rc = gatherRecords(ctx, B, slotBytes, out, outRecords);
+    // the pass's device work: zero fills (kernels, so that a captured graph holds no
+    // memset node), the persistent kernel, record sizes scanned and the total copied to
+    // pinned host memory
+    auto enqueue = [&]() -> hipError_t {
+        // recBytes[B] must read as zero for the exclusive scan; the kernel writes [0, B)
+        k_zero_u32<<<1, 64, 0, st>>>(ctx->recBytes.as<uint32_t>() + B, 1);
+        k_zero_u32<<<1, 64, 0, st>>>(ctx->counters.as<uint32_t>() + 4, 1);
+        void* args[] = {&fp};
+        hipError_t e = hipLaunchKernel(kernelFor(), dim3(uint32_t(G)), dim3(kWave), args, lds, st);
+        if(e != hipSuccess) return e;
+        return recordsDevicePart(ctx, B) == CARE_OK ? hipSuccess : hipErrorUnknown;
+    };
+    if(ctx->o.use_graphs){
+        // one executable graph per (parameters, grid, buffers); replayed while they hold
+        std::vector<uint8_t> key(sizeof(fp) + 3 * sizeof(uint64_t));
+        const uint64_t extra[3] = {G, uint64_t(lds), uint64_t(B)};
+        std::memcpy(key.data(), &fp, sizeof(fp));
+        std::memcpy(key.data() + sizeof(fp), extra, sizeof(extra));
+        if(!ctx->fusedExec || key != ctx->fusedKey){
+            HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            const hipError_t e = enqueue();
+            hipGraph_t g = nullptr;
+            const hipError_t ee = hipStreamEndCapture(st, &g);
+            if(e != hipSuccess || ee != hipSuccess){
+                if(g) (void)hipGraphDestroy(g);
+                return ctx->fail(CARE_EHIP, "capturing the fused pass failed: %s",
+                                 hipGetErrorString(e != hipSuccess ? e : ee));
+            }
+            if(ctx->fusedExec){ (void)hipGraphExecDestroy(ctx->fusedExec); ctx->fusedExec = nullptr; }
+            const hipError_t ie = hipGraphInstantiate(&ctx->fusedExec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if(ie != hipSuccess){ ctx->fusedExec = nullptr; return ctx->fail(CARE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie)); }
+            ctx->fusedKey = key;
+            ctx->graphsBuilt += 1;
+        }
+        HIPCHECK(hipEventRecord(ctx->ev[0], st));
+        HIPCHECK(hipGraphLaunch(ctx->fusedExec, st));
+        HIPCHECK(hipEventRecord(ctx->ev[1], st));
+    }else{
+        HIPCHECK(hipEventRecord(ctx->ev[0], st));
+        const hipError_t e = enqueue();
+        if(e != hipSuccess) return ctx->fail(CARE_EHIP, "fused pass launch failed: %s", hipGetErrorString(e));
+        HIPCHECK(hipEventRecord(ctx->ev[1], st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    const int rc = recordsFinish(ctx, B, slotBytes, out, outRecords);
     if(rc != CARE_OK) return rc;
     float ms;
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
@@ -2813,6 +2882,7 @@ int care_hip_create(const care_opts* opts, int device, care_ctx** out){
         delete ctx;
         return CARE_EHIP;
     }
+    if(hipHostMalloc(&ctx->pinned, 64, hipHostMallocDefault) != hipSuccess){ care_hip_destroy(ctx); return CARE_ENOMEM; }
     for(auto& e : ctx->ev){
         if(hipEventCreate(&e) != hipSuccess){ delete ctx; return CARE_EHIP; }
     }
@@ -2828,13 +2898,28 @@ void care_hip_destroy(care_ctx* ctx){
                    &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
-                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp, &ctx->scratch}){
+                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp, &ctx->scratch,
+                   &ctx->seam, &ctx->featBuf, &ctx->metaBuf}){
         b->release();
     }
+    if(ctx->fusedExec) (void)hipGraphExecDestroy(ctx->fusedExec);
+    if(ctx->pinned) (void)hipHostFree(ctx->pinned);
     for(auto& b : ctx->slotBufs) b.release();
     for(auto& e : ctx->ev) if(e) (void)hipEventDestroy(e);
     if(ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+int care_hip_device_info(int device, char* name, size_t cap, int* compute_units){
+    int count = 0;
+    if(hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return CARE_ENODEV;
+    hipDeviceProp_t prop;
+    if(hipGetDeviceProperties(&prop, device) != hipSuccess) return CARE_EHIP;
+    if(name && cap){
+        std::snprintf(name, cap, "%s %s", prop.name, prop.gcnArchName);
+    }
+    if(compute_units) *compute_units = prop.multiProcessorCount;
+    return CARE_OK;
 }
 
 const char* care_hip_last_error(const care_ctx* ctx){
@@ -2987,6 +3072,7 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
     ctx->t.records = ctx->numRecords;
     ctx->t.record_bytes = ctx->records.size();
     ctx->t.bucket_values = gathered;
+    ctx->t.graphs_built = ctx->graphsBuilt;
     if(num_records) *num_records = ctx->numRecords;
     if(num_bytes) *num_bytes = ctx->records.size();
     return CARE_OK;

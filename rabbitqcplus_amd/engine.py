@@ -37,7 +37,8 @@ class CareOpts(ctypes.Structure):
                 ("m_coverage", ctypes.c_float), ("paired_filter_threshold", ctypes.c_float),
                 ("batch_size", ctypes.c_int), ("keep_intermediates", ctypes.c_int),
                 ("engine", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
-                ("correction_type", ctypes.c_int), ("align_variant", ctypes.c_int)]
+                ("correction_type", ctypes.c_int), ("align_variant", ctypes.c_int),
+                ("use_graphs", ctypes.c_int)]
 
 
 class CareTimings(ctypes.Structure):
@@ -48,14 +49,15 @@ class CareTimings(ctypes.Structure):
                 ("fused_kernel_ms", ctypes.c_double),
                 ("anchors", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("records", ctypes.c_uint64), ("record_bytes", ctypes.c_uint64),
-                ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64)]
+                ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("graphs_built", ctypes.c_uint64)]
 
 
 EXPORTS = [
     "care_hip_default_opts", "care_hip_create", "care_hip_destroy", "care_hip_last_error",
     "care_hip_set_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
-    "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features",
+    "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
     "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
     "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
 ]
@@ -88,6 +90,7 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_inspect_batch.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
                                            vp, vp, vp, vp, vp]
     lib.care_hip_inspect_msa.argtypes = [vp, u64, i32, ctypes.POINTER(i32), vp, vp, vp, vp, vp]
+    lib.care_hip_device_info.argtypes = [i32, ctypes.c_char_p, sz, ctypes.POINTER(i32)]
     lib.care_hip_set_classifier.argtypes = [vp, vp, sz]
     lib.care_hip_extract_features.argtypes = [vp, u64, u64, vp, vp, sz, ctypes.POINTER(sz)]
     lib.care_mlp_create.argtypes = [i32, ctypes.POINTER(vp)]
@@ -106,6 +109,17 @@ def load_library(path: str = LIB_PATH):
         getattr(lib, name)
     _LIB = lib
     return lib
+
+
+def device_info(device: int = 0) -> tuple[str, int]:
+    """(device name and architecture, compute units)."""
+    lib = load_library()
+    buf = ctypes.create_string_buffer(256)
+    cus = ctypes.c_int(0)
+    rc = lib.care_hip_device_info(device, buf, 256, ctypes.byref(cus))
+    if rc != CARE_OK:
+        raise CareError(f"care_hip_device_info failed with code {rc}")
+    return buf.value.decode(errors="replace"), cus.value
 
 
 def default_opts(**kw) -> CareOpts:

@@ -173,3 +173,45 @@ def test_correction_reduces_error_rate():
     after = helpers.error_rate(s, helpers.apply_records(rs, data))
     print(f"error rate {before:.5f} -> {after:.5f}")
     assert after < 0.25 * before
+
+
+@pytest.mark.parametrize("name", ["se_higherr_q", "pe"])
+def test_fused_graph_replays_match_eager(name):
+    """The fused pass captured into a HIP graph and replayed: every replay equals the
+    eager pass, with eager work on another context between replays; a graph is reused
+    while the pass's arguments hold and recaptured when the anchor range changes."""
+    s, rs, o = helpers.case(name)
+    eager = engine.CareEngine(0, engine=2, **o)
+    eager.set_reads(rs)
+    eager.build_tables()
+    want = eager.correct()[1]
+    g = engine.CareEngine(0, engine=2, use_graphs=1, **o)
+    g.set_reads(rs)
+    g.build_tables()
+    for rep in range(3):
+        assert g.correct()[1] == want, f"replay {rep}"
+        assert eager.correct(0, 2000)[1] == eager.correct(0, 2000)[1]
+    assert g.timings()["graphs_built"] == 1
+    m = 1234
+    assert g.correct(0, m)[1] + g.correct(m, rs.n - m)[1] == want
+    assert g.timings()["graphs_built"] == 3
+    assert g.correct()[1] == want
+    g.close()
+    eager.close()
+
+
+def test_autotuner_choice_is_exact(tmp_path):
+    """The autotuner times every schedule on a sample and caches its choice; the chosen
+    schedule corrects the whole read set exactly like the oracle."""
+    from rabbitqcplus_amd import autotune
+    s, rs, o = helpers.case("se_higherr_q")
+    cache = str(tmp_path / "tune.json")
+    r = autotune.tune(rs, sample=4096, reps=2, cache=cache, **o)
+    assert not r["cached"] and r["choice"] in autotune.CANDIDATES
+    assert len(r["timings_ms"]) == len(autotune.CANDIDATES)
+    assert autotune.tune(rs, sample=4096, reps=2, cache=cache, **o)["cached"]
+    _, d = engine.correct_gpu(rs, **{**o, **r["choice"]})
+    cpu = oracle.CareCpu(rs, threads=8, **o)
+    _, d_cpu = cpu.correct(threads=8)
+    cpu.close()
+    assert d == d_cpu
