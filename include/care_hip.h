@@ -114,6 +114,15 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
  * include/minhasherlimit.hpp:10-16, include/groupbykey.hpp:99-250). */
 int care_hip_build_tables(care_ctx* ctx, int* maps_built);
 
+/* Table files in the reference's minhasher format (OrdinaryCpuMinhasher::writeToStream /
+ * loadFromStream, include/ordinaryminhasher.hpp:303-339, tables as in
+ * include/cpuhashtable.hpp:221-248,627-649): what --save-hashtables-to writes and
+ * --load-hashtables-from reads. save writes the built tables; load replaces
+ * care_hip_build_tables (after care_hip_set_reads; the file's k-mer length and bucket
+ * threshold must match the engine's options; at most num_hash_maps maps are used). */
+int care_hip_save_tables(care_ctx* ctx, const char* path);
+int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded);
+
 /* Corrects anchors [first, first+n) (replaces cpu::correct_cpu for that range).
  * On success the serialized records are held by the context: num_records records,
  * num_bytes bytes, in read-id order. For paired input first and n must be even. */

@@ -141,6 +141,8 @@ def ref_lib():
         lib.ref_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
         lib.ref_set_anchor_forest.argtypes = [vp, ctypes.c_char_p, ctypes.c_float]
         lib.ref_set_anchor_print.argtypes = [vp, ctypes.c_char_p]
+        lib.ref_save_tables.argtypes = [vp, ctypes.c_char_p]
+        lib.ref_load_tables.argtypes = [vp, ctypes.c_char_p]
         lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
         lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 8
         _ref = lib
@@ -327,6 +329,16 @@ class CareRef:
         cons = np.array([r[2] for r in rows])
         feats = np.array([[float(v) for v in r[3:]] for r in rows], np.float64).reshape(len(rows), -1)
         return rid, pos, cons, feats
+
+    def save_tables(self, path):
+        if self.lib.ref_save_tables(self.h, os.fsencode(path)) != 0:
+            raise RuntimeError("ref_save_tables failed")
+
+    def load_tables(self, path):
+        n = self.lib.ref_load_tables(self.h, os.fsencode(path))
+        if n < 0:
+            raise RuntimeError("ref_load_tables failed")
+        return n
 
     def set_anchor_forest(self, path, threshold=0.5):
         """Forest-classifier anchor correction with the forest file at `path`; None = classic."""

@@ -41,6 +41,7 @@
 #include <memory>
 #include <numeric>
 #include <stdexcept>
+#include <fstream>
 #include <vector>
 
 #include <omp.h>
@@ -235,6 +236,28 @@ ref_ctx* ref_create(const ref_opts* o, std::size_t n, const unsigned int* enc, s
 }
 
 void ref_destroy(ref_ctx* c){ delete c; }
+
+// The reference minhasher's own file writer (OrdinaryCpuMinhasher::writeToStream,
+// include/ordinaryminhasher.hpp:303-316); returns 0 on success.
+int ref_save_tables(ref_ctx* c, const char* path){
+    std::ofstream os(path, std::ios::binary);
+    if(!os) return -1;
+    c->mh->writeToStream(os);
+    return os ? 0 : -1;
+}
+
+// Replaces the context's tables with those of a minhasher file, read by the reference's
+// loader (OrdinaryCpuMinhasher::loadFromStream, :318-339); returns the maps loaded.
+int ref_load_tables(ref_ctx* c, const char* path){
+    std::ifstream is(path, std::ios::binary);
+    if(!is) return -1;
+    auto mh = std::make_unique<care::OrdinaryCpuMinhasher>(c->rs.getNumberOfReads(),
+                                                           calculateResultsPerMapThreshold(c->po.estimatedCoverage),
+                                                           c->po.kmerlength, c->po.hashtableLoadfactor);
+    const int loaded = mh->loadFromStream(is, c->po.numHashFunctions);
+    c->mh = std::move(mh);
+    return loaded;
+}
 
 // Switches the anchor correction to the feature print mode (include/corrector.hpp:1474-1493,
 // include/classification.hpp:50-57): the features of every decision column go to `path`

@@ -76,7 +76,7 @@ struct AlignBest {       // chosen alignment of a candidate (orientation 3 = non
 
 // ---------------------------------------------------------------- device helpers
 
-__device__ __forceinline__ uint64_t fmix64(uint64_t x){
+__host__ __device__ __forceinline__ uint64_t fmix64(uint64_t x){
     x ^= x >> 33;
     x *= 0xff51afd7ed558ccdull;
     x ^= x >> 33;
@@ -3016,6 +3016,147 @@ int care_hip_build_tables(care_ctx* ctx, int* maps_built){
     if(rc != CARE_OK) return rc;
     ctx->hasTables = true;
     if(maps_built) *maps_built = ctx->M;
+    return CARE_OK;
+}
+
+// ---------------------------------------------------------------- table files
+//
+// The reference's minhasher file (OrdinaryCpuMinhasher::writeToStream/loadFromStream,
+// include/ordinaryminhasher.hpp:303-339): int k, int threshold, float load factor, int map
+// count, then per map (CpuReadOnlyMultiValueHashTable, include/cpuhashtable.hpp:627-649)
+// u64 value count and the u32 values (the kept buckets, in key order, read ids ascending),
+// then the key -> (offset, count) table (AoSCpuSingleValueHashTable, :221-248): float load,
+// u64 keys, u64 max probes, u64 size, u64 capacity, u64 slot count and the slots of 16 bytes
+// {u64 key, u32 offset, u16 count, 2 padding bytes}; capacity = size_t(keys / load), slot =
+// fmix64(key) % capacity with linear probing (:96-125), empty = {max key, value-initialized
+// (offset, count) = (0, 0)} (:283-284: numeric_limits of a pair is its default value). Keys are inserted
+// in ascending order, as the reference's construction does, so the layout is the
+// reference's.
+
+constexpr float kTableFileLoad = 0.8f;   // ProgramOptions::hashtableLoadfactor default
+
+struct FileSlot { uint64_t key; uint32_t offset; uint16_t count; uint16_t pad; };
+static_assert(sizeof(FileSlot) == 16, "file slot layout");
+
+int care_hip_save_tables(care_ctx* ctx, const char* path){
+    if(!ctx || !path) return CARE_EINVAL;
+    if(!ctx->hasTables) return ctx->fail(CARE_ESTATE, "build_tables first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    std::FILE* f = std::fopen(path, "wb");
+    if(!f) return ctx->fail(CARE_EINVAL, "cannot open %s for writing", path);
+    struct Closer { std::FILE* f; ~Closer(){ if(f) std::fclose(f); } } closer{f};
+    bool ok = true;
+    auto put = [&](const void* p, size_t b){ ok = ok && std::fwrite(p, 1, b, f) == b; };
+    const int k = ctx->k, thr = ctx->thr, M = ctx->M;
+    const float lf = kTableFileLoad;
+    put(&k, 4); put(&thr, 4); put(&lf, 4); put(&M, 4);
+    std::vector<uint32_t> vals(ctx->n);
+    for(int m = 0; m < M; m++){
+        std::vector<Slot> slots(ctx->caps[m]);
+        HIPCHECK(hipMemcpy(slots.data(), ctx->slotBufs[m].p, slots.size() * sizeof(Slot), hipMemcpyDeviceToHost));
+        if(ctx->n) HIPCHECK(hipMemcpy(vals.data(), ctx->values.as<uint32_t>() + uint64_t(m) * ctx->n, ctx->n * 4,
+                                      hipMemcpyDeviceToHost));
+        std::vector<Slot> kept;
+        for(const Slot& sl : slots) if(sl.count) kept.push_back(sl);
+        std::sort(kept.begin(), kept.end(), [](const Slot& x, const Slot& y){ return x.key < y.key; });
+        std::vector<uint32_t> packed;
+        std::vector<uint64_t> offs(kept.size());
+        for(size_t i = 0; i < kept.size(); i++){
+            offs[i] = packed.size();
+            packed.insert(packed.end(), vals.begin() + kept[i].offset, vals.begin() + kept[i].offset + kept[i].count);
+        }
+        const uint64_t elements = packed.size();
+        put(&elements, 8);
+        put(packed.data(), elements * 4);
+        const uint64_t size = kept.size();
+        const uint64_t capacity = uint64_t(size / lf);
+        std::vector<FileSlot> table(capacity, FileSlot{~0ull, 0u, uint16_t(0), 0});
+        uint64_t maxProbes = 0;
+        for(size_t i = 0; i < kept.size(); i++){
+            uint64_t pos = fmix64(kept[i].key) % capacity, probes = 0;
+            while(table[pos].key != ~0ull){ pos = pos + 1 == capacity ? 0 : pos + 1; probes++; }
+            table[pos] = FileSlot{kept[i].key, uint32_t(offs[i]), uint16_t(kept[i].count), 0};
+            maxProbes = std::max(maxProbes, probes);
+        }
+        put(&lf, 4); put(&size, 8); put(&maxProbes, 8); put(&size, 8); put(&capacity, 8); put(&capacity, 8);
+        put(table.data(), capacity * sizeof(FileSlot));
+    }
+    if(!ok) return ctx->fail(CARE_EINVAL, "writing %s failed", path);
+    return CARE_OK;
+}
+
+int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
+    if(!ctx || !path) return CARE_EINVAL;
+    if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "set_reads first");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    std::FILE* f = std::fopen(path, "rb");
+    if(!f) return ctx->fail(CARE_EINVAL, "cannot open %s", path);
+    struct Closer { std::FILE* f; ~Closer(){ if(f) std::fclose(f); } } closer{f};
+    auto get = [&](void* p, size_t b){ return std::fread(p, 1, b, f) == b; };
+    int k = 0, thr = 0, maps = 0;
+    float lf = 0.0f;
+    if(!get(&k, 4) || !get(&thr, 4) || !get(&lf, 4) || !get(&maps, 4)) return ctx->fail(CARE_EINVAL, "%s: truncated header", path);
+    if(k != ctx->k) return ctx->fail(CARE_EINVAL, "%s: k-mer length %d, engine uses %d", path, k, ctx->k);
+    if(thr != ctx->thr) return ctx->fail(CARE_EINVAL, "%s: bucket threshold %d, engine uses %d", path, thr, ctx->thr);
+    if(maps < 1) return ctx->fail(CARE_EINVAL, "%s: no maps", path);
+    const int M = std::min(maps, ctx->o.num_hash_maps);
+    ctx->M = M;
+    HIPCHECK(ctx->sig.ensure(ctx->n * M * 8));
+    k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
+                                                                   ctx->n, ctx->k, M, ctx->sig.as<uint64_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(ctx->values.ensure(std::max<uint64_t>(1, ctx->n) * M * sizeof(uint32_t)));
+    ctx->slotBufs.resize(M);
+    ctx->caps.assign(M, 0);
+    std::vector<Slot*> ptrs48(kMaxMaps, nullptr);
+    std::vector<uint32_t> caps48(kMaxMaps, 1);
+    for(int m = 0; m < M; m++){
+        uint64_t elements = 0;
+        if(!get(&elements, 8)) return ctx->fail(CARE_EINVAL, "%s: truncated map %d", path, m);
+        if(elements > ctx->n) return ctx->fail(CARE_EINVAL, "%s: map %d holds %llu values for %llu reads", path, m,
+                                               (unsigned long long)elements, (unsigned long long)ctx->n);
+        std::vector<uint32_t> vals(elements);
+        if(!get(vals.data(), elements * 4)) return ctx->fail(CARE_EINVAL, "%s: truncated values of map %d", path, m);
+        for(uint32_t v : vals) if(v >= ctx->n) return ctx->fail(CARE_EINVAL, "%s: read id %u out of range", path, v);
+        float load = 0.0f;
+        uint64_t numKeys = 0, maxProbes = 0, size = 0, capacity = 0, slotCount = 0;
+        if(!get(&load, 4) || !get(&numKeys, 8) || !get(&maxProbes, 8) || !get(&size, 8) || !get(&capacity, 8) ||
+           !get(&slotCount, 8)) return ctx->fail(CARE_EINVAL, "%s: truncated table of map %d", path, m);
+        if(slotCount > (uint64_t(1) << 34)) return ctx->fail(CARE_EINVAL, "%s: implausible table size", path);
+        std::vector<FileSlot> table(slotCount);
+        if(!get(table.data(), slotCount * sizeof(FileSlot))) return ctx->fail(CARE_EINVAL, "%s: truncated slots of map %d", path, m);
+        std::vector<Slot> entries;
+        for(const FileSlot& fs : table){
+            if(fs.key == ~0ull) continue;                      // empty slot
+            if(uint64_t(fs.offset) + fs.count > elements) return ctx->fail(CARE_EINVAL, "%s: bucket past the values", path);
+            entries.push_back(Slot{fs.key, fs.offset, fs.count});
+        }
+        uint32_t cap = 64;
+        while(cap < 2u * entries.size()) cap <<= 1;
+        std::vector<Slot> slots(cap, Slot{0, 0, 0});
+        for(const Slot& e : entries){
+            if(e.count == 0) continue;
+            uint32_t h = uint32_t(fmix64(e.key)) & (cap - 1);
+            while(slots[h].count) h = (h + 1) & (cap - 1);
+            slots[h] = e;
+        }
+        ctx->caps[m] = cap;
+        HIPCHECK(ctx->slotBufs[m].ensure(size_t(cap) * sizeof(Slot)));
+        HIPCHECK(hipMemcpy(ctx->slotBufs[m].p, slots.data(), size_t(cap) * sizeof(Slot), hipMemcpyHostToDevice));
+        if(elements) HIPCHECK(hipMemcpy(ctx->values.as<uint32_t>() + uint64_t(m) * ctx->n, vals.data(), elements * 4,
+                                        hipMemcpyHostToDevice));
+        ptrs48[m] = ctx->slotBufs[m].as<Slot>();
+        caps48[m] = cap;
+    }
+    HIPCHECK(ctx->slotPtrs.ensure(sizeof(Slot*) * kMaxMaps));
+    HIPCHECK(ctx->capsDev.ensure(sizeof(uint32_t) * kMaxMaps));
+    HIPCHECK(hipMemcpy(ctx->slotPtrs.p, ptrs48.data(), sizeof(Slot*) * kMaxMaps, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(ctx->capsDev.p, caps48.data(), sizeof(uint32_t) * kMaxMaps, hipMemcpyHostToDevice));
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->hasTables = true;
+    if(maps_loaded) *maps_loaded = M;
     return CARE_OK;
 }
 

@@ -58,6 +58,7 @@ EXPORTS = [
     "care_hip_set_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
     "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
+    "care_hip_save_tables", "care_hip_load_tables",
     "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
     "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
 ]
@@ -91,6 +92,8 @@ def load_library(path: str = LIB_PATH):
                                            vp, vp, vp, vp, vp]
     lib.care_hip_inspect_msa.argtypes = [vp, u64, i32, ctypes.POINTER(i32), vp, vp, vp, vp, vp]
     lib.care_hip_device_info.argtypes = [i32, ctypes.c_char_p, sz, ctypes.POINTER(i32)]
+    lib.care_hip_save_tables.argtypes = [vp, ctypes.c_char_p]
+    lib.care_hip_load_tables.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(i32)]
     lib.care_hip_set_classifier.argtypes = [vp, vp, sz]
     lib.care_hip_extract_features.argtypes = [vp, u64, u64, vp, vp, sz, ctypes.POINTER(sz)]
     lib.care_mlp_create.argtypes = [i32, ctypes.POINTER(vp)]
@@ -186,6 +189,16 @@ class CareEngine:
     def build_tables(self) -> int:
         maps = ctypes.c_int(0)
         self._check(self.lib.care_hip_build_tables(self.h, ctypes.byref(maps)), "care_hip_build_tables")
+        return maps.value
+
+    def save_tables(self, path: str) -> None:
+        """Writes the tables in the reference's minhasher file format."""
+        self._check(self.lib.care_hip_save_tables(self.h, os.fsencode(path)), "care_hip_save_tables")
+
+    def load_tables(self, path: str) -> int:
+        """Loads tables from a reference minhasher file (instead of build_tables)."""
+        maps = ctypes.c_int(0)
+        self._check(self.lib.care_hip_load_tables(self.h, os.fsencode(path), ctypes.byref(maps)), "care_hip_load_tables")
         return maps.value
 
     def correct(self, first: int = 0, n: int | None = None) -> tuple[int, bytes]:
