@@ -142,6 +142,16 @@ def ref_lib():
         lib.ref_set_anchor_forest.argtypes = [vp, ctypes.c_char_p, ctypes.c_float]
         lib.ref_set_anchor_print.argtypes = [vp, ctypes.c_char_p]
         lib.ref_save_tables.argtypes = [vp, ctypes.c_char_p]
+        lib.ref_parse_fastq.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32]
+        lib.ref_parse_fastq.restype = vp
+        lib.ref_reads_count.argtypes = [vp]
+        lib.ref_reads_count.restype = sz
+        lib.ref_reads_max_length.argtypes = [vp]
+        lib.ref_reads_max_length.restype = i32
+        lib.ref_reads_pitch.argtypes = [vp]
+        lib.ref_reads_pitch.restype = sz
+        lib.ref_reads_copy.argtypes = [vp, vp, vp, vp, vp]
+        lib.ref_reads_free.argtypes = [vp]
         lib.ref_load_tables.argtypes = [vp, ctypes.c_char_p]
         lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
         lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 8
@@ -356,6 +366,33 @@ def write_constant_forest(path, prob):
     blob += struct.pack("<B", 0) + struct.pack("<d", 0.0) + struct.pack("<B", 3) + struct.pack("<ff", prob, prob)
     with open(path, "wb") as fh:
         fh.write(blob)
+
+
+def ref_parse_fastq(file1, file2=None, use_qualities=True):
+    """The reference's FASTQ -> read storage construction: dict of enc [n, pitch] u32,
+    lengths [n] i32, quals [n, maxLen] u8 (or None), ambiguous [n] u8, max_length."""
+    lib = ref_lib()
+    h = lib.ref_parse_fastq(os.fsencode(file1), None if file2 is None else os.fsencode(file2), int(use_qualities))
+    try:
+        n = lib.ref_reads_count(h)
+        ml = lib.ref_reads_max_length(h)
+        pitch = lib.ref_reads_pitch(h)
+        enc = np.zeros((n, pitch), np.uint32)
+        lens = np.zeros(n, np.int32)
+        quals = np.zeros((n, ml), np.uint8) if use_qualities else None
+        amb = np.zeros(n, np.uint8)
+        lib.ref_reads_copy(h, enc.ctypes.data, lens.ctypes.data, _p(quals), amb.ctypes.data)
+    finally:
+        lib.ref_reads_free(h)
+    return dict(enc=enc, lengths=lens, quals=quals, ambiguous=amb, max_length=ml)
+
+
+def write_fastq(path, seqs, quals, names=None):
+    """4-line FASTQ records from lists of byte strings."""
+    with open(path, "wb") as fh:
+        for i, (sq, q) in enumerate(zip(seqs, quals)):
+            name = names[i] if names else b"r%d" % i
+            fh.write(b"@" + name + b"\n" + sq + b"\n+\n" + q + b"\n")
 
 
 MLP_PARAMS = 64 * 16 + 64 + 64 * 64 + 64 + 64 + 1

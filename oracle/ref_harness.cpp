@@ -20,6 +20,7 @@
 #include <config.hpp>
 #include <options.hpp>
 #include <cpureadstorage.hpp>
+#include <chunkedreadstorageconstruction.hpp>
 #include <cpuminhasher.hpp>
 #include <cpuminhasherconstruction.hpp>
 #include <ordinaryminhasher.hpp>
@@ -258,6 +259,63 @@ int ref_load_tables(ref_ctx* c, const char* path){
     c->mh = std::move(mh);
     return loaded;
 }
+
+// The reference's read-storage construction from FASTQ files
+// (constructChunkedReadStorageFromFiles, include/chunkedreadstorageconstruction.hpp:29-480:
+// parser batches of 65,536 reads, lower case to upper case, other characters replaced
+// cyclically by A, C, G, T with the counter reset per batch, paired files interleaved),
+// exported as arrays.
+struct ref_reads {
+    std::size_t n = 0;
+    int maxLen = 0;
+    std::size_t pitchInts = 0;
+    std::vector<unsigned int> enc;
+    std::vector<int> lens;
+    std::vector<char> quals;      // maxLen bytes per read
+    std::vector<unsigned char> ambig;
+};
+
+ref_reads* ref_parse_fastq(const char* file1, const char* file2, int useQualities){
+    care::ProgramOptions po;
+    po.inputfiles.push_back(file1);
+    if(file2) po.inputfiles.push_back(file2);
+    po.pairType = file2 ? care::SequencePairType::PairedEnd : care::SequencePairType::SingleEnd;
+    po.useQualityScores = useQualities != 0;
+    po.qualityScoreBits = 8;
+    po.threads = 1;
+    po.showProgress = false;
+    auto storage = care::constructChunkedReadStorageFromFiles(po);
+    auto* r = new ref_reads;
+    r->n = storage->getNumberOfReads();
+    r->maxLen = storage->getSequenceLengthUpperBound();
+    r->pitchInts = SequenceHelpers::getEncodedNumInts2Bit(r->maxLen);
+    std::vector<care::read_number> ids(r->n);
+    std::iota(ids.begin(), ids.end(), care::read_number(0));
+    r->enc.assign(r->n * r->pitchInts, 0u);
+    r->lens.assign(r->n, 0);
+    storage->gatherSequences(r->enc.data(), r->pitchInts, ids.data(), r->n);
+    storage->gatherSequenceLengths(r->lens.data(), ids.data(), r->n);
+    if(useQualities){
+        r->quals.assign(r->n * std::size_t(r->maxLen), 0);
+        storage->gatherQualities(r->quals.data(), r->maxLen, ids.data(), r->n);
+    }
+    r->ambig.assign(r->n, 0);
+    std::vector<care::read_number> amb(storage->getNumberOfReadsWithN());
+    storage->getIdsOfAmbiguousReads(amb.data());
+    for(auto id : amb) r->ambig[id] = 1;
+    return r;
+}
+
+std::size_t ref_reads_count(const ref_reads* r){ return r->n; }
+int ref_reads_max_length(const ref_reads* r){ return r->maxLen; }
+std::size_t ref_reads_pitch(const ref_reads* r){ return r->pitchInts; }
+void ref_reads_copy(const ref_reads* r, unsigned int* enc, int* lens, char* quals, unsigned char* ambig){
+    std::copy(r->enc.begin(), r->enc.end(), enc);
+    std::copy(r->lens.begin(), r->lens.end(), lens);
+    if(quals) std::copy(r->quals.begin(), r->quals.end(), quals);
+    std::copy(r->ambig.begin(), r->ambig.end(), ambig);
+}
+void ref_reads_free(ref_reads* r){ delete r; }
 
 // Switches the anchor correction to the feature print mode (include/corrector.hpp:1474-1493,
 // include/classification.hpp:50-57): the features of every decision column go to `path`
