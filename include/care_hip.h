@@ -102,11 +102,36 @@ const char* care_hip_last_error(const care_ctx* ctx);
 /* Read storage (reference CpuReadStorage contract, include/cpureadstorage.hpp:16-79):
  * 2-bit rows of pitch_ints u32 (A=0 C=1 G=2 T=3, 16 bases per word, first base in
  * the high bits), lengths, optional Phred+33 qualities in rows of qual_pitch bytes
- * (bytes past a read's length are read as written), and one ambiguity flag per read
+ * (bytes past a read's length, up to the longest read's length, are read as written:
+ * reverse-complement candidates take the row of that width reversed, as the reference
+ * does with its storage pitch, src/correct_cpu.cpp:141, include/corrector.hpp:1227-1239;
+ * the reference's storage holds zeros there), and one ambiguity flag per read
  * (1 = the read contained non-ACGT bases before replacement). Copied to the device. */
 int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints,
                        const int32_t* lengths, const uint8_t* quals, size_t qual_pitch,
                        uint64_t num_reads, const uint8_t* ambiguous);
+
+/* Read storage built on the device from FASTQ text (replaces
+ * constructChunkedReadStorageFromFiles, include/chunkedreadstorageconstruction.hpp:29-480,
+ * with its reader include/readlibraryio.hpp): one text for single-end input, two for
+ * paired input (records of the two texts interleaved R1, R2, R1, ...). Records are
+ * 4-line FASTQ ("@header", sequence, "+...", quality; a '\r' before '\n' is dropped;
+ * blank lines at the end are ignored; at most 2 GiB per text). Lower-case acgt become
+ * upper case; every other sequence character becomes A, C, G, T in turn, the turn
+ * restarting at every batch of 65,536 reads, and marks the read ambiguous. Qualities are
+ * kept as given (8 bits) when use_quality_scores is set. Pitch = ceil(longest/16). */
+int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1,
+                             const char* fastq2, size_t bytes2);
+
+/* Shape of the resident read storage (all zero before any reads are set). */
+int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints);
+
+/* Copies reads [first, first+n) of the resident storage back in care_hip_set_reads'
+ * layout (any pointer may be null; qualities need qual_pitch >= the longest read).
+ * Reference counterparts: CpuReadStorage::gatherContiguousSequences, gatherQualities,
+ * gatherSequenceLengths, areSequencesAmbiguous, include/cpureadstorage.hpp:16-20,29-41,57-61. */
+int care_hip_inspect_reads(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* enc2bit, int32_t* lengths,
+                           uint8_t* quals, size_t qual_pitch, uint8_t* ambiguous);
 
 /* Minhash signatures of all reads and the device-resident tables: per map, the
  * ascending read ids of every signature value shared by 2..threshold reads,

@@ -2926,25 +2926,13 @@ const char* care_hip_last_error(const care_ctx* ctx){
     return ctx ? ctx->err.c_str() : "no context";
 }
 
-int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints, const int32_t* lengths,
-                       const uint8_t* quals, size_t qual_pitch, uint64_t num_reads, const uint8_t* ambiguous){
-    if(!ctx) return CARE_EINVAL;
-    if(!enc2bit || !lengths || !ambiguous || num_reads == 0) return ctx->fail(CARE_EINVAL, "empty or null read storage");
+// Shapes and options of a read set (shared by the array and the FASTQ entry points).
+int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_ints){
+    if(num_reads == 0) return ctx->fail(CARE_EINVAL, "empty read storage");
     if(num_reads >= 0xFFFFFFFFull) return ctx->fail(CARE_EINVAL, "read ids are 32-bit");
     if(ctx->o.paired && (num_reads & 1)) return ctx->fail(CARE_EINVAL, "paired input needs an even number of reads");
-    int maxLen = 0;
-    for(uint64_t i = 0; i < num_reads; i++){
-        if(lengths[i] < 0) return ctx->fail(CARE_EINVAL, "negative length at read %llu", (unsigned long long)i);
-        maxLen = std::max(maxLen, int(lengths[i]));
-    }
     if(maxLen > 32 * kMaxHiLoWords) return ctx->fail(CARE_EINVAL, "reads longer than %d bases are not supported", 32 * kMaxHiLoWords);
     if(pitch_ints * 16 < size_t(maxLen)) return ctx->fail(CARE_EINVAL, "pitch_ints too small for the longest read");
-    if(ctx->o.use_quality_scores){
-        if(!quals) return ctx->fail(CARE_EINVAL, "use_quality_scores set but no qualities given");
-        if(qual_pitch < size_t(maxLen)) return ctx->fail(CARE_EINVAL, "qual_pitch smaller than the longest read");
-    }
-    HIPCHECK(hipSetDevice(ctx->device));
-    hipStream_t st = ctx->stream;
     ctx->n = num_reads;
     ctx->maxLen = maxLen;
     ctx->pitch = pitch_ints;
@@ -2962,7 +2950,49 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
             return ctx->fail(CARE_EINVAL, "reads too long for the LDS-resident MSA");
     }
     ctx->dbgStride = int((4 + size_t(ctx->maxCols) * (16 + 16 + 4 + 4 + 1) + 15) / 16 * 16);
+    ctx->hasReads = false;
+    ctx->hasTables = false;
+    return CARE_OK;
+}
 
+// Quality-weight table and HiLo planes of the resident reads.
+int finishReads(care_ctx* ctx){
+    hipStream_t st = ctx->stream;
+    float qw[256];
+    for(int i = 0; i < 256; i++) qw[i] = qualityWeight(i);
+    HIPCHECK(ctx->qw.ensure(sizeof(qw)));
+    HIPCHECK(hipMemcpyAsync(ctx->qw.p, qw, sizeof(qw), hipMemcpyHostToDevice, st));
+    const uint64_t num_reads = ctx->n;
+    HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4 + 64));
+    const uint64_t hiloThreads = num_reads * uint64_t(ctx->W);
+    k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
+                                                                  num_reads, ctx->W, ctx->hilo.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    ctx->hasReads = true;
+    ctx->hasTables = false;
+    return CARE_OK;
+}
+
+int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints, const int32_t* lengths,
+                       const uint8_t* quals, size_t qual_pitch, uint64_t num_reads, const uint8_t* ambiguous){
+    if(!ctx) return CARE_EINVAL;
+    if(!enc2bit || !lengths || !ambiguous || num_reads == 0) return ctx->fail(CARE_EINVAL, "empty or null read storage");
+    int maxLen = 0;
+    for(uint64_t i = 0; i < num_reads; i++){
+        if(lengths[i] < 0) return ctx->fail(CARE_EINVAL, "negative length at read %llu", (unsigned long long)i);
+        maxLen = std::max(maxLen, int(lengths[i]));
+    }
+    if(ctx->o.use_quality_scores){
+        if(!quals) return ctx->fail(CARE_EINVAL, "use_quality_scores set but no qualities given");
+        if(qual_pitch < size_t(maxLen)) return ctx->fail(CARE_EINVAL, "qual_pitch smaller than the longest read");
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    {
+        const int rc = configureReads(ctx, num_reads, maxLen, pitch_ints);
+        if(rc != CARE_OK) return rc;
+    }
+    hipStream_t st = ctx->stream;
     HIPCHECK(ctx->enc.ensure(num_reads * pitch_ints * 4));
     HIPCHECK(hipMemcpyAsync(ctx->enc.p, enc2bit, num_reads * pitch_ints * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(ctx->len.ensure(num_reads * 4));
@@ -2970,7 +3000,9 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
     HIPCHECK(ctx->ambig.ensure(num_reads));
     HIPCHECK(hipMemcpyAsync(ctx->ambig.p, ambiguous, num_reads, hipMemcpyHostToDevice, st));
     if(ctx->o.use_quality_scores){
-        ctx->qpitch = qual_pitch;
+        // the reference reverses quality rows of its storage's pitch, the longest read
+        // (src/correct_cpu.cpp:141, include/corrector.hpp:1227-1239), whatever the caller's pitch
+        ctx->qpitch = size_t(maxLen);
         ctx->qdpitch = (qual_pitch + 15) / 16 * 16;
         HIPCHECK(ctx->qual.ensure(num_reads * ctx->qdpitch));
         if(ctx->qdpitch == qual_pitch){
@@ -2988,18 +3020,238 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
             HIPCHECK(hipStreamSynchronize(st));
         }
     }
-    float qw[256];
-    for(int i = 0; i < 256; i++) qw[i] = qualityWeight(i);
-    HIPCHECK(ctx->qw.ensure(sizeof(qw)));
-    HIPCHECK(hipMemcpyAsync(ctx->qw.p, qw, sizeof(qw), hipMemcpyHostToDevice, st));
-    HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4 + 64));
-    const uint64_t hiloThreads = num_reads * uint64_t(ctx->W);
-    k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), pitch_ints, ctx->len.as<int>(),
-                                                                  num_reads, ctx->W, ctx->hilo.as<uint32_t>());
+    return finishReads(ctx);
+}
+
+// ---------------------------------------------------------------- FASTQ on the device
+//
+// Read-storage construction from FASTQ text (constructChunkedReadStorageFromFiles,
+// include/chunkedreadstorageconstruction.hpp:29-480): 4-line records; the sequence line
+// upper-cased, every other character than ACGTacgt replaced by A, C, G, T in turn with the
+// counter restarting at every parser batch of 65,536 reads (:67-92, :106, :260), and the
+// read flagged ambiguous; paired files interleaved (R1, R2, R1, ...,
+// include/readlibraryio.hpp:337-). Line ends are '\n'; a '\r' before it is not part of the
+// line. Newline positions come from a stream compaction of the text; then one thread per
+// record scans its lines, a scan of the per-read replacement counts gives every read its
+// counter start, and one thread per read encodes it. Records are 4 lines (sequence and
+// quality on one line each); wrapped multi-line records fail the separator check and are
+// rejected, as is FASTA.
+
+constexpr uint64_t kParserBatch = 65536;
+
+__global__ void k_newline_flags(const char* __restrict__ t, uint64_t n, uint8_t* __restrict__ f){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < n) f[i] = t[i] == '\n';
+}
+
+__device__ __forceinline__ int fastqCode(char c){
+    switch(c){
+        case 'A': case 'a': return 0;
+        case 'C': case 'c': return 1;
+        case 'G': case 'g': return 2;
+        case 'T': case 't': return 3;
+        default: return -1;
+    }
+}
+
+// Record r of one file: lines 4r..4r+3; line i spans [i ? nl[i-1]+1 : 0, i < numNl ? nl[i] : bytes).
+__global__ void k_fastq_records(const char* __restrict__ t, uint64_t bytes, const uint64_t* __restrict__ nl,
+                                uint64_t numNl, uint64_t records, int stride, int slot,
+                                uint64_t* __restrict__ seqStart, uint64_t* __restrict__ qualStart,
+                                int* __restrict__ len, uint64_t* __restrict__ ncount, int* __restrict__ err){
+    const uint64_t rec = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(rec >= records) return;
+    auto lineStart = [&](uint64_t i){ return i ? nl[i - 1] + 1 : uint64_t(0); };
+    auto lineEnd = [&](uint64_t i){
+        uint64_t e = i < numNl ? nl[i] : bytes;
+        const uint64_t s = lineStart(i);
+        if(e > s && t[e - 1] == '\r') e--;
+        return e;
+    };
+    const uint64_t l0 = 4 * rec;
+    const uint64_t h = lineStart(l0), ss = lineStart(l0 + 1), se = lineEnd(l0 + 1);
+    const uint64_t ps = lineStart(l0 + 2), qs = lineStart(l0 + 3), qe = lineEnd(l0 + 3);
+    const uint64_t r = uint64_t(stride) * rec + uint64_t(slot);
+    bool ok = h < bytes && t[h] == '@' && ps < bytes && t[ps] == '+' && qe - qs == se - ss;
+    uint64_t nc = 0;
+    for(uint64_t p = ss; p < se; p++) nc += fastqCode(t[p]) < 0;
+    seqStart[r] = ss;
+    qualStart[r] = qs;
+    len[r] = int(se - ss);
+    ncount[r] = nc;
+    if(!ok) atomicOr(err, 1);
+}
+
+// One thread per read: 2-bit row (MSB first, 16 bases per word, zero-padded to the pitch),
+// ambiguity flag, qualities zero-padded to the device pitch.
+__global__ void k_fastq_encode(const char* __restrict__ t1, const char* __restrict__ t2, int paired,
+                               const uint64_t* __restrict__ seqStart, const uint64_t* __restrict__ qualStart,
+                               const int* __restrict__ len, const uint64_t* __restrict__ prefix, uint64_t n,
+                               size_t pitch, uint32_t* __restrict__ enc, uint8_t* __restrict__ ambig,
+                               uint8_t* __restrict__ qual, size_t qdpitch){
+    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(r >= n) return;
+    const char* t = (paired && (r & 1)) ? t2 : t1;
+    const int L = len[r];
+    const uint64_t b0 = r & ~(kParserBatch - 1);
+    int counter = int((prefix[r] - prefix[b0]) & 3);
+    const char* sq = t + seqStart[r];
+    bool amb = false;
+    uint32_t* row = enc + pitch * r;
+    for(size_t w = 0; w < pitch; w++){
+        uint32_t word = 0;
+        for(int j = 0; j < 16; j++){
+            const int pos = int(w) * 16 + j;
+            if(pos >= L) break;
+            int code = fastqCode(sq[pos]);
+            if(code < 0){ code = counter; counter = (counter + 1) & 3; amb = true; }
+            word |= uint32_t(code) << (30 - 2 * j);
+        }
+        row[w] = word;
+    }
+    ambig[r] = amb;
+    if(qual){
+        const char* qs = t + qualStart[r];
+        uint8_t* qrow = qual + qdpitch * r;
+        for(size_t j = 0; j < qdpitch; j++) qrow[j] = j < size_t(L) ? uint8_t(qs[j]) : uint8_t(0);
+    }
+}
+
+namespace {
+struct ParsedFile {
+    DBuf text, flags, nl, count;
+    uint64_t bytes = 0, numNl = 0, records = 0;
+    void release(){ text.release(); flags.release(); nl.release(); count.release(); }
+};
+}  // namespace
+
+static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t bytes, ParsedFile& f){
+    hipStream_t st = ctx->stream;
+    // blank lines at the end of the text are not records (the reference's reader skips them)
+    while(bytes && (data[bytes - 1] == '\n' || data[bytes - 1] == '\r')) bytes--;
+    if(bytes > size_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "FASTQ text above 2 GiB in one call");
+    f.bytes = bytes;
+    if(bytes == 0){ f.numNl = 0; f.records = 0; return CARE_OK; }
+    HIPCHECK(f.text.ensure(bytes + 1));
+    HIPCHECK(hipMemcpyAsync(f.text.p, data, bytes, hipMemcpyHostToDevice, st));
+    HIPCHECK(f.flags.ensure(bytes + 1));
+    k_newline_flags<<<dim3(uint32_t((bytes + 255) / 256)), 256, 0, st>>>(f.text.as<char>(), bytes, f.flags.as<uint8_t>());
     HIPCHECK(hipGetLastError());
+    HIPCHECK(f.count.ensure(16));
+    hipcub::CountingInputIterator<uint64_t> it(0);
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, f.flags.as<uint8_t>(), (uint64_t*)nullptr,
+                                           f.count.as<uint64_t>(), int(bytes), st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    // newlines <= bytes; the positions are written first into the flags' own size of u64
+    HIPCHECK(f.nl.ensure((bytes + 1) * sizeof(uint64_t) / 8 * 8 + 8));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceSelect::Flagged(ctx->scanTmp.p, tb, it, f.flags.as<uint8_t>(), f.nl.as<uint64_t>(),
+                                           f.count.as<uint64_t>(), int(bytes), st));
+    uint64_t numNl = 0;
+    HIPCHECK(hipMemcpyAsync(&numNl, f.count.p, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
-    ctx->hasReads = true;
-    ctx->hasTables = false;
+    f.numNl = numNl;
+    const uint64_t lines = bytes ? numNl + 1 : 0;
+    if(lines % 4) return ctx->fail(CARE_EINVAL, "FASTQ text of %llu lines is not 4-line records",
+                                   (unsigned long long)lines);
+    f.records = lines / 4;
+    f.flags.release();
+    return CARE_OK;
+}
+
+int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, const char* fastq2, size_t bytes2){
+    if(!ctx || !fastq1) return CARE_EINVAL;
+    const bool paired = ctx->o.paired != 0;
+    if(paired != (fastq2 != nullptr)) return ctx->fail(CARE_EINVAL, paired ? "paired input needs two FASTQ texts"
+                                                                            : "single-end input takes one FASTQ text");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    ParsedFile f1, f2;
+    struct Rel { ParsedFile& a; ParsedFile& b; ~Rel(){ a.release(); b.release(); } } rel{f1, f2};
+    int rc = uploadAndSplitLines(ctx, fastq1, bytes1, f1);
+    if(rc != CARE_OK) return rc;
+    if(paired){
+        rc = uploadAndSplitLines(ctx, fastq2, bytes2, f2);
+        if(rc != CARE_OK) return rc;
+        if(f1.records != f2.records) return ctx->fail(CARE_EINVAL, "paired FASTQ texts hold %llu and %llu records",
+                                                      (unsigned long long)f1.records, (unsigned long long)f2.records);
+    }
+    const uint64_t n = paired ? 2 * f1.records : f1.records;
+    if(n == 0) return ctx->fail(CARE_EINVAL, "no FASTQ records");
+    struct Tmp { DBuf seqStart, qualStart, lens, ncount, prefix, err; ~Tmp(){ seqStart.release(); qualStart.release();
+        lens.release(); ncount.release(); prefix.release(); err.release(); } } tmp;
+    HIPCHECK(tmp.seqStart.ensure(n * 8)); HIPCHECK(tmp.qualStart.ensure(n * 8));
+    HIPCHECK(tmp.lens.ensure(n * 4)); HIPCHECK(tmp.ncount.ensure((n + 1) * 8)); HIPCHECK(tmp.prefix.ensure((n + 1) * 8));
+    HIPCHECK(tmp.err.ensure(4));
+    k_zero_u32<<<1, 64, 0, st>>>(tmp.err.as<uint32_t>(), 1);
+    const int stride = paired ? 2 : 1;
+    for(int fi = 0; fi < (paired ? 2 : 1); fi++){
+        ParsedFile& f = fi ? f2 : f1;
+        k_fastq_records<<<dim3(uint32_t((f.records + 255) / 256)), 256, 0, st>>>(
+            f.text.as<char>(), f.bytes, f.nl.as<uint64_t>(), f.numNl, f.records, stride, fi,
+            tmp.seqStart.as<uint64_t>(), tmp.qualStart.as<uint64_t>(), tmp.lens.as<int>(), tmp.ncount.as<uint64_t>(),
+            tmp.err.as<int>());
+        HIPCHECK(hipGetLastError());
+    }
+    int err = 0;
+    std::vector<int> lens(n);
+    HIPCHECK(hipMemcpyAsync(&err, tmp.err.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(lens.data(), tmp.lens.p, n * 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if(err) return ctx->fail(CARE_EINVAL, "malformed FASTQ record (header '@', separator '+', quality length)");
+    const int maxLen = *std::max_element(lens.begin(), lens.end());
+    const size_t pitch = size_t(std::max(1, (maxLen + 15) / 16));
+    rc = configureReads(ctx, n, maxLen, pitch);
+    if(rc != CARE_OK) return rc;
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, tmp.ncount.as<uint64_t>(), tmp.prefix.as<uint64_t>(), int(n), st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, tmp.ncount.as<uint64_t>(), tmp.prefix.as<uint64_t>(), int(n), st));
+    HIPCHECK(ctx->enc.ensure(n * pitch * 4));
+    HIPCHECK(ctx->len.ensure(n * 4));
+    HIPCHECK(hipMemcpyAsync(ctx->len.p, tmp.lens.p, n * 4, hipMemcpyDeviceToDevice, st));
+    HIPCHECK(ctx->ambig.ensure(n));
+    uint8_t* qual = nullptr;
+    if(ctx->o.use_quality_scores){
+        ctx->qpitch = size_t(maxLen);
+        ctx->qdpitch = (size_t(maxLen) + 15) / 16 * 16;
+        HIPCHECK(ctx->qual.ensure(n * ctx->qdpitch));
+        qual = ctx->qual.as<uint8_t>();
+    }
+    k_fastq_encode<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(
+        f1.text.as<char>(), paired ? f2.text.as<char>() : nullptr, paired, tmp.seqStart.as<uint64_t>(),
+        tmp.qualStart.as<uint64_t>(), tmp.lens.as<int>(), tmp.prefix.as<uint64_t>(), n, pitch, ctx->enc.as<uint32_t>(),
+        ctx->ambig.as<uint8_t>(), qual, ctx->qdpitch);
+    HIPCHECK(hipGetLastError());
+    return finishReads(ctx);
+}
+
+int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints){
+    if(!ctx) return CARE_EINVAL;
+    if(num_reads) *num_reads = ctx->hasReads ? ctx->n : 0;
+    if(max_length) *max_length = ctx->hasReads ? ctx->maxLen : 0;
+    if(pitch_ints) *pitch_ints = ctx->hasReads ? ctx->pitch : 0;
+    return CARE_OK;
+}
+
+int care_hip_inspect_reads(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* enc2bit, int32_t* lengths,
+                           uint8_t* quals, size_t qual_pitch, uint8_t* ambiguous){
+    if(!ctx) return CARE_EINVAL;
+    if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "no reads");
+    if(first > ctx->n || n > ctx->n - first) return ctx->fail(CARE_EINVAL, "range out of bounds");
+    HIPCHECK(hipSetDevice(ctx->device));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if(enc2bit && n) HIPCHECK(hipMemcpy(enc2bit, ctx->enc.as<uint32_t>() + ctx->pitch * first, n * ctx->pitch * 4, hipMemcpyDeviceToHost));
+    if(lengths && n) HIPCHECK(hipMemcpy(lengths, ctx->len.as<int>() + first, n * 4, hipMemcpyDeviceToHost));
+    if(ambiguous && n) HIPCHECK(hipMemcpy(ambiguous, ctx->ambig.as<uint8_t>() + first, n, hipMemcpyDeviceToHost));
+    if(quals && n){
+        if(!ctx->o.use_quality_scores) return ctx->fail(CARE_EINVAL, "no qualities");
+        if(qual_pitch < size_t(ctx->maxLen)) return ctx->fail(CARE_EINVAL, "qual_pitch smaller than the longest read");
+        HIPCHECK(hipMemcpy2D(quals, qual_pitch, ctx->qual.as<uint8_t>() + ctx->qdpitch * first, ctx->qdpitch,
+                             std::min(qual_pitch, ctx->qdpitch), n, hipMemcpyDeviceToHost));
+    }
     return CARE_OK;
 }
 

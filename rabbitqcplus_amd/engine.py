@@ -58,7 +58,8 @@ EXPORTS = [
     "care_hip_set_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
     "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
-    "care_hip_save_tables", "care_hip_load_tables",
+    "care_hip_save_tables", "care_hip_load_tables", "care_hip_set_reads_fastq", "care_hip_read_layout",
+    "care_hip_inspect_reads",
     "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
     "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
 ]
@@ -83,6 +84,9 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_last_error.argtypes = [vp]
     lib.care_hip_last_error.restype = ctypes.c_char_p
     lib.care_hip_set_reads.argtypes = [vp, vp, sz, vp, vp, sz, u64, vp]
+    lib.care_hip_set_reads_fastq.argtypes = [vp, vp, sz, vp, sz]
+    lib.care_hip_read_layout.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(i32), ctypes.POINTER(sz)]
+    lib.care_hip_inspect_reads.argtypes = [vp, u64, u64, vp, vp, vp, sz, vp]
     lib.care_hip_build_tables.argtypes = [vp, ctypes.POINTER(i32)]
     lib.care_hip_correct.argtypes = [vp, u64, u64, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     lib.care_hip_copy_records.argtypes = [vp, vp, sz]
@@ -186,6 +190,47 @@ class CareEngine:
                                          rs.n, rs.ambiguous.ctypes.data)
         self._check(rc, "care_hip_set_reads")
 
+    def set_reads_fastq(self, file1, file2=None) -> int:
+        """Builds the read storage on the device from FASTQ files (paths or bytes);
+        two files for paired input. Returns the number of reads."""
+        def text(f):
+            if isinstance(f, (bytes, bytearray, memoryview)):
+                return np.frombuffer(f, np.uint8)
+            return np.fromfile(f, np.uint8)
+        t1 = text(file1)
+        t2 = None if file2 is None else text(file2)
+        self.storage = None
+        rc = self.lib.care_hip_set_reads_fastq(self.h, t1.ctypes.data, t1.size,
+                                               None if t2 is None else t2.ctypes.data, 0 if t2 is None else t2.size)
+        self._check(rc, "care_hip_set_reads_fastq")
+        return self.read_layout()[0]
+
+    def read_layout(self) -> tuple[int, int, int]:
+        """(number of reads, longest read, pitch in u32 words) of the resident storage."""
+        n, ml, pitch = ctypes.c_uint64(0), ctypes.c_int(0), ctypes.c_size_t(0)
+        self._check(self.lib.care_hip_read_layout(self.h, ctypes.byref(n), ctypes.byref(ml), ctypes.byref(pitch)),
+                    "care_hip_read_layout")
+        return n.value, ml.value, pitch.value
+
+    @property
+    def num_reads(self) -> int:
+        return self.read_layout()[0]
+
+    def inspect_reads(self, first: int = 0, n: int | None = None) -> dict:
+        """Reads [first, first+n) of the resident storage: enc [n, pitch] u32, lengths,
+        quals [n, longest] u8 (None without quality scores), ambiguous."""
+        total, ml, pitch = self.read_layout()
+        n = total - first if n is None else n
+        enc = np.zeros((max(n, 1), pitch), np.uint32)
+        lens = np.zeros(max(n, 1), np.int32)
+        amb = np.zeros(max(n, 1), np.uint8)
+        quals = np.zeros((max(n, 1), ml), np.uint8) if self.opts.use_quality_scores else None
+        self._check(self.lib.care_hip_inspect_reads(self.h, first, n, enc.ctypes.data, lens.ctypes.data,
+                                                    None if quals is None else quals.ctypes.data, ml,
+                                                    amb.ctypes.data), "care_hip_inspect_reads")
+        return dict(enc=enc[:n], lengths=lens[:n], quals=None if quals is None else quals[:n], ambiguous=amb[:n],
+                    max_length=ml)
+
     def build_tables(self) -> int:
         maps = ctypes.c_int(0)
         self._check(self.lib.care_hip_build_tables(self.h, ctypes.byref(maps)), "care_hip_build_tables")
@@ -202,7 +247,7 @@ class CareEngine:
         return maps.value
 
     def correct(self, first: int = 0, n: int | None = None) -> tuple[int, bytes]:
-        n = self.storage.n - first if n is None else n
+        n = self.num_reads - first if n is None else n
         nrec, nbytes = ctypes.c_size_t(0), ctypes.c_size_t(0)
         self._check(self.lib.care_hip_correct(self.h, first, n, ctypes.byref(nrec), ctypes.byref(nbytes)),
                     "care_hip_correct")
@@ -221,7 +266,7 @@ class CareEngine:
     def extract_features(self, first: int = 0, n: int | None = None, cap: int | None = None):
         """Decision columns of anchors [first, first+n): (features [m,16] f32, meta [m,4] u32),
         ordered by (read id, position)."""
-        n = self.storage.n - first if n is None else n
+        n = self.num_reads - first if n is None else n
         total = ctypes.c_size_t(0)
         if cap is None:
             self._check(self.lib.care_hip_extract_features(self.h, first, n, None, None, 0, ctypes.byref(total)),
@@ -240,7 +285,7 @@ class CareEngine:
         return {k: getattr(t, k) for k, _ in CareTimings._fields_}
 
     def signatures(self, first: int = 0, n: int | None = None) -> np.ndarray:
-        n = self.storage.n - first if n is None else n
+        n = self.num_reads - first if n is None else n
         out = np.zeros((n, self.opts.num_hash_maps), np.uint64)
         self._check(self.lib.care_hip_inspect_signatures(self.h, first, n, out.ctypes.data), "inspect_signatures")
         return out

@@ -24,6 +24,7 @@ def test_host_storage_matches_reference_parser(tmp_path, paired):
     assert ref["ambiguous"].sum() > 100
     w = min(ref["enc"].shape[1], rs.enc.shape[1])
     assert np.array_equal(ref["enc"][:, :w], rs.enc[:, :w])
+    # whole rows: the reference's storage holds zeros past each read's end, and the
+    # correction reads them (reverse-complement candidates reverse the whole row)
     ml = ref["max_length"]
-    valid = np.arange(ml)[None, :] < ref["lengths"][:, None]
-    assert np.array_equal(np.where(valid, ref["quals"], 0), np.where(valid, rs.quals[:, :ml], 0))
+    assert np.array_equal(ref["quals"], rs.quals[:, :ml])
