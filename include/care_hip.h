@@ -123,6 +123,22 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
 int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1,
                              const char* fastq2, size_t bytes2);
 
+/* Output construction: the corrected FASTQ text of input file `file` (0, or 1 for the
+ * second file of a pair) from serialized records (any order; the concatenation of what
+ * care_hip_copy_records returns, over one or more care_hip_correct calls) and the FASTQ
+ * text of care_hip_set_reads_fastq, kept on the device. Replaces the reference's
+ * sortSerializedResultsByReadIdAscending (include/sortserializedresults.hpp:17-78) and
+ * constructOutputFileFromCorrectionResultsOutToQueue (src/correctionresultoutput.cpp:1589-1632,
+ * merge :448-975, combine rule combineMultipleCorrectionResults1_rawtcs2 :1256-1550): each
+ * read is written as "@" header "\n" sequence "\n+\n" quality "\n", the sequence being the
+ * original text with the combined correction applied; quality_labels appends " care:q=N"
+ * (outputCorrectionQualityLabels) to the headers of reads up to the last corrected one.
+ * With no records the input text is returned unchanged (the reference then hands QC
+ * nothing and QC reads the input itself, src/seqc.cpp:662-672). *written = size of the
+ * text; dst may be null to ask for the size; otherwise cap must be at least the size. */
+int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file,
+                         int quality_labels, char* dst, size_t cap, size_t* written);
+
 /* Shape of the resident read storage (all zero before any reads are set). */
 int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints);
 

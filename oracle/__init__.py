@@ -152,6 +152,8 @@ def ref_lib():
         lib.ref_reads_pitch.restype = sz
         lib.ref_reads_copy.argtypes = [vp, vp, vp, vp, vp]
         lib.ref_reads_free.argtypes = [vp]
+        lib.ref_output_fastq.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, sz, i32, ctypes.c_char_p,
+                                         ctypes.c_char_p, ctypes.c_char_p]
         lib.ref_load_tables.argtypes = [vp, ctypes.c_char_p]
         lib.ref_orientation.argtypes = [i32, i32, i32, i32, i32, i32, i32, i32, ctypes.c_float, i32, ctypes.c_float]
         lib.ref_msa.argtypes = [i32, i32, i32, vp, vp, vp, i32, vp, i32, vp, vp, vp] + [vp] * 8
@@ -366,6 +368,24 @@ def write_constant_forest(path, prob):
     blob += struct.pack("<B", 0) + struct.pack("<d", 0.0) + struct.pack("<B", 3) + struct.pack("<ff", prob, prob)
     with open(path, "wb") as fh:
         fh.write(blob)
+
+
+def ref_output_fastq(file1, file2, records: bytes, labels=False, tmpdir=None):
+    """The reference's output construction (sort + merge into QC's queues): ([text per
+    queue], number of records). Single-end runs have one queue."""
+    import tempfile
+    lib = ref_lib()
+    with tempfile.TemporaryDirectory(dir=tmpdir) as d:
+        q1, q2 = os.path.join(d, "q1.out"), os.path.join(d, "q2.out")
+        buf = ctypes.create_string_buffer(records, max(1, len(records)))
+        rc = lib.ref_output_fastq(os.fsencode(file1), None if file2 is None else os.fsencode(file2), buf,
+                                  len(records), int(labels), os.fsencode(d), os.fsencode(q1), os.fsencode(q2))
+        if rc < 0:
+            raise RuntimeError("ref_output_fastq failed")
+        outs = [open(q1, "rb").read()]
+        if file2 is not None:
+            outs.append(open(q2, "rb").read())
+    return outs, rc
 
 
 def ref_parse_fastq(file1, file2=None, use_qualities=True):

@@ -34,6 +34,8 @@
 #include <correctedsequence.hpp>
 #include <serializedobjectstorage.hpp>
 #include <sequencehelpers.hpp>
+#include <correctionresultoutput.hpp>
+#include <sortserializedresults.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -44,6 +46,8 @@
 #include <stdexcept>
 #include <fstream>
 #include <vector>
+#include <atomic>
+#include <thread>
 
 #include <omp.h>
 
@@ -548,6 +552,73 @@ int ref_msa(int useQ, int la, int nc, const char* anchor, const char* aq, const 
         }
     }
     return ncol;
+}
+
+// Output construction as RabbitQCPlus runs it: the serialized records are sorted by read
+// id (sortSerializedResultsByReadIdAscending, include/sortserializedresults.hpp:17-78) and
+// merged with the original FASTQ file(s) into the queues QC reads from
+// (constructOutputFileFromCorrectionResultsOutToQueue, src/correctionresultoutput.cpp:1589-1632,
+// merge :448-975, combine rule :1256-1550). Single-end runs pass one queue twice (src/seqc.cpp:256),
+// paired runs two queues and two output names (src/peqc.cpp:135-148,300). What reaches queue 1
+// (and queue 2) is written to q1_path (q2_path). Scratch files go to `tmpdir`. Returns the
+// number of records (the reference's changNum) or -1.
+int ref_output_fastq(const char* in1, const char* in2, const uint8_t* records, size_t bytes, int labels,
+                     const char* tmpdir, const char* q1_path, const char* q2_path){
+    try{
+        care::ProgramOptions po;
+        po.outputCorrectionQualityLabels = labels != 0;
+        po.pairType = in2 ? care::SequencePairType::PairedEnd : care::SequencePairType::SingleEnd;
+        std::vector<std::string> inputs{in1};
+        std::vector<std::string> outputs{std::string(tmpdir) + "/tmp1.fq"};
+        if(in2){
+            inputs.push_back(in2);
+            outputs.push_back(std::string(tmpdir) + "/tmp2.fq");
+        }
+        care::SerializedObjectStorage storage(size_t(1) << 30, size_t(1) << 30, std::string(tmpdir) + "/");
+        size_t p = 0;
+        while(p + 8 <= bytes){
+            uint32_t flags;
+            std::memcpy(&flags, records + p + 4, 4);
+            const size_t len = 8 + (flags & ((1u << 29) - 1));
+            if(p + len > bytes) return -1;
+            storage.insert(records + p, records + p + len);
+            p += len;
+        }
+        care::sortSerializedResultsByReadIdAscending<care::EncodedTempCorrectedSequence>(storage, size_t(1) << 30);
+        moodycamel::ReaderWriterQueue<std::pair<char*, int>> q1, q2;
+        auto* second = in2 ? &q2 : &q1;
+        std::atomic_int producerDone{0}, careStartWrite{0};
+        int changNum = 0;
+        std::atomic<bool> finished{false};
+        std::thread worker([&]{
+            care::constructOutputFileFromCorrectionResultsOutToQueue(inputs, storage, care::FileFormat::FASTQ, outputs,
+                                                                     false, po, &q1, second, &producerDone,
+                                                                     &careStartWrite, &changNum);
+            finished = true;
+        });
+        std::ofstream o1(q1_path, std::ios::binary), o2(q2_path, std::ios::binary);
+        auto drain = [](moodycamel::ReaderWriterQueue<std::pair<char*, int>>& q, std::ofstream& o){
+            std::pair<char*, int> item;
+            bool any = false;
+            while(q.try_dequeue(item)){
+                o.write(item.first, item.second);
+                delete[] item.first;
+                any = true;
+            }
+            return any;
+        };
+        while(!finished){
+            const bool a = drain(q1, o1);
+            const bool b = in2 ? drain(q2, o2) : false;
+            if(!a && !b) std::this_thread::sleep_for(std::chrono::microseconds(200));
+        }
+        worker.join();
+        drain(q1, o1);
+        if(in2) drain(q2, o2);
+        return changNum;
+    }catch(...){
+        return -1;
+    }
 }
 
 }  // extern "C"

@@ -2310,6 +2310,13 @@ struct care_ctx {
     unsigned long long featCap = 0;
     std::vector<uint8_t> records;
     size_t numRecords = 0;
+    // FASTQ text kept on the device by care_hip_set_reads_fastq for the output merge, and
+    // per read: header start (after '@'), header length, sequence and quality starts
+    DBuf fqText[2], fqHdr, fqHdrLen, fqSeq, fqQual;
+    size_t fqRaw[2] = {0, 0};
+    bool hasFastq = false;
+    // output merge working memory
+    DBuf mRec, mOff, mKeyIn, mKeyOut, mIdxIn, mIdxOut, mRunBeg, mRunEnd, mPlan, mSize, mOutOff, mText, mErr;
 
     // last batch (for inspection)
     uint64_t lastB0 = 0, lastB = 0, lastTotal = 0;
@@ -2899,7 +2906,10 @@ void care_hip_destroy(care_ctx* ctx){
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
                    &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp, &ctx->scratch,
-                   &ctx->seam, &ctx->featBuf, &ctx->metaBuf}){
+                   &ctx->seam, &ctx->featBuf, &ctx->metaBuf, &ctx->fqText[0], &ctx->fqText[1], &ctx->fqHdr,
+                   &ctx->fqHdrLen, &ctx->fqSeq, &ctx->fqQual, &ctx->mRec, &ctx->mOff, &ctx->mKeyIn, &ctx->mKeyOut,
+                   &ctx->mIdxIn, &ctx->mIdxOut, &ctx->mRunBeg, &ctx->mRunEnd, &ctx->mPlan, &ctx->mSize, &ctx->mOutOff,
+                   &ctx->mText, &ctx->mErr}){
         b->release();
     }
     if(ctx->fusedExec) (void)hipGraphExecDestroy(ctx->fusedExec);
@@ -2952,6 +2962,7 @@ int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_i
     ctx->dbgStride = int((4 + size_t(ctx->maxCols) * (16 + 16 + 4 + 4 + 1) + 15) / 16 * 16);
     ctx->hasReads = false;
     ctx->hasTables = false;
+    ctx->hasFastq = false;
     return CARE_OK;
 }
 
@@ -3030,8 +3041,10 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
 // upper-cased, every other character than ACGTacgt replaced by A, C, G, T in turn with the
 // counter restarting at every parser batch of 65,536 reads (:67-92, :106, :260), and the
 // read flagged ambiguous; paired files interleaved (R1, R2, R1, ...,
-// include/readlibraryio.hpp:337-). Line ends are '\n'; a '\r' before it is not part of the
-// line. Newline positions come from a stream compaction of the text; then one thread per
+// include/readlibraryio.hpp:337-). Line ends are '\n'; a '\r' before it is dropped when
+// what it ends (header after '@', sequence, quality) has more than one character, as the
+// reference's reader does (include/kseqpp/kseqpp.hpp:286-288). Newline positions come from
+// a stream compaction of the text; then one thread per
 // record scans its lines, a scan of the per-read replacement counts gives every read its
 // counter start, and one thread per read encodes it. Records are 4 lines (sequence and
 // quality on one line each); wrapped multi-line records fail the separator check and are
@@ -3057,24 +3070,29 @@ __device__ __forceinline__ int fastqCode(char c){
 // Record r of one file: lines 4r..4r+3; line i spans [i ? nl[i-1]+1 : 0, i < numNl ? nl[i] : bytes).
 __global__ void k_fastq_records(const char* __restrict__ t, uint64_t bytes, const uint64_t* __restrict__ nl,
                                 uint64_t numNl, uint64_t records, int stride, int slot,
+                                uint64_t* __restrict__ hdrStart, uint32_t* __restrict__ hdrLen,
                                 uint64_t* __restrict__ seqStart, uint64_t* __restrict__ qualStart,
                                 int* __restrict__ len, uint64_t* __restrict__ ncount, int* __restrict__ err){
     const uint64_t rec = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(rec >= records) return;
     auto lineStart = [&](uint64_t i){ return i ? nl[i - 1] + 1 : uint64_t(0); };
-    auto lineEnd = [&](uint64_t i){
+    // end of the content starting at s on line i
+    auto contentEnd = [&](uint64_t i, uint64_t s){
         uint64_t e = i < numNl ? nl[i] : bytes;
-        const uint64_t s = lineStart(i);
-        if(e > s && t[e - 1] == '\r') e--;
+        if(e > s + 1 && t[e - 1] == '\r') e--;
         return e;
     };
     const uint64_t l0 = 4 * rec;
-    const uint64_t h = lineStart(l0), ss = lineStart(l0 + 1), se = lineEnd(l0 + 1);
-    const uint64_t ps = lineStart(l0 + 2), qs = lineStart(l0 + 3), qe = lineEnd(l0 + 3);
+    const uint64_t h = lineStart(l0);
+    const uint64_t ss = lineStart(l0 + 1), se = contentEnd(l0 + 1, ss);
+    const uint64_t ps = lineStart(l0 + 2), qs = lineStart(l0 + 3), qe = contentEnd(l0 + 3, qs);
     const uint64_t r = uint64_t(stride) * rec + uint64_t(slot);
-    bool ok = h < bytes && t[h] == '@' && ps < bytes && t[ps] == '+' && qe - qs == se - ss;
+    const bool ok = h < bytes && t[h] == '@' && ps < bytes && t[ps] == '+' && qe - qs == se - ss;
+    const uint64_t he = ok ? contentEnd(l0, h + 1) : h;
     uint64_t nc = 0;
     for(uint64_t p = ss; p < se; p++) nc += fastqCode(t[p]) < 0;
+    hdrStart[r] = h + 1;
+    hdrLen[r] = uint32_t(ok ? he - (h + 1) : 0);
     seqStart[r] = ss;
     qualStart[r] = qs;
     len[r] = int(se - ss);
@@ -3119,23 +3137,27 @@ __global__ void k_fastq_encode(const char* __restrict__ t1, const char* __restri
 
 namespace {
 struct ParsedFile {
-    DBuf text, flags, nl, count;
+    DBuf* text = nullptr;          // the context's resident copy of the text
+    DBuf flags, nl, count;
     uint64_t bytes = 0, numNl = 0, records = 0;
-    void release(){ text.release(); flags.release(); nl.release(); count.release(); }
+    void release(){ flags.release(); nl.release(); count.release(); }
 };
 }  // namespace
 
-static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t bytes, ParsedFile& f){
+static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t raw, int slot, ParsedFile& f){
     hipStream_t st = ctx->stream;
+    if(raw > size_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "FASTQ text above 2 GiB in one call");
+    f.text = &ctx->fqText[slot];
+    ctx->fqRaw[slot] = raw;
+    HIPCHECK(f.text->ensure(raw + 1));
+    if(raw) HIPCHECK(hipMemcpyAsync(f.text->p, data, raw, hipMemcpyHostToDevice, st));
     // blank lines at the end of the text are not records (the reference's reader skips them)
+    size_t bytes = raw;
     while(bytes && (data[bytes - 1] == '\n' || data[bytes - 1] == '\r')) bytes--;
-    if(bytes > size_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "FASTQ text above 2 GiB in one call");
     f.bytes = bytes;
     if(bytes == 0){ f.numNl = 0; f.records = 0; return CARE_OK; }
-    HIPCHECK(f.text.ensure(bytes + 1));
-    HIPCHECK(hipMemcpyAsync(f.text.p, data, bytes, hipMemcpyHostToDevice, st));
     HIPCHECK(f.flags.ensure(bytes + 1));
-    k_newline_flags<<<dim3(uint32_t((bytes + 255) / 256)), 256, 0, st>>>(f.text.as<char>(), bytes, f.flags.as<uint8_t>());
+    k_newline_flags<<<dim3(uint32_t((bytes + 255) / 256)), 256, 0, st>>>(f.text->as<char>(), bytes, f.flags.as<uint8_t>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(f.count.ensure(16));
     hipcub::CountingInputIterator<uint64_t> it(0);
@@ -3167,21 +3189,23 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, c
                                                                             : "single-end input takes one FASTQ text");
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
+    ctx->hasFastq = false;
     ParsedFile f1, f2;
     struct Rel { ParsedFile& a; ParsedFile& b; ~Rel(){ a.release(); b.release(); } } rel{f1, f2};
-    int rc = uploadAndSplitLines(ctx, fastq1, bytes1, f1);
+    int rc = uploadAndSplitLines(ctx, fastq1, bytes1, 0, f1);
     if(rc != CARE_OK) return rc;
     if(paired){
-        rc = uploadAndSplitLines(ctx, fastq2, bytes2, f2);
+        rc = uploadAndSplitLines(ctx, fastq2, bytes2, 1, f2);
         if(rc != CARE_OK) return rc;
         if(f1.records != f2.records) return ctx->fail(CARE_EINVAL, "paired FASTQ texts hold %llu and %llu records",
                                                       (unsigned long long)f1.records, (unsigned long long)f2.records);
     }
     const uint64_t n = paired ? 2 * f1.records : f1.records;
     if(n == 0) return ctx->fail(CARE_EINVAL, "no FASTQ records");
-    struct Tmp { DBuf seqStart, qualStart, lens, ncount, prefix, err; ~Tmp(){ seqStart.release(); qualStart.release();
-        lens.release(); ncount.release(); prefix.release(); err.release(); } } tmp;
-    HIPCHECK(tmp.seqStart.ensure(n * 8)); HIPCHECK(tmp.qualStart.ensure(n * 8));
+    struct Tmp { DBuf lens, ncount, prefix, err; ~Tmp(){ lens.release(); ncount.release(); prefix.release();
+        err.release(); } } tmp;
+    HIPCHECK(ctx->fqHdr.ensure(n * 8)); HIPCHECK(ctx->fqHdrLen.ensure(n * 4));
+    HIPCHECK(ctx->fqSeq.ensure(n * 8)); HIPCHECK(ctx->fqQual.ensure(n * 8));
     HIPCHECK(tmp.lens.ensure(n * 4)); HIPCHECK(tmp.ncount.ensure((n + 1) * 8)); HIPCHECK(tmp.prefix.ensure((n + 1) * 8));
     HIPCHECK(tmp.err.ensure(4));
     k_zero_u32<<<1, 64, 0, st>>>(tmp.err.as<uint32_t>(), 1);
@@ -3189,9 +3213,9 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, c
     for(int fi = 0; fi < (paired ? 2 : 1); fi++){
         ParsedFile& f = fi ? f2 : f1;
         k_fastq_records<<<dim3(uint32_t((f.records + 255) / 256)), 256, 0, st>>>(
-            f.text.as<char>(), f.bytes, f.nl.as<uint64_t>(), f.numNl, f.records, stride, fi,
-            tmp.seqStart.as<uint64_t>(), tmp.qualStart.as<uint64_t>(), tmp.lens.as<int>(), tmp.ncount.as<uint64_t>(),
-            tmp.err.as<int>());
+            f.text->as<char>(), f.bytes, f.nl.as<uint64_t>(), f.numNl, f.records, stride, fi,
+            ctx->fqHdr.as<uint64_t>(), ctx->fqHdrLen.as<uint32_t>(), ctx->fqSeq.as<uint64_t>(), ctx->fqQual.as<uint64_t>(),
+            tmp.lens.as<int>(), tmp.ncount.as<uint64_t>(), tmp.err.as<int>());
         HIPCHECK(hipGetLastError());
     }
     int err = 0;
@@ -3221,11 +3245,340 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, c
         qual = ctx->qual.as<uint8_t>();
     }
     k_fastq_encode<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(
-        f1.text.as<char>(), paired ? f2.text.as<char>() : nullptr, paired, tmp.seqStart.as<uint64_t>(),
-        tmp.qualStart.as<uint64_t>(), tmp.lens.as<int>(), tmp.prefix.as<uint64_t>(), n, pitch, ctx->enc.as<uint32_t>(),
+        f1.text->as<char>(), paired ? f2.text->as<char>() : nullptr, paired, ctx->fqSeq.as<uint64_t>(),
+        ctx->fqQual.as<uint64_t>(), tmp.lens.as<int>(), tmp.prefix.as<uint64_t>(), n, pitch, ctx->enc.as<uint32_t>(),
         ctx->ambig.as<uint8_t>(), qual, ctx->qdpitch);
     HIPCHECK(hipGetLastError());
-    return finishReads(ctx);
+    rc = finishReads(ctx);
+    if(rc == CARE_OK) ctx->hasFastq = true;
+    return rc;
+}
+
+// ---------------------------------------------------------------- output merge
+//
+// The corrected FASTQ stream of one input file, as the reference hands it to QC
+// (constructOutputFileFromCorrectionResultsOutToQueue, src/correctionresultoutput.cpp:1589-1632):
+// records ordered by read id (sortSerializedResultsByReadIdAscending,
+// include/sortserializedresults.hpp:17-78), the records of each read combined with its
+// original FASTQ text (combineMultipleCorrectionResults1_rawtcs2, :1256-1550), and every
+// read written as "@" header [" care:q=N"] "\n" sequence "\n+\n" quality "\n" (:726-760).
+// On the device: a stable radix sort of (read id, record index), per-read runs of the
+// sorted records, one thread per read for the combine decision and the output size, a
+// scan of the sizes, and one wavefront per read writing its text.
+
+__device__ __forceinline__ uint32_t ldU32(const uint8_t* p){
+    return uint32_t(p[0]) | uint32_t(p[1]) << 8 | uint32_t(p[2]) << 16 | uint32_t(p[3]) << 24;
+}
+
+// One serialized record (EncodedTempCorrectedSequence, include/correctedsequence.hpp:151-387):
+// read id, flags (hq << 31 | useEdits << 30 | candidate << 29 | body bytes), then either
+// (int count, int positions[count], char bases[count]) or (int length, char sequence[length]),
+// then an int shift for candidate records.
+struct RecView {
+    const uint8_t* body;
+    uint32_t flags;
+    __device__ bool hq() const { return flags >> 31; }
+    __device__ bool useEdits() const { return (flags >> 30) & 1u; }
+    __device__ bool isCandidate() const { return (flags >> 29) & 1u; }
+    __device__ int count() const { return int(ldU32(body)); }
+    __device__ int editPos(int i) const { return int(ldU32(body + 4 + 4 * i)); }
+    __device__ uint8_t editBase(int ne, int i) const { return body[4 + 4 * ne + i]; }
+    __device__ uint8_t seqByte(int i) const { return body[4 + i]; }
+};
+
+__device__ __forceinline__ RecView recAt(const uint8_t* recs, const uint64_t* off, uint32_t idx){
+    const uint8_t* p = recs + off[idx];
+    return RecView{p + 8, ldU32(p + 4)};
+}
+
+// isEqualSequenceOneHasEdits (src/correctionresultoutput.cpp:1327-1356), step for step
+__device__ bool recEqualOneHasEdits(const uint8_t* orig, int L, const RecView& e, const RecView& q){
+    const int ne = e.count();
+    int k = 0, pos = 0;
+    while(pos != L && k != ne){
+        if(pos != e.editPos(k)){
+            if(orig[pos] != q.seqByte(pos)) return false;
+        }else{
+            if(e.editBase(ne, k) != q.seqByte(pos)) return false;
+            ++k;
+        }
+        ++pos;
+    }
+    for(; pos != L; ++pos) if(orig[pos] != q.seqByte(pos)) return false;
+    return true;
+}
+
+// isEqualSequence (src/correctionresultoutput.cpp:1358-1384)
+__device__ bool recEqual(const uint8_t* orig, int L, const RecView& a, const RecView& b){
+    const int la = a.useEdits() ? L : a.count();
+    const int lb = b.useEdits() ? L : b.count();
+    if(la != lb) return false;
+    if(a.useEdits() && b.useEdits()){
+        const int na = a.count(), nb = b.count();
+        if(na != nb) return false;
+        for(int i = 0; i < na; i++)
+            if(a.editPos(i) != b.editPos(i) || a.editBase(na, i) != b.editBase(nb, i)) return false;
+        return true;
+    }
+    if(!a.useEdits() && !b.useEdits()){
+        for(int i = 0; i < la; i++) if(a.seqByte(i) != b.seqByte(i)) return false;
+        return true;
+    }
+    const RecView& e = a.useEdits() ? a : b;
+    const RecView& q = a.useEdits() ? b : a;
+    if(e.count() == 0){
+        for(int i = 0; i < L; i++) if(orig[i] != q.seqByte(i)) return false;
+        return true;
+    }
+    return recEqualOneHasEdits(orig, L, e, q);
+}
+
+struct MergeParams {
+    const uint8_t* recs; uint64_t recBytes; const uint64_t* off; uint32_t m;
+    const uint32_t* keysSorted; const uint32_t* idxSorted;
+    const uint32_t* runBeg; const uint32_t* runEnd;
+    const char* text; const uint64_t* hdr; const uint32_t* hdrLen; const uint64_t* seq; const uint64_t* qual;
+    const int* len;
+    uint64_t n, count; int stride, file, labels;
+    int2* plan; uint64_t* sizes; const uint64_t* outOff; char* out;
+};
+
+// Record headers: read id keys, identity values, and a flag for records that do not fit
+// the buffer or name a read past the storage.
+__global__ void k_merge_keys(const uint8_t* __restrict__ recs, uint64_t recBytes, const uint64_t* __restrict__ off,
+                             uint32_t m, uint64_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ idx,
+                             int* __restrict__ err){
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i >= m) return;
+    const uint64_t o = off[i];
+    uint32_t id = 0;
+    bool ok = o + 8 <= recBytes;
+    if(ok){
+        id = ldU32(recs + o);
+        const uint32_t flags = ldU32(recs + o + 4);
+        const uint64_t nb = flags & ((1u << 29) - 1);
+        ok = o + 8 + nb <= recBytes && nb >= 4 && uint64_t(id) < n;
+        if(ok){
+            const int64_t c = int32_t(ldU32(recs + o + 8));
+            const int64_t need = 4 + ((flags >> 30) & 1u ? 5 * c : c) + ((flags >> 29) & 1u ? 4 : 0);
+            ok = c >= 0 && need <= int64_t(nb);
+        }
+    }
+    keys[i] = id;
+    idx[i] = i;
+    if(!ok) atomicOr(err, 1);
+}
+
+__global__ void k_fill_u32(uint32_t* __restrict__ v, uint64_t n, uint32_t x){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < n) v[i] = x;
+}
+
+// [runBeg[r], runEnd[r]) = positions of read r's records in the sorted order
+__global__ void k_merge_runs(const uint32_t* __restrict__ keys, uint32_t m, uint32_t* __restrict__ runBeg,
+                             uint32_t* __restrict__ runEnd){
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= m) return;
+    const uint32_t r = keys[j];
+    if(j == 0 || keys[j - 1] != r) runBeg[r] = j;
+    if(j == m - 1 || keys[j + 1] != r) runEnd[r] = j + 1;
+}
+
+constexpr int kLabelBytes = 9;     // " care:q=N"
+
+// One thread per read of the file: the combine decision (record index or -1, label) and
+// the size of the read's output text.
+__global__ void k_merge_plan(MergeParams p){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i >= p.count) return;
+    const uint64_t r = uint64_t(p.stride) * i + uint64_t(p.file);
+    const uint32_t b = p.runBeg[r], e = p.runEnd[r];
+    const int L = p.len[r];
+    int choice = -1, label = 0;
+    if(b < e){
+        int anchor = -1;
+        for(uint32_t j = b; j < e; j++){
+            if(!recAt(p.recs, p.off, p.idxSorted[j]).isCandidate()){ anchor = int(j); break; }
+        }
+        if(anchor >= 0){
+            const RecView an = recAt(p.recs, p.off, p.idxSorted[anchor]);
+            if(an.hq()){
+                choice = int(p.idxSorted[anchor]); label = 3;
+            }else if(e - b >= 3){
+                const uint8_t* orig = reinterpret_cast<const uint8_t*>(p.text + p.seq[r]);
+                const RecView first = recAt(p.recs, p.off, p.idxSorted[b]);
+                bool same = true;
+                for(uint32_t j = b + 1; j < e && same; j++) same = recEqual(orig, L, first, recAt(p.recs, p.off, p.idxSorted[j]));
+                if(same){ choice = int(p.idxSorted[b]); label = 2; }
+            }else{
+                choice = int(p.idxSorted[anchor]); label = 1;
+            }
+        }
+    }
+    // the reference stops visiting reads after the last record's read: no label there
+    const bool labelled = p.labels && r <= p.keysSorted[p.m - 1];
+    int seqLen = L;
+    if(choice >= 0){
+        const RecView c = recAt(p.recs, p.off, uint32_t(choice));
+        if(!c.useEdits()) seqLen = c.count();
+    }
+    p.plan[i] = make_int2(choice, label | (labelled ? 0x100 : 0));
+    p.sizes[i] = 1 + uint64_t(p.hdrLen[r]) + (labelled ? kLabelBytes : 0) + 1 + uint64_t(seqLen) + 3 + uint64_t(L) + 1;
+}
+
+// One wavefront per read: lanes write consecutive bytes of the read's output text.
+__global__ void __launch_bounds__(256) k_merge_write(MergeParams p){
+    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if(i >= p.count) return;
+    const uint64_t r = uint64_t(p.stride) * i + uint64_t(p.file);
+    const int2 pl = p.plan[i];
+    const int hl = int(p.hdrLen[r]);
+    const int lb = (pl.y & 0x100) ? kLabelBytes : 0;
+    const int L = p.len[r];
+    const uint8_t* hdr = reinterpret_cast<const uint8_t*>(p.text + p.hdr[r]);
+    const uint8_t* orig = reinterpret_cast<const uint8_t*>(p.text + p.seq[r]);
+    const uint8_t* qual = reinterpret_cast<const uint8_t*>(p.text + p.qual[r]);
+    RecView c{nullptr, 0};
+    int ne = 0, sl = L;
+    const bool take = pl.x >= 0;
+    if(take){
+        c = recAt(p.recs, p.off, uint32_t(pl.x));
+        if(c.useEdits()) ne = c.count(); else sl = c.count();
+    }
+    const bool full = take && !c.useEdits();
+    char* out = p.out + p.outOff[i];
+    const uint64_t total = p.outOff[i + 1] - p.outOff[i];
+    const int s0 = 2 + hl + lb;                  // first sequence byte
+    for(uint64_t k = lane; k < total; k += 64){
+        const int kk = int(k);
+        uint8_t ch;
+        if(kk == 0) ch = '@';
+        else if(kk <= hl) ch = hdr[kk - 1];
+        else if(kk <= hl + lb){
+            const int t = kk - 1 - hl;
+            ch = t < kLabelBytes - 1 ? uint8_t(" care:q="[t]) : uint8_t('0' + (pl.y & 0xff));
+        }else if(kk == hl + lb + 1) ch = '\n';
+        else{
+            const int q = kk - s0;
+            if(q < sl){
+                if(full) ch = c.seqByte(q);
+                else{
+                    ch = orig[q];
+                    for(int x = 0; x < ne; x++) if(c.editPos(x) == q) ch = c.editBase(ne, x);
+                }
+            }else if(q < sl + 3) ch = (q == sl + 1) ? '+' : '\n';
+            else if(q < sl + 3 + L) ch = qual[q - sl - 3];
+            else ch = '\n';
+        }
+        out[k] = char(ch);
+    }
+}
+
+int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file, int quality_labels,
+                         char* dst, size_t cap, size_t* written){
+    if(!ctx || !written) return CARE_EINVAL;
+    *written = 0;
+    if(!ctx->hasFastq) return ctx->fail(CARE_ESTATE, "the output merge needs reads set by care_hip_set_reads_fastq");
+    const bool paired = ctx->o.paired != 0;
+    if(file < 0 || file > (paired ? 1 : 0)) return ctx->fail(CARE_EINVAL, "file %d out of range", file);
+    if(record_bytes && !records) return ctx->fail(CARE_EINVAL, "null records");
+    // record boundaries (the serialized stream carries its own sizes)
+    std::vector<uint64_t> off;
+    for(size_t p = 0; p < record_bytes;){
+        if(p + 8 > record_bytes) return ctx->fail(CARE_EINVAL, "truncated record at byte %zu", p);
+        uint32_t flags;
+        std::memcpy(&flags, records + p + 4, 4);
+        const size_t len = 8 + (flags & ((1u << 29) - 1));
+        if(p + len > record_bytes) return ctx->fail(CARE_EINVAL, "truncated record at byte %zu", p);
+        off.push_back(p);
+        p += len;
+    }
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const uint64_t m64 = off.size();
+    if(m64 == 0){
+        // no records: the reference hands QC nothing and QC reads the input text itself
+        const size_t raw = ctx->fqRaw[file];
+        *written = raw;
+        if(!dst) return CARE_OK;
+        if(cap < raw) return ctx->fail(CARE_EINVAL, "output needs %zu bytes, cap is %zu", raw, cap);
+        if(raw) HIPCHECK(hipMemcpy(dst, ctx->fqText[file].p, raw, hipMemcpyDeviceToHost));
+        return CARE_OK;
+    }
+    if(m64 >= uint64_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "too many records");
+    const uint32_t m = uint32_t(m64);
+    const uint64_t n = ctx->n;
+    const uint64_t count = paired ? n / 2 : n;
+    HIPCHECK(ctx->mRec.ensure(record_bytes));
+    HIPCHECK(hipMemcpyAsync(ctx->mRec.p, records, record_bytes, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->mOff.ensure(m64 * 8));
+    HIPCHECK(hipMemcpyAsync(ctx->mOff.p, off.data(), m64 * 8, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->mKeyIn.ensure(m64 * 4)); HIPCHECK(ctx->mKeyOut.ensure(m64 * 4));
+    HIPCHECK(ctx->mIdxIn.ensure(m64 * 4)); HIPCHECK(ctx->mIdxOut.ensure(m64 * 4));
+    HIPCHECK(ctx->mErr.ensure(4));
+    k_fill_u32<<<1, 64, 0, st>>>(ctx->mErr.as<uint32_t>(), 1, 0u);
+    const uint32_t mb = (m + 255) / 256;
+    k_merge_keys<<<mb, 256, 0, st>>>(ctx->mRec.as<uint8_t>(), record_bytes, ctx->mOff.as<uint64_t>(), m, n,
+                                     ctx->mKeyIn.as<uint32_t>(), ctx->mIdxIn.as<uint32_t>(), ctx->mErr.as<int>());
+    HIPCHECK(hipGetLastError());
+    int endBit = 1;
+    while(endBit < 32 && (uint64_t(1) << endBit) < n) endBit++;
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ctx->mKeyIn.as<uint32_t>(), ctx->mKeyOut.as<uint32_t>(),
+                                                ctx->mIdxIn.as<uint32_t>(), ctx->mIdxOut.as<uint32_t>(), int(m), 0, endBit, st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceRadixSort::SortPairs(ctx->scanTmp.p, tb, ctx->mKeyIn.as<uint32_t>(), ctx->mKeyOut.as<uint32_t>(),
+                                                ctx->mIdxIn.as<uint32_t>(), ctx->mIdxOut.as<uint32_t>(), int(m), 0, endBit, st));
+    HIPCHECK(ctx->mRunBeg.ensure(n * 4)); HIPCHECK(ctx->mRunEnd.ensure(n * 4));
+    const uint32_t nb = uint32_t((n + 255) / 256);
+    k_fill_u32<<<nb, 256, 0, st>>>(ctx->mRunBeg.as<uint32_t>(), n, 0u);
+    k_fill_u32<<<nb, 256, 0, st>>>(ctx->mRunEnd.as<uint32_t>(), n, 0u);
+    int err = 0;
+    HIPCHECK(hipMemcpyAsync(&err, ctx->mErr.p, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if(err) return ctx->fail(CARE_EINVAL, "malformed record (sizes, or a read id past the %llu reads)",
+                             (unsigned long long)n);
+    k_merge_runs<<<mb, 256, 0, st>>>(ctx->mKeyOut.as<uint32_t>(), m, ctx->mRunBeg.as<uint32_t>(), ctx->mRunEnd.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(ctx->mPlan.ensure(count * sizeof(int2)));
+    HIPCHECK(ctx->mSize.ensure((count + 1) * 8));
+    HIPCHECK(ctx->mOutOff.ensure((count + 1) * 8));
+    MergeParams mp{};
+    mp.recs = ctx->mRec.as<uint8_t>(); mp.recBytes = record_bytes; mp.off = ctx->mOff.as<uint64_t>(); mp.m = m;
+    mp.keysSorted = ctx->mKeyOut.as<uint32_t>(); mp.idxSorted = ctx->mIdxOut.as<uint32_t>();
+    mp.runBeg = ctx->mRunBeg.as<uint32_t>(); mp.runEnd = ctx->mRunEnd.as<uint32_t>();
+    mp.text = ctx->fqText[file].as<char>(); mp.hdr = ctx->fqHdr.as<uint64_t>(); mp.hdrLen = ctx->fqHdrLen.as<uint32_t>();
+    mp.seq = ctx->fqSeq.as<uint64_t>(); mp.qual = ctx->fqQual.as<uint64_t>(); mp.len = ctx->len.as<int>();
+    mp.n = n; mp.count = count; mp.stride = paired ? 2 : 1; mp.file = file; mp.labels = quality_labels != 0;
+    mp.plan = ctx->mPlan.as<int2>(); mp.sizes = ctx->mSize.as<uint64_t>(); mp.outOff = ctx->mOutOff.as<uint64_t>();
+    const uint32_t cb = uint32_t((count + 255) / 256);
+    k_merge_plan<<<cb, 256, 0, st>>>(mp);
+    HIPCHECK(hipGetLastError());
+    tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->mSize.as<uint64_t>(), ctx->mOutOff.as<uint64_t>(),
+                                              int(count + 1), st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    // the last size is a zero written here, so outOff[count] is the total
+    k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(ctx->mSize.as<uint64_t>() + count), 2, 0u);
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->mSize.as<uint64_t>(), ctx->mOutOff.as<uint64_t>(),
+                                              int(count + 1), st));
+    uint64_t total = 0;
+    HIPCHECK(hipMemcpyAsync(&total, ctx->mOutOff.as<uint64_t>() + count, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    *written = total;
+    if(!dst) return CARE_OK;
+    if(cap < total) return ctx->fail(CARE_EINVAL, "output needs %llu bytes, cap is %zu", (unsigned long long)total, cap);
+    HIPCHECK(ctx->mText.ensure(std::max<uint64_t>(total, 1)));
+    mp.out = ctx->mText.as<char>();
+    const uint64_t wb = (count * 64 + 255) / 256;
+    k_merge_write<<<dim3(uint32_t(wb)), 256, 0, st>>>(mp);
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipMemcpyAsync(dst, ctx->mText.p, total, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    return CARE_OK;
 }
 
 int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints){

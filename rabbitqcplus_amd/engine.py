@@ -59,7 +59,7 @@ EXPORTS = [
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
     "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
     "care_hip_save_tables", "care_hip_load_tables", "care_hip_set_reads_fastq", "care_hip_read_layout",
-    "care_hip_inspect_reads",
+    "care_hip_inspect_reads", "care_hip_write_fastq",
     "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
     "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
 ]
@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_set_reads_fastq.argtypes = [vp, vp, sz, vp, sz]
     lib.care_hip_read_layout.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(i32), ctypes.POINTER(sz)]
     lib.care_hip_inspect_reads.argtypes = [vp, u64, u64, vp, vp, vp, sz, vp]
+    lib.care_hip_write_fastq.argtypes = [vp, vp, sz, i32, i32, vp, sz, ctypes.POINTER(sz)]
     lib.care_hip_build_tables.argtypes = [vp, ctypes.POINTER(i32)]
     lib.care_hip_correct.argtypes = [vp, u64, u64, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     lib.care_hip_copy_records.argtypes = [vp, vp, sz]
@@ -204,6 +205,18 @@ class CareEngine:
                                                None if t2 is None else t2.ctypes.data, 0 if t2 is None else t2.size)
         self._check(rc, "care_hip_set_reads_fastq")
         return self.read_layout()[0]
+
+    def write_fastq(self, records: bytes, file: int = 0, quality_labels: bool = False) -> bytes:
+        """Corrected FASTQ text of input file `file` from serialized records (output merge
+        on the device; needs reads set by set_reads_fastq)."""
+        rec = np.frombuffer(records, np.uint8) if len(records) else np.zeros(1, np.uint8)
+        size = ctypes.c_size_t(0)
+        self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file, int(quality_labels),
+                                                  None, 0, ctypes.byref(size)), "care_hip_write_fastq")
+        out = np.empty(max(size.value, 1), np.uint8)
+        self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file, int(quality_labels),
+                                                  out.ctypes.data, out.size, ctypes.byref(size)), "care_hip_write_fastq")
+        return out[:size.value].tobytes()
 
     def read_layout(self) -> tuple[int, int, int]:
         """(number of reads, longest read, pitch in u32 words) of the resident storage."""
