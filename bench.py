@@ -234,6 +234,32 @@ def main():
     log(f"GPU plain: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
         f"errors {plain_err}")
 
+    # FASTQ text -> device read storage, and records -> corrected FASTQ text (the stages
+    # either side of the correction pass; each timed once)
+    texts = sim.fastq_texts(s)
+    text_bytes = sum(len(t) for t in texts)
+    eng = engine.CareEngine(0, **opts)
+    ta = time.time()
+    eng.set_reads_fastq(*texts)
+    tb = time.time()
+    outs = [eng.write_fastq(data, f) for f in range(len(texts))]
+    tc = time.time()
+    same_storage = bool(np.array_equal(eng.inspect_reads()["enc"], rs.enc))
+    eng.write_fastq(data, 0)
+    tio = eng.timings()
+    eng.close()
+    out_bytes = sum(len(o) for o in outs)
+    io_line = {
+        "fastq_bytes": text_bytes, "storage_equal_to_array_path": same_storage,
+        "parse_device_ms": round(tio["fastq_device_ms"], 3), "parse_wall_ms": round(1e3 * (tb - ta), 2),
+        "parse_device_gbps": round(text_bytes / (tio["fastq_device_ms"] / 1e3) / 1e9, 2),
+        "merge_output_bytes": out_bytes, "merge_device_ms_file0": round(tio["merge_device_ms"], 3),
+        "merge_wall_ms_all_files": round(1e3 * (tc - tb), 2),
+    }
+    log(f"FASTQ parse {text_bytes / 1e6:.1f} MB: device {tio['fastq_device_ms']:.2f} ms, wall {1e3 * (tb - ta):.1f} ms; "
+        f"merge device {tio['merge_device_ms']:.2f} ms (file 0), wall {1e3 * (tc - tb):.1f} ms")
+    del texts, outs
+
     learned_line = None
     if not args.no_learned and os.path.exists(learned.DEFAULT_CLASSIFIER) and rs.max_length <= 340:
         params, _ = learned.load_classifier()
@@ -283,6 +309,7 @@ def main():
         "errors": plain_err,
         "learned": learned_line,
         "records": nrec,
+        "fastq_io": io_line,
         "table_build_ms": round(1e3 * (t3 - t2), 2),
         "stage_ms": {"candidates": round(t["candidates_ms"], 3), "align": round(t["align_ms"], 3),
                      "msa": round(t["msa_ms"], 3), "output": round(t["output_ms"], 3)},

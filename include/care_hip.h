@@ -88,6 +88,9 @@ typedef struct care_timings {
     uint64_t bucket_values;   /* values gathered from the tables */
     uint64_t batches;
     uint64_t graphs_built;    /* HIP graphs captured and instantiated by this context */
+    double fastq_device_ms;   /* last care_hip_set_reads_fastq: device span after the text upload */
+    double merge_device_ms;   /* last care_hip_write_fastq: device span from the record upload to the text
+                                 download (both excluded) */
 } care_timings;
 
 void care_hip_default_opts(care_opts* o);
@@ -115,11 +118,13 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
  * constructChunkedReadStorageFromFiles, include/chunkedreadstorageconstruction.hpp:29-480,
  * with its reader include/readlibraryio.hpp): one text for single-end input, two for
  * paired input (records of the two texts interleaved R1, R2, R1, ...). Records are
- * 4-line FASTQ ("@header", sequence, "+...", quality; a '\r' before '\n' is dropped;
- * blank lines at the end are ignored; at most 2 GiB per text). Lower-case acgt become
- * upper case; every other sequence character becomes A, C, G, T in turn, the turn
- * restarting at every batch of 65,536 reads, and marks the read ambiguous. Qualities are
- * kept as given (8 bits) when use_quality_scores is set. Pitch = ceil(longest/16). */
+ * 4-line FASTQ ("@header", sequence, "+...", quality; a '\r' ending a header, sequence
+ * or quality of more than one character is dropped, as the reference's reader does,
+ * include/kseqpp/kseqpp.hpp:286-288; blank lines at the end are ignored; at most 2 GiB
+ * per text). Lower-case acgt become upper case; every other sequence character becomes
+ * A, C, G, T in turn, the turn restarting at every batch of 65,536 reads, and marks the
+ * read ambiguous. Qualities are kept as given (8 bits) when use_quality_scores is set.
+ * Pitch = ceil(longest/16). The text stays on the device for care_hip_write_fastq. */
 int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1,
                              const char* fastq2, size_t bytes2);
 

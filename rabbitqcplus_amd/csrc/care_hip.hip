@@ -245,7 +245,7 @@ k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ l
     out[3 * W + w] = rl;
 }
 
-// Rows of `sp` bytes to rows of `dp` >= sp bytes, zero padded.
+// Rows of `sp` bytes to rows of `dp` bytes: truncated, or zero padded past sp.
 __global__ void k_repack_rows(const uint8_t* __restrict__ src, size_t sp, uint8_t* __restrict__ dst, size_t dp,
                               uint64_t total){
     const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -2231,6 +2231,11 @@ k_fused(FusedParams fp){
     }
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ v, uint64_t n, uint32_t x){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < n) v[i] = x;
+}
+
 __global__ void k_zero_u32(uint32_t* v, int n){
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i < n) v[i] = 0u;
@@ -2315,6 +2320,7 @@ struct care_ctx {
     DBuf fqText[2], fqHdr, fqHdrLen, fqSeq, fqQual;
     size_t fqRaw[2] = {0, 0};
     bool hasFastq = false;
+    double fastqMs = 0, mergeMs = 0;     // device spans of the last FASTQ parse and output merge
     // output merge working memory
     DBuf mRec, mOff, mKeyIn, mKeyOut, mIdxIn, mIdxOut, mRunBeg, mRunEnd, mPlan, mSize, mOutOff, mText, mErr;
 
@@ -2967,7 +2973,7 @@ int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_i
 }
 
 // Quality-weight table and HiLo planes of the resident reads.
-int finishReads(care_ctx* ctx){
+int finishReads(care_ctx* ctx, hipEvent_t done = nullptr){
     hipStream_t st = ctx->stream;
     float qw[256];
     for(int i = 0; i < 256; i++) qw[i] = qualityWeight(i);
@@ -2979,6 +2985,7 @@ int finishReads(care_ctx* ctx){
     k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
                                                                   num_reads, ctx->W, ctx->hilo.as<uint32_t>());
     HIPCHECK(hipGetLastError());
+    if(done) HIPCHECK(hipEventRecord(done, st));
     HIPCHECK(hipStreamSynchronize(st));
     ctx->hasReads = true;
     ctx->hasTables = false;
@@ -3052,9 +3059,66 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
 
 constexpr uint64_t kParserBatch = 65536;
 
-__global__ void k_newline_flags(const char* __restrict__ t, uint64_t n, uint8_t* __restrict__ f){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i < n) f[i] = t[i] == '\n';
+// Newline positions by a two-pass compaction: every thread takes 16 bytes of the text;
+// pass 1 counts the '\n' bytes per block, a scan of the block counts gives each block its
+// first output slot, pass 2 recounts and writes the positions in text order. The text
+// buffer is padded to a multiple of 16 bytes, so the 16-byte loads stay inside it.
+constexpr int kNlBlock = 256;
+constexpr int kNlPerBlock = 16 * kNlBlock;
+
+__device__ __forceinline__ uint32_t newlineMask(const char* __restrict__ t, uint64_t base, uint64_t bytes){
+    const uint4 v = *reinterpret_cast<const uint4*>(t + base);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t mask = 0;
+    #pragma unroll
+    for(int k = 0; k < 16; k++){
+        const bool nlb = ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) == uint32_t('\n');
+        if(nlb && base + k < bytes) mask |= 1u << k;
+    }
+    return mask;
+}
+
+// Block-wide exclusive prefix of one value per thread (kNlBlock threads); returns the
+// prefix and sets *total.
+__device__ __forceinline__ int blockExclusiveSum(int v, int* sWave, int* total){
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int incl = waveInclusiveSum(v);
+    if(lane == 63) sWave[wave] = incl;
+    __syncthreads();
+    int before = 0, all = 0;
+    #pragma unroll
+    for(int w = 0; w < kNlBlock / 64; w++){
+        const int x = sWave[w];
+        before += w < wave ? x : 0;
+        all += x;
+    }
+    *total = all;
+    return before + incl - v;
+}
+
+__global__ void __launch_bounds__(kNlBlock) k_newline_count(const char* __restrict__ t, uint64_t bytes,
+                                                            uint32_t* __restrict__ blockCnt){
+    __shared__ int sWave[kNlBlock / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kNlPerBlock + 16ull * threadIdx.x;
+    const int c = base < bytes ? __popc(newlineMask(t, base, bytes)) : 0;
+    int total;
+    blockExclusiveSum(c, sWave, &total);
+    if(threadIdx.x == 0) blockCnt[blockIdx.x] = uint32_t(total);
+}
+
+__global__ void __launch_bounds__(kNlBlock) k_newline_write(const char* __restrict__ t, uint64_t bytes,
+                                                            const uint32_t* __restrict__ blockOff,
+                                                            uint32_t* __restrict__ nl){
+    __shared__ int sWave[kNlBlock / 64];
+    const uint64_t base = uint64_t(blockIdx.x) * kNlPerBlock + 16ull * threadIdx.x;
+    uint32_t mask = base < bytes ? newlineMask(t, base, bytes) : 0u;
+    int total;
+    uint32_t o = blockOff[blockIdx.x] + uint32_t(blockExclusiveSum(__popc(mask), sWave, &total));
+    while(mask){
+        const int k = __ffs(mask) - 1;
+        nl[o++] = uint32_t(base + k);
+        mask &= mask - 1;
+    }
 }
 
 __device__ __forceinline__ int fastqCode(char c){
@@ -3067,18 +3131,37 @@ __device__ __forceinline__ int fastqCode(char c){
     }
 }
 
+// Inclusive sum over the 16 lanes of a DPP row.
+__device__ __forceinline__ int rowInclusiveSum(int v){
+    int t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false); v += t;
+    t = __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false); v += t;
+    return v;
+}
+
+__device__ __forceinline__ int rowBroadcastLast(int v){
+    return __shfl(v, (threadIdx.x & 63 & ~15) | 15, 64);
+}
+
+constexpr int kFqBlock = 256;        // 16 records (or reads) per block, one 16-lane row each
+
 // Record r of one file: lines 4r..4r+3; line i spans [i ? nl[i-1]+1 : 0, i < numNl ? nl[i] : bytes).
-__global__ void k_fastq_records(const char* __restrict__ t, uint64_t bytes, const uint64_t* __restrict__ nl,
-                                uint64_t numNl, uint64_t records, int stride, int slot,
-                                uint64_t* __restrict__ hdrStart, uint32_t* __restrict__ hdrLen,
-                                uint64_t* __restrict__ seqStart, uint64_t* __restrict__ qualStart,
-                                int* __restrict__ len, uint64_t* __restrict__ ncount, int* __restrict__ err){
-    const uint64_t rec = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(rec >= records) return;
-    auto lineStart = [&](uint64_t i){ return i ? nl[i - 1] + 1 : uint64_t(0); };
+// One 16-lane row per record: every lane finds the lines, the lanes count the sequence's
+// non-ACGT bytes over 16-byte slices, and lane 0 writes the record.
+__global__ void __launch_bounds__(kFqBlock) k_fastq_records(
+        const char* __restrict__ t, uint64_t bytes, const uint32_t* __restrict__ nl, uint64_t numNl, uint64_t records,
+        int stride, int slot, uint64_t* __restrict__ hdrStart, uint32_t* __restrict__ hdrLen,
+        uint64_t* __restrict__ seqStart, uint64_t* __restrict__ qualStart, int* __restrict__ len,
+        uint64_t* __restrict__ ncount, int* __restrict__ err){
+    const uint64_t rec = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
+    if(rec >= records) return;                   // whole rows leave together
+    auto lineStart = [&](uint64_t i){ return i ? uint64_t(nl[i - 1]) + 1 : uint64_t(0); };
     // end of the content starting at s on line i
     auto contentEnd = [&](uint64_t i, uint64_t s){
-        uint64_t e = i < numNl ? nl[i] : bytes;
+        uint64_t e = i < numNl ? uint64_t(nl[i]) : bytes;
         if(e > s + 1 && t[e - 1] == '\r') e--;
         return e;
     };
@@ -3086,99 +3169,117 @@ __global__ void k_fastq_records(const char* __restrict__ t, uint64_t bytes, cons
     const uint64_t h = lineStart(l0);
     const uint64_t ss = lineStart(l0 + 1), se = contentEnd(l0 + 1, ss);
     const uint64_t ps = lineStart(l0 + 2), qs = lineStart(l0 + 3), qe = contentEnd(l0 + 3, qs);
+    int nc = 0;
+    for(uint64_t p = ss + 16 * sub; p < se; p += 256){
+        const int m = se - p < 16 ? int(se - p) : 16;
+        for(int k = 0; k < m; k++) nc += fastqCode(t[p + k]) < 0;
+    }
+    nc = rowBroadcastLast(rowInclusiveSum(nc));
+    if(sub != 0) return;
     const uint64_t r = uint64_t(stride) * rec + uint64_t(slot);
     const bool ok = h < bytes && t[h] == '@' && ps < bytes && t[ps] == '+' && qe - qs == se - ss;
     const uint64_t he = ok ? contentEnd(l0, h + 1) : h;
-    uint64_t nc = 0;
-    for(uint64_t p = ss; p < se; p++) nc += fastqCode(t[p]) < 0;
     hdrStart[r] = h + 1;
     hdrLen[r] = uint32_t(ok ? he - (h + 1) : 0);
     seqStart[r] = ss;
     qualStart[r] = qs;
     len[r] = int(se - ss);
-    ncount[r] = nc;
+    ncount[r] = uint64_t(nc);
     if(!ok) atomicOr(err, 1);
 }
 
-// One thread per read: 2-bit row (MSB first, 16 bases per word, zero-padded to the pitch),
-// ambiguity flag, qualities zero-padded to the device pitch.
-__global__ void k_fastq_encode(const char* __restrict__ t1, const char* __restrict__ t2, int paired,
-                               const uint64_t* __restrict__ seqStart, const uint64_t* __restrict__ qualStart,
-                               const int* __restrict__ len, const uint64_t* __restrict__ prefix, uint64_t n,
-                               size_t pitch, uint32_t* __restrict__ enc, uint8_t* __restrict__ ambig,
-                               uint8_t* __restrict__ qual, size_t qdpitch){
-    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+// One 16-lane row per read: each lane encodes a run of the read's 2-bit words (MSB first,
+// 16 bases per word, zero-padded to the pitch) with its replacement counter taken from the
+// read's counter start plus the non-ACGT bytes of the lanes before it; then the lanes copy
+// the qualities in 16-byte chunks, zero-padded to the device pitch.
+__global__ void __launch_bounds__(kFqBlock) k_fastq_encode(
+        const char* __restrict__ t1, const char* __restrict__ t2, int paired, const uint64_t* __restrict__ seqStart,
+        const uint64_t* __restrict__ qualStart, const int* __restrict__ len, const uint64_t* __restrict__ prefix,
+        uint64_t n, size_t pitch, uint32_t* __restrict__ enc, uint8_t* __restrict__ ambig,
+        uint8_t* __restrict__ qual, size_t qdpitch){
+    const uint64_t r = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4;
+    const int sub = threadIdx.x & 15;
     if(r >= n) return;
     const char* t = (paired && (r & 1)) ? t2 : t1;
     const int L = len[r];
-    const uint64_t b0 = r & ~(kParserBatch - 1);
-    int counter = int((prefix[r] - prefix[b0]) & 3);
     const char* sq = t + seqStart[r];
-    bool amb = false;
+    const int wpl = int((pitch + 15) / 16);
+    const int w0 = sub * wpl, w1 = min(int(pitch), w0 + wpl);
+    int cnt = 0;
+    for(int pos = w0 * 16; pos < min(L, w1 * 16); pos++) cnt += fastqCode(sq[pos]) < 0;
+    const int incl = rowInclusiveSum(cnt);
+    const int any = rowBroadcastLast(incl);
+    const uint64_t b0 = r & ~(kParserBatch - 1);
+    int counter = int((prefix[r] - prefix[b0] + uint64_t(incl - cnt)) & 3);
     uint32_t* row = enc + pitch * r;
-    for(size_t w = 0; w < pitch; w++){
+    for(int w = w0; w < w1; w++){
         uint32_t word = 0;
         for(int j = 0; j < 16; j++){
-            const int pos = int(w) * 16 + j;
+            const int pos = w * 16 + j;
             if(pos >= L) break;
             int code = fastqCode(sq[pos]);
-            if(code < 0){ code = counter; counter = (counter + 1) & 3; amb = true; }
+            if(code < 0){ code = counter; counter = (counter + 1) & 3; }
             word |= uint32_t(code) << (30 - 2 * j);
         }
         row[w] = word;
     }
-    ambig[r] = amb;
+    if(sub == 0) ambig[r] = any > 0;
     if(qual){
         const char* qs = t + qualStart[r];
         uint8_t* qrow = qual + qdpitch * r;
-        for(size_t j = 0; j < qdpitch; j++) qrow[j] = j < size_t(L) ? uint8_t(qs[j]) : uint8_t(0);
+        for(int c = sub; c < int(qdpitch / 16); c += 16){
+            uint32_t w[4] = {0, 0, 0, 0};
+            #pragma unroll
+            for(int k = 0; k < 16; k++){
+                const int idx = 16 * c + k;
+                if(idx < L) w[k >> 2] |= uint32_t(uint8_t(qs[idx])) << (8 * (k & 3));
+            }
+            *reinterpret_cast<uint4*>(qrow + 16 * c) = make_uint4(w[0], w[1], w[2], w[3]);
+        }
     }
 }
 
 namespace {
 struct ParsedFile {
     DBuf* text = nullptr;          // the context's resident copy of the text
-    DBuf flags, nl, count;
+    DBuf blockCnt, blockOff, nl;
     uint64_t bytes = 0, numNl = 0, records = 0;
-    void release(){ flags.release(); nl.release(); count.release(); }
+    void release(){ blockCnt.release(); blockOff.release(); nl.release(); }
 };
 }  // namespace
 
 static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t raw, int slot, ParsedFile& f){
     hipStream_t st = ctx->stream;
-    if(raw > size_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "FASTQ text above 2 GiB in one call");
     f.text = &ctx->fqText[slot];
-    ctx->fqRaw[slot] = raw;
-    HIPCHECK(f.text->ensure(raw + 1));
-    if(raw) HIPCHECK(hipMemcpyAsync(f.text->p, data, raw, hipMemcpyHostToDevice, st));
     // blank lines at the end of the text are not records (the reference's reader skips them)
     size_t bytes = raw;
     while(bytes && (data[bytes - 1] == '\n' || data[bytes - 1] == '\r')) bytes--;
     f.bytes = bytes;
     if(bytes == 0){ f.numNl = 0; f.records = 0; return CARE_OK; }
-    HIPCHECK(f.flags.ensure(bytes + 1));
-    k_newline_flags<<<dim3(uint32_t((bytes + 255) / 256)), 256, 0, st>>>(f.text->as<char>(), bytes, f.flags.as<uint8_t>());
+    const uint32_t nb = uint32_t((bytes + kNlPerBlock - 1) / kNlPerBlock);
+    HIPCHECK(f.blockCnt.ensure((size_t(nb) + 1) * 4));
+    HIPCHECK(f.blockOff.ensure((size_t(nb) + 1) * 4));
+    k_newline_count<<<nb, kNlBlock, 0, st>>>(f.text->as<char>(), bytes, f.blockCnt.as<uint32_t>());
     HIPCHECK(hipGetLastError());
-    HIPCHECK(f.count.ensure(16));
-    hipcub::CountingInputIterator<uint64_t> it(0);
+    k_fill_u32<<<1, 64, 0, st>>>(f.blockCnt.as<uint32_t>() + nb, 1, 0u);
     size_t tb = 0;
-    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, it, f.flags.as<uint8_t>(), (uint64_t*)nullptr,
-                                           f.count.as<uint64_t>(), int(bytes), st));
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, f.blockCnt.as<uint32_t>(), f.blockOff.as<uint32_t>(),
+                                              int(nb + 1), st));
     HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
-    // newlines <= bytes; the positions are written first into the flags' own size of u64
-    HIPCHECK(f.nl.ensure((bytes + 1) * sizeof(uint64_t) / 8 * 8 + 8));
     tb = ctx->scanTmp.bytes;
-    HIPCHECK(hipcub::DeviceSelect::Flagged(ctx->scanTmp.p, tb, it, f.flags.as<uint8_t>(), f.nl.as<uint64_t>(),
-                                           f.count.as<uint64_t>(), int(bytes), st));
-    uint64_t numNl = 0;
-    HIPCHECK(hipMemcpyAsync(&numNl, f.count.p, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, f.blockCnt.as<uint32_t>(), f.blockOff.as<uint32_t>(),
+                                              int(nb + 1), st));
+    uint32_t numNl = 0;
+    HIPCHECK(hipMemcpyAsync(&numNl, f.blockOff.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     f.numNl = numNl;
-    const uint64_t lines = bytes ? numNl + 1 : 0;
+    const uint64_t lines = numNl + 1;
     if(lines % 4) return ctx->fail(CARE_EINVAL, "FASTQ text of %llu lines is not 4-line records",
                                    (unsigned long long)lines);
     f.records = lines / 4;
-    f.flags.release();
+    HIPCHECK(f.nl.ensure(std::max<size_t>(size_t(numNl) * 4, 4)));
+    k_newline_write<<<nb, kNlBlock, 0, st>>>(f.text->as<char>(), bytes, f.blockOff.as<uint32_t>(), f.nl.as<uint32_t>());
+    HIPCHECK(hipGetLastError());
     return CARE_OK;
 }
 
@@ -3190,6 +3291,14 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, c
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     ctx->hasFastq = false;
+    for(int slot = 0; slot < (paired ? 2 : 1); slot++){
+        const size_t raw = slot ? bytes2 : bytes1;
+        if(raw > size_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "FASTQ text above 2 GiB in one call");
+        ctx->fqRaw[slot] = raw;
+        HIPCHECK(ctx->fqText[slot].ensure((raw + 15) / 16 * 16 + 16));
+        if(raw) HIPCHECK(hipMemcpyAsync(ctx->fqText[slot].p, slot ? fastq2 : fastq1, raw, hipMemcpyHostToDevice, st));
+    }
+    HIPCHECK(hipEventRecord(ctx->ev[6], st));
     ParsedFile f1, f2;
     struct Rel { ParsedFile& a; ParsedFile& b; ~Rel(){ a.release(); b.release(); } } rel{f1, f2};
     int rc = uploadAndSplitLines(ctx, fastq1, bytes1, 0, f1);
@@ -3212,8 +3321,8 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, c
     const int stride = paired ? 2 : 1;
     for(int fi = 0; fi < (paired ? 2 : 1); fi++){
         ParsedFile& f = fi ? f2 : f1;
-        k_fastq_records<<<dim3(uint32_t((f.records + 255) / 256)), 256, 0, st>>>(
-            f.text->as<char>(), f.bytes, f.nl.as<uint64_t>(), f.numNl, f.records, stride, fi,
+        k_fastq_records<<<dim3(uint32_t((f.records * 16 + kFqBlock - 1) / kFqBlock)), kFqBlock, 0, st>>>(
+            f.text->as<char>(), f.bytes, f.nl.as<uint32_t>(), f.numNl, f.records, stride, fi,
             ctx->fqHdr.as<uint64_t>(), ctx->fqHdrLen.as<uint32_t>(), ctx->fqSeq.as<uint64_t>(), ctx->fqQual.as<uint64_t>(),
             tmp.lens.as<int>(), tmp.ncount.as<uint64_t>(), tmp.err.as<int>());
         HIPCHECK(hipGetLastError());
@@ -3244,14 +3353,18 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, c
         HIPCHECK(ctx->qual.ensure(n * ctx->qdpitch));
         qual = ctx->qual.as<uint8_t>();
     }
-    k_fastq_encode<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(
+    k_fastq_encode<<<dim3(uint32_t((n * 16 + kFqBlock - 1) / kFqBlock)), kFqBlock, 0, st>>>(
         f1.text->as<char>(), paired ? f2.text->as<char>() : nullptr, paired, ctx->fqSeq.as<uint64_t>(),
         ctx->fqQual.as<uint64_t>(), tmp.lens.as<int>(), tmp.prefix.as<uint64_t>(), n, pitch, ctx->enc.as<uint32_t>(),
         ctx->ambig.as<uint8_t>(), qual, ctx->qdpitch);
     HIPCHECK(hipGetLastError());
-    rc = finishReads(ctx);
-    if(rc == CARE_OK) ctx->hasFastq = true;
-    return rc;
+    rc = finishReads(ctx, ctx->ev[7]);
+    if(rc != CARE_OK) return rc;
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
+    ctx->fastqMs = ms;
+    ctx->hasFastq = true;
+    return CARE_OK;
 }
 
 // ---------------------------------------------------------------- output merge
@@ -3367,11 +3480,6 @@ __global__ void k_merge_keys(const uint8_t* __restrict__ recs, uint64_t recBytes
     keys[i] = id;
     idx[i] = i;
     if(!ok) atomicOr(err, 1);
-}
-
-__global__ void k_fill_u32(uint32_t* __restrict__ v, uint64_t n, uint32_t x){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i < n) v[i] = x;
 }
 
 // [runBeg[r], runEnd[r]) = positions of read r's records in the sorted order
@@ -3517,6 +3625,7 @@ int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_by
     HIPCHECK(ctx->mKeyIn.ensure(m64 * 4)); HIPCHECK(ctx->mKeyOut.ensure(m64 * 4));
     HIPCHECK(ctx->mIdxIn.ensure(m64 * 4)); HIPCHECK(ctx->mIdxOut.ensure(m64 * 4));
     HIPCHECK(ctx->mErr.ensure(4));
+    HIPCHECK(hipEventRecord(ctx->ev[6], st));
     k_fill_u32<<<1, 64, 0, st>>>(ctx->mErr.as<uint32_t>(), 1, 0u);
     const uint32_t mb = (m + 255) / 256;
     k_merge_keys<<<mb, 256, 0, st>>>(ctx->mRec.as<uint8_t>(), record_bytes, ctx->mOff.as<uint64_t>(), m, n,
@@ -3576,8 +3685,12 @@ int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_by
     const uint64_t wb = (count * 64 + 255) / 256;
     k_merge_write<<<dim3(uint32_t(wb)), 256, 0, st>>>(mp);
     HIPCHECK(hipGetLastError());
+    HIPCHECK(hipEventRecord(ctx->ev[7], st));
     HIPCHECK(hipMemcpyAsync(dst, ctx->mText.p, total, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
+    ctx->mergeMs = ms;
     return CARE_OK;
 }
 
@@ -3602,8 +3715,15 @@ int care_hip_inspect_reads(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* 
     if(quals && n){
         if(!ctx->o.use_quality_scores) return ctx->fail(CARE_EINVAL, "no qualities");
         if(qual_pitch < size_t(ctx->maxLen)) return ctx->fail(CARE_EINVAL, "qual_pitch smaller than the longest read");
-        HIPCHECK(hipMemcpy2D(quals, qual_pitch, ctx->qual.as<uint8_t>() + ctx->qdpitch * first, ctx->qdpitch,
-                             std::min(qual_pitch, ctx->qdpitch), n, hipMemcpyDeviceToHost));
+        // rows repacked to the caller's pitch on the device, then one linear copy
+        struct Staging { DBuf b; ~Staging(){ b.release(); } } hold;
+        HIPCHECK(hold.b.ensure(n * qual_pitch));
+        const uint64_t total = n * qual_pitch;
+        k_repack_rows<<<dim3(uint32_t((total + 255) / 256)), 256, 0, ctx->stream>>>(
+            ctx->qual.as<uint8_t>() + ctx->qdpitch * first, ctx->qdpitch, hold.b.as<uint8_t>(), qual_pitch, total);
+        HIPCHECK(hipGetLastError());
+        HIPCHECK(hipMemcpyAsync(quals, hold.b.p, total, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
     }
     return CARE_OK;
 }
@@ -3907,6 +4027,8 @@ int care_hip_extract_features(care_ctx* ctx, uint64_t first, uint64_t n, float* 
 int care_hip_get_timings(const care_ctx* ctx, care_timings* out){
     if(!ctx || !out) return CARE_EINVAL;
     *out = ctx->t;
+    out->fastq_device_ms = ctx->fastqMs;
+    out->merge_device_ms = ctx->mergeMs;
     return CARE_OK;
 }
 

@@ -50,7 +50,8 @@ class CareTimings(ctypes.Structure):
                 ("anchors", ctypes.c_uint64), ("candidates", ctypes.c_uint64),
                 ("records", ctypes.c_uint64), ("record_bytes", ctypes.c_uint64),
                 ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64),
-                ("graphs_built", ctypes.c_uint64)]
+                ("graphs_built", ctypes.c_uint64), ("fastq_device_ms", ctypes.c_double),
+                ("merge_device_ms", ctypes.c_double)]
 
 
 EXPORTS = [
@@ -201,6 +202,7 @@ class CareEngine:
         t1 = text(file1)
         t2 = None if file2 is None else text(file2)
         self.storage = None
+        self._fastq_bytes = [t1.size] + ([] if t2 is None else [t2.size])
         rc = self.lib.care_hip_set_reads_fastq(self.h, t1.ctypes.data, t1.size,
                                                None if t2 is None else t2.ctypes.data, 0 if t2 is None else t2.size)
         self._check(rc, "care_hip_set_reads_fastq")
@@ -211,9 +213,16 @@ class CareEngine:
         on the device; needs reads set by set_reads_fastq)."""
         rec = np.frombuffer(records, np.uint8) if len(records) else np.zeros(1, np.uint8)
         size = ctypes.c_size_t(0)
-        self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file, int(quality_labels),
-                                                  None, 0, ctypes.byref(size)), "care_hip_write_fastq")
-        out = np.empty(max(size.value, 1), np.uint8)
+        # every read's output is at most its input text, a label and its records' bytes
+        fq = getattr(self, "_fastq_bytes", None)
+        if fq is not None and 0 <= file < len(fq):
+            bound = fq[file] + 9 * (self.num_reads // len(fq)) + len(records) + 64
+        else:
+            self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file,
+                                                      int(quality_labels), None, 0, ctypes.byref(size)),
+                        "care_hip_write_fastq")
+            bound = size.value
+        out = np.empty(max(bound, 1), np.uint8)
         self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file, int(quality_labels),
                                                   out.ctypes.data, out.size, ctypes.byref(size)), "care_hip_write_fastq")
         return out[:size.value].tobytes()

@@ -153,3 +153,18 @@ def workload(name: str, seed: int = 12345, **overrides) -> SimReads:
     kw.update(overrides)
     kw["seed"] = seed
     return simulate(**kw)
+
+
+def fastq_texts(s: SimReads) -> list[bytes]:
+    """4-line FASTQ text of simulated reads: one text, or R1/R2 texts for pairs
+    (reads 2i and 2i+1 are mates)."""
+    n = int(s.lengths.shape[0])
+    parts = [range(n)] if not s.paired else [range(0, n, 2), range(1, n, 2)]
+    out = []
+    for part in parts:
+        chunks = []
+        for i in part:
+            L = int(s.lengths[i])
+            chunks.append(b"@r%d\n%s\n+\n%s\n" % (i, s.seqs[i, :L].tobytes(), s.quals[i, :L].tobytes()))
+        out.append(b"".join(chunks))
+    return out
