@@ -100,6 +100,8 @@ def cpu_lib():
         lib.carecpu_last_seconds.argtypes = [vp]
         lib.carecpu_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
         lib.carecpu_set_classifier.argtypes = [vp, vp, i32]
+        lib.carecpu_set_candidate_correction.argtypes = [vp, i32, i32]
+        lib.carecpu_set_candidate_correction.restype = None
         lib.carecpu_set_classifier.restype = i32
         lib.carecpu_mlp_num_params.restype = i32
         lib.carecpu_features.argtypes = [vp, u64, u64, i32, vp, vp, sz]
@@ -141,6 +143,10 @@ def ref_lib():
         lib.ref_flag_pairs.argtypes = [vp, i32, vp, i32, vp, vp]
         lib.ref_set_anchor_forest.argtypes = [vp, ctypes.c_char_p, ctypes.c_float]
         lib.ref_set_anchor_print.argtypes = [vp, ctypes.c_char_p]
+        lib.ref_set_candidate_correction.argtypes = [vp, i32, i32]
+        lib.ref_set_candidate_correction.restype = None
+        lib.ref_set_record_order.argtypes = [vp, i32]
+        lib.ref_set_record_order.restype = None
         lib.ref_save_tables.argtypes = [vp, ctypes.c_char_p]
         lib.ref_parse_fastq.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32]
         lib.ref_parse_fastq.restype = vp
@@ -258,6 +264,9 @@ class CareCpu:
     def last_seconds(self):
         return self.lib.carecpu_last_seconds(self.h)
 
+    def set_candidate_correction(self, on=True, new_columns=15):
+        self.lib.carecpu_set_candidate_correction(self.h, int(on), int(new_columns))
+
     def set_classifier(self, params):
         """MLP parameters (float32, MLP_PARAMS values) for the learned seam; None = classic."""
         if params is None:
@@ -305,6 +314,14 @@ class CareRef:
             self.close()
         except Exception:
             pass
+
+    def set_candidate_correction(self, on=True, new_columns=15):
+        """Candidate correction; run correct(threads=1) for the deterministic order."""
+        self.lib.ref_set_candidate_correction(self.h, int(on), int(new_columns))
+
+    def set_record_order(self, sorted_by_read_id=True):
+        """Records of correct() sorted by read id (default) or in insertion order."""
+        self.lib.ref_set_record_order(self.h, int(sorted_by_read_id))
 
     def correct(self, threads=8, num=0):
         cnt = self.lib.ref_correct(self.h, threads, num)
@@ -386,6 +403,19 @@ def ref_output_fastq(file1, file2, records: bytes, labels=False, tmpdir=None):
         if file2 is not None:
             outs.append(open(q2, "rb").read())
     return outs, rc
+
+
+def sort_records(data: bytes) -> bytes:
+    """Serialized records ordered by read id, ties in stream order."""
+    spans = []
+    p = 0
+    while p < len(data):
+        flags = int.from_bytes(data[p + 4:p + 8], "little")
+        nb = 8 + (flags & ((1 << 29) - 1))
+        spans.append((int.from_bytes(data[p:p + 4], "little"), p, nb))
+        p += nb
+    spans.sort(key=lambda x: x[0])
+    return b"".join(data[q:q + nb] for _, q, nb in spans)
 
 
 def ref_parse_fastq(file1, file2=None, use_qualities=True):

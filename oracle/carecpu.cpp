@@ -542,6 +542,8 @@ struct carecpu_ctx {
     std::vector<Table> tables;
     std::vector<uint8_t> records;
     std::vector<float> mlp;         // classifier parameters (empty: classic decision)
+    bool correctCandidates = false; // --candidateCorrection
+    int newColumns = 15;            // --candidateCorrectionNewColumns
     double seconds = 0.0;
 
     const uint32_t* row(uint32_t id) const { return enc.data() + pitch * id; }
@@ -592,12 +594,61 @@ struct carecpu_ctx {
 
     // One anchor through CpuErrorCorrector::process (include/corrector.hpp:145-312),
     // given its candidate ids and (PE) pair flags. Appends its record to `out`.
-    void processAnchor(uint32_t a, const std::vector<uint32_t>& cids, const std::vector<uint8_t>& pairFlags,
-                       std::vector<uint8_t>& out, carecpu_trace* tr, FeatureSink* sink = nullptr) const;
+    // One anchor's candidate corrections go to `cands` as (candidate id, record) pairs.
+    // Returns whether the anchor was corrected as an HQ anchor.
+    bool processAnchor(uint32_t a, const std::vector<uint32_t>& cids, const std::vector<uint8_t>& pairFlags,
+                       std::vector<uint8_t>& out, carecpu_trace* tr, FeatureSink* sink = nullptr,
+                       std::vector<std::pair<uint32_t, std::vector<uint8_t>>>* cands = nullptr) const;
 };
 
-void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, const std::vector<uint8_t>& pairFlags,
-                                std::vector<uint8_t>& out, carecpu_trace* tr, FeatureSink* sink) const {
+// Serialized record (include/correctedsequence.hpp:299-387): read id, flags
+// (hq << 31 | useEdits << 30 | candidate << 29 | body bytes), then either the edits
+// (count, positions, bases) or the sequence (length, bases), then a candidate's shift.
+void appendRecord(std::vector<uint8_t>& out, uint32_t id, bool hq, bool candidate, int shift,
+                  const std::string& corrected, const std::string& original, bool ambiguous){
+    const int L = int(corrected.size());
+    std::vector<std::pair<int, char>> edits;
+    bool useEdits = false;
+    if(!ambiguous){
+        const int maxEdits = L / 7;
+        int count = 0;
+        for(int i = 0; i < L && count <= maxEdits; i++){
+            if(corrected[i] != original[i]){
+                edits.emplace_back(i, corrected[i]);
+                count++;
+            }
+        }
+        useEdits = count <= maxEdits;
+    }
+    // HQ records without edits are not stored (src/correct_cpu.cpp:263-273)
+    if(hq && useEdits && edits.empty()) return;
+    uint32_t numBytes = useEdits ? uint32_t(4 + edits.size() * 5) : uint32_t(4 + L);
+    if(candidate) numBytes += 4;
+    const uint32_t flags = (uint32_t(hq) << 31) | (uint32_t(useEdits) << 30) | (uint32_t(candidate) << 29) | numBytes;
+    const size_t start = out.size();
+    out.resize(start + 8 + numBytes);
+    uint8_t* ptr = out.data() + start;
+    std::memcpy(ptr, &id, 4);
+    std::memcpy(ptr + 4, &flags, 4);
+    ptr += 8;
+    if(useEdits){
+        const int ne = int(edits.size());
+        std::memcpy(ptr, &ne, 4);
+        ptr += 4;
+        for(const auto& e : edits){ std::memcpy(ptr, &e.first, 4); ptr += 4; }
+        for(const auto& e : edits){ *ptr++ = uint8_t(e.second); }
+    }else{
+        std::memcpy(ptr, &L, 4);
+        ptr += 4;
+        std::memcpy(ptr, corrected.data(), L);
+        ptr += L;
+    }
+    if(candidate) std::memcpy(ptr, &shift, 4);
+}
+
+bool carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, const std::vector<uint8_t>& pairFlags,
+                                std::vector<uint8_t>& out, carecpu_trace* tr, FeatureSink* sink,
+                                std::vector<std::pair<uint32_t, std::vector<uint8_t>>>* cands) const {
     const int La = lens[a];
     const std::vector<uint8_t> anchor = decodeBases(a);
     const bool useQ = o.use_quality_scores != 0;
@@ -616,7 +667,7 @@ void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
             if(tr->is_paired) tr->is_paired[i] = pairFlags.empty() ? 0 : pairFlags[i];
         }
     }
-    if(nc0 == 0) return;
+    if(nc0 == 0) return false;
 
     // alignments in both orientations and orientation choice
     // (include/corrector.hpp:925-979)
@@ -656,7 +707,7 @@ void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
         r.reverseQual = (flag == 2);
         rows.push_back(std::move(r));
     }
-    if(rows.empty()) return;
+    if(rows.empty()) return false;
 
     // filterCandidatesByAlignmentMismatchRatio (include/corrector.hpp:1053-1149) or,
     // for paired input, ...WithPairFlags (:1151-1211)
@@ -691,7 +742,7 @@ void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
         }
     }
     if(tr) tr->num_filtered = int(kept.size());
-    if(kept.empty()) return;
+    if(kept.empty()) return false;
 
     // overlap weights (include/corrector.hpp:1260-1280)
     for(auto& r : kept) r.weight = overlapWeight(r.nOps, r.overlap, o.max_error_rate);
@@ -742,7 +793,7 @@ void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
 
     // correctAnchorClassic (include/corrector.hpp:1423-1446): getMSAProperties and
     // getCorrectedAnchor (src/msa.cpp:304-453)
-    if(msa.ncand == 0) return;   // cannot be corrected without candidates
+    if(msa.ncand == 0) return false;   // cannot be corrected without candidates
     const Properties p = msaProperties(msa, o.estimated_errorrate, o.estimated_coverage, o.m_coverage);
     std::string corrected(La, 'A');
     if(p.allGood){
@@ -787,42 +838,40 @@ void carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
         if(tr->corrected) std::memcpy(tr->corrected, corrected.data(), La);
     }
 
-    // makeOutputOfTask (include/corrector.hpp:1639-1761) and the record encoding
-    // (include/correctedsequence.hpp:299-387); HQ records without edits are not
-    // stored (src/correct_cpu.cpp:263-273).
-    std::vector<std::pair<int, char>> edits;
-    bool useEdits = false;
-    if(!ambig[a]){
-        const int maxEdits = La / 7;
-        int count = 0;
-        for(int i = 0; i < La && count <= maxEdits; i++){
-            if(corrected[i] != kDecode[anchor[i]]){
-                edits.emplace_back(i, corrected[i]);
-                count++;
+    // correctCandidatesClassic (include/corrector.hpp:1504-1512, src/msa.cpp:457-522): in an
+    // HQ anchor, every final candidate that starts within newColumns columns of the anchor
+    // and ends at most newColumns columns after it takes the consensus over its columns
+    // (the support/coverage test there is overwritten with true); reverse-complement
+    // candidates are turned forward (include/corrector.hpp:1684-1700).
+    if(cands && correctCandidates && p.isHQ){
+        for(const auto& r : kept){
+            const int qb = msa.ab + r.shift;
+            const int qe = qb + r.length;
+            if(!(msa.ab - newColumns <= qb && qb <= msa.ab + newColumns && qe <= msa.ab + La + newColumns)) continue;
+            std::string corr(msa.consensus.begin() + qb, msa.consensus.begin() + qe);
+            if(r.orient == 2){
+                std::string fwdc(corr.size(), 'A');
+                for(size_t j = 0; j < corr.size(); j++){
+                    const char c = corr[corr.size() - 1 - j];
+                    fwdc[j] = c == 'A' ? 'T' : c == 'C' ? 'G' : c == 'G' ? 'C' : 'A';
+                }
+                corr.swap(fwdc);
             }
+            const std::vector<uint8_t> ob = decodeBases(r.id);
+            std::string orig(ob.size(), 'A');
+            for(size_t j = 0; j < ob.size(); j++) orig[j] = kDecode[ob[j]];
+            std::vector<uint8_t> rec;
+            appendRecord(rec, r.id, false, true, r.shift, corr, orig, ambig[r.id] != 0);
+            cands->emplace_back(r.id, std::move(rec));
         }
-        useEdits = count <= maxEdits;
     }
-    if(p.isHQ && useEdits && edits.empty()) return;
-    uint32_t numBytes = useEdits ? uint32_t(4 + edits.size() * 5) : uint32_t(4 + La);
-    const uint32_t flags = (uint32_t(p.isHQ) << 31) | (uint32_t(useEdits) << 30) | (0u << 29) | numBytes;
-    const size_t start = out.size();
-    out.resize(start + 8 + numBytes);
-    uint8_t* ptr = out.data() + start;
-    std::memcpy(ptr, &a, 4);
-    std::memcpy(ptr + 4, &flags, 4);
-    ptr += 8;
-    if(useEdits){
-        const int ne = int(edits.size());
-        std::memcpy(ptr, &ne, 4);
-        ptr += 4;
-        for(const auto& e : edits){ std::memcpy(ptr, &e.first, 4); ptr += 4; }
-        for(const auto& e : edits){ *ptr++ = uint8_t(e.second); }
-    }else{
-        std::memcpy(ptr, &La, 4);
-        ptr += 4;
-        std::memcpy(ptr, corrected.data(), La);
-    }
+
+    // makeOutputOfTask (include/corrector.hpp:1639-1761) and the record encoding
+    // (include/correctedsequence.hpp:299-387)
+    std::string original(La, 'A');
+    for(int i = 0; i < La; i++) original[i] = kDecode[anchor[i]];
+    appendRecord(out, a, p.isHQ, false, 0, corrected, original, ambig[a] != 0);
+    return p.isHQ;
 }
 
 extern "C" {
@@ -954,7 +1003,10 @@ size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads
     // units: single anchors, or mate pairs for paired input
     if(paired && ((first & 1) || (n & 1))) return 0;
     const long long units = paired ? (long long)(n / 2) : (long long)n;
-    std::vector<std::vector<uint8_t>> outs(units);
+    const bool withCands = ctx->correctCandidates;
+    std::vector<std::vector<uint8_t>> outs(n);
+    std::vector<std::vector<std::pair<uint32_t, std::vector<uint8_t>>>> cands(withCands ? n : 0);
+    std::vector<uint8_t> hq(n, 0);
     const auto t0 = std::chrono::steady_clock::now();
     #pragma omp parallel for schedule(dynamic, 16)
     for(long long u = 0; u < units; u++){
@@ -965,29 +1017,67 @@ size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads
             const auto c1 = ctx->candidates(a1);
             std::vector<uint8_t> f0, f1;
             flagPairs(c0, c1, f0, f1);
-            ctx->processAnchor(a0, c0, f0, outs[u], nullptr);
-            ctx->processAnchor(a1, c1, f1, outs[u], nullptr);
+            hq[2 * u] = ctx->processAnchor(a0, c0, f0, outs[2 * u], nullptr, nullptr, withCands ? &cands[2 * u] : nullptr);
+            hq[2 * u + 1] = ctx->processAnchor(a1, c1, f1, outs[2 * u + 1], nullptr, nullptr,
+                                               withCands ? &cands[2 * u + 1] : nullptr);
         }else{
             const uint32_t a = uint32_t(first + u);
             const auto c = ctx->candidates(a);
-            ctx->processAnchor(a, c, std::vector<uint8_t>{}, outs[u], nullptr);
+            hq[u] = ctx->processAnchor(a, c, std::vector<uint8_t>{}, outs[u], nullptr, nullptr,
+                                       withCands ? &cands[u] : nullptr);
         }
     }
     const auto t1 = std::chrono::steady_clock::now();
     ctx->seconds = std::chrono::duration<double>(t1 - t0).count();
     ctx->records.clear();
     size_t count = 0;
-    for(auto& v : outs){
-        size_t p = 0;
+    auto countIn = [](const std::vector<uint8_t>& v){
+        size_t p = 0, c = 0;
         while(p < v.size()){
             uint32_t flags;
             std::memcpy(&flags, v.data() + p + 4, 4);
             p += 8 + (flags & ((1u << 29) - 1));
-            count++;
+            c++;
         }
-        ctx->records.insert(ctx->records.end(), v.begin(), v.end());
+        return c;
+    };
+    if(!withCands){
+        for(auto& v : outs){
+            count += countIn(v);
+            ctx->records.insert(ctx->records.end(), v.begin(), v.end());
+        }
+        return count;
+    }
+    // The reference's records in the order a one-thread run inserts them
+    // (src/correct_cpu.cpp:166-281: per 16-read batch, the anchors' records, then the
+    // candidates' records in anchor order), with a candidate's record dropped when the
+    // candidate was corrected as an HQ anchor before (include/corrector.hpp:1676-1680,
+    // flags set at :284-290: reads below the anchor, in this range).
+    std::vector<std::pair<uint32_t, const std::vector<uint8_t>*>> stream;
+    for(uint64_t s0 = first; s0 < first + n;){
+        const uint64_t e0 = std::min<uint64_t>(first + n, (s0 / 16 + 1) * 16);
+        for(uint64_t a = s0; a < e0; a++){
+            if(!outs[a - first].empty()) stream.emplace_back(uint32_t(a), &outs[a - first]);
+        }
+        for(uint64_t a = s0; a < e0; a++){
+            for(const auto& cr : cands[a - first]){
+                const uint32_t c = cr.first;
+                if(c < a && c >= first && hq[c - first]) continue;
+                if(!cr.second.empty()) stream.emplace_back(c, &cr.second);
+            }
+        }
+        s0 = e0;
+    }
+    for(const auto& e : stream){
+        count += countIn(*e.second);
+        ctx->records.insert(ctx->records.end(), e.second->begin(), e.second->end());
     }
     return count;
+}
+
+void carecpu_set_candidate_correction(carecpu_ctx* ctx, int on, int new_columns){
+    ctx->correctCandidates = on != 0;
+    ctx->newColumns = new_columns;
 }
 
 int carecpu_set_classifier(carecpu_ctx* ctx, const float* params, int count){

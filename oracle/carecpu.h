@@ -105,6 +105,10 @@ int carecpu_trace_anchor(carecpu_ctx* ctx, uint32_t read_id, carecpu_trace* trac
  * Records are serialized exactly as the reference's EncodedTempCorrectedSequence
  * (readId u32, flags u32, payload), ordered by read id. */
 size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads);
+
+/* Candidate correction (--candidateCorrection, include/options.hpp:41,52): records of
+ * corrected candidates of HQ anchors, in a one-thread run's order. */
+void carecpu_set_candidate_correction(carecpu_ctx* ctx, int on, int new_columns);
 size_t carecpu_records_bytes(carecpu_ctx* ctx);
 const uint8_t* carecpu_records_data(carecpu_ctx* ctx);
 double carecpu_last_seconds(carecpu_ctx* ctx);

@@ -184,6 +184,7 @@ struct ref_ctx {
     std::vector<std::uint8_t> records;
     std::vector<std::size_t> recordOffsets;
     double seconds = 0.0;
+    bool sortRecords = true;        // false: keep correct_cpu's insertion order
 };
 
 static void fill_program_options(care::ProgramOptions& po, const ref_opts* o, int maxLen){
@@ -321,6 +322,18 @@ void ref_reads_copy(const ref_reads* r, unsigned int* enc, int* lens, char* qual
 }
 void ref_reads_free(ref_reads* r){ delete r; }
 
+// Records of ref_correct in read-id order (1, the default) or in the order correct_cpu
+// inserted them into its SerializedObjectStorage (0; deterministic with one thread).
+void ref_set_record_order(ref_ctx* c, int sorted){ c->sortRecords = sorted != 0; }
+
+// Candidate correction (--candidateCorrection, --candidateCorrectionNewColumns,
+// include/options.hpp:41,52; classic candidate correction, include/corrector.hpp:1504-1512).
+void ref_set_candidate_correction(ref_ctx* c, int on, int newColumns){
+    c->po.correctCandidates = on != 0;
+    c->po.new_columns_to_correct = newColumns;
+    c->po.correctionTypeCands = care::CorrectionType::Classic;
+}
+
 // Switches the anchor correction to the feature print mode (include/corrector.hpp:1474-1493,
 // include/classification.hpp:50-57): the features of every decision column go to `path`
 // (sample rate 1); no anchor is corrected. Run with one thread for a whole-line file.
@@ -360,7 +373,7 @@ std::size_t ref_correct(ref_ctx* c, int threads, long long fixedNumberOfReads){
     for(std::size_t i = 0; i < num; i++){
         order[i] = {care::EncodedTempCorrectedSequence::parseReadId(storage.getPointer(i)), i};
     }
-    std::sort(order.begin(), order.end());
+    if(c->sortRecords) std::sort(order.begin(), order.end());
     c->records.clear();
     c->recordOffsets.clear();
     for(const auto& p : order){
