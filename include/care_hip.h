@@ -66,6 +66,10 @@ typedef struct care_opts {
                                        most 352 bases (else 1), 1 = one wavefront per pair */
     int use_graphs;                 /* fused engine: 1 = capture a pass's device work into a HIP graph
                                        and replay it while the pass's arguments and buffers hold */
+    int correct_candidates;         /* --candidateCorrection (staged engine, reads of at most 340 bases):
+                                       HQ anchors also correct their candidates (include/corrector.hpp:
+                                       1504-1512, src/msa.cpp:457-522) */
+    int new_columns_to_correct;     /* --candidateCorrectionNewColumns, 15 */
 } care_opts;
 
 typedef struct care_ctx care_ctx;
@@ -171,7 +175,13 @@ int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded);
 
 /* Corrects anchors [first, first+n) (replaces cpu::correct_cpu for that range).
  * On success the serialized records are held by the context: num_records records,
- * num_bytes bytes, in read-id order. For paired input first and n must be even. */
+ * num_bytes bytes, in the order a one-thread run of the reference inserts them into its
+ * SerializedObjectStorage (src/correct_cpu.cpp:166-281): read-id order; with
+ * correct_candidates, per 16-read group of read ids the anchors' records, then their
+ * candidates' records. A candidate record is dropped when the candidate was corrected as
+ * an HQ anchor earlier in the call (read id below the anchor's, not below `first`;
+ * include/corrector.hpp:1676-1680); the reference's choice there depends on its thread
+ * timing, and matches this one with one thread. For paired input first and n must be even. */
 int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_records, size_t* num_bytes);
 
 /* Copies the records of the last care_hip_correct call (cap must be >= num_bytes). */

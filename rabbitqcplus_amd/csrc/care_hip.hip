@@ -933,6 +933,10 @@ struct MsaParams {
     const float* seamB;             // b1 [64], b2 [64], b3
     float estCov;
     float* featOut; uint32_t* metaOut; unsigned long long* featCount; unsigned long long featCap;
+    // candidate correction: per working-list entry a record slot and its size; per read
+    // whether it was corrected as an HQ anchor
+    int corrCands; int newCols;
+    uint8_t* candSlots; int candSlotBytes; uint32_t* candBytes; uint8_t* hqAnchor;
 };
 
 // Per-anchor candidate lists of the MSA stage: the staged kernels index batch-wide
@@ -1618,6 +1622,81 @@ __device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, c
     sSup = reinterpret_cast<float*>(tp + (max(tileBytes, kSeamLdsBytes) + 15) / 16 * 16);
 }
 
+// Candidate correction of an HQ anchor (correctCandidatesClassic, include/corrector.hpp:
+// 1504-1512, src/msa.cpp:457-522, records as makeOutputOfTask, :1676-1757): every final
+// candidate that starts within newCols columns of the anchor and ends at most newCols
+// columns after it takes the consensus over its columns (the support and coverage test
+// there is overwritten with true); a reverse-complement candidate's correction is turned
+// forward; the record holds edits against the stored read when there are at most
+// length/7 and the read has no N, else the sequence, then the candidate's shift. One
+// candidate at a time, lanes over positions; the consensus is staged into the free tile
+// region.
+template<int KC>
+__device__ __forceinline__ void emitCandidateRecords(const MsaParams& p, const AnchorLists& lst, uint32_t cons, int ncol,
+                                                 int ab, int beg, int nk, int La, const MsaTile& tile, int lane){
+    uint8_t* sCons = reinterpret_cast<uint8_t*>(tile.shift);
+    __syncthreads();
+    #pragma unroll
+    for(int k = 0; k < KC; k++){
+        const int c = lane + kWave * k;
+        if(c < ncol) sCons[c] = uint8_t((cons >> (2 * k)) & 3u);
+    }
+    __syncthreads();
+    const uint64_t below = lanesBelow(lane);
+    for(int j = 0; j < nk; j++){
+        const int shift = lst.kShift[beg + j];
+        const int Lc = lst.kLen[beg + j];
+        const int qb = ab + shift, qe = qb + Lc;
+        if(!(ab - p.newCols <= qb && qb <= ab + p.newCols && qe <= ab + La + p.newCols)) continue;
+        const uint32_t id = lst.kId[beg + j];
+        const bool rc = lst.kOrient[beg + j] == 2;
+        const uint32_t* crow = p.enc + p.pitch * id;
+        int D = 0;
+        for(int base = 0; base < Lc; base += kWave){
+            const int pos = base + lane;
+            bool diff = false;
+            if(pos < Lc){
+                const int cb = rc ? 3 - int(sCons[qe - 1 - pos]) : int(sCons[qb + pos]);
+                diff = cb != baseAt(crow, pos);
+            }
+            D += __popcll(__ballot(diff));
+        }
+        const bool useEdits = p.ambig[id] == 0 && D <= Lc / 7;
+        const uint32_t payload = (useEdits ? uint32_t(4 + 5 * D) : uint32_t(4 + Lc)) + 4;
+        uint8_t* rec = p.candSlots + size_t(beg + j) * p.candSlotBytes;
+        if(lane == 0){
+            const uint32_t flags = (uint32_t(useEdits) << 30) | (1u << 29) | payload;
+            const int count = useEdits ? D : Lc;
+            memcpy(rec, &id, 4);
+            memcpy(rec + 4, &flags, 4);
+            memcpy(rec + 8, &count, 4);
+            memcpy(rec + 8 + payload - 4, &shift, 4);
+            p.candBytes[beg + j] = 8 + payload;
+        }
+        int running = 0;
+        for(int base = 0; base < Lc; base += kWave){
+            const int pos = base + lane;
+            int cb = 0;
+            bool diff = false;
+            if(pos < Lc){
+                cb = rc ? 3 - int(sCons[qe - 1 - pos]) : int(sCons[qb + pos]);
+                diff = cb != baseAt(crow, pos);
+            }
+            if(useEdits){
+                const uint64_t mask = __ballot(diff);
+                if(diff){
+                    const int r = running + __popcll(mask & below);
+                    memcpy(rec + 12 + 4 * r, &pos, 4);
+                    rec[12 + 4 * D + r] = uint8_t("ACGT"[cb]);
+                }
+                running += __popcll(mask);
+            }else if(pos < Lc){
+                rec[12 + pos] = uint8_t("ACGT"[cb]);
+            }
+        }
+    }
+}
+
 // Filters, MSA, refinement, decision and record of anchor `a` by one wavefront
 // (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734). Its candidates
 // are lst.ids/lst.best/lst.paired[beg, beg+n0); the record goes to slot `i`.
@@ -1630,6 +1709,10 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     int* info = p.info ? p.info + size_t(i) * kAnchorInfo : nullptr;
     if(lane == 0) p.recBytes[i] = 0;
     if(info && lane < kAnchorInfo) info[lane] = 0;
+    if(p.corrCands){
+        for(int q = beg + lane; q < beg + n0; q += kWave) p.candBytes[q] = 0;
+        if(lane == 0) p.hqAnchor[a] = 0;
+    }
     if(n0 == 0) return;
     const uint64_t below = lanesBelow(lane);
 
@@ -1954,6 +2037,11 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
         info[4] = isHQ; info[5] = 1; info[6] = mincov; info[7] = maxcov;
         info[9] = __float_as_int(avgSupport); info[10] = __float_as_int(minSupport);
     }
+    // corrected (it has candidates) as an HQ anchor (include/corrector.hpp:284-290)
+    if(p.corrCands){
+        if(lane == 0) p.hqAnchor[a] = isHQ;
+        if(isHQ) emitCandidateRecords<KC>(p, lst, m.cons, m.ncol, m.ab, beg, nk, La, tile, lane);
+    }
 
     // 5) record (include/corrector.hpp:1639-1761, include/correctedsequence.hpp:299-387);
     // HQ records without edits are not stored (src/correct_cpu.cpp:263-273)
@@ -2256,6 +2344,63 @@ __global__ void k_widen(const uint32_t* in, uint64_t* out, int n){
     if(i < n) out[i] = in[i];
 }
 
+// Candidate records of reads corrected as HQ anchors earlier in the same call are not
+// kept (include/corrector.hpp:1676-1680): with one thread the reference processes reads
+// in ascending order, so "earlier" is a read id below the anchor's and not below the
+// call's first anchor.
+__global__ void k_cand_filter(uint32_t* __restrict__ candBytes, const uint32_t* __restrict__ kId,
+                              const uint32_t* __restrict__ anchorOf, uint64_t b0, uint64_t first,
+                              const uint8_t* __restrict__ hqAnchor, uint32_t total){
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if(q >= total || candBytes[q] == 0) return;
+    const uint64_t c = kId[q];
+    const uint64_t a = b0 + anchorOf[q];
+    if(c < a && c >= first && hqAnchor[c]) candBytes[q] = 0;
+}
+
+// Position of every anchor record and candidate record of a batch in the order a
+// one-thread run inserts them (src/correct_cpu.cpp:166-281): per 16-read batch of read
+// ids, the anchors' records, then their candidates' records in anchor order. Anchor i of
+// the 16-read group starting at batch index is (ending at ie) goes to offsets[is] + i;
+// working-list entry q of anchor i to ie + q. Sizes, and the slot each entry comes from.
+__global__ void k_insertion_order(const uint32_t* __restrict__ recBytes, const uint32_t* __restrict__ candBytes,
+                                  const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ anchorOf,
+                                  uint64_t b0, int B, uint32_t total, const uint8_t* slots, int slotBytes,
+                                  const uint8_t* candSlots, int candSlotBytes, uint64_t* __restrict__ size,
+                                  uint64_t* __restrict__ src){
+    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(e < uint64_t(B)){
+        const int i = int(e);
+        const uint64_t a = b0 + uint64_t(i);
+        const int64_t gs = int64_t(a & ~uint64_t(15)) - int64_t(b0);
+        const int is = gs < 0 ? 0 : int(gs);
+        const uint64_t pos = uint64_t(offsets[is]) + uint64_t(i);
+        size[pos] = recBytes[i];
+        src[pos] = reinterpret_cast<uint64_t>(slots + size_t(i) * slotBytes);
+    }else if(e < uint64_t(B) + total){
+        const uint32_t q = uint32_t(e - uint64_t(B));
+        const uint64_t a = b0 + anchorOf[q];
+        const int64_t ge = int64_t((a | 15) + 1) - int64_t(b0);
+        const int ie = ge > B ? B : int(ge);
+        const uint64_t pos = uint64_t(ie) + q;
+        size[pos] = candBytes[q];
+        src[pos] = reinterpret_cast<uint64_t>(candSlots + size_t(q) * candSlotBytes);
+    }
+}
+
+// One wavefront per entry: copies its record to its offset in the batch's stream.
+__global__ void __launch_bounds__(256) k_gather_entries(const uint64_t* __restrict__ size, const uint64_t* __restrict__ src,
+                                                        const uint64_t* __restrict__ off, uint8_t* __restrict__ out,
+                                                        uint64_t entries){
+    const uint64_t e = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if(e >= entries) return;
+    const uint64_t nb = size[e];
+    const uint8_t* s = reinterpret_cast<const uint8_t*>(src[e]);
+    uint8_t* d = out + off[e];
+    for(uint64_t q = lane; q < nb; q += 64) d[q] = s[q];
+}
+
 // ---------------------------------------------------------------- host side
 
 struct DBuf {
@@ -2315,6 +2460,10 @@ struct care_ctx {
     unsigned long long featCap = 0;
     std::vector<uint8_t> records;
     size_t numRecords = 0;
+    // candidate correction: record slots and sizes per working-list entry, the insertion
+    // order of a batch's records, per read the HQ-anchor flag; first anchor of the call
+    DBuf candSlots, candBytes, insSize, insOff, insSrc, hqAnchor;
+    uint64_t callFirst = 0;
     // FASTQ text kept on the device by care_hip_set_reads_fastq for the output merge, and
     // per read: header start (after '@'), header length, sequence and quality starts
     DBuf fqText[2], fqHdr, fqHdrLen, fqSeq, fqQual;
@@ -2505,6 +2654,8 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.metaOut = ctx->featActive ? ctx->metaBuf.as<uint32_t>() : nullptr;
     mp.featCount = ctx->counters.as<unsigned long long>() + 4;
     mp.featCap = ctx->featCap;
+    mp.corrCands = 0; mp.newCols = ctx->o.new_columns_to_correct;
+    mp.candSlots = nullptr; mp.candSlotBytes = 0; mp.candBytes = nullptr; mp.hqAnchor = nullptr;
 }
 
 void fillCandCommon(const care_ctx* ctx, CandParams& cp){
@@ -2700,6 +2851,74 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     return CARE_OK;
 }
 
+// Slot of one candidate record: header, count, edits (5 bytes each, at most length/7 + 1)
+// or the sequence, shift.
+int candidateSlotBytes(int maxLen){
+    return ((8 + 4 + std::max(maxLen, 5 * (maxLen / 7 + 1)) + 4) + 15) / 16 * 16;
+}
+
+// Candidate correction: drops the candidate records the reference would not keep, then
+// gathers the batch's anchor and candidate records in the reference's insertion order
+// and appends them to `out` (after the MSA kernel of the batch).
+int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int slotBytes, int candSlotBytes,
+                         std::vector<uint8_t>& out, size_t& outRecords){
+    hipStream_t st = ctx->stream;
+    if(total){
+        k_cand_filter<<<(total + 255) / 256, 256, 0, st>>>(ctx->candBytes.as<uint32_t>(), ctx->kId.as<uint32_t>(),
+                                                           ctx->anchorOf.as<uint32_t>(), b0, ctx->callFirst,
+                                                           ctx->hqAnchor.as<uint8_t>(), total);
+        HIPCHECK(hipGetLastError());
+    }
+    const uint64_t entries = uint64_t(B) + total;
+    HIPCHECK(ctx->insSize.ensure((entries + 1) * 8));
+    HIPCHECK(ctx->insOff.ensure((entries + 1) * 8));
+    HIPCHECK(ctx->insSrc.ensure(entries * 8));
+    k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(ctx->insSize.as<uint64_t>() + entries), 2, 0u);
+    k_insertion_order<<<dim3(uint32_t((entries + 255) / 256)), 256, 0, st>>>(
+        ctx->recBytes.as<uint32_t>(), ctx->candBytes.as<uint32_t>(), ctx->offsets.as<uint32_t>(), ctx->anchorOf.as<uint32_t>(),
+        b0, B, total, ctx->slots.as<uint8_t>(), slotBytes, ctx->candSlots.as<uint8_t>(), candSlotBytes,
+        ctx->insSize.as<uint64_t>(), ctx->insSrc.as<uint64_t>());
+    HIPCHECK(hipGetLastError());
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->insSize.as<uint64_t>(), ctx->insOff.as<uint64_t>(),
+                                              int(entries + 1), st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->insSize.as<uint64_t>(), ctx->insOff.as<uint64_t>(),
+                                              int(entries + 1), st));
+    uint64_t outBytes = 0;
+    int errFlag = 0;
+    HIPCHECK(hipMemcpyAsync(&outBytes, ctx->insOff.as<uint64_t>() + entries, 8, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&errFlag, ctx->counters.as<int>() + 2, 4, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if(errFlag) return ctx->fail(CARE_EINVAL, "MSA column bound exceeded in batch at %llu", (unsigned long long)b0);
+    HIPCHECK(ctx->outBuf.ensure(std::max<uint64_t>(outBytes, 1)));
+    k_gather_entries<<<dim3(uint32_t((entries * 64 + 255) / 256)), 256, 0, st>>>(
+        ctx->insSize.as<uint64_t>(), ctx->insSrc.as<uint64_t>(), ctx->insOff.as<uint64_t>(), ctx->outBuf.as<uint8_t>(),
+        entries);
+    HIPCHECK(hipGetLastError());
+    const size_t old = out.size();
+    out.resize(old + outBytes);
+    if(outBytes) HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->outBuf.p, outBytes, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipEventRecord(ctx->ev[4], st));
+    HIPCHECK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1])); ctx->t.candidates_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[5])); ctx->t.cand_kernel_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2])); ctx->t.align_ms += ms; ctx->t.align_kernel_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3])); ctx->t.msa_ms += ms; ctx->t.msa_kernel_ms += ms;
+    HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4])); ctx->t.output_ms += ms;
+    ctx->t.candidates += total;
+    size_t q = old;
+    while(q < out.size()){
+        uint32_t flags;
+        memcpy(&flags, out.data() + q + 4, 4);
+        q += 8 + (flags & ((1u << 29) - 1));
+        outRecords++;
+    }
+    return CARE_OK;
+}
+
 int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     const int capPerAnchor = ctx->M * ctx->thr;
@@ -2798,6 +3017,15 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     mp.msaDbg = ctx->o.keep_intermediates ? ctx->msaDbg.as<uint8_t>() : nullptr;
     mp.dbgStride = ctx->dbgStride;
     mp.error = ctx->counters.as<int>() + 2;
+    const bool cands = ctx->o.correct_candidates != 0;
+    const int candSlotBytes = candidateSlotBytes(ctx->maxLen);
+    if(cands){
+        HIPCHECK(ctx->candSlots.ensure(T * candSlotBytes));
+        HIPCHECK(ctx->candBytes.ensure(T * 4));
+        mp.corrCands = 1;
+        mp.candSlots = ctx->candSlots.as<uint8_t>(); mp.candSlotBytes = candSlotBytes;
+        mp.candBytes = ctx->candBytes.as<uint32_t>(); mp.hqAnchor = ctx->hqAnchor.as<uint8_t>();
+    }
     const int kc = waveMsaColumnsPerLane(maxCols);
     if(kc > 0){
         // one wavefront per anchor, MSA columns in registers
@@ -2817,6 +3045,11 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
+
+    if(cands){
+        ctx->lastB0 = b0; ctx->lastB = uint64_t(B); ctx->lastTotal = total;
+        return gatherInsertionOrder(ctx, b0, B, total, slotBytes, candSlotBytes, out, outRecords);
+    }
 
     // record compaction
     k_widen<<<(B + 1 + 255) / 256, 256, 0, st>>>(ctx->recBytes.as<uint32_t>(), ctx->recOff64.as<uint64_t>(), B + 1);
@@ -2877,6 +3110,7 @@ void care_hip_default_opts(care_opts* o){
     o->estimated_errorrate = 0.06f;
     o->m_coverage = 0.6f;
     o->paired_filter_threshold = 0.06f;
+    o->new_columns_to_correct = 15;
 }
 
 int care_hip_create(const care_opts* opts, int device, care_ctx** out){
@@ -2885,6 +3119,7 @@ int care_hip_create(const care_opts* opts, int device, care_ctx** out){
     if(opts->num_hash_maps < 1 || opts->num_hash_maps > kMaxMaps) return CARE_EINVAL;
     if(opts->kmerlength < 0 || opts->kmerlength > 32) return CARE_EINVAL;
     if(!(opts->estimated_coverage > 0.0f)) return CARE_EINVAL;
+    if(opts->new_columns_to_correct < 0) return CARE_EINVAL;
     int count = 0;
     if(hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return CARE_ENODEV;
     auto* ctx = new (std::nothrow) care_ctx;
@@ -2915,7 +3150,8 @@ void care_hip_destroy(care_ctx* ctx){
                    &ctx->seam, &ctx->featBuf, &ctx->metaBuf, &ctx->fqText[0], &ctx->fqText[1], &ctx->fqHdr,
                    &ctx->fqHdrLen, &ctx->fqSeq, &ctx->fqQual, &ctx->mRec, &ctx->mOff, &ctx->mKeyIn, &ctx->mKeyOut,
                    &ctx->mIdxIn, &ctx->mIdxOut, &ctx->mRunBeg, &ctx->mRunEnd, &ctx->mPlan, &ctx->mSize, &ctx->mOutOff,
-                   &ctx->mText, &ctx->mErr}){
+                   &ctx->mText, &ctx->mErr, &ctx->candSlots, &ctx->candBytes, &ctx->insSize, &ctx->insOff,
+                   &ctx->insSrc, &ctx->hqAnchor}){
         b->release();
     }
     if(ctx->fusedExec) (void)hipGraphExecDestroy(ctx->fusedExec);
@@ -3904,10 +4140,16 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
         size_t freeB = 0, totalB = 0;
         HIPCHECK(hipMemGetInfo(&freeB, &totalB));
         const size_t budget = std::min<size_t>(size_t(16) << 30, freeB / 4);
-        const size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + size_t(recordSlotBytes(ctx->maxLen)) + 512;
+        size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + size_t(recordSlotBytes(ctx->maxLen)) + 512;
+        // candidate correction: a record slot per working-list entry (about threshold per anchor)
+        if(ctx->o.correct_candidates) perAnchor += size_t(ctx->thr) * (candidateSlotBytes(ctx->maxLen) + 36);
         B = int(std::min<size_t>(size_t(1) << 22, std::max<size_t>(1024, budget / perAnchor)));
     }
     if(ctx->o.paired && (B & 1)) B++;
+    const bool cands = ctx->o.correct_candidates != 0;
+    // candidate correction orders records by 16-read groups of read ids: batches end on
+    // group boundaries
+    if(cands) B = std::max(16, B & ~15);
     // the fused kernel keeps MSA columns in registers (<= 1024 columns) and prefetches a
     // candidate's four planes into 3 uint2 per lane (4 * (W + 1) <= 192)
     const bool fusedFits = waveMsaColumnsPerLane(ctx->maxCols) > 0 && 4 * (ctx->W + 1) <= 3 * kWave;
@@ -3916,16 +4158,25 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
     if(ctx->o.correction_type == 1 && !ctx->hasSeam) return ctx->fail(CARE_ESTATE, "learned correction: set_classifier first");
     if((ctx->o.correction_type == 1 || ctx->featActive) && waveMsaColumnsPerLane(ctx->maxCols) == 0)
         return ctx->fail(CARE_EINVAL, "learned correction and feature output need reads of at most 340 bases");
+    if(cands && waveMsaColumnsPerLane(ctx->maxCols) == 0)
+        return ctx->fail(CARE_EINVAL, "candidate correction needs reads of at most 340 bases");
+    if(cands && ctx->o.engine == 2) return ctx->fail(CARE_EINVAL, "candidate correction runs on the staged engine");
+    if(cands){
+        HIPCHECK(ctx->hqAnchor.ensure(ctx->n));
+        ctx->callFirst = first;
+    }
     const auto t0 = std::chrono::steady_clock::now();
     if(fused){
         const int rc = runFused(ctx, first, n, ctx->records, ctx->numRecords);
         if(rc != CARE_OK) return rc;
     }else{
-        for(uint64_t b0 = first; b0 < first + n; b0 += uint64_t(B)){
-            const int cur = int(std::min<uint64_t>(uint64_t(B), first + n - b0));
-            const int rc = runBatch(ctx, b0, cur, ctx->records, ctx->numRecords);
+        for(uint64_t b0 = first; b0 < first + n;){
+            uint64_t e = std::min<uint64_t>(b0 + uint64_t(B), first + n);
+            if(cands && e < first + n && (e & ~uint64_t(15)) > b0) e &= ~uint64_t(15);
+            const int rc = runBatch(ctx, b0, int(e - b0), ctx->records, ctx->numRecords);
             if(rc != CARE_OK) return rc;
             ctx->t.batches++;
+            b0 = e;
         }
     }
     unsigned long long cnt[4] = {0, 0, 0, 0};
