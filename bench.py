@@ -15,6 +15,7 @@ Prints one JSON line (see the keys below). Options:
   --cpu-sample S  anchors for the CPU baseline leg (default 300000; 0 disables)
   --cpu-threads T threads for the CPU baseline (default min(16, cores))
   --no-learned    skip the learned leg (the classifier seam with the shipped classifier)
+  --no-candidates skip the candidate-correction leg
   --no-autotune   skip the autotuner (which times the schedules on 65,536 anchors first)
 """
 from __future__ import annotations
@@ -161,6 +162,8 @@ def main():
                     help="auto = staged kernels; fused = one persistent kernel")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-candidates", action="store_true",
+                    help="skip the candidate-correction leg (--candidateCorrection)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --engine/--waves-per-cu as given instead of the autotuner's choice")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
@@ -278,6 +281,22 @@ def main():
                         "classifier": os.path.relpath(learned.DEFAULT_CLASSIFIER, ROOT)}
         log(f"GPU learned: {lvalue:.0f} reads/s, {lms:.1f} ms/pass; errors {lerr}")
 
+    # candidate correction (opt-in in the reference): the same passes with HQ anchors also
+    # correcting their candidates; records then hold several results per read
+    cand_line = None
+    if not args.no_candidates and rs.max_length <= 340:
+        copts = {k: v for k, v in opts.items() if k not in ("engine", "waves_per_cu", "use_graphs")}
+        eng = engine.CareEngine(0, correct_candidates=1, engine=1, **copts)
+        eng.set_reads(rs)
+        eng.build_tables()
+        cms, crec, cdata, ct = timed_passes(eng, args.steps, args.warmup, log, "candidates")
+        eng.close()
+        cvalue = rs.n / (cms / 1e3)
+        cand_line = {"value": round(cvalue, 1), "unit": "reads/s", "ms_per_step": round(cms, 3),
+                     "vs_plain": round(cvalue / value, 4), "records": crec, "record_bytes": len(cdata)}
+        log(f"GPU candidates: {cvalue:.0f} reads/s, {cms:.1f} ms/pass, {crec} records")
+        del cdata
+
     cpu = None
     if args.cpu_sample > 0:
         log(f"CPU baseline, {args.cpu_sample} anchors")
@@ -308,6 +327,7 @@ def main():
         "error_rate_after": plain_err["after"] / plain_err["bases"],
         "errors": plain_err,
         "learned": learned_line,
+        "candidate_correction": cand_line,
         "records": nrec,
         "fastq_io": io_line,
         "table_build_ms": round(1e3 * (t3 - t2), 2),
