@@ -148,6 +148,16 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1,
 int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file,
                          int quality_labels, char* dst, size_t cap, size_t* written);
 
+/* Preprocessed-reads files in the reference's format (--save-preprocessedreads-to /
+ * --load-preprocessedreads-from; ChunkedReadStorage::saveToFile / loadFromFile,
+ * include/chunkedreadstorage.hpp:158-398, lengths as include/lengthstorage.hpp:184-204):
+ * save writes the resident storage (qualities when use_quality_scores is set, zero past
+ * each read's length; ambiguous read ids ascending, where the reference writes its hash
+ * set's order); load replaces care_hip_set_reads (qualities are required when
+ * use_quality_scores is set, as 8-bit scores, and skipped otherwise). */
+int care_hip_save_reads(care_ctx* ctx, const char* path);
+int care_hip_load_reads(care_ctx* ctx, const char* path);
+
 /* Shape of the resident read storage (all zero before any reads are set). */
 int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints);
 

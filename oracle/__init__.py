@@ -158,6 +158,9 @@ def ref_lib():
         lib.ref_reads_pitch.restype = sz
         lib.ref_reads_copy.argtypes = [vp, vp, vp, vp, vp]
         lib.ref_reads_free.argtypes = [vp]
+        lib.ref_reads_save.argtypes = [vp, ctypes.c_char_p]
+        lib.ref_reads_load.argtypes = [ctypes.c_char_p, i32, i32]
+        lib.ref_reads_load.restype = vp
         lib.ref_output_fastq.argtypes = [ctypes.c_char_p, ctypes.c_char_p, vp, sz, i32, ctypes.c_char_p,
                                          ctypes.c_char_p, ctypes.c_char_p]
         lib.ref_load_tables.argtypes = [vp, ctypes.c_char_p]
@@ -418,11 +421,28 @@ def sort_records(data: bytes) -> bytes:
     return b"".join(data[q:q + nb] for _, q, nb in spans)
 
 
-def ref_parse_fastq(file1, file2=None, use_qualities=True):
+def ref_parse_fastq(file1, file2=None, use_qualities=True, save_to=None):
     """The reference's FASTQ -> read storage construction: dict of enc [n, pitch] u32,
-    lengths [n] i32, quals [n, maxLen] u8 (or None), ambiguous [n] u8, max_length."""
+    lengths [n] i32, quals [n, maxLen] u8 (or None), ambiguous [n] u8, max_length.
+    save_to: also write the reference's preprocessed-reads file there."""
     lib = ref_lib()
     h = lib.ref_parse_fastq(os.fsencode(file1), None if file2 is None else os.fsencode(file2), int(use_qualities))
+    if save_to is not None and lib.ref_reads_save(h, os.fsencode(save_to)) != 0:
+        lib.ref_reads_free(h)
+        raise RuntimeError("ref_reads_save failed")
+    return _reads_dict(lib, h, use_qualities)
+
+
+def ref_load_reads(path, paired=False, use_qualities=True):
+    """Read storage from a preprocessed-reads file, by the reference's loader."""
+    lib = ref_lib()
+    h = lib.ref_reads_load(os.fsencode(path), int(paired), int(use_qualities))
+    if not h:
+        raise RuntimeError(f"the reference could not load {path}")
+    return _reads_dict(lib, h, use_qualities)
+
+
+def _reads_dict(lib, h, use_qualities):
     try:
         n = lib.ref_reads_count(h)
         ml = lib.ref_reads_max_length(h)

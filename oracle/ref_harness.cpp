@@ -278,18 +278,10 @@ struct ref_reads {
     std::vector<int> lens;
     std::vector<char> quals;      // maxLen bytes per read
     std::vector<unsigned char> ambig;
+    std::unique_ptr<care::ChunkedReadStorage> storage;
 };
 
-ref_reads* ref_parse_fastq(const char* file1, const char* file2, int useQualities){
-    care::ProgramOptions po;
-    po.inputfiles.push_back(file1);
-    if(file2) po.inputfiles.push_back(file2);
-    po.pairType = file2 ? care::SequencePairType::PairedEnd : care::SequencePairType::SingleEnd;
-    po.useQualityScores = useQualities != 0;
-    po.qualityScoreBits = 8;
-    po.threads = 1;
-    po.showProgress = false;
-    auto storage = care::constructChunkedReadStorageFromFiles(po);
+static ref_reads* readsFromStorage(std::unique_ptr<care::ChunkedReadStorage> storage, bool useQualities){
     auto* r = new ref_reads;
     r->n = storage->getNumberOfReads();
     r->maxLen = storage->getSequenceLengthUpperBound();
@@ -308,7 +300,38 @@ ref_reads* ref_parse_fastq(const char* file1, const char* file2, int useQualitie
     std::vector<care::read_number> amb(storage->getNumberOfReadsWithN());
     storage->getIdsOfAmbiguousReads(amb.data());
     for(auto id : amb) r->ambig[id] = 1;
+    r->storage = std::move(storage);
     return r;
+}
+
+ref_reads* ref_parse_fastq(const char* file1, const char* file2, int useQualities){
+    care::ProgramOptions po;
+    po.inputfiles.push_back(file1);
+    if(file2) po.inputfiles.push_back(file2);
+    po.pairType = file2 ? care::SequencePairType::PairedEnd : care::SequencePairType::SingleEnd;
+    po.useQualityScores = useQualities != 0;
+    po.qualityScoreBits = 8;
+    po.threads = 1;
+    po.showProgress = false;
+    return readsFromStorage(care::constructChunkedReadStorageFromFiles(po), useQualities != 0);
+}
+
+// The reference's preprocessed-reads file (--save-preprocessedreads-to,
+// ChunkedReadStorage::saveToFile, include/chunkedreadstorage.hpp:246-398).
+int ref_reads_save(const ref_reads* r, const char* path){
+    try{ r->storage->saveToFile(path); return 0; }catch(...){ return -1; }
+}
+
+// Read storage loaded from such a file (--load-preprocessedreads-from,
+// ChunkedReadStorage::loadFromFile, include/chunkedreadstorage.hpp:158-244); null on error.
+ref_reads* ref_reads_load(const char* path, int paired, int useQualities){
+    try{
+        auto storage = std::make_unique<care::ChunkedReadStorage>(paired != 0, useQualities != 0, 8);
+        storage->loadFromFile(path);
+        return readsFromStorage(std::move(storage), useQualities != 0);
+    }catch(...){
+        return nullptr;
+    }
 }
 
 std::size_t ref_reads_count(const ref_reads* r){ return r->n; }

@@ -3930,6 +3930,261 @@ int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_by
     return CARE_OK;
 }
 
+// ---------------------------------------------------------------- preprocessed-reads file
+//
+// The reference's read-storage file (--save-preprocessedreads-to / --load-preprocessedreads-from;
+// ChunkedReadStorage::saveToFile / loadFromFile, include/chunkedreadstorage.hpp:158-398):
+//   u64 reads, i32 longest, i32 shortest, u8 has-qualities, i32 quality bits (8),
+//   u64 bytes of each of the four sections below, then
+//   lengths (LengthStore::writeToStream, include/lengthstorage.hpp:184-204): i32 32, i32 bits,
+//     i32 min, i32 max, u32 mask, i64 count, u64 words, u64 bytes, u32 words of length - min
+//     as a big-endian bit stream of `bits` bits each (bits = bit length of max - min);
+//   sequences: u64 pitch (ceil(longest/16)), u64 words, 2-bit rows;
+//   qualities: u64 pitch (ceil(longest/4), 0 without qualities), u64 words, rows of bytes,
+//     zero past each read's length;
+//   ambiguous reads: u64 count, u32 read ids.
+// The reference writes the ambiguous ids in its hash set's iteration order; here they are
+// ascending (its loader puts them back into a set).
+
+// One thread per output word of the length bit stream.
+__global__ void k_pack_lengths(const int* __restrict__ len, uint64_t n, int minLen, int bits,
+                               uint32_t* __restrict__ words, uint64_t nwords){
+    const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(w >= nwords) return;
+    const uint64_t b0 = 32 * w, b1 = b0 + 32;
+    const uint64_t i0 = b0 / uint64_t(bits);
+    const uint64_t i1 = min(n, (b1 + uint64_t(bits) - 1) / uint64_t(bits));
+    uint32_t word = 0;
+    for(uint64_t i = i0; i < i1; i++){
+        const uint32_t v = uint32_t(len[i] - minLen);
+        for(int k = 0; k < bits; k++){
+            const uint64_t pos = i * uint64_t(bits) + uint64_t(k);
+            if(pos >= b0 && pos < b1) word |= ((v >> (bits - 1 - k)) & 1u) << (31 - int(pos - b0));
+        }
+    }
+    words[w] = word;
+}
+
+// One thread per read: its length from the bit stream.
+__global__ void k_unpack_lengths(const uint32_t* __restrict__ words, uint64_t n, int minLen, int bits,
+                                 int* __restrict__ len){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i >= n) return;
+    uint32_t v = 0;
+    for(int k = 0; k < bits; k++){
+        const uint64_t pos = i * uint64_t(bits) + uint64_t(k);
+        v = (v << 1) | ((words[pos >> 5] >> (31 - int(pos & 31))) & 1u);
+    }
+    len[i] = minLen + int(v);
+}
+
+// Quality rows of `sp` bytes to rows of `dp` bytes, zero past each read's length.
+__global__ void k_quality_rows(const uint8_t* __restrict__ src, size_t sp, const int* __restrict__ len,
+                               uint8_t* __restrict__ dst, size_t dp, uint64_t total){
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(t >= total) return;
+    const uint64_t r = t / dp;
+    const uint64_t c = t - r * dp;
+    dst[t] = (c < sp && c < uint64_t(len[r])) ? src[r * sp + c] : uint8_t(0);
+}
+
+__global__ void k_set_flags(const uint32_t* __restrict__ ids, uint64_t m, uint8_t* __restrict__ flags){
+    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(i < m) flags[ids[i]] = 1;
+}
+
+int care_hip_save_reads(care_ctx* ctx, const char* path){
+    if(!ctx || !path) return CARE_EINVAL;
+    if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "no reads");
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    const uint64_t n = ctx->n;
+    const int maxLen = ctx->maxLen;
+    // shortest read
+    int minLen = 0;
+    {
+        struct Tmp { DBuf out; ~Tmp(){ out.release(); } } tmp;
+        HIPCHECK(tmp.out.ensure(4));
+        size_t tb = 0;
+        HIPCHECK(hipcub::DeviceReduce::Min(nullptr, tb, ctx->len.as<int>(), tmp.out.as<int>(), int(n), st));
+        HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+        tb = ctx->scanTmp.bytes;
+        HIPCHECK(hipcub::DeviceReduce::Min(ctx->scanTmp.p, tb, ctx->len.as<int>(), tmp.out.as<int>(), int(n), st));
+        HIPCHECK(hipMemcpyAsync(&minLen, tmp.out.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    const int diff = maxLen - minLen;
+    int bits = 0;
+    while(bits < 31 && (uint32_t(1) << bits) <= uint32_t(diff)) bits++;
+    if(diff == 0) bits = 0;
+    const uint32_t mask = bits ? uint32_t((uint64_t(1) << bits) - 1) : 0u;
+    const uint64_t lwords = bits ? (uint64_t(bits) * n + 31) / 32 : 0;
+    std::vector<uint32_t> lw(lwords);
+    const size_t refPitch = size_t((maxLen + 15) / 16);
+    const bool q = ctx->o.use_quality_scores != 0;
+    const size_t qPitch = q ? size_t((maxLen + 3) / 4) : 0;
+    std::vector<uint32_t> seq(n * refPitch), qual(n * qPitch);
+    std::vector<uint8_t> amb(n);
+    {
+        struct Tmp { DBuf words, rows; ~Tmp(){ words.release(); rows.release(); } } tmp;
+        if(lwords){
+            HIPCHECK(tmp.words.ensure(lwords * 4));
+            k_pack_lengths<<<dim3(uint32_t((lwords + 255) / 256)), 256, 0, st>>>(ctx->len.as<int>(), n, minLen, bits,
+                                                                               tmp.words.as<uint32_t>(), lwords);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(lw.data(), tmp.words.p, lwords * 4, hipMemcpyDeviceToHost, st));
+        }
+        const uint64_t seqBytes = n * refPitch * 4;
+        if(refPitch == ctx->pitch){
+            HIPCHECK(hipMemcpyAsync(seq.data(), ctx->enc.p, seqBytes, hipMemcpyDeviceToHost, st));
+        }else{
+            HIPCHECK(tmp.rows.ensure(seqBytes));
+            k_repack_rows<<<dim3(uint32_t((seqBytes + 255) / 256)), 256, 0, st>>>(ctx->enc.as<uint8_t>(), ctx->pitch * 4,
+                                                                                tmp.rows.as<uint8_t>(), refPitch * 4, seqBytes);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(seq.data(), tmp.rows.p, seqBytes, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+        if(q){
+            const uint64_t qBytes = n * qPitch * 4;
+            HIPCHECK(tmp.rows.ensure(qBytes));
+            k_quality_rows<<<dim3(uint32_t((qBytes + 255) / 256)), 256, 0, st>>>(ctx->qual.as<uint8_t>(), ctx->qdpitch,
+                                                                                ctx->len.as<int>(), tmp.rows.as<uint8_t>(),
+                                                                                qPitch * 4, qBytes);
+            HIPCHECK(hipGetLastError());
+            HIPCHECK(hipMemcpyAsync(qual.data(), tmp.rows.p, qBytes, hipMemcpyDeviceToHost, st));
+        }
+        HIPCHECK(hipMemcpyAsync(amb.data(), ctx->ambig.p, n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    std::vector<uint32_t> ambIds;
+    for(uint64_t i = 0; i < n; i++) if(amb[i]) ambIds.push_back(uint32_t(i));
+    std::FILE* f = std::fopen(path, "wb");
+    if(!f) return ctx->fail(CARE_EINVAL, "cannot open %s for writing", path);
+    struct Closer { std::FILE* f; ~Closer(){ if(f) std::fclose(f); } } closer{f};
+    bool ok = true;
+    auto put = [&](const void* p, size_t b){ ok = ok && (b == 0 || std::fwrite(p, 1, b, f) == b); };
+    const uint64_t nr = n;
+    const uint8_t hasQ = q ? 1 : 0;
+    const int qbits = 8, dtb = 32;
+    const int64_t count = int64_t(n);
+    const uint64_t lbytes = lwords * 4;
+    const uint64_t lengthsBytes = 4 * 4 + 4 + 8 + 8 + 8 + lbytes;
+    const uint64_t sequencesBytes = 16 + seq.size() * 4;
+    const uint64_t qualitiesBytes = 16 + qual.size() * 4;
+    const uint64_t ambigBytes = 8 + ambIds.size() * 4;
+    put(&nr, 8); put(&maxLen, 4); put(&minLen, 4); put(&hasQ, 1); put(&qbits, 4);
+    put(&lengthsBytes, 8); put(&sequencesBytes, 8); put(&qualitiesBytes, 8); put(&ambigBytes, 8);
+    put(&dtb, 4); put(&bits, 4); put(&minLen, 4); put(&maxLen, 4); put(&mask, 4); put(&count, 8);
+    put(&lwords, 8); put(&lbytes, 8); put(lw.data(), lbytes);
+    const uint64_t sp = refPitch, sn = seq.size();
+    put(&sp, 8); put(&sn, 8); put(seq.data(), sn * 4);
+    const uint64_t qp = qPitch, qn = qual.size();
+    put(&qp, 8); put(&qn, 8); put(qual.data(), qn * 4);
+    const uint64_t na = ambIds.size();
+    put(&na, 8); put(ambIds.data(), na * 4);
+    if(!ok) return ctx->fail(CARE_EINVAL, "write to %s failed", path);
+    return CARE_OK;
+}
+
+int care_hip_load_reads(care_ctx* ctx, const char* path){
+    if(!ctx || !path) return CARE_EINVAL;
+    std::FILE* f = std::fopen(path, "rb");
+    if(!f) return ctx->fail(CARE_EINVAL, "cannot open %s", path);
+    struct Closer { std::FILE* f; ~Closer(){ if(f) std::fclose(f); } } closer{f};
+    bool ok = true;
+    auto get = [&](void* p, size_t b){ ok = ok && (b == 0 || std::fread(p, 1, b, f) == b); return ok; };
+    auto skip = [&](uint64_t b){ ok = ok && std::fseek(f, long(b), SEEK_CUR) == 0; };
+    uint64_t nr = 0, sec[4] = {0, 0, 0, 0};
+    int upper = 0, lower = 0, qbits = 0;
+    uint8_t hasQ = 0;
+    get(&nr, 8); get(&upper, 4); get(&lower, 4); get(&hasQ, 1); get(&qbits, 4); get(sec, 32);
+    int dtb = 0, bits = 0, minLen = 0, maxLen = 0;
+    uint32_t mask = 0;
+    int64_t count = 0;
+    uint64_t lwords = 0, lbytes = 0;
+    get(&dtb, 4); get(&bits, 4); get(&minLen, 4); get(&maxLen, 4); get(&mask, 4); get(&count, 8);
+    get(&lwords, 8); get(&lbytes, 8);
+    if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated header", path);
+    if(dtb != 32 || bits < 0 || bits > 31 || count != int64_t(nr) || lbytes != lwords * 4 || maxLen < minLen || minLen < 0
+       || (bits == 0 && maxLen != minLen) || (bits > 0 && lwords < (uint64_t(bits) * nr + 31) / 32) || nr == 0
+       || nr >= 0xFFFFFFFFull)
+        return ctx->fail(CARE_EINVAL, "%s: not a preprocessed-reads file of this format", path);
+    std::vector<uint32_t> lw(lwords);
+    get(lw.data(), lbytes);
+    uint64_t sp = 0, sn = 0;
+    get(&sp, 8); get(&sn, 8);
+    if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
+    if(sp * 16 < uint64_t(maxLen) || sn != sp * nr) return ctx->fail(CARE_EINVAL, "%s: bad sequence section", path);
+    std::vector<uint32_t> seq(sn);
+    get(seq.data(), sn * 4);
+    uint64_t qp = 0, qn = 0;
+    get(&qp, 8); get(&qn, 8);
+    if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
+    const bool useQ = ctx->o.use_quality_scores != 0;
+    std::vector<uint32_t> qual;
+    if(useQ){
+        // include/chunkedreadstorage.hpp:201-232
+        if(!hasQ) return ctx->fail(CARE_EINVAL, "%s: quality scores expected, but none are present", path);
+        if(qbits != 8) return ctx->fail(CARE_EINVAL, "%s uses %d bits per quality score, 8 are required", path, qbits);
+        if(qp * 4 < uint64_t(maxLen) || qn != qp * nr) return ctx->fail(CARE_EINVAL, "%s: bad quality section", path);
+        qual.resize(qn);
+        get(qual.data(), qn * 4);
+    }else{
+        skip(qn * 4);
+    }
+    uint64_t na = 0;
+    get(&na, 8);
+    if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
+    if(na > nr) return ctx->fail(CARE_EINVAL, "%s: bad ambiguous-read section", path);
+    std::vector<uint32_t> ambIds(na);
+    get(ambIds.data(), na * 4);
+    if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
+    for(uint32_t id : ambIds) if(id >= nr) return ctx->fail(CARE_EINVAL, "%s: ambiguous read id %u out of range", path, id);
+    HIPCHECK(hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    {
+        const int rc = configureReads(ctx, nr, maxLen, size_t(sp));
+        if(rc != CARE_OK) return rc;
+    }
+    const uint64_t n = nr;
+    struct Tmp { DBuf words, rows, ids; ~Tmp(){ words.release(); rows.release(); ids.release(); } } tmp;
+    HIPCHECK(ctx->len.ensure(n * 4));
+    if(bits){
+        HIPCHECK(tmp.words.ensure(lwords * 4));
+        HIPCHECK(hipMemcpyAsync(tmp.words.p, lw.data(), lwords * 4, hipMemcpyHostToDevice, st));
+        k_unpack_lengths<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(tmp.words.as<uint32_t>(), n, minLen, bits,
+                                                                         ctx->len.as<int>());
+    }else{
+        k_fill_u32<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(ctx->len.as<uint32_t>(), n, uint32_t(minLen));
+    }
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(ctx->enc.ensure(sn * 4));
+    HIPCHECK(hipMemcpyAsync(ctx->enc.p, seq.data(), sn * 4, hipMemcpyHostToDevice, st));
+    HIPCHECK(ctx->ambig.ensure(n));
+    HIPCHECK(hipMemsetAsync(ctx->ambig.p, 0, n, st));
+    if(na){
+        HIPCHECK(tmp.ids.ensure(na * 4));
+        HIPCHECK(hipMemcpyAsync(tmp.ids.p, ambIds.data(), na * 4, hipMemcpyHostToDevice, st));
+        k_set_flags<<<dim3(uint32_t((na + 255) / 256)), 256, 0, st>>>(tmp.ids.as<uint32_t>(), na, ctx->ambig.as<uint8_t>());
+        HIPCHECK(hipGetLastError());
+    }
+    if(useQ){
+        ctx->qpitch = size_t(maxLen);
+        ctx->qdpitch = (size_t(maxLen) + 15) / 16 * 16;
+        HIPCHECK(ctx->qual.ensure(n * ctx->qdpitch));
+        HIPCHECK(tmp.rows.ensure(qn * 4));
+        HIPCHECK(hipMemcpyAsync(tmp.rows.p, qual.data(), qn * 4, hipMemcpyHostToDevice, st));
+        const uint64_t total = n * ctx->qdpitch;
+        k_quality_rows<<<dim3(uint32_t((total + 255) / 256)), 256, 0, st>>>(tmp.rows.as<uint8_t>(), qp * 4,
+                                                                           ctx->len.as<int>(), ctx->qual.as<uint8_t>(),
+                                                                           ctx->qdpitch, total);
+        HIPCHECK(hipGetLastError());
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    return finishReads(ctx);
+}
+
 int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints){
     if(!ctx) return CARE_EINVAL;
     if(num_reads) *num_reads = ctx->hasReads ? ctx->n : 0;

@@ -61,7 +61,7 @@ EXPORTS = [
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
     "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
     "care_hip_save_tables", "care_hip_load_tables", "care_hip_set_reads_fastq", "care_hip_read_layout",
-    "care_hip_inspect_reads", "care_hip_write_fastq",
+    "care_hip_inspect_reads", "care_hip_write_fastq", "care_hip_save_reads", "care_hip_load_reads",
     "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
     "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
 ]
@@ -90,6 +90,8 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_read_layout.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(i32), ctypes.POINTER(sz)]
     lib.care_hip_inspect_reads.argtypes = [vp, u64, u64, vp, vp, vp, sz, vp]
     lib.care_hip_write_fastq.argtypes = [vp, vp, sz, i32, i32, vp, sz, ctypes.POINTER(sz)]
+    lib.care_hip_save_reads.argtypes = [vp, ctypes.c_char_p]
+    lib.care_hip_load_reads.argtypes = [vp, ctypes.c_char_p]
     lib.care_hip_build_tables.argtypes = [vp, ctypes.POINTER(i32)]
     lib.care_hip_correct.argtypes = [vp, u64, u64, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     lib.care_hip_copy_records.argtypes = [vp, vp, sz]
@@ -227,6 +229,16 @@ class CareEngine:
         self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file, int(quality_labels),
                                                   out.ctypes.data, out.size, ctypes.byref(size)), "care_hip_write_fastq")
         return out[:size.value].tobytes()
+
+    def save_reads(self, path: str) -> None:
+        """Writes the resident read storage as the reference's preprocessed-reads file."""
+        self._check(self.lib.care_hip_save_reads(self.h, os.fsencode(path)), "care_hip_save_reads")
+
+    def load_reads(self, path: str) -> int:
+        """Loads the read storage from a preprocessed-reads file; returns the number of reads."""
+        self.storage = None
+        self._check(self.lib.care_hip_load_reads(self.h, os.fsencode(path)), "care_hip_load_reads")
+        return self.read_layout()[0]
 
     def read_layout(self) -> tuple[int, int, int]:
         """(number of reads, longest read, pitch in u32 words) of the resident storage."""

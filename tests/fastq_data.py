@@ -49,3 +49,34 @@ def write(tmp_path, data, paired):
     open(p1, "wb").write(text(data, range(0, n, 2)))
     open(p2, "wb").write(text(data, range(1, n, 2)))
     return (p1, p2)
+
+
+def parse_reads_file(path):
+    """Fields of a preprocessed-reads file (ChunkedReadStorage::saveToFile,
+    include/chunkedreadstorage.hpp:246-398; lengths include/lengthstorage.hpp:184-204)."""
+    import struct
+    b = open(path, "rb").read()
+    p = 0
+    n, upper, lower = struct.unpack_from("<Qii", b, p); p += 16
+    has_q = b[p]; p += 1
+    (qbits,) = struct.unpack_from("<i", b, p); p += 4
+    sections = struct.unpack_from("<QQQQ", b, p); p += 32
+    dtb, bits, mn, mx, mask, count, lwords, lbytes = struct.unpack_from("<iiiiIqQQ", b, p); p += 44
+    lw = np.frombuffer(b, "<u4", lwords, p).copy(); p += lbytes
+    sp, sn = struct.unpack_from("<QQ", b, p); p += 16
+    seq = np.frombuffer(b, "<u4", sn, p).copy(); p += 4 * sn
+    qp, qn = struct.unpack_from("<QQ", b, p); p += 16
+    qual = np.frombuffer(b, "<u4", qn, p).copy(); p += 4 * qn
+    (na,) = struct.unpack_from("<Q", b, p); p += 8
+    amb = np.frombuffer(b, "<u4", na, p).copy(); p += 4 * na
+    assert p == len(b)
+    if bits:
+        pos = np.arange(n, dtype=np.int64)[:, None] * bits + np.arange(bits)[None, :]
+        bit = (lw[pos >> 5] >> (31 - (pos & 31)).astype(np.uint32)) & 1
+        lengths = mn + (bit.astype(np.int64) << np.arange(bits - 1, -1, -1)).sum(1)
+    else:
+        lengths = np.full(n, mn, np.int64)
+    return dict(n=n, upper=upper, lower=lower, has_q=has_q, qbits=qbits, sections=sections, dtb=dtb, bits=bits,
+                min=mn, max=mx, mask=mask, count=count, length_words=lw, lengths=lengths.astype(np.int32),
+                seq_pitch=sp, enc=seq.reshape(n, sp) if sp else seq, qual_pitch=qp,
+                quals=qual.view(np.uint8).reshape(n, 4 * qp) if qp else None, ambiguous_ids=amb)

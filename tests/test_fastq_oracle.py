@@ -28,3 +28,29 @@ def test_host_storage_matches_reference_parser(tmp_path, paired):
     # correction reads them (reverse-complement candidates reverse the whole row)
     ml = ref["max_length"]
     assert np.array_equal(ref["quals"], rs.quals[:, :ml])
+
+
+@pytest.mark.parametrize("paired, use_q", [(False, True), (True, False)])
+def test_reference_reads_file_layout(tmp_path, paired, use_q):
+    """The reference's preprocessed-reads file decodes (fastq_data.parse_reads_file) to
+    the storage its parser built: pins the layout the engine writes and reads."""
+    data = fastq_data.make(seed=8 + paired, n_reads=70_000 if not paired else 2 * 36_000, paired=paired)
+    files = fastq_data.write(tmp_path, data, paired)
+    path = str(tmp_path / "reads.bin")
+    ref = oracle.ref_parse_fastq(*files, use_qualities=use_q, save_to=path)
+    f = fastq_data.parse_reads_file(path)
+    n, ml = len(ref["lengths"]), ref["max_length"]
+    assert (f["n"], f["upper"], f["lower"], f["has_q"], f["qbits"]) == (n, ml, ref["lengths"].min(), int(use_q), 8)
+    assert f["dtb"] == 32 and f["count"] == n and f["mask"] == (1 << f["bits"]) - 1
+    assert f["bits"] == int(ref["lengths"].max() - ref["lengths"].min()).bit_length()
+    assert np.array_equal(f["lengths"], ref["lengths"])
+    assert f["seq_pitch"] == (ml + 15) // 16 and np.array_equal(f["enc"], ref["enc"])
+    if use_q:
+        assert f["qual_pitch"] == (ml + 3) // 4
+        assert np.array_equal(f["quals"][:, :ml], ref["quals"]) and not f["quals"][:, ml:].any()
+    else:
+        assert f["qual_pitch"] == 0 and f["quals"] is None
+    assert sorted(f["ambiguous_ids"].tolist()) == np.nonzero(ref["ambiguous"])[0].tolist()
+    back = oracle.ref_load_reads(path, paired, use_q)
+    for k in ("enc", "lengths", "ambiguous"):
+        assert np.array_equal(back[k], ref[k])

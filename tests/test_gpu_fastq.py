@@ -1,16 +1,20 @@
-"""Read storage built on the GPU from FASTQ text (care_hip_set_reads_fastq).
+"""Read storage built on the GPU from FASTQ text (care_hip_set_reads_fastq), and the
+preprocessed-reads file (care_hip_save_reads / care_hip_load_reads).
 
 Bar: bit-exact. The device storage (2-bit rows, lengths, ambiguity flags, qualities) equals
 the host construction rabbitqcplus_amd.reads.make_storage, which the CPU suite pins to the
 reference's own FASTQ -> storage construction on the same seeded data (more than one
 parser batch of 65,536 reads, N and IUPAC codes, lower case); corrections from the
-FASTQ-built storage equal the golden records of the same reads.
+FASTQ-built storage equal the golden records of the same reads. The engine's reads file
+equals the reference's file (the ambiguous-id list compared in sorted order, which the
+reference writes in its hash set's order), and the files load both ways.
 """
 import os
 
 import numpy as np
 import pytest
 
+import oracle
 from rabbitqcplus_amd import engine, reads
 from tests import fastq_data, helpers
 
@@ -123,5 +127,89 @@ def test_paired_fastq_needs_two_texts_of_equal_records():
         with pytest.raises(engine.CareError, match="records"):
             eng.set_reads_fastq(rec * 2, rec * 3)
         assert eng.set_reads_fastq(rec * 2, rec * 2) == 4
+    finally:
+        eng.close()
+
+
+def _ambig_sorted(path):
+    """File bytes with the ambiguous-id list sorted (the reference writes its hash set's order)."""
+    f = fastq_data.parse_reads_file(path)
+    blob = open(path, "rb").read()
+    na = len(f["ambiguous_ids"])
+    return blob[:len(blob) - 4 * na] + np.sort(f["ambiguous_ids"]).astype("<u4").tobytes()
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("paired, use_q", [(False, True), (True, True), (False, False)])
+def test_reads_file_equals_reference_and_loads_both_ways(tmp_path, paired, use_q):
+    data = fastq_data.make(seed=12 + paired, n_reads=70_000 if not paired else 2 * 36_000, paired=paired)
+    files = fastq_data.write(tmp_path, data, paired)
+    theirs = str(tmp_path / "ref.bin")
+    ref = oracle.ref_parse_fastq(*files, use_qualities=use_q, save_to=theirs)
+    print(f"[reads file paired={paired} q={use_q}] reads={len(ref['lengths'])}", flush=True)
+    texts = _texts(data, paired)
+    ours = str(tmp_path / "ours.bin")
+    eng = engine.CareEngine(0, paired=int(paired), use_quality_scores=int(use_q))
+    try:
+        eng.set_reads_fastq(*texts)
+        eng.save_reads(ours)
+    finally:
+        eng.close()
+    assert np.array_equal(np.sort(fastq_data.parse_reads_file(ours)["ambiguous_ids"]),
+                          fastq_data.parse_reads_file(ours)["ambiguous_ids"])
+    assert _ambig_sorted(ours) == _ambig_sorted(theirs)
+    # the reference's file into the engine
+    rs = _host_storage(data, paired, use_q)
+    eng = engine.CareEngine(0, paired=int(paired), use_quality_scores=int(use_q))
+    try:
+        assert eng.load_reads(theirs) == rs.n
+        _assert_same(eng.inspect_reads(), rs, use_q)
+    finally:
+        eng.close()
+    # the engine's file into the reference
+    back = oracle.ref_load_reads(ours, paired, use_q)
+    assert np.array_equal(back["enc"], rs.enc) and np.array_equal(back["lengths"], rs.lengths)
+    assert np.array_equal(back["ambiguous"], rs.ambiguous)
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+def test_records_from_loaded_reads_equal_golden(tmp_path):
+    s, rs, o = helpers.case("se_q")
+    data = dict(seq_matrix=s.seqs, qual_matrix=s.quals, lengths=s.lengths)
+    files = fastq_data.write(tmp_path, data, False)
+    path = str(tmp_path / "ref.bin")
+    oracle.ref_parse_fastq(*files, use_qualities=True, save_to=path)
+    golden = np.load(os.path.join(os.path.dirname(__file__), "golden", "records_se_q.npz"))["records"].tobytes()
+    eng = engine.CareEngine(0, **o)
+    try:
+        eng.load_reads(path)
+        eng.build_tables()
+        assert eng.correct()[1] == golden
+    finally:
+        eng.close()
+
+
+def test_reads_file_errors(tmp_path):
+    data = fastq_data.make(seed=30, n_reads=500, paired=False)
+    text = _texts(data, False)[0]
+    noq = str(tmp_path / "noq.bin")
+    eng = engine.CareEngine(0, paired=0, use_quality_scores=0)
+    try:
+        eng.set_reads_fastq(text)
+        eng.save_reads(noq)
+    finally:
+        eng.close()
+    eng = engine.CareEngine(0, paired=0, use_quality_scores=1)
+    try:
+        with pytest.raises(engine.CareError, match="quality scores expected"):
+            eng.load_reads(noq)
+        eng.set_reads_fastq(text)
+        withq = str(tmp_path / "q.bin")
+        eng.save_reads(withq)
+        for cut in (1000, 30000):
+            short = str(tmp_path / "short.bin")
+            open(short, "wb").write(open(withq, "rb").read()[:cut])
+            with pytest.raises(engine.CareError, match="truncated"):
+                eng.load_reads(short)
     finally:
         eng.close()
