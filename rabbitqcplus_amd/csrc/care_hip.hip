@@ -293,8 +293,10 @@ struct CandParams {
     const uint64_t* sig; uint64_t n; int M; int k; int thr; int excludeAmbiguous;
     const int* len; const uint8_t* ambig;
     const Slot* const* slots; const uint32_t* caps; const uint32_t* values;
+    const uint2* bucketOf;          // per (read, map), read-major: (offset, count) of the read's own bucket
     uint64_t b0; int B; int capPerAnchor; int hashSlots;
     uint32_t* raw; uint32_t* uniq; unsigned long long* gathered;
+    uint32_t* gath;                 // per anchor: bucket values gathered (staged kernel)
 };
 
 __device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint64_t key, uint32_t* offset){
@@ -305,6 +307,22 @@ __device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint
         if(s.key == key){ *offset = s.offset; return s.count; }
         h = (h + 1) & (cap - 1);
     }
+}
+
+// Every read's own bucket in every map, read-major. A candidate lookup queries map m with
+// the anchor's own signature for m (include/corrector.hpp:571-650), and every read's
+// signature was inserted into every map, so the answer is the bucket the read belongs to
+// (or nothing when that bucket was not kept); resolving it once per table set turns each
+// pass's 48 signature loads and hash probes per anchor into one 384-byte read.
+__global__ void k_bucket_of(const uint64_t* __restrict__ sig, uint64_t n, int M, const Slot* const* slots,
+                            const uint32_t* __restrict__ caps, uint2* __restrict__ out){
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(t >= n * uint64_t(M)) return;
+    const int m = int(t / n);
+    const uint64_t a = t - uint64_t(m) * n;
+    uint32_t off = 0;
+    const uint32_t cnt = lookup(slots[m], caps[m], sig[t], &off);
+    out[a * uint64_t(M) + uint64_t(m)] = make_uint2(cnt ? off : 0u, cnt);
 }
 
 // Inclusive prefix sum over the wave (DPP row shifts, then row broadcasts).
@@ -384,9 +402,9 @@ k_candidates(CandParams p){
         const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
         uint32_t cnt = 0, off = 0;
         if(valid && tid < p.M){
-            const uint64_t key = p.sig[uint64_t(tid) * p.n + a];
-            cnt = lookup(p.slots[tid], p.caps[tid], key, &off);
-            if(cnt > uint32_t(p.thr)) cnt = 0;
+            const uint2 b = p.bucketOf[a * uint64_t(p.M) + uint64_t(tid)];
+            off = b.x;
+            cnt = b.y > uint32_t(p.thr) ? 0u : b.y;
         }
         const int incl = waveInclusiveSum(int(cnt));
         if(tid < kMaxMaps){ sOff[tid] = off; sPre[tid + 1] = uint32_t(incl); }
@@ -403,27 +421,60 @@ k_candidates(CandParams p){
     gatherInsert(p, a, T, mapOf, sOff, sPre, hset, H, out, &sUnique, tid, kBlock);
     __syncthreads();
     const int U = sUnique;
-    int P = 1;
-    while(P < U) P <<= 1;
-    for(int q = tid; q < P; q += kBlock) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
-    __syncthreads();
-    for(int kk = 2; kk <= P; kk <<= 1){
-        for(int j = kk >> 1; j > 0; j >>= 1){
-            for(int idx = tid; idx < P; idx += kBlock){
-                const int ixj = idx ^ j;
-                if(ixj > idx){
-                    const uint32_t x = hset[idx], y = hset[ixj];
-                    const bool up = (idx & kk) == 0;
-                    if((x > y) == up){ hset[idx] = y; hset[ixj] = x; }
-                }
+    if(U <= kBlock){
+        // rank sort (the ids are distinct): one pass over the set per thread, reading
+        // four ids per LDS access, then every id goes straight to its rank
+        const int U4 = (U + 3) & ~3;
+        for(int q = tid; q < U4; q += kBlock) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
+        __syncthreads();
+        if(tid < U){
+            const uint32_t v = hset[tid];
+            int rank = 0;
+            const uint4* h4 = reinterpret_cast<const uint4*>(hset);
+            for(int q = 0; q < U4 / 4; q++){
+                const uint4 w = h4[q];
+                rank += int(w.x < v) + int(w.y < v) + int(w.z < v) + int(w.w < v);
             }
-            __syncthreads();
+            out[rank] = v;
         }
+    }else{
+        int P = 1;
+        while(P < U) P <<= 1;
+        for(int q = tid; q < P; q += kBlock) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
+        __syncthreads();
+        for(int kk = 2; kk <= P; kk <<= 1){
+            for(int j = kk >> 1; j > 0; j >>= 1){
+                for(int idx = tid; idx < P; idx += kBlock){
+                    const int ixj = idx ^ j;
+                    if(ixj > idx){
+                        const uint32_t x = hset[idx], y = hset[ixj];
+                        const bool up = (idx & kk) == 0;
+                        if((x > y) == up){ hset[idx] = y; hset[ixj] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for(int q = tid; q < U; q += kBlock) out[q] = hset[q];
     }
-    for(int q = tid; q < U; q += kBlock) out[q] = hset[q];
     if(tid == 0){
         p.uniq[i] = uint32_t(U);
-        atomicAdd(p.gathered, (unsigned long long)T);
+        p.gath[i] = uint32_t(T);
+    }
+}
+
+// Sum of the per-anchor gathered-value counts of a batch into the pass counter.
+__global__ void k_add_gathered(const uint32_t* __restrict__ gath, int B, unsigned long long* __restrict__ total){
+    __shared__ unsigned long long part[kBlock / kWave];
+    unsigned long long s = 0;
+    for(int q = threadIdx.x; q < B; q += blockDim.x) s += gath[q];
+    for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if(threadIdx.x == 0){
+        unsigned long long t = 0;
+        for(int w = 0; w < int(blockDim.x / kWave); w++) t += part[w];
+        *total += t;
     }
 }
 
@@ -2149,9 +2200,9 @@ __device__ __forceinline__ int candidatesWave(const CandParams& p, uint64_t a, u
     const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
     uint32_t cnt = 0, off = 0;
     if(valid && lane < p.M){
-        const uint64_t key = p.sig[uint64_t(lane) * p.n + a];
-        cnt = lookup(p.slots[lane], p.caps[lane], key, &off);
-        if(cnt > uint32_t(p.thr)) cnt = 0;
+        const uint2 b = p.bucketOf[a * uint64_t(p.M) + uint64_t(lane)];
+        off = b.x;
+        cnt = b.y > uint32_t(p.thr) ? 0u : b.y;
     }
     const int incl = waveInclusiveSum(int(cnt));
     if(lane < kMaxMaps){ sOff[lane] = off; sPre[lane + 1] = uint32_t(incl); }
@@ -2438,7 +2489,7 @@ struct care_ctx {
     bool hasReads = false;
     bool hasTables = false;
 
-    DBuf enc, len, qual, ambig, sig, hilo, qw;
+    DBuf enc, len, qual, ambig, sig, hilo, qw, bucketOf, gath;
     std::vector<DBuf> slotBufs;
     std::vector<uint32_t> caps;
     DBuf slotPtrs, capsDev, values;
@@ -2660,12 +2711,14 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
 
 void fillCandCommon(const care_ctx* ctx, CandParams& cp){
     cp.sig = ctx->sig.as<uint64_t>(); cp.n = ctx->n; cp.M = ctx->M; cp.k = ctx->k; cp.thr = ctx->thr;
+    cp.bucketOf = ctx->bucketOf.as<uint2>();
     cp.excludeAmbiguous = ctx->o.exclude_ambiguous; cp.len = ctx->len.as<int>(); cp.ambig = ctx->ambig.as<uint8_t>();
     cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
     cp.capPerAnchor = ctx->M * ctx->thr;
     int P = 64;
     while(P <= cp.capPerAnchor) P <<= 1;      // hash slots > the most values an anchor can gather
     cp.hashSlots = P;
+    cp.gath = nullptr;
 }
 
 void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
@@ -2934,13 +2987,18 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     HIPCHECK(hipEventRecord(ctx->ev[0], st));
     CandParams cp;
     cp.sig = ctx->sig.as<uint64_t>(); cp.n = ctx->n; cp.M = ctx->M; cp.k = ctx->k; cp.thr = ctx->thr;
+    cp.bucketOf = ctx->bucketOf.as<uint2>();
     cp.excludeAmbiguous = ctx->o.exclude_ambiguous; cp.len = ctx->len.as<int>(); cp.ambig = ctx->ambig.as<uint8_t>();
     cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
     cp.b0 = b0; cp.B = B; cp.capPerAnchor = capPerAnchor; cp.hashSlots = P; cp.raw = ctx->raw.as<uint32_t>(); cp.uniq = ctx->uniq.as<uint32_t>();
     cp.gathered = ctx->counters.as<unsigned long long>();
+    HIPCHECK(ctx->gath.ensure(size_t(B) * 4));
+    cp.gath = ctx->gath.as<uint32_t>();
     const size_t candLds = candLdsBytes(P, capPerAnchor);
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
     k_candidates<<<B, kBlock, candLds, st>>>(cp);
+    HIPCHECK(hipGetLastError());
+    k_add_gathered<<<1, kBlock, 0, st>>>(ctx->gath.as<uint32_t>(), B, ctx->counters.as<unsigned long long>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
     size_t tb = 0;
@@ -3142,7 +3200,7 @@ void care_hip_destroy(care_ctx* ctx){
     if(!ctx) return;
     (void)hipSetDevice(ctx->device);
     if(ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->qw, &ctx->slotPtrs,
+    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->qw, &ctx->bucketOf, &ctx->gath, &ctx->slotPtrs,
                    &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
@@ -4219,6 +4277,19 @@ int care_hip_inspect_reads(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* 
     return CARE_OK;
 }
 
+// Resolves every read's bucket per map (after the tables are built or loaded).
+static int resolveBuckets(care_ctx* ctx){
+    hipStream_t st = ctx->stream;
+    const uint64_t total = ctx->n * uint64_t(ctx->M);
+    HIPCHECK(ctx->bucketOf.ensure(total * sizeof(uint2)));
+    k_bucket_of<<<dim3(uint32_t((total + 255) / 256)), 256, 0, st>>>(ctx->sig.as<uint64_t>(), ctx->n, ctx->M,
+                                                                    ctx->slotPtrs.as<const Slot*>(),
+                                                                    ctx->capsDev.as<uint32_t>(), ctx->bucketOf.as<uint2>());
+    HIPCHECK(hipGetLastError());
+    HIPCHECK(hipStreamSynchronize(st));
+    return CARE_OK;
+}
+
 int care_hip_build_tables(care_ctx* ctx, int* maps_built){
     if(!ctx) return CARE_EINVAL;
     if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "set_reads first");
@@ -4228,7 +4299,9 @@ int care_hip_build_tables(care_ctx* ctx, int* maps_built){
     k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
                                                                    ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>());
     HIPCHECK(hipGetLastError());
-    const int rc = buildTables(ctx);
+    int rc = buildTables(ctx);
+    if(rc != CARE_OK) return rc;
+    rc = resolveBuckets(ctx);
     if(rc != CARE_OK) return rc;
     ctx->hasTables = true;
     if(maps_built) *maps_built = ctx->M;
@@ -4371,6 +4444,8 @@ int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
     HIPCHECK(hipMemcpy(ctx->slotPtrs.p, ptrs48.data(), sizeof(Slot*) * kMaxMaps, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(ctx->capsDev.p, caps48.data(), sizeof(uint32_t) * kMaxMaps, hipMemcpyHostToDevice));
     HIPCHECK(hipStreamSynchronize(st));
+    const int rc = resolveBuckets(ctx);
+    if(rc != CARE_OK) return rc;
     ctx->hasTables = true;
     if(maps_loaded) *maps_loaded = M;
     return CARE_OK;
