@@ -2151,6 +2151,8 @@ k_msa_wave(MsaParams p){
 
 // ---------------------------------------------------------------- fused persistent kernel
 
+constexpr int kFusedChunk = 4;      // work units taken per counter increment
+
 struct FusedParams {
     MsaParams m;             // storage, thresholds and record slots (per-wave lists set in the kernel)
     CandParams c;            // tables
@@ -2196,7 +2198,7 @@ __device__ inline WaveScratch carveScratch(uint8_t* base, int cap){
 // Candidate ids of anchor a by one wave (include/corrector.hpp:571-650): the same
 // lookup / LDS hash-set dedup / sort as k_candidates. Returns the count; ids in out[].
 __device__ __forceinline__ int candidatesWave(const CandParams& p, uint64_t a, uint32_t* hset, uint32_t* sOff, uint32_t* sPre,
-                              int* sUnique, uint32_t* out, int lane){
+                              int* sUnique, uint32_t* out, int lane, unsigned long long& gathered){
     const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
     uint32_t cnt = 0, off = 0;
     if(valid && lane < p.M){
@@ -2208,7 +2210,7 @@ __device__ __forceinline__ int candidatesWave(const CandParams& p, uint64_t a, u
     if(lane < kMaxMaps){ sOff[lane] = off; sPre[lane + 1] = uint32_t(incl); }
     if(lane == 0){ sPre[0] = 0; *sUnique = 0; }
     const int T = __builtin_amdgcn_readlane(incl, 63);
-    if(lane == 0 && T) atomicAdd(p.gathered, (unsigned long long)T);
+    gathered += uint64_t(T);
     uint8_t* mapOf = reinterpret_cast<uint8_t*>(hset + p.hashSlots);
     for(int q = incl - int(cnt); q < incl; q++) mapOf[q] = uint8_t(lane);
     int H = 64;
@@ -2219,24 +2221,43 @@ __device__ __forceinline__ int candidatesWave(const CandParams& p, uint64_t a, u
     gatherInsert(p, a, T, mapOf, sOff, sPre, hset, H, out, sUnique, lane, kWave);
     __syncthreads();
     const int U = *sUnique;
-    int P = 1;
-    while(P < U) P <<= 1;
-    for(int q = lane; q < P; q += kWave) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
-    __syncthreads();
-    for(int kk = 2; kk <= P; kk <<= 1){
-        for(int j = kk >> 1; j > 0; j >>= 1){
-            for(int idx = lane; idx < P; idx += kWave){
-                const int ixj = idx ^ j;
-                if(ixj > idx){
-                    const uint32_t x = hset[idx], y = hset[ixj];
-                    const bool up = (idx & kk) == 0;
-                    if((x > y) == up){ hset[idx] = y; hset[ixj] = x; }
-                }
-            }
-            __syncthreads();
+    if(U <= 4 * kWave){
+        // rank sort of the distinct ids, up to four per lane
+        const int U4 = (U + 3) & ~3;
+        for(int q = lane; q < U4; q += kWave) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
+        __syncthreads();
+        uint32_t v[4];
+        int rank[4] = {0, 0, 0, 0};
+        #pragma unroll
+        for(int e = 0; e < 4; e++) v[e] = lane + kWave * e < U ? hset[lane + kWave * e] : 0xFFFFFFFFu;
+        const uint4* h4 = reinterpret_cast<const uint4*>(hset);
+        for(int q = 0; q < U4 / 4; q++){
+            const uint4 w = h4[q];
+            #pragma unroll
+            for(int e = 0; e < 4; e++) rank[e] += int(w.x < v[e]) + int(w.y < v[e]) + int(w.z < v[e]) + int(w.w < v[e]);
         }
+        #pragma unroll
+        for(int e = 0; e < 4; e++) if(lane + kWave * e < U) out[rank[e]] = v[e];
+    }else{
+        int P = 1;
+        while(P < U) P <<= 1;
+        for(int q = lane; q < P; q += kWave) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
+        __syncthreads();
+        for(int kk = 2; kk <= P; kk <<= 1){
+            for(int j = kk >> 1; j > 0; j >>= 1){
+                for(int idx = lane; idx < P; idx += kWave){
+                    const int ixj = idx ^ j;
+                    if(ixj > idx){
+                        const uint32_t x = hset[idx], y = hset[ixj];
+                        const bool up = (idx & kk) == 0;
+                        if((x > y) == up){ hset[idx] = y; hset[ixj] = x; }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for(int q = lane; q < U; q += kWave) out[q] = hset[q];
     }
-    for(int q = lane; q < U; q += kWave) out[q] = hset[q];
     __syncthreads();
     return U;
 }
@@ -2334,23 +2355,30 @@ k_fused(FusedParams fp){
     __syncthreads();
     AnchorLists lst{ws.idsA, ws.best, nullptr, ws.kId, ws.kShift, ws.kLen, ws.kNops, ws.kOvl, ws.kW, ws.kOrient,
                     ws.kPaired};
+    // work is taken kFusedChunk units at a time; the statistics are summed per wave and
+    // added once at the end (one global atomic per unit serializes the whole device)
+    unsigned long long gathered = 0, aligned = 0;
+    uint32_t unit = 0, chunkEnd = 0;
     for(;;){
-        if(lane == 0) sUnit = atomicAdd(fp.counter, 1u);
-        __syncthreads();
-        const uint32_t unit = sUnit;
-        __syncthreads();
+        if(unit == chunkEnd){
+            if(lane == 0) sUnit = atomicAdd(fp.counter, uint32_t(kFusedChunk));
+            __syncthreads();
+            unit = sUnit;
+            chunkEnd = unit + uint32_t(kFusedChunk);
+            __syncthreads();
+        }
         if(unit >= fp.units) break;
         if(!PAIRED){
             const uint64_t a = fp.first + unit;
-            const int U = candidatesWave(fp.c, a, hset, sOff, sPre, &sUnique, ws.idsA, lane);
-            if(lane == 0 && U) atomicAdd(fp.aligned, (unsigned long long)U);
+            const int U = candidatesWave(fp.c, a, hset, sOff, sPre, &sUnique, ws.idsA, lane, gathered);
+            aligned += uint64_t(U);
             alignAllWave(fp.al, a, ws.idsA, U, ws.best, planeA, planeC, lane);
             msaAnchorWave<KC>(fp.m, lst, a, int(unit), 0, U, tile, sSup, lane);
         }else{
             const uint64_t a0 = fp.first + 2 * uint64_t(unit);
-            const int U0 = candidatesWave(fp.c, a0, hset, sOff, sPre, &sUnique, ws.idsA, lane);
-            const int U1 = candidatesWave(fp.c, a0 + 1, hset, sOff, sPre, &sUnique, ws.idsB, lane);
-            if(lane == 0 && U0 + U1) atomicAdd(fp.aligned, (unsigned long long)(U0 + U1));
+            const int U0 = candidatesWave(fp.c, a0, hset, sOff, sPre, &sUnique, ws.idsA, lane, gathered);
+            const int U1 = candidatesWave(fp.c, a0 + 1, hset, sOff, sPre, &sUnique, ws.idsB, lane, gathered);
+            aligned += uint64_t(U0 + U1);
             // pair flags: the reference's merge (include/util.hpp:1016-1200) flags x in one
             // mate's list exactly when x^1 (the other read of x's pair) is in the other list
             for(int j = lane; j < U0; j += kWave) ws.pairedA[j] = sortedContains(ws.idsB, U1, ws.idsA[j] ^ 1u);
@@ -2367,6 +2395,11 @@ k_fused(FusedParams fp){
             }
         }
         __syncthreads();
+        unit++;
+    }
+    if(lane == 0){
+        if(gathered) atomicAdd(fp.c.gathered, gathered);
+        if(aligned) atomicAdd(fp.aligned, aligned);
     }
 }
 
