@@ -297,6 +297,7 @@ struct CandParams {
     uint64_t b0; int B; int capPerAnchor; int hashSlots;
     uint32_t* raw; uint32_t* uniq; unsigned long long* gathered;
     uint32_t* gath;                 // per anchor: bucket values gathered (staged kernel)
+    const uint32_t* order;          // processing order of the batch's anchors, or null
 };
 
 __device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint64_t key, uint32_t* offset){
@@ -307,6 +308,19 @@ __device__ __forceinline__ uint32_t lookup(const Slot* slots, uint32_t cap, uint
         if(s.key == key){ *offset = s.offset; return s.count; }
         h = (h + 1) & (cap - 1);
     }
+}
+
+// Anchor processed by this workgroup (one anchor per workgroup, `B` in the batch). With
+// an order, anchors that share a map-0 signature (overlapping reads, whose candidate sets
+// largely coincide) run one after another on the same XCD: workgroups go to the 8 XCDs in
+// turn, so workgroup b takes position (b mod 8) * B/8 + b/8 of the order, and each XCD's
+// L2 sees a contiguous stretch of it. Results do not depend on the order.
+__device__ __forceinline__ int batchAnchor(const uint32_t* order, int B){
+    const int b = blockIdx.x;
+    if(!order) return b;
+    const int x = b & 7, j = b >> 3;
+    const int q = B >> 3, r = B & 7;
+    return int(order[x * q + min(x, r) + j]);
 }
 
 // Every read's own bucket in every map, read-major. A candidate lookup queries map m with
@@ -395,7 +409,7 @@ k_candidates(CandParams p){
     __shared__ uint32_t sOff[kMaxMaps];
     __shared__ uint32_t sPre[kMaxMaps + 1];
     __shared__ int sUnique;
-    const int i = blockIdx.x;
+    const int i = batchAnchor(p.order, p.B);
     const uint64_t a = p.b0 + i;
     const int tid = threadIdx.x;
     if(tid < kWave){
@@ -549,6 +563,7 @@ struct AlignParams {
     int maxLen;
     int minOverlap; float maxErrorRate; float minOverlapRatio; float orientErrorRate;
     AlignBest* best; int* dbg;   // dbg: [total][11] or null
+    const uint32_t* order;       // processing order of the batch's anchors, or null
 };
 
 struct ShdResult { int score, overlap, shift, nOps, valid; };
@@ -940,8 +955,8 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
 template<int WM>
 __global__ void __launch_bounds__(kWave)
 k_align_lanes(AlignParams p){
-    const int i = blockIdx.x;
-    if(i >= p.B) return;
+    if(int(blockIdx.x) >= p.B) return;
+    const int i = batchAnchor(p.order, p.B);
     const uint32_t beg = p.offsets[i];
     const int n = int(p.offsets[i + 1] - beg);
     if(n == 0) return;
@@ -988,6 +1003,7 @@ struct MsaParams {
     // whether it was corrected as an HQ anchor
     int corrCands; int newCols;
     uint8_t* candSlots; int candSlotBytes; uint32_t* candBytes; uint8_t* hqAnchor;
+    const uint32_t* order; int B;   // processing order of the batch's anchors, or null
 };
 
 // Per-anchor candidate lists of the MSA stage: the staged kernels index batch-wide
@@ -2143,7 +2159,7 @@ k_msa_wave(MsaParams p){
     carveWaveTile(p, smem, sQw, tile, sSup);
     if(p.useQ) for(int q = lane; q < 256; q += kWave) sQw[q] = p.qweight[q];
     __syncthreads();
-    const int i = blockIdx.x;
+    const int i = batchAnchor(p.order, p.B);
     const int beg = int(p.offsets[i]);
     const int n0 = int(p.offsets[i + 1]) - beg;
     msaAnchorWave<KC>(p, listsOf(p), p.b0 + i, i, beg, n0, tile, sSup, lane);
@@ -2522,7 +2538,7 @@ struct care_ctx {
     bool hasReads = false;
     bool hasTables = false;
 
-    DBuf enc, len, qual, ambig, sig, hilo, qw, bucketOf, gath;
+    DBuf enc, len, qual, ambig, sig, hilo, qw, bucketOf, gath, order, orderFlags, batchOrd;
     std::vector<DBuf> slotBufs;
     std::vector<uint32_t> caps;
     DBuf slotPtrs, capsDev, values;
@@ -2740,6 +2756,7 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.featCap = ctx->featCap;
     mp.corrCands = 0; mp.newCols = ctx->o.new_columns_to_correct;
     mp.candSlots = nullptr; mp.candSlotBytes = 0; mp.candBytes = nullptr; mp.hqAnchor = nullptr;
+    mp.order = nullptr; mp.B = 0;
 }
 
 void fillCandCommon(const care_ctx* ctx, CandParams& cp){
@@ -2752,6 +2769,7 @@ void fillCandCommon(const care_ctx* ctx, CandParams& cp){
     while(P <= cp.capPerAnchor) P <<= 1;      // hash slots > the most values an anchor can gather
     cp.hashSlots = P;
     cp.gath = nullptr;
+    cp.order = nullptr;
 }
 
 void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
@@ -2759,6 +2777,7 @@ void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
     ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
     ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
     ap.maxLen = ctx->maxLen;
+    ap.order = nullptr;
 }
 
 int recordSlotBytes(int maxLen){
@@ -3005,6 +3024,42 @@ int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int 
     return CARE_OK;
 }
 
+// Processing order of the anchors of batch [b0, b0+B): the context's order restricted to
+// the batch, as batch indices.
+__global__ void k_order_flags(const uint32_t* __restrict__ order, uint64_t n, uint64_t b0, uint64_t B,
+                              uint8_t* __restrict__ flags){
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(t < n) flags[t] = order[t] >= b0 && order[t] < b0 + B;
+}
+
+__global__ void k_order_rebase(uint32_t* __restrict__ v, uint64_t m, uint32_t b0){
+    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if(t < m) v[t] -= b0;
+}
+
+int batchOrder(care_ctx* ctx, uint64_t b0, int B, const uint32_t** out){
+    hipStream_t st = ctx->stream;
+    if(b0 == 0 && uint64_t(B) == ctx->n){ *out = ctx->order.as<uint32_t>(); return CARE_OK; }
+    const uint64_t n = ctx->n;
+    HIPCHECK(ctx->orderFlags.ensure(n + 16));
+    HIPCHECK(ctx->batchOrd.ensure(size_t(B) * 4 + 16));
+    k_order_flags<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(ctx->order.as<uint32_t>(), n, b0, uint64_t(B),
+                                                                 ctx->orderFlags.as<uint8_t>());
+    HIPCHECK(hipGetLastError());
+    uint32_t* count = reinterpret_cast<uint32_t*>(ctx->batchOrd.as<uint8_t>() + size_t(B) * 4);
+    size_t tb = 0;
+    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, tb, ctx->order.as<uint32_t>(), ctx->orderFlags.as<uint8_t>(),
+                                           ctx->batchOrd.as<uint32_t>(), count, int(n), st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+    tb = ctx->scanTmp.bytes;
+    HIPCHECK(hipcub::DeviceSelect::Flagged(ctx->scanTmp.p, tb, ctx->order.as<uint32_t>(), ctx->orderFlags.as<uint8_t>(),
+                                           ctx->batchOrd.as<uint32_t>(), count, int(n), st));
+    k_order_rebase<<<dim3(uint32_t((B + 255) / 256)), 256, 0, st>>>(ctx->batchOrd.as<uint32_t>(), uint64_t(B), uint32_t(b0));
+    HIPCHECK(hipGetLastError());
+    *out = ctx->batchOrd.as<uint32_t>();
+    return CARE_OK;
+}
+
 int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     const int capPerAnchor = ctx->M * ctx->thr;
@@ -3027,6 +3082,12 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     cp.gathered = ctx->counters.as<unsigned long long>();
     HIPCHECK(ctx->gath.ensure(size_t(B) * 4));
     cp.gath = ctx->gath.as<uint32_t>();
+    const uint32_t* order = nullptr;
+    {
+        const int rc = batchOrder(ctx, b0, B, &order);
+        if(rc != CARE_OK) return rc;
+    }
+    cp.order = order;
     const size_t candLds = candLdsBytes(P, capPerAnchor);
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
     k_candidates<<<B, kBlock, candLds, st>>>(cp);
@@ -3069,6 +3130,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
     ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
     ap.offsets = ctx->offsets.as<uint32_t>(); ap.B = B; ap.maxLen = ctx->maxLen;
+    ap.order = order;
     const int wm = ctx->o.align_variant == 1 ? 0 : alignLaneWords(ctx->W);
     if(total){
         switch(wm){
@@ -3108,6 +3170,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     mp.msaDbg = ctx->o.keep_intermediates ? ctx->msaDbg.as<uint8_t>() : nullptr;
     mp.dbgStride = ctx->dbgStride;
     mp.error = ctx->counters.as<int>() + 2;
+    mp.order = order; mp.B = B;
     const bool cands = ctx->o.correct_candidates != 0;
     const int candSlotBytes = candidateSlotBytes(ctx->maxLen);
     if(cands){
@@ -3233,7 +3296,7 @@ void care_hip_destroy(care_ctx* ctx){
     if(!ctx) return;
     (void)hipSetDevice(ctx->device);
     if(ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->qw, &ctx->bucketOf, &ctx->gath, &ctx->slotPtrs,
+    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->qw, &ctx->bucketOf, &ctx->gath, &ctx->order, &ctx->orderFlags, &ctx->batchOrd, &ctx->slotPtrs,
                    &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
@@ -4319,7 +4382,24 @@ static int resolveBuckets(care_ctx* ctx){
                                                                     ctx->slotPtrs.as<const Slot*>(),
                                                                     ctx->capsDev.as<uint32_t>(), ctx->bucketOf.as<uint2>());
     HIPCHECK(hipGetLastError());
-    HIPCHECK(hipStreamSynchronize(st));
+    // processing order: reads sorted by their map-0 signature (reads that share it overlap)
+    {
+        struct Tmp { DBuf keys, ids; ~Tmp(){ keys.release(); ids.release(); } } tmp;
+        const uint64_t n = ctx->n;
+        HIPCHECK(ctx->order.ensure(n * 4));
+        HIPCHECK(tmp.keys.ensure(n * 8));
+        HIPCHECK(tmp.ids.ensure(n * 4));
+        k_iota<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(tmp.ids.as<uint32_t>(), n);
+        HIPCHECK(hipGetLastError());
+        size_t tb = 0;
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ctx->sig.as<uint64_t>(), tmp.keys.as<uint64_t>(),
+                                                    tmp.ids.as<uint32_t>(), ctx->order.as<uint32_t>(), int(n), 0, 64, st));
+        HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
+        tb = ctx->scanTmp.bytes;
+        HIPCHECK(hipcub::DeviceRadixSort::SortPairs(ctx->scanTmp.p, tb, ctx->sig.as<uint64_t>(), tmp.keys.as<uint64_t>(),
+                                                    tmp.ids.as<uint32_t>(), ctx->order.as<uint32_t>(), int(n), 0, 64, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
     return CARE_OK;
 }
 
