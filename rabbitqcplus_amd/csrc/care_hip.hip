@@ -830,12 +830,14 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
     const int W = p.W;
     const int La = p.len[a];
     const uint32_t* arow = p.hilo + a * uint64_t(4 * W);
-    uint32_t AH[2 * WM + 2], AL[2 * WM + 2];
+    // anchor planes and its validity plane (positions < La), zero words on both sides
+    uint32_t AH[2 * WM + 2], AL[2 * WM + 2], AV[2 * WM + 2];
     #pragma unroll
     for(int i = 0; i < 2 * WM + 2; i++){
         const bool in = i >= 1 && i <= W;
         AH[i] = in ? arow[i - 1] : 0u;
         AL[i] = in ? arow[W + i - 1] : 0u;
+        AV[i] = in ? overlapMask(La - 32 * (i - 1)) : 0u;
     }
     const int mo = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
     const int lastF = La - mo;
@@ -844,7 +846,7 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
         const bool valid = c < n;
         uint32_t id = 0;
         int Lc = 0;
-        uint32_t FH[WM], FL[WM], RH[WM], RL[WM];
+        uint32_t FH[WM], FL[WM], RH[WM], RL[WM], CV[WM];
         if(valid){
             id = ids[c];
             Lc = p.len[id];
@@ -857,10 +859,13 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
             FL[w] = in ? crow[W + w] : 0u;
             RH[w] = in ? crow[2 * W + w] : 0u;
             RL[w] = in ? crow[3 * W + w] : 0u;
+            CV[w] = in ? overlapMask(Lc - 32 * w) : 0u;
         }
         const int total = La + Lc;
-        int bF = total, sF = -Lc, bR = total, sR = -Lc;
-        // anchor shifted: shifts 0..lastF
+        // gF, gR: best score minus total (score = total - 2 * overlap + mismatches)
+        int gF = 0, sF = -Lc, gR = 0, sR = -Lc;
+        // anchor shifted: shifts 0..lastF. A position counts when it lies inside both the
+        // candidate (CV) and the shifted anchor (shifted AV): exactly the overlap
         bool oF = valid, oR = valid;
         #pragma unroll
         for(int ps = 0; ps <= WM; ps++){
@@ -870,8 +875,8 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                 const int sft = 32 * ps + r;
                 const int ov = min(La - sft, Lc);
                 const int e = int(float(ov) * p.maxErrorRate);
-                const int mF = min(e, bF - total + 2 * ov);
-                const int mR = min(e, bR - total + 2 * ov);
+                const int mF = min(e, gF + 2 * ov);
+                const int mR = min(e, gR + 2 * ov);
                 oF = oF && mF > 0;
                 oR = oR && mR > 0;
                 if(__builtin_amdgcn_ballot_w64(oF || oR) == 0) break;
@@ -883,13 +888,13 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                     if(w == 2 && __builtin_amdgcn_ballot_w64((oF && dF < mF) || (oR && dR < mR)) == 0) break;
                     const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
                     const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
-                    // top min(32, max(0, ov - 32w)) bits: positions [32w, 32w+32) inside the overlap
-                    const uint32_t m = overlapMask(ov - 32 * w);
+                    const uint32_t xv = uint32_t((((uint64_t(AV[w + ps + 1]) << 32) | AV[w + ps + 2]) << r) >> 32);
+                    const uint32_t m = xv & CV[w];
                     dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                     dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
                 }
-                if(oF && dF < mF){ bF = dF + total - 2 * ov; sF = sft; }
-                if(oR && dR < mR){ bR = dR + total - 2 * ov; sR = sft; }
+                if(oF && dF < mF){ gF = dF - 2 * ov; sF = sft; }
+                if(oR && dR < mR){ gR = dR - 2 * ov; sR = sft; }
             }
         }
         // candidate shifted: shifts -1..-(Lc-mo), here t = 1..Lc-mo with the anchor
@@ -905,28 +910,28 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                 const int t = 32 * pt + r;
                 const int ov = min(La, Lc - t);
                 const int e = int(float(ov) * p.maxErrorRate);
-                const int mF = min(e, bF - total + 2 * ov);
-                const int mR = min(e, bR - total + 2 * ov);
+                const int mF = min(e, gF + 2 * ov);
+                const int mR = min(e, gR + 2 * ov);
                 const bool inRange = t <= lastB;
                 oF = oF && inRange && mF > 0;
                 oR = oR && inRange && mR > 0;
                 if(__builtin_amdgcn_ballot_w64(oF || oR) == 0) break;
-                const int end = t + ov;       // window [t, end) in candidate positions
                 int dF = 0, dR = 0;
                 #pragma unroll
                 for(int w = pt; w < WM; w++){
                     if(w == pt + 2 && __builtin_amdgcn_ballot_w64((oF && dF < mF) || (oR && dR < mR)) == 0) break;
                     const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
                     const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
-                    uint32_t m = overlapMask(end - 32 * w);
-                    if(w == pt) m &= 0xFFFFFFFFu >> r;
+                    const uint32_t xv = uint32_t(((uint64_t(AV[w - pt]) << 32) | AV[w - pt + 1]) >> r);
+                    const uint32_t m = xv & CV[w];
                     dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                     dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
                 }
-                if(oF && dF < mF){ bF = dF + total - 2 * ov; sF = -t; }
-                if(oR && dR < mR){ bR = dR + total - 2 * ov; sR = -t; }
+                if(oF && dF < mF){ gF = dF - 2 * ov; sF = -t; }
+                if(oR && dR < mR){ gR = dR - 2 * ov; sR = -t; }
             }
         }
+        const int bF = gF + total, bR = gR + total;
         if(!valid) continue;
         ShdResult f, rr;
         {
