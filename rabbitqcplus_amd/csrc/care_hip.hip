@@ -865,11 +865,14 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
         // gF, gR: best score minus total (score = total - 2 * overlap + mismatches)
         int gF = 0, sF = -Lc, gR = 0, sR = -Lc;
         // anchor shifted: shifts 0..lastF. A position counts when it lies inside both the
-        // candidate (CV) and the shifted anchor (shifted AV): exactly the overlap
-        bool oF = valid, oR = valid;
+        // candidate (CV) and the shifted anchor (shifted AV): exactly the overlap. The
+        // lanes whose scan in an orientation is still running are kept as wave masks
+        // (scalar registers); a lane's bit is read back with an inverse ballot.
+        const uint64_t start = __builtin_amdgcn_ballot_w64(valid);
+        uint64_t liveF = start, liveR = start;
         #pragma unroll
         for(int ps = 0; ps <= WM; ps++){
-            if(32 * ps > lastF || __ballot(oF || oR) == 0) break;
+            if(32 * ps > lastF || (liveF | liveR) == 0) break;
             const int rEnd = min(31, lastF - 32 * ps);
             for(int r = 0; r <= rEnd; r++){
                 const int sft = 32 * ps + r;
@@ -877,15 +880,16 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                 const int e = int(float(ov) * p.maxErrorRate);
                 const int mF = min(e, gF + 2 * ov);
                 const int mR = min(e, gR + 2 * ov);
-                oF = oF && mF > 0;
-                oR = oR && mR > 0;
-                if(__builtin_amdgcn_ballot_w64(oF || oR) == 0) break;
+                liveF &= __builtin_amdgcn_ballot_w64(mF > 0);
+                liveR &= __builtin_amdgcn_ballot_w64(mR > 0);
+                if((liveF | liveR) == 0) break;
                 int dF = 0, dR = 0;
                 #pragma unroll
                 for(int w = 0; w < WM; w++){
                     // once no lane can still stay below its bound, the remaining words
                     // cannot make any shift admissible (mismatch counts only grow)
-                    if(w == 2 && __builtin_amdgcn_ballot_w64((oF && dF < mF) || (oR && dR < mR)) == 0) break;
+                    if(w == 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                  (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
                     const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
                     const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
                     const uint32_t xv = uint32_t((((uint64_t(AV[w + ps + 1]) << 32) | AV[w + ps + 2]) << r) >> 32);
@@ -893,18 +897,22 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                     dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                     dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
                 }
-                if(oF && dF < mF){ gF = dF - 2 * ov; sF = sft; }
-                if(oR && dR < mR){ gR = dR - 2 * ov; sR = sft; }
+                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
+                    gF = dF - 2 * ov; sF = sft;
+                }
+                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
+                    gR = dR - 2 * ov; sR = sft;
+                }
             }
         }
         // candidate shifted: shifts -1..-(Lc-mo), here t = 1..Lc-mo with the anchor
         // shifted right by t against the candidate's positions [t, t+ov)
         const int lastB = Lc - mo;
         const int lastBw = p.maxLen - mo;
-        oF = valid; oR = valid;
+        liveF = start; liveR = start;
         #pragma unroll
         for(int pt = 0; pt <= WM; pt++){
-            if(32 * pt > lastBw || __ballot(oF || oR) == 0) break;
+            if(32 * pt > lastBw || (liveF | liveR) == 0) break;
             const int rEnd = min(31, lastBw - 32 * pt);
             for(int r = (pt == 0 ? 1 : 0); r <= rEnd; r++){
                 const int t = 32 * pt + r;
@@ -912,14 +920,15 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                 const int e = int(float(ov) * p.maxErrorRate);
                 const int mF = min(e, gF + 2 * ov);
                 const int mR = min(e, gR + 2 * ov);
-                const bool inRange = t <= lastB;
-                oF = oF && inRange && mF > 0;
-                oR = oR && inRange && mR > 0;
-                if(__builtin_amdgcn_ballot_w64(oF || oR) == 0) break;
+                const uint64_t inRange = __builtin_amdgcn_ballot_w64(t <= lastB);
+                liveF &= inRange & __builtin_amdgcn_ballot_w64(mF > 0);
+                liveR &= inRange & __builtin_amdgcn_ballot_w64(mR > 0);
+                if((liveF | liveR) == 0) break;
                 int dF = 0, dR = 0;
                 #pragma unroll
                 for(int w = pt; w < WM; w++){
-                    if(w == pt + 2 && __builtin_amdgcn_ballot_w64((oF && dF < mF) || (oR && dR < mR)) == 0) break;
+                    if(w == pt + 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                       (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
                     const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
                     const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
                     const uint32_t xv = uint32_t(((uint64_t(AV[w - pt]) << 32) | AV[w - pt + 1]) >> r);
@@ -927,8 +936,12 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
                     dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                     dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
                 }
-                if(oF && dF < mF){ gF = dF - 2 * ov; sF = -t; }
-                if(oR && dR < mR){ gR = dR - 2 * ov; sR = -t; }
+                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
+                    gF = dF - 2 * ov; sF = -t;
+                }
+                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
+                    gR = dR - 2 * ov; sR = -t;
+                }
             }
         }
         const int bF = gF + total, bR = gR + total;
