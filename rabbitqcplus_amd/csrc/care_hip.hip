@@ -477,18 +477,22 @@ k_candidates(CandParams p){
     }
 }
 
-// Sum of the per-anchor gathered-value counts of a batch into the pass counter.
+// Sum of the per-anchor gathered-value counts of a batch into the pass counter: each
+// block sums kAddPerBlock counts and adds them with one atomic.
+constexpr int kAddPerBlock = 16 * kBlock;
+
 __global__ void k_add_gathered(const uint32_t* __restrict__ gath, int B, unsigned long long* __restrict__ total){
     __shared__ unsigned long long part[kBlock / kWave];
     unsigned long long s = 0;
-    for(int q = threadIdx.x; q < B; q += blockDim.x) s += gath[q];
+    const int lo = blockIdx.x * kAddPerBlock, hi = min(B, lo + kAddPerBlock);
+    for(int q = lo + int(threadIdx.x); q < hi; q += blockDim.x) s += gath[q];
     for(int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
     if((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
     __syncthreads();
     if(threadIdx.x == 0){
         unsigned long long t = 0;
         for(int w = 0; w < int(blockDim.x / kWave); w++) t += part[w];
-        *total += t;
+        if(t) atomicAdd(total, t);
     }
 }
 
@@ -3110,7 +3114,8 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
     k_candidates<<<B, kBlock, candLds, st>>>(cp);
     HIPCHECK(hipGetLastError());
-    k_add_gathered<<<1, kBlock, 0, st>>>(ctx->gath.as<uint32_t>(), B, ctx->counters.as<unsigned long long>());
+    k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(ctx->gath.as<uint32_t>(), B,
+                                                                          ctx->counters.as<unsigned long long>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
     size_t tb = 0;
