@@ -1545,8 +1545,7 @@ __device__ __forceinline__ uint64_t lanesBelow(int lane){ return lane ? (~0ull >
 template<int KC>
 struct WaveMsa {
     float w[KC][4];
-    uint32_t c01[KC];   // count of A (low 16 bits) and C (high 16 bits)
-    uint32_t c23[KC];   // count of G and T
+    uint64_t cnt[KC];   // counts of A, C, G, T in 16-bit fields (low to high)
     float sup[KC];
     uint32_t cons;      // 2 bits per slot
     uint32_t abase;     // 2 bits per slot: anchor base of anchor columns
@@ -1563,8 +1562,7 @@ __device__ __forceinline__ float weightOf(const WaveMsa<KC>& m, int k, int b){
 
 template<int KC>
 __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
-    const uint32_t v = b < 2 ? m.c01[k] : m.c23[k];
-    return int((b & 1) ? (v >> 16) : (v & 0xFFFFu));
+    return int(uint32_t(m.cnt[k] >> (16 * b)) & 0xFFFFu);
 }
 
 // MSA of the anchor and the first nk working candidates (src/msa.cpp:9-173), one wave:
@@ -1593,8 +1591,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     for(int k = 0; k < KC; k++){
         #pragma unroll
         for(int b = 0; b < 4; b++) m.w[k][b] = 0.0f;
-        m.c01[k] = 0;
-        m.c23[k] = 0;
+        m.cnt[k] = 0;
         const int pa = lane + kWave * k - ab;
         if(pa >= 0 && pa < La){
             const int b = baseAt(arow, pa);
@@ -1603,8 +1600,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
             m.w[k][1] = b == 1 ? x : 0.0f;
             m.w[k][2] = b == 2 ? x : 0.0f;
             m.w[k][3] = b == 3 ? x : 0.0f;
-            m.c01[k] = b == 0 ? 1u : (b == 1 ? 0x10000u : 0u);
-            m.c23[k] = b == 2 ? 1u : (b == 3 ? 0x10000u : 0u);
+            m.cnt[k] = uint64_t(1) << (16 * b);
             m.abase |= uint32_t(b) << (2 * k);
             m.isAnchor |= 1u << k;
         }
@@ -1637,9 +1633,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
                     m.w[k][1] += b == 1 ? x : 0.0f;
                     m.w[k][2] += b == 2 ? x : 0.0f;
                     m.w[k][3] += b == 3 ? x : 0.0f;
-                    const uint32_t inc = 1u << (16 * (b & 1));
-                    m.c01[k] += b < 2 ? inc : 0u;
-                    m.c23[k] += b < 2 ? 0u : inc;
+                    m.cnt[k] += uint64_t(1) << (16 * b);
                 }
             }
         }
