@@ -232,7 +232,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant),
                 "kernel": dominant, "avg_launch_ms": round(kernel_ms[dominant], 4),
-                "algorithmic_bytes_per_launch": int(per_launch[dominant]), "launches_per_step": int(batches)}
+                "algorithmic_bytes_per_launch": int(per_launch[dominant]), "launches_per_step": int(batches),
+                "limiter": "vector ALU issue and LDS/memory latency, not HBM bandwidth "
+                           "(counters: profiles/pmc_staged_se_5mbp_150bp_1M.md)"}
     plain_err = evaluate.error_counts(s, rs, data)
     log(f"GPU plain: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
         f"errors {plain_err}")
