@@ -1567,7 +1567,11 @@ __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
 
 // MSA of the anchor and the first nk working candidates (src/msa.cpp:9-173), one wave:
 // every lane adds the anchor then the candidates, in list order, to the columns it owns.
-template<int KC>
+// AO (anchor columns only): slot column c is anchor position c, and only the columns
+// under the anchor are built; every later step of the plain correction (refinement,
+// properties, decision, features) reads no other column. Otherwise slot column c is MSA
+// column c (anchor position c - ab), all ncol columns.
+template<int KC, bool AO>
 __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int La,
                                              const uint32_t* arow, int beg, int nk,
                              WaveMsa<KC>& m, const MsaTile& t, int lane){
@@ -1581,18 +1585,19 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     hi = waveReduce(hi, OpMax(), INT_MIN);
     const int ncol = hi - lo;
     const int ab = max(-lo, 0);
-    if(ncol > KC * kWave || ncol > p.maxCols) return false;
+    if((AO ? La : ncol) > KC * kWave || ncol > p.maxCols) return false;
     m.ncol = ncol;
     m.ab = ab;
     m.abase = 0;
     m.isAnchor = 0;
+    const int off = AO ? 0 : ab;       // slot column of anchor position 0
     const uint8_t* aq = p.useQ ? p.qual + p.qdpitch * a : nullptr;
     #pragma unroll
     for(int k = 0; k < KC; k++){
         #pragma unroll
         for(int b = 0; b < 4; b++) m.w[k][b] = 0.0f;
         m.cnt[k] = 0;
-        const int pa = lane + kWave * k - ab;
+        const int pa = lane + kWave * k - off;
         if(pa >= 0 && pa < La){
             const int b = baseAt(arow, pa);
             const float x = 1.0f * (p.useQ ? t.qw[aq[pa]] : 1.0f);
@@ -1613,11 +1618,13 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
         stageTile(p, lst, t, beg, tb, tn, lane, kWave);
         __syncthreads();
         for(int j = 0; j < tn; j++){
-            const int cs = ab + t.shift[j];          // first column of candidate j (>= 0)
+            const int cs = off + t.shift[j];         // slot column of candidate j's first base
             const int ln = t.len[j];
             const float wj = t.w[j];
-            const int k0 = cs >> 6;
-            const int k1 = (cs + ln - 1) >> 6;
+            // slots holding its columns (AO: those under the anchor; lanes past the anchor
+            // end add to columns that are never read)
+            const int k0 = (AO ? max(cs, 0) : cs) >> 6;
+            const int k1 = ((AO ? min(cs + ln, La) : cs + ln) - 1) >> 6;
             #pragma unroll
             for(int k = 0; k < KC; k++){
                 if(k < k0 || k > k1) continue;        // wave-uniform
@@ -1649,7 +1656,8 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
         if(m.w[k][3] > cw){ cb = 3; cw = m.w[k][3]; }
         m.cons |= uint32_t(cb) << (2 * k);
         const float colW = m.w[k][0] + m.w[k][1] + m.w[k][2] + m.w[k][3];
-        m.sup[k] = (lane + kWave * k < ncol) ? cw / colW : 0.0f;
+        const bool inMsa = AO ? ((m.isAnchor >> k) & 1u) != 0 : lane + kWave * k < ncol;
+        m.sup[k] = inMsa ? cw / colW : 0.0f;
     }
     return true;
 }
@@ -1783,7 +1791,7 @@ __device__ __forceinline__ void emitCandidateRecords(const MsaParams& p, const A
 // Filters, MSA, refinement, decision and record of anchor `a` by one wavefront
 // (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734). Its candidates
 // are lst.ids/lst.best/lst.paired[beg, beg+n0); the record goes to slot `i`.
-template<int KC>
+template<int KC, bool AO>
 __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int i, int beg,
                                               int n0, const MsaTile& tile,
                               float* sSup, int lane){
@@ -1867,7 +1875,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
     WaveMsa<KC> m;
     int nk = n2;
-    if(!buildMsaWave<KC>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
+    if(!buildMsaWave<KC, AO>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
         if(lane == 0) atomicAdd(p.error, 1);
         return;
     }
@@ -1887,19 +1895,20 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
             if(sig != -1) colCand = min(colCand, lane + kWave * k);
         }
         if(__ballot(mis) == 0) break;
-        const int col = waveReduce(colCand, OpMin(), INT_MAX);
-        if(col == INT_MAX) break;
+        const int scol = waveReduce(colCand, OpMin(), INT_MAX);     // slot column
+        if(scol == INT_MAX) break;
+        const int col = scol + (AO ? m.ab : 0);                     // MSA column
         int mySig = -1;
         #pragma unroll
         for(int k = 0; k < KC; k++){
-            if(k == (col >> 6)){
+            if(k == (scol >> 6)){
                 const int ci = int((m.cons >> (2 * k)) & 3u);
                 for(int b = 0; b < 4; b++){
                     if(b != ci && p.sigThr <= countOf(m, k, b)) mySig = b;
                 }
             }
         }
-        const int sigBase = __builtin_amdgcn_readlane(mySig, col & 63);
+        const int sigBase = __builtin_amdgcn_readlane(mySig, scol & 63);
         const int origBase = baseAt(arow, col - m.ab);
         const bool keepMatching = origBase == sigBase;
         bool anyRemove = false, veryGood = false;
@@ -1944,14 +1953,15 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
         }
         nk = nn;
         rounds++;
-        if(!buildMsaWave<KC>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
+        if(!buildMsaWave<KC, AO>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
             if(lane == 0) atomicAdd(p.error, 1);
             return;
         }
     }
     const int ab = m.ab;
     const int ncol = m.ncol;
-    if(p.msaDbg){
+    const int off = AO ? 0 : ab;       // slot column of anchor position 0
+    if(!AO && p.msaDbg){
         uint8_t* d = p.msaDbg + size_t(i) * p.dbgStride;
         const int C = p.maxCols;
         int* dn = reinterpret_cast<int*>(d);
@@ -1987,7 +1997,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
             const int cv = countOf(m, k, 0) + countOf(m, k, 1) + countOf(m, k, 2) + countOf(m, k, 3);
             mincov = min(mincov, cv);
             maxcov = max(maxcov, cv);
-            sSup[c - ab] = m.sup[k];
+            sSup[c - off] = m.sup[k];
         }
     }
     const float minSupport = __int_as_float(waveReduce(minSupBits, OpMin(), INT_MAX));
@@ -2033,7 +2043,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
             if(mask == 0) continue;
             const int n = __popcll(mask);
             const int e = __popcll(mask & below);
-            const int i = lane + kWave * k - ab;
+            const int i = lane + kWave * k - off;
             unsigned long long fbase = 0;
             if(p.featOut){
                 if(lane == 0) fbase = atomicAdd(p.featCount, (unsigned long long)n);
@@ -2121,7 +2131,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
         info[9] = __float_as_int(avgSupport); info[10] = __float_as_int(minSupport);
     }
     // corrected (it has candidates) as an HQ anchor (include/corrector.hpp:284-290)
-    if(p.corrCands){
+    if(!AO && p.corrCands){
         if(lane == 0) p.hqAnchor[a] = isHQ;
         if(isHQ) emitCandidateRecords<KC>(p, lst, m.cons, m.ncol, m.ab, beg, nk, La, tile, lane);
     }
@@ -2147,7 +2157,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     #pragma unroll
     for(int k = 0; k < KC; k++){
         const bool anc = (m.isAnchor >> k) & 1u;
-        const int q = lane + kWave * k - ab;
+        const int q = lane + kWave * k - off;
         const int cb = int((corrBases >> (2 * k)) & 3u);
         if(useEdits){
             const bool e = (editBits >> k) & 1u;
@@ -2164,7 +2174,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     }
 }
 
-template<int KC>
+template<int KC, bool AO>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 4 : (KC <= 12 ? 3 : 2))))
 k_msa_wave(MsaParams p){
     extern __shared__ uint8_t smem[];
@@ -2178,7 +2188,7 @@ k_msa_wave(MsaParams p){
     const int i = batchAnchor(p.order, p.B);
     const int beg = int(p.offsets[i]);
     const int n0 = int(p.offsets[i + 1]) - beg;
-    msaAnchorWave<KC>(p, listsOf(p), p.b0 + i, i, beg, n0, tile, sSup, lane);
+    msaAnchorWave<KC, AO>(p, listsOf(p), p.b0 + i, i, beg, n0, tile, sSup, lane);
 }
 
 // ---------------------------------------------------------------- fused persistent kernel
@@ -2405,7 +2415,7 @@ k_fused(FusedParams fp){
             const int U = candidatesWave(fp.c, a, hset, sOff, sPre, &sUnique, ws.idsA, lane, gathered);
             aligned += uint64_t(U);
             alignAllWave(fp.al, a, ws.idsA, U, ws.best, planeA, planeC, lane);
-            msaAnchorWave<KC>(fp.m, lst, a, int(unit), 0, U, tile, sSup, lane);
+            msaAnchorWave<KC, true>(fp.m, lst, a, int(unit), 0, U, tile, sSup, lane);
         }else{
             const uint64_t a0 = fp.first + 2 * uint64_t(unit);
             const int U0 = candidatesWave(fp.c, a0, hset, sOff, sPre, &sUnique, ws.idsA, lane, gathered);
@@ -2422,7 +2432,7 @@ k_fused(FusedParams fp){
                 lst.paired = mate == 0 ? ws.pairedA : ws.pairedB;
                 const int U = mate == 0 ? U0 : U1;
                 alignAllWave(fp.al, a, lst.ids, U, ws.best, planeA, planeC, lane);
-                msaAnchorWave<KC>(fp.m, lst, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
+                msaAnchorWave<KC, true>(fp.m, lst, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
                 __syncthreads();
             }
         }
@@ -2722,6 +2732,19 @@ int waveMsaColumnsPerLane(int maxCols){
     return 0;
 }
 
+// Register columns per lane of k_msa_wave when only the columns under the anchor are
+// built (reads of at most maxLen bases; 0: too long).
+int waveMsaAnchorSlots(int maxLen){
+    const int need = (maxLen + kWave - 1) / kWave;
+    if(need <= 2) return 2;
+    if(need <= 3) return 3;
+    if(need <= 4) return 4;
+    if(need <= 8) return 8;
+    if(need <= 12) return 12;
+    if(need <= 16) return 16;
+    return 0;
+}
+
 // dynamic LDS of k_msa_wave: candidate tile + anchor-column supports
 size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
     size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * qStride(maxLen) : 0);
@@ -2866,7 +2889,7 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     hipStream_t st = ctx->stream;
     const bool paired = ctx->o.paired != 0;
     const uint64_t units = paired ? n / 2 : n;
-    const int kc = waveMsaColumnsPerLane(ctx->maxCols);
+    const int kc = waveMsaAnchorSlots(ctx->maxLen);
     if(n == 0) return CARE_OK;
     if(n > uint64_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "too many anchors for one fused pass");
     const int B = int(n);
@@ -2906,6 +2929,8 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
                                  waveMsaLdsBytes(fp.m.tile, ctx->W, ctx->maxLen, q)});
     auto kernelFor = [&]() -> const void* {
         switch(kc){
+            case 2: return paired ? (const void*)k_fused<2, true> : (const void*)k_fused<2, false>;
+            case 3: return paired ? (const void*)k_fused<3, true> : (const void*)k_fused<3, false>;
             case 4: return paired ? (const void*)k_fused<4, true> : (const void*)k_fused<4, false>;
             case 8: return paired ? (const void*)k_fused<8, true> : (const void*)k_fused<8, false>;
             case 12: return paired ? (const void*)k_fused<12, true> : (const void*)k_fused<12, false>;
@@ -3197,17 +3222,31 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
         mp.candSlots = ctx->candSlots.as<uint8_t>(); mp.candSlotBytes = candSlotBytes;
         mp.candBytes = ctx->candBytes.as<uint32_t>(); mp.hqAnchor = ctx->hqAnchor.as<uint8_t>();
     }
-    const int kc = waveMsaColumnsPerLane(maxCols);
+    // the whole MSA is kept for candidate correction and the intermediate dumps; the
+    // plain correction builds the columns under the anchor only
+    const bool wholeMsa = cands || ctx->o.keep_intermediates;
+    const int kc = wholeMsa ? waveMsaColumnsPerLane(maxCols) : waveMsaAnchorSlots(ctx->maxLen);
     if(kc > 0){
         // one wavefront per anchor, MSA columns in registers
         const bool q = ctx->o.use_quality_scores != 0;
         mp.tile = q ? 32 : 64;
         const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q);
-        switch(kc){
-            case 4: k_msa_wave<4><<<B, kWave, lds, st>>>(mp); break;
-            case 8: k_msa_wave<8><<<B, kWave, lds, st>>>(mp); break;
-            case 12: k_msa_wave<12><<<B, kWave, lds, st>>>(mp); break;
-            default: k_msa_wave<16><<<B, kWave, lds, st>>>(mp); break;
+        if(wholeMsa){
+            switch(kc){
+                case 4: k_msa_wave<4, false><<<B, kWave, lds, st>>>(mp); break;
+                case 8: k_msa_wave<8, false><<<B, kWave, lds, st>>>(mp); break;
+                case 12: k_msa_wave<12, false><<<B, kWave, lds, st>>>(mp); break;
+                default: k_msa_wave<16, false><<<B, kWave, lds, st>>>(mp); break;
+            }
+        }else{
+            switch(kc){
+                case 2: k_msa_wave<2, true><<<B, kWave, lds, st>>>(mp); break;
+                case 3: k_msa_wave<3, true><<<B, kWave, lds, st>>>(mp); break;
+                case 4: k_msa_wave<4, true><<<B, kWave, lds, st>>>(mp); break;
+                case 8: k_msa_wave<8, true><<<B, kWave, lds, st>>>(mp); break;
+                case 12: k_msa_wave<12, true><<<B, kWave, lds, st>>>(mp); break;
+                default: k_msa_wave<16, true><<<B, kWave, lds, st>>>(mp); break;
+            }
         }
     }else{
         const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen, ctx->W, mp.tile, ctx->o.use_quality_scores != 0);
