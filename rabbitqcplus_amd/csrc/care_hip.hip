@@ -216,17 +216,18 @@ k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restri
 
 // HiLo planes (include/sequencehelpers.hpp:646-689): bit 31-(i%32) of word i/32 holds
 // the high (hi plane) / low (lo plane) bit of base i. Layout per read:
-// [fwd hi W][fwd lo W][rc hi W][rc lo W].
+// [fwd hi W][fwd lo W][rc hi W][rc lo W]. Also the reverse complement in the 2-bit row
+// format of `enc` (base i at bits 31-2(i%16) of word i/16), 2W words per read.
 __global__ void __launch_bounds__(256)
 k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n, int W,
-       uint32_t* __restrict__ hilo){
+       uint32_t* __restrict__ hilo, uint32_t* __restrict__ rcenc){
     const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(t >= n * uint64_t(W)) return;
     const uint64_t r = t / W;
     const int w = int(t % W);
     const int L = len[r];
     const uint32_t* row = enc + pitch * r;
-    uint32_t fh = 0, fl = 0, rh = 0, rl = 0;
+    uint32_t fh = 0, fl = 0, rh = 0, rl = 0, r0 = 0, r1 = 0;
     for(int j = 0; j < 32; j++){
         const int i = w * 32 + j;
         if(i < L){
@@ -236,6 +237,8 @@ k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ l
             fl |= uint32_t(b & 1) << (31 - j);
             rh |= uint32_t(rb >> 1) << (31 - j);
             rl |= uint32_t(rb & 1) << (31 - j);
+            if(j < 16) r0 |= uint32_t(rb) << (30 - 2 * j);
+            else r1 |= uint32_t(rb) << (30 - 2 * (j - 16));
         }
     }
     uint32_t* out = hilo + r * uint64_t(4 * W);
@@ -243,6 +246,9 @@ k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ l
     out[W + w] = fl;
     out[2 * W + w] = rh;
     out[3 * W + w] = rl;
+    uint32_t* rout = rcenc + r * uint64_t(2 * W);
+    rout[2 * w] = r0;
+    rout[2 * w + 1] = r1;
 }
 
 // Rows of `sp` bytes to rows of `dp` bytes: truncated, or zero padded past sp.
@@ -1011,6 +1017,7 @@ struct MsaParams {
     float avgThr, minThr, minCovThr; int pct; float pctHalf; int sigThr;
     int maxCols; int maxLen;
     const uint32_t* hilo; int W; int tile;
+    const uint32_t* rcenc;          // reverse-complement 2-bit rows, 2W words per read
     uint8_t* slots; int slotBytes; uint32_t* recBytes;
     int* info;                      // [B][kAnchorInfo] or null
     uint8_t* msaDbg; int dbgStride; // per anchor: 1 int ncol + cols*(4 int + 4 float + int + float + 1 byte)
@@ -1063,27 +1070,39 @@ __device__ __forceinline__ float orientedQualityWeight(const MsaParams& p, uint3
 }
 
 // LDS staging area for a tile of candidates (order kept): shift, length, overlap weight,
-// oriented HiLo planes (W words each) and oriented qualities (maxLen bytes each).
+// oriented bases (2-bit rows of tileRowWords words, base i at bits 31-2(i%16) of word
+// i/16) and qualities (qStride bytes each; see stageTile for their placement).
 struct MsaTile {
     int* shift;
     int* len;
     float* w;
     uint32_t* id;
     int* orient;
-    uint32_t* hi;
-    uint32_t* lo;
+    uint32_t* rows;      // 2W words reserved per candidate
     uint8_t* q;
     const float* qw;     // quality-weight table in LDS
 };
 
-// Row stride of the tile's quality rows: 16-byte aligned, so that forward rows are
-// written with one 16-byte store per chunk.
+// Row stride of the tile's quality rows: 16-byte aligned, so that rows are written with
+// one 16-byte store per chunk.
 __host__ __device__ inline int qStride(int maxLen){ return (maxLen + 15) & ~15; }
 
+// 2-bit words of a tile row
+__host__ __device__ inline int tileRowWords(int maxLen){ return (maxLen + 15) >> 4; }
+
+// Offset of position 0 of a reverse-complement candidate's quality row in its tile slot:
+// the reversed row is stored right-aligned (quality rows are maxLen = qpitch bytes, so
+// the slot of qStride(maxLen) bytes holds the whole 16-byte-chunked row).
+__device__ __forceinline__ int rcQualityOffset(const MsaParams& p){ return qStride(p.maxLen) - int(p.qpitch); }
+
 // Stages candidates [tb, tb+tn) of the working list into the LDS tile: one thread per
-// candidate for the metadata and its oriented HiLo planes (8-byte loads); qualities as
-// 16-byte chunks of the device rows, spread over the threads, reverse-complement rows
-// reversed byte by byte. Caller synchronizes before (tile free) and after (tile complete).
+// candidate for the metadata and its oriented 2-bit row (the stored row or its reverse
+// complement); qualities as 16-byte chunks of the device rows, spread over the threads. A
+// reverse-complement candidate reads the pitch-wide quality row reversed
+// (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239): each chunk is
+// byte-reversed in registers and stored mirrored in the slot, so that position pos of the
+// reversed row sits at rcQualityOffset + pos. Caller synchronizes before (tile free) and
+// after (tile complete).
 __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists& lst, const MsaTile& t, int beg,
                                           int tb, int tn, int tid, int nthreads){
     const int W = p.W;
@@ -1096,39 +1115,30 @@ __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists&
         t.w[q] = lst.kW[d];
         t.id[q] = id;
         t.orient[q] = orient;
-        // planes [hi W words | lo W words] of the orientation: W pairs, 8-byte aligned (rows
-        // are 16W bytes, the reverse-complement planes start 8W bytes in)
-        const uint2* src = reinterpret_cast<const uint2*>(p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 1 ? 0 : 2 * W));
-        for(int k2 = 0; k2 < W; k2++){
-            const uint2 v = src[k2];
-            const int w0 = 2 * k2, w1 = 2 * k2 + 1;
-            if(w0 < W) t.hi[q * W + w0] = v.x; else t.lo[q * W + w0 - W] = v.x;
-            if(w1 < W) t.hi[q * W + w1] = v.y; else t.lo[q * W + w1 - W] = v.y;
-        }
+        // words [0, RW) of the oriented row (the enc pitch holds at least RW words)
+        const uint32_t* src = orient == 1 ? p.enc + p.pitch * id : p.rcenc + uint64_t(id) * uint64_t(2 * W);
+        const int RW = tileRowWords(p.maxLen);
+        for(int k = 0; k < RW; k++) t.rows[q * 2 * W + k] = src[k];
     }
     if(!p.useQ) return;
     __syncthreads();
     const int qs = qStride(p.maxLen);
-    const int L = p.maxLen;
-    const int qp = int(p.qpitch);
-    const int nc = (qp + 15) >> 4;       // chunks per row; device rows are qdpitch >= 16*nc bytes
+    const int nc = qs >> 4;              // chunks per row; device rows are qdpitch >= qs bytes
+    // u / nc as a multiply-high: exact for u * nc < 2^32
+    const uint64_t magic = ((uint64_t(1) << 32) + uint64_t(nc) - 1) / uint64_t(nc);
     for(int u = tid; u < tn * nc; u += nthreads){
-        const int j = u / nc;
+        const int j = int((uint64_t(u) * magic) >> 32);
         const int c = u - j * nc;
         const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * t.id[j] + 16 * c);
         uint8_t* dst = t.q + j * qs;
         if(t.orient[j] == 1){
-            // positions past maxLen are never read; the chunk fits the row when 16c < qs
-            if(16 * c < qs) *reinterpret_cast<uint4*>(dst + 16 * c) = v;
+            *reinterpret_cast<uint4*>(dst + 16 * c) = v;
         }else{
-            // reverse-complement candidates read the pitch-wide quality row reversed
-            // (reverseQualitiesOfRCAlignments, include/corrector.hpp:1227-1239)
-            const uint32_t wv[4] = {v.x, v.y, v.z, v.w};
-            #pragma unroll
-            for(int k = 0; k < 16; k++){
-                const int pos = qp - 1 - (16 * c + k);
-                if(pos >= 0 && pos < L) dst[pos] = uint8_t(wv[k >> 2] >> (8 * (k & 3)));
-            }
+            // byte s of the row goes to qs - 1 - s
+            uint4 r;
+            r.x = __builtin_bswap32(v.w); r.y = __builtin_bswap32(v.z);
+            r.z = __builtin_bswap32(v.y); r.w = __builtin_bswap32(v.x);
+            *reinterpret_cast<uint4*>(dst + qs - 16 * (c + 1)) = r;
         }
     }
 }
@@ -1185,10 +1195,9 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
             for(int j = 0; j < tn; j++){
                 const int pos = pa - t.shift[j];
                 if(pos >= 0 && pos < t.len[j]){
-                    const int sh5 = 31 - (pos & 31);
-                    const int wd = j * W + (pos >> 5);
-                    const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
-                    const float x = t.w[j] * (p.useQ ? t.qw[t.q[j * qStride(p.maxLen) + pos]] : 1.0f);
+                    const int b = baseAt(t.rows + j * 2 * W, pos);
+                    const int qo = t.orient[j] == 1 ? 0 : rcQualityOffset(p);
+                    const float x = t.w[j] * (p.useQ ? t.qw[t.q[j * qStride(p.maxLen) + qo + pos]] : 1.0f);
                     if(b == 0){ n0++; w0 += x; } else if(b == 1){ n1++; w1 += x; }
                     else if(b == 2){ n2++; w2 += x; } else { n3++; w3 += x; }
                     cv++;
@@ -1242,9 +1251,8 @@ k_msa(MsaParams p){
         tile.w = reinterpret_cast<float*>(tile.len + T);
         tile.id = reinterpret_cast<uint32_t*>(tile.w + T);
         tile.orient = reinterpret_cast<int*>(tile.id + T);
-        tile.hi = reinterpret_cast<uint32_t*>(tile.orient + T);
-        tile.lo = tile.hi + T * p.W;
-        tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
+        tile.rows = reinterpret_cast<uint32_t*>(tile.orient + T);
+        tile.q = reinterpret_cast<uint8_t*>(tile.rows + T * 2 * p.W);
         tile.qw = sQw;
         if(p.useQ) for(int q = threadIdx.x; q < 256; q += kBlock) sQw[q] = p.qweight[q];
     }
@@ -1610,30 +1618,42 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
             m.isAnchor |= 1u << k;
         }
     }
-    const int W = p.W;
+    const int RS = 2 * p.W;                  // tile row stride in words
     const int T = p.tile;
+    const int qs = qStride(p.maxLen);
+    const int qrc = rcQualityOffset(p);
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();
         stageTile(p, lst, t, beg, tb, tn, lane, kWave);
         __syncthreads();
+        // lane q holds candidate tb+q's shift, length, weight and orientation; the loop
+        // below reads them with readlane (no LDS round trip per candidate)
+        int mShift = 0, mLen = 0, mRc = 0;
+        float mW = 0.0f;
+        if(lane < tn){ mShift = t.shift[lane]; mLen = t.len[lane]; mW = t.w[lane]; mRc = t.orient[lane] != 1; }
         for(int j = 0; j < tn; j++){
-            const int cs = off + t.shift[j];         // slot column of candidate j's first base
-            const int ln = t.len[j];
-            const float wj = t.w[j];
+            const int cs = off + __builtin_amdgcn_readlane(mShift, j);   // slot column of its first base
+            const int ln = __builtin_amdgcn_readlane(mLen, j);
+            const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mW), j));
+            const bool rc = __builtin_amdgcn_readlane(mRc, j) != 0;
             // slots holding its columns (AO: those under the anchor; lanes past the anchor
             // end add to columns that are never read)
             const int k0 = (AO ? max(cs, 0) : cs) >> 6;
             const int k1 = ((AO ? min(cs + ln, La) : cs + ln) - 1) >> 6;
+            // slot k covers candidate position pos0 + 64k: base word w0 + 4k at the same bit
+            // offset, quality byte q0 + 64k
+            const int pos0 = lane - cs;
+            const int w0 = j * RS + (pos0 >> 4);
+            const int sA = 30 - 2 * (pos0 & 15);
+            const int q0 = j * qs + (rc ? qrc : 0) + pos0;
             #pragma unroll
             for(int k = 0; k < KC; k++){
                 if(k < k0 || k > k1) continue;        // wave-uniform
-                const int pos = lane + kWave * k - cs;
-                if(pos >= 0 && pos < ln){
-                    const int sh5 = 31 - (pos & 31);
-                    const int wd = j * W + (pos >> 5);
-                    const int b = int(((t.hi[wd] >> sh5) & 1u) << 1 | ((t.lo[wd] >> sh5) & 1u));
-                    const float x = wj * (p.useQ ? t.qw[t.q[j * qStride(p.maxLen) + pos]] : 1.0f);
+                const int pos = pos0 + kWave * k;
+                if(unsigned(pos) < unsigned(ln)){
+                    const int b = int((t.rows[w0 + 4 * k] >> sA) & 3u);
+                    const float x = wj * (p.useQ ? t.qw[t.q[q0 + kWave * k]] : 1.0f);
                     // selects instead of branches: the other weights add +0.0f, which leaves
                     // them bit-identical (weights start at +0.0f and only grow)
                     m.w[k][0] += b == 0 ? x : 0.0f;
@@ -1705,9 +1725,8 @@ __device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, c
     tile.w = reinterpret_cast<float*>(tile.len + T);
     tile.id = reinterpret_cast<uint32_t*>(tile.w + T);
     tile.orient = reinterpret_cast<int*>(tile.id + T);
-    tile.hi = reinterpret_cast<uint32_t*>(tile.orient + T);
-    tile.lo = tile.hi + T * p.W;
-    tile.q = reinterpret_cast<uint8_t*>(tile.lo + T * p.W);
+    tile.rows = reinterpret_cast<uint32_t*>(tile.orient + T);
+    tile.q = reinterpret_cast<uint8_t*>(tile.rows + T * 2 * p.W);
     tile.qw = qw;
     const size_t tileBytes = size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * qStride(p.maxLen) : 0);
     sSup = reinterpret_cast<float*>(tp + (max(tileBytes, kSeamLdsBytes) + 15) / 16 * 16);
@@ -2564,7 +2583,7 @@ struct care_ctx {
     bool hasReads = false;
     bool hasTables = false;
 
-    DBuf enc, len, qual, ambig, sig, hilo, qw, bucketOf, gath, order, orderFlags, batchOrd;
+    DBuf enc, len, qual, ambig, sig, hilo, rcenc, qw, bucketOf, gath, order, orderFlags, batchOrd;
     std::vector<DBuf> slotBufs;
     std::vector<uint32_t> caps;
     DBuf slotPtrs, capsDev, values;
@@ -2783,7 +2802,7 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.pctHalf = mp.pct * 0.5f;
     mp.sigThr = int(int(ctx->o.estimated_coverage) * 0.3f);
     mp.maxCols = ctx->maxCols; mp.maxLen = ctx->maxLen;
-    mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W;
+    mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W; mp.rcenc = ctx->rcenc.as<uint32_t>();
     mp.info = nullptr; mp.msaDbg = nullptr; mp.dbgStride = ctx->dbgStride;
     mp.seamMode = ctx->o.correction_type == 1 ? 1 : 0;
     mp.seamW = ctx->seam.as<uint16_t>();
@@ -3352,7 +3371,7 @@ void care_hip_destroy(care_ctx* ctx){
     if(!ctx) return;
     (void)hipSetDevice(ctx->device);
     if(ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->qw, &ctx->bucketOf, &ctx->gath, &ctx->order, &ctx->orderFlags, &ctx->batchOrd, &ctx->slotPtrs,
+    for(DBuf* b : {&ctx->enc, &ctx->len, &ctx->qual, &ctx->ambig, &ctx->sig, &ctx->hilo, &ctx->rcenc, &ctx->qw, &ctx->bucketOf, &ctx->gath, &ctx->order, &ctx->orderFlags, &ctx->batchOrd, &ctx->slotPtrs,
                    &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
@@ -3427,9 +3446,11 @@ int finishReads(care_ctx* ctx, hipEvent_t done = nullptr){
     HIPCHECK(hipMemcpyAsync(ctx->qw.p, qw, sizeof(qw), hipMemcpyHostToDevice, st));
     const uint64_t num_reads = ctx->n;
     HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4 + 64));
+    HIPCHECK(ctx->rcenc.ensure(num_reads * 2 * ctx->W * 4 + 64));
     const uint64_t hiloThreads = num_reads * uint64_t(ctx->W);
     k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
-                                                                  num_reads, ctx->W, ctx->hilo.as<uint32_t>());
+                                                                  num_reads, ctx->W, ctx->hilo.as<uint32_t>(),
+                                                                  ctx->rcenc.as<uint32_t>());
     HIPCHECK(hipGetLastError());
     if(done) HIPCHECK(hipEventRecord(done, st));
     HIPCHECK(hipStreamSynchronize(st));
