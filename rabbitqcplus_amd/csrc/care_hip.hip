@@ -1654,13 +1654,18 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
                 if(unsigned(pos) < unsigned(ln)){
                     const int b = int((t.rows[w0 + 4 * k] >> sA) & 3u);
                     const float x = wj * (p.useQ ? t.qw[t.q[q0 + kWave * k]] : 1.0f);
-                    // selects instead of branches: the other weights add +0.0f, which leaves
-                    // them bit-identical (weights start at +0.0f and only grow)
-                    m.w[k][0] += b == 0 ? x : 0.0f;
-                    m.w[k][1] += b == 1 ? x : 0.0f;
-                    m.w[k][2] += b == 2 ? x : 0.0f;
-                    m.w[k][3] += b == 3 ? x : 0.0f;
-                    m.cnt[k] += uint64_t(1) << (16 * b);
+                    // inc holds 1 in the 16-bit field of base b, so bytes 0 and 2 of its
+                    // halves are the one-hot factors of A, C (low half) and G, T (high half).
+                    // fma(x, 1, w) rounds w + x exactly like the add and fma(x, 0, w) = w (no
+                    // weight is ever -0.0: weights start at +0.0f and add finite products),
+                    // so each weight is the sum of its base's x in list order, branch-free.
+                    const uint64_t inc = uint64_t(1) << (16 * b);
+                    const uint32_t incLo = uint32_t(inc), incHi = uint32_t(inc >> 32);
+                    m.w[k][0] = __builtin_fmaf(x, float(incLo & 0xFFu), m.w[k][0]);
+                    m.w[k][1] = __builtin_fmaf(x, float((incLo >> 16) & 0xFFu), m.w[k][1]);
+                    m.w[k][2] = __builtin_fmaf(x, float(incHi & 0xFFu), m.w[k][2]);
+                    m.w[k][3] = __builtin_fmaf(x, float((incHi >> 16) & 0xFFu), m.w[k][3]);
+                    m.cnt[k] += inc;
                 }
             }
         }
