@@ -70,6 +70,9 @@ typedef struct care_opts {
                                        HQ anchors also correct their candidates (include/corrector.hpp:
                                        1504-1512, src/msa.cpp:457-522) */
     int new_columns_to_correct;     /* --candidateCorrectionNewColumns, 15 */
+    int msa_tile;                   /* staged engine: candidates staged into LDS per step of the
+                                       one-wavefront MSA, a multiple of 4 in 4..64; 0 = automatic
+                                       (24 with quality scores, else 64) */
 } care_opts;
 
 typedef struct care_ctx care_ctx;

@@ -2769,6 +2769,12 @@ int waveMsaAnchorSlots(int maxLen){
     return 0;
 }
 
+// Candidates per LDS tile of the one-wavefront MSA. With qualities (220 bytes per
+// candidate at 150 bp) 24 keeps the workgroup near 7 KB, so about 23 one-wavefront
+// workgroups fit a compute unit's LDS, while two tiles still cover a typical candidate
+// list; without qualities 64 fit in the same space.
+int defaultWaveMsaTile(bool useQ){ return useQ ? 24 : 64; }
+
 // dynamic LDS of k_msa_wave: candidate tile + anchor-column supports
 size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
     size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * qStride(maxLen) : 0);
@@ -2924,7 +2930,7 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     fillCandCommon(ctx, fp.c);
     fillAlignCommon(ctx, fp.al);
     const bool q = ctx->o.use_quality_scores != 0;
-    fp.m.tile = q ? 32 : 64;
+    fp.m.tile = defaultWaveMsaTile(q);
     fp.cap = fp.c.capPerAnchor;
     int cus = 0;
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
@@ -3253,7 +3259,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     if(kc > 0){
         // one wavefront per anchor, MSA columns in registers
         const bool q = ctx->o.use_quality_scores != 0;
-        mp.tile = q ? 32 : 64;
+        mp.tile = ctx->o.msa_tile ? ctx->o.msa_tile : defaultWaveMsaTile(q);
         const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q);
         if(wholeMsa){
             switch(kc){
@@ -3354,6 +3360,7 @@ int care_hip_create(const care_opts* opts, int device, care_ctx** out){
     if(opts->kmerlength < 0 || opts->kmerlength > 32) return CARE_EINVAL;
     if(!(opts->estimated_coverage > 0.0f)) return CARE_EINVAL;
     if(opts->new_columns_to_correct < 0) return CARE_EINVAL;
+    if(opts->msa_tile != 0 && (opts->msa_tile < 4 || opts->msa_tile > 64 || (opts->msa_tile & 3))) return CARE_EINVAL;
     int count = 0;
     if(hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return CARE_ENODEV;
     auto* ctx = new (std::nothrow) care_ctx;

@@ -36,6 +36,14 @@ def test_sim_library_exports_every_declared_symbol():
         assert hasattr(lib, n), n
 
 
+@pytest.mark.parametrize("tile", [2, 6, 68, -4])
+def test_create_rejects_bad_msa_tile(tile):
+    """Options are checked before any device call: an MSA tile outside 4..64 or not a
+    multiple of 4 is CARE_EINVAL (1), also where there is no GPU."""
+    with pytest.raises(engine.CareError, match="failed with code 1$"):
+        engine.CareEngine(0, msa_tile=tile)
+
+
 def test_missing_library_fails_loudly(tmp_path):
     saved = engine._LIB
     engine._LIB = None

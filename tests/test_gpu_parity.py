@@ -166,6 +166,20 @@ def test_batching_invariance():
     assert outs[0] == outs[1] == outs[2]
 
 
+@pytest.mark.parametrize("name", ["se_q", "varlen_q", "se_higherr"])
+def test_msa_tile_invariance(name):
+    """The number of candidates the one-wavefront MSA stages into LDS per step (4..64) does
+    not change the records; they equal the oracle's."""
+    s, rs, o = helpers.case(name)
+    print(f"[case {name}] msa tiles", flush=True)
+    cpu = oracle.CareCpu(rs, threads=8, **o)
+    _, d_cpu = cpu.correct(threads=8)
+    cpu.close()
+    for tile in (4, 12, 64):
+        _, d = engine.correct_gpu(rs, engine=1, msa_tile=tile, **o)
+        assert d == d_cpu, tile
+
+
 def test_correction_reduces_error_rate():
     s, rs, o = helpers.case("se_higherr")
     _, data = engine.correct_gpu(rs, **o)
