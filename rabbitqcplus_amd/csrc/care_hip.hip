@@ -1547,6 +1547,10 @@ __device__ __forceinline__ int waveReduce(int v, Op op, int identity){
     return __builtin_amdgcn_readlane(v, 63);
 }
 
+// pointers into LDS (address space 3, 32-bit)
+typedef const __attribute__((address_space(3))) uint32_t* LdsU32;
+typedef const __attribute__((address_space(3))) uint8_t* LdsU8;
+
 __device__ __forceinline__ uint64_t lanesBelow(int lane){ return lane ? (~0ull >> (64 - lane)) : 0ull; }
 
 // Per-lane MSA columns: lane owns columns lane + 64*k, k < KC.
@@ -1578,8 +1582,9 @@ __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
 // AO (anchor columns only): slot column c is anchor position c, and only the columns
 // under the anchor are built; every later step of the plain correction (refinement,
 // properties, decision, features) reads no other column. Otherwise slot column c is MSA
-// column c (anchor position c - ab), all ncol columns.
-template<int KC, bool AO>
+// column c (anchor position c - ab), all ncol columns. QM: 0 = no quality scores,
+// 1 = quality scores, 2 = as p.useQ says at run time.
+template<int KC, bool AO, int QM>
 __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int La,
                                              const uint32_t* arow, int beg, int nk,
                              WaveMsa<KC>& m, const MsaTile& t, int lane){
@@ -1599,7 +1604,8 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     m.abase = 0;
     m.isAnchor = 0;
     const int off = AO ? 0 : ab;       // slot column of anchor position 0
-    const uint8_t* aq = p.useQ ? p.qual + p.qdpitch * a : nullptr;
+    const bool useQ = QM == 2 ? p.useQ != 0 : QM == 1;
+    const uint8_t* aq = useQ ? p.qual + p.qdpitch * a : nullptr;
     #pragma unroll
     for(int k = 0; k < KC; k++){
         #pragma unroll
@@ -1608,7 +1614,7 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
         const int pa = lane + kWave * k - off;
         if(pa >= 0 && pa < La){
             const int b = baseAt(arow, pa);
-            const float x = 1.0f * (p.useQ ? t.qw[aq[pa]] : 1.0f);
+            const float x = 1.0f * (useQ ? t.qw[aq[pa]] : 1.0f);
             m.w[k][0] = b == 0 ? x : 0.0f;
             m.w[k][1] = b == 1 ? x : 0.0f;
             m.w[k][2] = b == 2 ? x : 0.0f;
@@ -1644,16 +1650,21 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
             // slot k covers candidate position pos0 + 64k: base word w0 + 4k at the same bit
             // offset, quality byte q0 + 64k
             const int pos0 = lane - cs;
-            const int w0 = j * RS + (pos0 >> 4);
             const int sA = 30 - 2 * (pos0 & 15);
-            const int q0 = j * qs + (rc ? qrc : 0) + pos0;
+            // LDS addresses of slot 0's base word and quality byte, kept in registers (the
+            // compiler would otherwise rebuild them inside every slot's branch; the empty asm
+            // statement emits no instruction). Slot k reads rowp[4k] and qp[64k], which lie
+            // in candidate j's row whenever the slot's position is inside the candidate.
+            LdsU32 rowp = (LdsU32)(t.rows) + (j * RS + (pos0 >> 4));
+            LdsU8 qp = (LdsU8)(t.q) + (j * qs + (rc ? qrc : 0) + pos0);
+            asm volatile("" : "+v"(rowp), "+v"(qp));
             #pragma unroll
             for(int k = 0; k < KC; k++){
                 if(k < k0 || k > k1) continue;        // wave-uniform
                 const int pos = pos0 + kWave * k;
                 if(unsigned(pos) < unsigned(ln)){
-                    const int b = int((t.rows[w0 + 4 * k] >> sA) & 3u);
-                    const float x = wj * (p.useQ ? t.qw[t.q[q0 + kWave * k]] : 1.0f);
+                    const int b = int((rowp[4 * k] >> sA) & 3u);
+                    const float x = wj * (useQ ? t.qw[qp[kWave * k]] : 1.0f);
                     // inc holds 1 in the 16-bit field of base b, so bytes 0 and 2 of its
                     // halves are the one-hot factors of A, C (low half) and G, T (high half).
                     // fma(x, 1, w) rounds w + x exactly like the add and fma(x, 0, w) = w (no
@@ -1815,7 +1826,7 @@ __device__ __forceinline__ void emitCandidateRecords(const MsaParams& p, const A
 // Filters, MSA, refinement, decision and record of anchor `a` by one wavefront
 // (include/corrector.hpp:982-1446,1639-1761; src/msa.cpp:9-453,532-734). Its candidates
 // are lst.ids/lst.best/lst.paired[beg, beg+n0); the record goes to slot `i`.
-template<int KC, bool AO>
+template<int KC, bool AO, int QM>
 __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int i, int beg,
                                               int n0, const MsaTile& tile,
                               float* sSup, int lane){
@@ -1899,7 +1910,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
     WaveMsa<KC> m;
     int nk = n2;
-    if(!buildMsaWave<KC, AO>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
+    if(!buildMsaWave<KC, AO, QM>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
         if(lane == 0) atomicAdd(p.error, 1);
         return;
     }
@@ -1977,7 +1988,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
         }
         nk = nn;
         rounds++;
-        if(!buildMsaWave<KC, AO>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
+        if(!buildMsaWave<KC, AO, QM>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
             if(lane == 0) atomicAdd(p.error, 1);
             return;
         }
@@ -2198,7 +2209,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     }
 }
 
-template<int KC, bool AO>
+template<int KC, bool AO, int QM>
 __global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 4 : (KC <= 12 ? 3 : 2))))
 k_msa_wave(MsaParams p){
     extern __shared__ uint8_t smem[];
@@ -2212,7 +2223,7 @@ k_msa_wave(MsaParams p){
     const int i = batchAnchor(p.order, p.B);
     const int beg = int(p.offsets[i]);
     const int n0 = int(p.offsets[i + 1]) - beg;
-    msaAnchorWave<KC, AO>(p, listsOf(p), p.b0 + i, i, beg, n0, tile, sSup, lane);
+    msaAnchorWave<KC, AO, QM>(p, listsOf(p), p.b0 + i, i, beg, n0, tile, sSup, lane);
 }
 
 // ---------------------------------------------------------------- fused persistent kernel
@@ -2439,7 +2450,7 @@ k_fused(FusedParams fp){
             const int U = candidatesWave(fp.c, a, hset, sOff, sPre, &sUnique, ws.idsA, lane, gathered);
             aligned += uint64_t(U);
             alignAllWave(fp.al, a, ws.idsA, U, ws.best, planeA, planeC, lane);
-            msaAnchorWave<KC, true>(fp.m, lst, a, int(unit), 0, U, tile, sSup, lane);
+            msaAnchorWave<KC, true, 2>(fp.m, lst, a, int(unit), 0, U, tile, sSup, lane);
         }else{
             const uint64_t a0 = fp.first + 2 * uint64_t(unit);
             const int U0 = candidatesWave(fp.c, a0, hset, sOff, sPre, &sUnique, ws.idsA, lane, gathered);
@@ -2456,7 +2467,7 @@ k_fused(FusedParams fp){
                 lst.paired = mate == 0 ? ws.pairedA : ws.pairedB;
                 const int U = mate == 0 ? U0 : U1;
                 alignAllWave(fp.al, a, lst.ids, U, ws.best, planeA, planeC, lane);
-                msaAnchorWave<KC, true>(fp.m, lst, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
+                msaAnchorWave<KC, true, 2>(fp.m, lst, a, int(2 * unit + mate), 0, U, tile, sSup, lane);
                 __syncthreads();
             }
         }
@@ -3262,20 +3273,30 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
         mp.tile = ctx->o.msa_tile ? ctx->o.msa_tile : defaultWaveMsaTile(q);
         const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q);
         if(wholeMsa){
+            // candidate correction and the dumps: quality mode read at run time
             switch(kc){
-                case 4: k_msa_wave<4, false><<<B, kWave, lds, st>>>(mp); break;
-                case 8: k_msa_wave<8, false><<<B, kWave, lds, st>>>(mp); break;
-                case 12: k_msa_wave<12, false><<<B, kWave, lds, st>>>(mp); break;
-                default: k_msa_wave<16, false><<<B, kWave, lds, st>>>(mp); break;
+                case 4: k_msa_wave<4, false, 2><<<B, kWave, lds, st>>>(mp); break;
+                case 8: k_msa_wave<8, false, 2><<<B, kWave, lds, st>>>(mp); break;
+                case 12: k_msa_wave<12, false, 2><<<B, kWave, lds, st>>>(mp); break;
+                default: k_msa_wave<16, false, 2><<<B, kWave, lds, st>>>(mp); break;
+            }
+        }else if(q){
+            switch(kc){
+                case 2: k_msa_wave<2, true, 1><<<B, kWave, lds, st>>>(mp); break;
+                case 3: k_msa_wave<3, true, 1><<<B, kWave, lds, st>>>(mp); break;
+                case 4: k_msa_wave<4, true, 1><<<B, kWave, lds, st>>>(mp); break;
+                case 8: k_msa_wave<8, true, 1><<<B, kWave, lds, st>>>(mp); break;
+                case 12: k_msa_wave<12, true, 1><<<B, kWave, lds, st>>>(mp); break;
+                default: k_msa_wave<16, true, 1><<<B, kWave, lds, st>>>(mp); break;
             }
         }else{
             switch(kc){
-                case 2: k_msa_wave<2, true><<<B, kWave, lds, st>>>(mp); break;
-                case 3: k_msa_wave<3, true><<<B, kWave, lds, st>>>(mp); break;
-                case 4: k_msa_wave<4, true><<<B, kWave, lds, st>>>(mp); break;
-                case 8: k_msa_wave<8, true><<<B, kWave, lds, st>>>(mp); break;
-                case 12: k_msa_wave<12, true><<<B, kWave, lds, st>>>(mp); break;
-                default: k_msa_wave<16, true><<<B, kWave, lds, st>>>(mp); break;
+                case 2: k_msa_wave<2, true, 0><<<B, kWave, lds, st>>>(mp); break;
+                case 3: k_msa_wave<3, true, 0><<<B, kWave, lds, st>>>(mp); break;
+                case 4: k_msa_wave<4, true, 0><<<B, kWave, lds, st>>>(mp); break;
+                case 8: k_msa_wave<8, true, 0><<<B, kWave, lds, st>>>(mp); break;
+                case 12: k_msa_wave<12, true, 0><<<B, kWave, lds, st>>>(mp); break;
+                default: k_msa_wave<16, true, 0><<<B, kWave, lds, st>>>(mp); break;
             }
         }
     }else{
