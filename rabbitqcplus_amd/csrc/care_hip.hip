@@ -572,6 +572,9 @@ struct AlignParams {
     const uint32_t* offsets; int B;     // lane kernel: candidates of anchor b0+i are [offsets[i], offsets[i+1])
     int maxLen;
     int minOverlap; float maxErrorRate; float minOverlapRatio; float orientErrorRate;
+    // int(float(ov) * maxErrorRate) == mulhi(ov << ePre, eMul) for every ov <= maxLen when
+    // eExact (checked on the host)
+    uint32_t eMul; int ePre; int eExact;
     AlignBest* best; int* dbg;   // dbg: [total][11] or null
     const uint32_t* order;       // processing order of the batch's anchors, or null
 };
@@ -834,6 +837,99 @@ __device__ __forceinline__ uint32_t overlapMask(int k){
     return uint32_t(0xFFFFFFFF00000000ull >> c);
 }
 
+// The two scan directions of one lane (src/cpu_alignment.cpp:32-497). UNI: every valid
+// lane's candidate has the anchor's length, so the overlap of a shift, its max_errors
+// (the host-checked integer form, on the scalar unit) and the overlap masks (anchor
+// validity shifted AND unshifted) are wave-uniform; the results are the same as those of
+// the general form.
+template<int WM, bool UNI>
+__device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t* AH, const uint32_t* AL, const uint32_t* AV,
+                                           const uint32_t* FH, const uint32_t* FL, const uint32_t* RH, const uint32_t* RL,
+                                           const uint32_t* CV, int La, int Lc, int lastF, int mo, uint64_t start,
+                                           int& gF, int& sF, int& gR, int& sR){
+    gF = 0; sF = -Lc; gR = 0; sR = -Lc;
+    // anchor shifted: shifts 0..lastF. A position counts when it lies inside both the
+    // candidate (CV) and the shifted anchor (shifted AV): exactly the overlap. The
+    // lanes whose scan in an orientation is still running are kept as wave masks
+    // (scalar registers); a lane's bit is read back with an inverse ballot.
+    uint64_t liveF = start, liveR = start;
+    #pragma unroll
+    for(int ps = 0; ps <= WM; ps++){
+        if(32 * ps > lastF || (liveF | liveR) == 0) break;
+        const int rEnd = min(31, lastF - 32 * ps);
+        for(int r = 0; r <= rEnd; r++){
+            const int sft = 32 * ps + r;
+            const int ov = UNI ? La - sft : min(La - sft, Lc);
+            const int e = UNI ? int(__umulhi(uint32_t(ov) << p.ePre, p.eMul)) : int(float(ov) * p.maxErrorRate);
+            const int mF = min(e, gF + 2 * ov);
+            const int mR = min(e, gR + 2 * ov);
+            liveF &= __builtin_amdgcn_ballot_w64(mF > 0);
+            liveR &= __builtin_amdgcn_ballot_w64(mR > 0);
+            if((liveF | liveR) == 0) break;
+            int dF = 0, dR = 0;
+            #pragma unroll
+            for(int w = 0; w < WM; w++){
+                // once no lane can still stay below its bound, the remaining words
+                // cannot make any shift admissible (mismatch counts only grow)
+                if(w == 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                              (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
+                const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
+                const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
+                const uint32_t xv = uint32_t((((uint64_t(AV[w + ps + 1]) << 32) | AV[w + ps + 2]) << r) >> 32);
+                // UNI: candidate positions < La - sft are all inside the candidate (Lc == La)
+                const uint32_t m = UNI ? xv : xv & CV[w];
+                dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
+                dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
+            }
+            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
+                gF = dF - 2 * ov; sF = sft;
+            }
+            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
+                gR = dR - 2 * ov; sR = sft;
+            }
+        }
+    }
+    // candidate shifted: shifts -1..-(Lc-mo), here t = 1..Lc-mo with the anchor
+    // shifted right by t against the candidate's positions [t, t+ov)
+    const int lastB = (UNI ? La : Lc) - mo;     // (valid lanes of a UNI scan have Lc == La)
+    const int lastBw = UNI ? lastB : p.maxLen - mo;
+    liveF = start; liveR = start;
+    #pragma unroll
+    for(int pt = 0; pt <= WM; pt++){
+        if(32 * pt > lastBw || (liveF | liveR) == 0) break;
+        const int rEnd = min(31, lastBw - 32 * pt);
+        for(int r = (pt == 0 ? 1 : 0); r <= rEnd; r++){
+            const int t = 32 * pt + r;
+            const int ov = UNI ? La - t : min(La, Lc - t);
+            const int e = UNI ? int(__umulhi(uint32_t(ov) << p.ePre, p.eMul)) : int(float(ov) * p.maxErrorRate);
+            const int mF = min(e, gF + 2 * ov);
+            const int mR = min(e, gR + 2 * ov);
+            const uint64_t inRange = UNI ? ~0ull : __builtin_amdgcn_ballot_w64(t <= lastB);
+            liveF &= inRange & __builtin_amdgcn_ballot_w64(mF > 0);
+            liveR &= inRange & __builtin_amdgcn_ballot_w64(mR > 0);
+            if((liveF | liveR) == 0) break;
+            int dF = 0, dR = 0;
+            #pragma unroll
+            for(int w = pt; w < WM; w++){
+                if(w == pt + 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                   (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
+                const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
+                const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
+                const uint32_t xv = uint32_t(((uint64_t(AV[w - pt]) << 32) | AV[w - pt + 1]) >> r);
+                const uint32_t m = xv & (UNI ? AV[w + 1] : CV[w]);
+                dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
+                dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
+            }
+            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
+                gF = dF - 2 * ov; sF = -t;
+            }
+            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
+                gR = dR - 2 * ov; sR = -t;
+            }
+        }
+    }
+}
+
 template<int WM>
 __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, const uint32_t* ids, int n,
                                            AlignBest* best, int* dbg, int lane){
@@ -873,87 +969,12 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
         }
         const int total = La + Lc;
         // gF, gR: best score minus total (score = total - 2 * overlap + mismatches)
-        int gF = 0, sF = -Lc, gR = 0, sR = -Lc;
-        // anchor shifted: shifts 0..lastF. A position counts when it lies inside both the
-        // candidate (CV) and the shifted anchor (shifted AV): exactly the overlap. The
-        // lanes whose scan in an orientation is still running are kept as wave masks
-        // (scalar registers); a lane's bit is read back with an inverse ballot.
+        int gF, sF, gR, sR;
         const uint64_t start = __builtin_amdgcn_ballot_w64(valid);
-        uint64_t liveF = start, liveR = start;
-        #pragma unroll
-        for(int ps = 0; ps <= WM; ps++){
-            if(32 * ps > lastF || (liveF | liveR) == 0) break;
-            const int rEnd = min(31, lastF - 32 * ps);
-            for(int r = 0; r <= rEnd; r++){
-                const int sft = 32 * ps + r;
-                const int ov = min(La - sft, Lc);
-                const int e = int(float(ov) * p.maxErrorRate);
-                const int mF = min(e, gF + 2 * ov);
-                const int mR = min(e, gR + 2 * ov);
-                liveF &= __builtin_amdgcn_ballot_w64(mF > 0);
-                liveR &= __builtin_amdgcn_ballot_w64(mR > 0);
-                if((liveF | liveR) == 0) break;
-                int dF = 0, dR = 0;
-                #pragma unroll
-                for(int w = 0; w < WM; w++){
-                    // once no lane can still stay below its bound, the remaining words
-                    // cannot make any shift admissible (mismatch counts only grow)
-                    if(w == 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
-                                  (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
-                    const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
-                    const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
-                    const uint32_t xv = uint32_t((((uint64_t(AV[w + ps + 1]) << 32) | AV[w + ps + 2]) << r) >> 32);
-                    const uint32_t m = xv & CV[w];
-                    dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
-                    dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
-                }
-                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
-                    gF = dF - 2 * ov; sF = sft;
-                }
-                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
-                    gR = dR - 2 * ov; sR = sft;
-                }
-            }
-        }
-        // candidate shifted: shifts -1..-(Lc-mo), here t = 1..Lc-mo with the anchor
-        // shifted right by t against the candidate's positions [t, t+ov)
-        const int lastB = Lc - mo;
-        const int lastBw = p.maxLen - mo;
-        liveF = start; liveR = start;
-        #pragma unroll
-        for(int pt = 0; pt <= WM; pt++){
-            if(32 * pt > lastBw || (liveF | liveR) == 0) break;
-            const int rEnd = min(31, lastBw - 32 * pt);
-            for(int r = (pt == 0 ? 1 : 0); r <= rEnd; r++){
-                const int t = 32 * pt + r;
-                const int ov = min(La, Lc - t);
-                const int e = int(float(ov) * p.maxErrorRate);
-                const int mF = min(e, gF + 2 * ov);
-                const int mR = min(e, gR + 2 * ov);
-                const uint64_t inRange = __builtin_amdgcn_ballot_w64(t <= lastB);
-                liveF &= inRange & __builtin_amdgcn_ballot_w64(mF > 0);
-                liveR &= inRange & __builtin_amdgcn_ballot_w64(mR > 0);
-                if((liveF | liveR) == 0) break;
-                int dF = 0, dR = 0;
-                #pragma unroll
-                for(int w = pt; w < WM; w++){
-                    if(w == pt + 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
-                                       (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
-                    const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
-                    const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
-                    const uint32_t xv = uint32_t(((uint64_t(AV[w - pt]) << 32) | AV[w - pt + 1]) >> r);
-                    const uint32_t m = xv & CV[w];
-                    dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
-                    dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
-                }
-                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
-                    gF = dF - 2 * ov; sF = -t;
-                }
-                if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
-                    gR = dR - 2 * ov; sR = -t;
-                }
-            }
-        }
+        if(p.eExact && __builtin_amdgcn_ballot_w64(valid && Lc != La) == 0)
+            scanShifts<WM, true>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, lastF, mo, start, gF, sF, gR, sR);
+        else
+            scanShifts<WM, false>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, lastF, mo, start, gF, sF, gR, sR);
         const int bF = gF + total, bR = gR + total;
         if(!valid) continue;
         ShdResult f, rr;
@@ -2852,11 +2873,33 @@ void fillCandCommon(const care_ctx* ctx, CandParams& cp){
     cp.order = nullptr;
 }
 
+// max_errors = int(float(ov) * rate) (src/cpu_alignment.cpp) as the integer
+// mulhi(ov << pre, mul) = (ov * mul) >> (32 - pre): rate = mul * 2^-(32-pre) exactly (mul
+// its 24-bit significand), so the integer form is the floor of the exact product; it equals
+// the float form unless rounding the product reaches the next integer, which is checked for
+// every overlap up to maxLen (as is ov << pre < 2^32).
+bool errorBoundAsInteger(float rate, int maxLen, uint32_t& mul, int& pre){
+    if(!(rate > 0.0f) || !std::isfinite(rate)) return false;
+    int ex = 0;
+    const float fr = std::frexp(rate, &ex);          // rate = fr * 2^ex, fr in [0.5, 1)
+    mul = uint32_t(std::ldexp(fr, 24));
+    pre = 32 - (24 - ex);
+    if(pre < 0 || pre > 31 || (uint64_t(uint32_t(maxLen)) << pre) >= (uint64_t(1) << 32)) return false;
+    for(int ov = 0; ov <= maxLen; ov++){
+        const float prod = float(ov) * rate;
+        const uint32_t hi = uint32_t((uint64_t(uint32_t(ov) << pre) * mul) >> 32);
+        if(int(prod) != int(hi)) return false;
+    }
+    return true;
+}
+
 void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
     ap.hilo = ctx->hilo.as<uint32_t>(); ap.W = ctx->W; ap.len = ctx->len.as<int>();
     ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
     ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
     ap.maxLen = ctx->maxLen;
+    ap.eExact = errorBoundAsInteger(ap.maxErrorRate, ctx->maxLen, ap.eMul, ap.ePre) ? 1 : 0;
+    if(!ap.eExact){ ap.eMul = 0; ap.ePre = 0; }
     ap.order = nullptr;
 }
 
@@ -3207,10 +3250,8 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     HIPCHECK(hipEventRecord(ctx->ev[1], st));
 
     AlignParams ap;
-    ap.hilo = ctx->hilo.as<uint32_t>(); ap.W = ctx->W; ap.len = ctx->len.as<int>();
+    fillAlignCommon(ctx, ap);
     ap.ids = ctx->ids.as<uint32_t>(); ap.anchorOf = ctx->anchorOf.as<uint32_t>(); ap.b0 = b0; ap.total = total;
-    ap.minOverlap = ctx->o.min_overlap; ap.maxErrorRate = ctx->o.max_error_rate; ap.minOverlapRatio = ctx->o.min_overlap_ratio;
-    ap.orientErrorRate = ctx->o.estimated_errorrate * 4.0f;
     ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
     ap.offsets = ctx->offsets.as<uint32_t>(); ap.B = B; ap.maxLen = ctx->maxLen;
     ap.order = order;
