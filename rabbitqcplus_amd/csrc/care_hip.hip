@@ -863,9 +863,10 @@ __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t*
             const int e = UNI ? int(__umulhi(uint32_t(ov) << p.ePre, p.eMul)) : int(float(ov) * p.maxErrorRate);
             const int mF = min(e, gF + 2 * ov);
             const int mR = min(e, gR + 2 * ov);
+            // (a lane leaves its live mask for good; the block loop ends the scan once no
+            // lane is live, which rarely happens inside a block)
             liveF &= __builtin_amdgcn_ballot_w64(mF > 0);
             liveR &= __builtin_amdgcn_ballot_w64(mR > 0);
-            if((liveF | liveR) == 0) break;
             int dF = 0, dR = 0;
             #pragma unroll
             for(int w = 0; w < WM; w++){
@@ -907,7 +908,6 @@ __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t*
             const uint64_t inRange = UNI ? ~0ull : __builtin_amdgcn_ballot_w64(t <= lastB);
             liveF &= inRange & __builtin_amdgcn_ballot_w64(mF > 0);
             liveR &= inRange & __builtin_amdgcn_ballot_w64(mR > 0);
-            if((liveF | liveR) == 0) break;
             int dF = 0, dR = 0;
             #pragma unroll
             for(int w = pt; w < WM; w++){
