@@ -1863,67 +1863,101 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     if(n0 == 0) return;
     const uint64_t below = lanesBelow(lane);
 
-    // 1) drop candidates without an orientation (include/corrector.hpp:982-1050)
-    int n1 = 0;
-    for(int base = 0; base < n0; base += kWave){
-        const int j = base + lane;
-        AlignBest b{};
-        bool keep = false;
-        if(j < n0){ b = lst.best[beg + j]; keep = b.orientation != 3; }
-        const uint64_t mask = __ballot(keep);
-        if(keep){
-            const int d = beg + n1 + __popcll(mask & below);
-            const uint32_t id = lst.ids[beg + j];
-            lst.kId[d] = id;
-            lst.kShift[d] = b.shift;
-            lst.kLen[d] = p.len[id];
-            lst.kNops[d] = b.nOps;
-            lst.kOvl[d] = b.overlap;
-            lst.kOrient[d] = uint8_t(b.orientation);
-            lst.kPaired[d] = lst.paired ? lst.paired[beg + j] : 0;
-        }
-        n1 += __popcll(mask);
-    }
-    if(n1 == 0) return;
-
-    // 2) mismatch-ratio filter (include/corrector.hpp:1053-1149; paired :1151-1211)
-    float threshold = -std::numeric_limits<float>::max();
-    if(!p.isPaired){
-        int c0 = 0, c1 = 0, c2 = 0;
-        for(int base = 0; base < n1; base += kWave){
-            const int j = base + lane;
-            float ratio = 1e30f;
-            if(j < n1) ratio = float(lst.kNops[beg + j]) / float(lst.kOvl[beg + j]);
-            c0 += __popcll(__ballot(j < n1 && ratio < p.ratio2));
-            c1 += __popcll(__ballot(j < n1 && ratio < p.ratio3));
-            c2 += __popcll(__ballot(j < n1 && ratio < p.ratio4));
-        }
-        if(c0 > 0 || c1 > 0 || c2 > 0){
-            if(c0 >= p.goodCount) threshold = p.ratio2;
-            else if(c1 >= p.goodCount) threshold = p.ratio3;
-            else if(c2 >= p.goodCount) threshold = p.ratio4;
-        }
-    }
+    // 1) drop candidates without an orientation (include/corrector.hpp:982-1050), then
+    // 2) the mismatch-ratio filter (include/corrector.hpp:1053-1149; paired :1151-1211)
     int n2 = 0;
-    for(int base = 0; base < n1; base += kWave){
-        const int j = base + lane;
-        bool keep = false;
-        uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0;
-        if(j < n1){
-            id = lst.kId[beg + j]; s = lst.kShift[beg + j]; l = lst.kLen[beg + j]; no = lst.kNops[beg + j];
-            ov = lst.kOvl[beg + j]; o = lst.kOrient[beg + j]; pr = lst.kPaired[beg + j];
-            const float ratio = float(no) / float(ov);
-            keep = p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold);
+    if(n0 <= kWave){
+        // at most one candidate per lane: both filters on the lane's registers and one
+        // compaction, no intermediate list
+        AlignBest b{};
+        bool k1 = false;
+        if(lane < n0){ b = lst.best[beg + lane]; k1 = b.orientation != 3; }
+        if(__ballot(k1) == 0) return;
+        const float ratio = k1 ? float(b.nOps) / float(b.overlap) : 1e30f;
+        float threshold = -std::numeric_limits<float>::max();
+        if(!p.isPaired){
+            const int c0 = __popcll(__ballot(k1 && ratio < p.ratio2));
+            const int c1 = __popcll(__ballot(k1 && ratio < p.ratio3));
+            const int c2 = __popcll(__ballot(k1 && ratio < p.ratio4));
+            if(c0 > 0 || c1 > 0 || c2 > 0){
+                if(c0 >= p.goodCount) threshold = p.ratio2;
+                else if(c1 >= p.goodCount) threshold = p.ratio3;
+                else if(c2 >= p.goodCount) threshold = p.ratio4;
+            }
         }
+        const uint8_t pr = (k1 && lst.paired) ? lst.paired[beg + lane] : 0;
+        const bool keep = k1 && (p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold));
         const uint64_t mask = __ballot(keep);
         if(keep){
-            const int d = beg + n2 + __popcll(mask & below);
-            lst.kId[d] = id; lst.kShift[d] = s; lst.kLen[d] = l; lst.kNops[d] = no; lst.kOvl[d] = ov;
-            lst.kOrient[d] = o; lst.kPaired[d] = pr;
+            const int d = beg + __popcll(mask & below);
+            const uint32_t id = lst.ids[beg + lane];
+            const int no = b.nOps, ov = b.overlap;
+            lst.kId[d] = id; lst.kShift[d] = b.shift; lst.kLen[d] = p.len[id]; lst.kNops[d] = no; lst.kOvl[d] = ov;
+            lst.kOrient[d] = uint8_t(b.orientation); lst.kPaired[d] = pr;
             // overlap weight (include/hostdevicefunctions.cuh:93-97)
             lst.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
         }
-        n2 += __popcll(mask);
+        n2 = __popcll(mask);
+    }else{
+        // more candidates than lanes: step 1 into the working list, then step 2 over it
+        int n1 = 0;
+        for(int base = 0; base < n0; base += kWave){
+            const int j = base + lane;
+            AlignBest b{};
+            bool keep = false;
+            if(j < n0){ b = lst.best[beg + j]; keep = b.orientation != 3; }
+            const uint64_t mask = __ballot(keep);
+            if(keep){
+                const int d = beg + n1 + __popcll(mask & below);
+                const uint32_t id = lst.ids[beg + j];
+                lst.kId[d] = id;
+                lst.kShift[d] = b.shift;
+                lst.kLen[d] = p.len[id];
+                lst.kNops[d] = b.nOps;
+                lst.kOvl[d] = b.overlap;
+                lst.kOrient[d] = uint8_t(b.orientation);
+                lst.kPaired[d] = lst.paired ? lst.paired[beg + j] : 0;
+            }
+            n1 += __popcll(mask);
+        }
+        if(n1 == 0) return;
+        float threshold = -std::numeric_limits<float>::max();
+        if(!p.isPaired){
+            int c0 = 0, c1 = 0, c2 = 0;
+            for(int base = 0; base < n1; base += kWave){
+                const int j = base + lane;
+                float ratio = 1e30f;
+                if(j < n1) ratio = float(lst.kNops[beg + j]) / float(lst.kOvl[beg + j]);
+                c0 += __popcll(__ballot(j < n1 && ratio < p.ratio2));
+                c1 += __popcll(__ballot(j < n1 && ratio < p.ratio3));
+                c2 += __popcll(__ballot(j < n1 && ratio < p.ratio4));
+            }
+            if(c0 > 0 || c1 > 0 || c2 > 0){
+                if(c0 >= p.goodCount) threshold = p.ratio2;
+                else if(c1 >= p.goodCount) threshold = p.ratio3;
+                else if(c2 >= p.goodCount) threshold = p.ratio4;
+            }
+        }
+        for(int base = 0; base < n1; base += kWave){
+            const int j = base + lane;
+            bool keep = false;
+            uint32_t id = 0; int s = 0, l = 0, no = 0, ov = 0; uint8_t o = 0, pr = 0;
+            if(j < n1){
+                id = lst.kId[beg + j]; s = lst.kShift[beg + j]; l = lst.kLen[beg + j]; no = lst.kNops[beg + j];
+                ov = lst.kOvl[beg + j]; o = lst.kOrient[beg + j]; pr = lst.kPaired[beg + j];
+                const float ratio = float(no) / float(ov);
+                keep = p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold);
+            }
+            const uint64_t mask = __ballot(keep);
+            if(keep){
+                const int d = beg + n2 + __popcll(mask & below);
+                lst.kId[d] = id; lst.kShift[d] = s; lst.kLen[d] = l; lst.kNops[d] = no; lst.kOvl[d] = ov;
+                lst.kOrient[d] = o; lst.kPaired[d] = pr;
+                // overlap weight (include/hostdevicefunctions.cuh:93-97)
+                lst.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
+            }
+            n2 += __popcll(mask);
+        }
     }
     if(info && lane == 0) info[0] = n2;
     if(n2 == 0) return;
