@@ -1116,6 +1116,8 @@ __host__ __device__ inline int tileRowWords(int maxLen){ return (maxLen + 15) >>
 // the slot of qStride(maxLen) bytes holds the whole 16-byte-chunked row).
 __device__ __forceinline__ int rcQualityOffset(const MsaParams& p){ return qStride(p.maxLen) - int(p.qpitch); }
 
+__device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile& t, int tn, int tid, int nthreads);
+
 // Stages candidates [tb, tb+tn) of the working list into the LDS tile: one thread per
 // candidate for the metadata and its oriented 2-bit row (the stored row or its reverse
 // complement); qualities as 16-byte chunks of the device rows, spread over the threads. A
@@ -1143,6 +1145,11 @@ __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists&
     }
     if(!p.useQ) return;
     __syncthreads();
+    stageQualities(p, t, tn, tid, nthreads);
+}
+
+// Quality rows of the tile's tn candidates (ids and orientations already in the tile).
+__device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile& t, int tn, int tid, int nthreads){
     const int qs = qStride(p.maxLen);
     const int nc = qs >> 4;              // chunks per row; device rows are qdpitch >= qs bytes
     // u / nc as a multiply-high: exact for u * nc < 2^32
@@ -1162,6 +1169,32 @@ __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists&
             *reinterpret_cast<uint4*>(dst + qs - 16 * (c + 1)) = r;
         }
     }
+}
+
+// Working-list entry held by lane i of a wave for candidate i (lists of at most 64).
+struct LaneCand {
+    uint32_t id;
+    int shift, len, orient;
+    float w;
+};
+
+// stageTile for candidates held in lanes: lane tb+q stages candidate tb+q into slot q
+// (its oriented 2-bit row, id and orientation), then the qualities. Same tile contents as
+// stageTile except the shift/length/weight arrays, which the caller reads from the lanes.
+__device__ __forceinline__ void stageTileLanes(const MsaParams& p, const MsaTile& t, const LaneCand& c, int tb, int tn,
+                                               int lane){
+    const int W = p.W;
+    const int q = lane - tb;
+    if(q >= 0 && q < tn){
+        t.id[q] = c.id;
+        t.orient[q] = c.orient;
+        const uint32_t* src = c.orient == 1 ? p.enc + p.pitch * c.id : p.rcenc + uint64_t(c.id) * uint64_t(2 * W);
+        const int RW = tileRowWords(p.maxLen);
+        for(int k = 0; k < RW; k++) t.rows[q * 2 * W + k] = src[k];
+    }
+    if(!p.useQ) return;
+    __syncthreads();
+    stageQualities(p, t, tn, lane, kWave);
 }
 
 // Builds the MSA of the anchor and the first `nk` working candidates in LDS
@@ -1608,12 +1641,15 @@ __device__ __forceinline__ int countOf(const WaveMsa<KC>& m, int k, int b){
 template<int KC, bool AO, int QM>
 __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLists& lst, uint64_t a, int La,
                                              const uint32_t* arow, int beg, int nk,
-                             WaveMsa<KC>& m, const MsaTile& t, int lane){
+                             WaveMsa<KC>& m, const MsaTile& t, int lane, bool inLanes = false,
+                             const LaneCand& lc = LaneCand{}){
+    // inLanes (wave-uniform, nk <= 64): lane i holds candidate i of the working list, which
+    // the list in memory holds as well
     int lo = 0, hi = La;
     for(int j = lane; j < nk; j += kWave){
-        const int s = lst.kShift[beg + j];
+        const int s = inLanes ? lc.shift : lst.kShift[beg + j];
         lo = min(lo, s);
-        hi = max(hi, s + lst.kLen[beg + j]);
+        hi = max(hi, s + (inLanes ? lc.len : lst.kLen[beg + j]));
     }
     lo = waveReduce(lo, OpMin(), INT_MAX);
     hi = waveReduce(hi, OpMax(), INT_MIN);
@@ -1652,18 +1688,24 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();
-        stageTile(p, lst, t, beg, tb, tn, lane, kWave);
+        if(inLanes) stageTileLanes(p, t, lc, tb, tn, lane);
+        else stageTile(p, lst, t, beg, tb, tn, lane, kWave);
         __syncthreads();
-        // lane q holds candidate tb+q's shift, length, weight and orientation; the loop
+        // lane src+q holds candidate tb+q's shift, length, weight and orientation; the loop
         // below reads them with readlane (no LDS round trip per candidate)
-        int mShift = 0, mLen = 0, mRc = 0;
+        int mShift = 0, mLen = 0, mRc = 0, src = tb;
         float mW = 0.0f;
-        if(lane < tn){ mShift = t.shift[lane]; mLen = t.len[lane]; mW = t.w[lane]; mRc = t.orient[lane] != 1; }
+        if(inLanes){
+            mShift = lc.shift; mLen = lc.len; mW = lc.w; mRc = lc.orient != 1;
+        }else{
+            src = 0;
+            if(lane < tn){ mShift = t.shift[lane]; mLen = t.len[lane]; mW = t.w[lane]; mRc = t.orient[lane] != 1; }
+        }
         for(int j = 0; j < tn; j++){
-            const int cs = off + __builtin_amdgcn_readlane(mShift, j);   // slot column of its first base
-            const int ln = __builtin_amdgcn_readlane(mLen, j);
-            const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mW), j));
-            const bool rc = __builtin_amdgcn_readlane(mRc, j) != 0;
+            const int cs = off + __builtin_amdgcn_readlane(mShift, src + j);   // slot column of its first base
+            const int ln = __builtin_amdgcn_readlane(mLen, src + j);
+            const float wj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mW), src + j));
+            const bool rc = __builtin_amdgcn_readlane(mRc, src + j) != 0;
             // slots holding its columns (AO: those under the anchor; lanes past the anchor
             // end add to columns that are never read)
             const int k0 = (AO ? max(cs, 0) : cs) >> 6;
@@ -1866,7 +1908,9 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     // 1) drop candidates without an orientation (include/corrector.hpp:982-1050), then
     // 2) the mismatch-ratio filter (include/corrector.hpp:1053-1149; paired :1151-1211)
     int n2 = 0;
-    if(n0 <= kWave){
+    LaneCand lc{};              // lists of at most 64: lane i's copy of working candidate i
+    const bool inLanes = n0 <= kWave;
+    if(inLanes){
         // at most one candidate per lane: both filters on the lane's registers and one
         // compaction, no intermediate list
         AlignBest b{};
@@ -1888,16 +1932,29 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
         const uint8_t pr = (k1 && lst.paired) ? lst.paired[beg + lane] : 0;
         const bool keep = k1 && (p.isPaired ? (pr != 0 || ratio < p.pairedThreshold) : (ratio < threshold));
         const uint64_t mask = __ballot(keep);
+        n2 = __popcll(mask);
+        uint32_t id = 0;
+        int L = 0;
+        float w = 0.0f;
         if(keep){
             const int d = beg + __popcll(mask & below);
-            const uint32_t id = lst.ids[beg + lane];
+            id = lst.ids[beg + lane];
+            L = p.len[id];
             const int no = b.nOps, ov = b.overlap;
-            lst.kId[d] = id; lst.kShift[d] = b.shift; lst.kLen[d] = p.len[id]; lst.kNops[d] = no; lst.kOvl[d] = ov;
-            lst.kOrient[d] = uint8_t(b.orientation); lst.kPaired[d] = pr;
             // overlap weight (include/hostdevicefunctions.cuh:93-97)
-            lst.kW[d] = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
+            w = 1.0f - sqrtf(no / (ov * p.maxErrorRate));
+            lst.kId[d] = id; lst.kShift[d] = b.shift; lst.kLen[d] = L; lst.kNops[d] = no; lst.kOvl[d] = ov;
+            lst.kOrient[d] = uint8_t(b.orientation); lst.kPaired[d] = pr;
+            lst.kW[d] = w;
         }
-        n2 = __popcll(mask);
+        // the same entries into lanes 0..n2-1 (kept lanes in order, the others after them:
+        // a permutation of the 64 lanes)
+        const int dl = keep ? __popcll(mask & below) : n2 + __popcll(~mask & below);
+        lc.id = uint32_t(__builtin_amdgcn_ds_permute(4 * dl, int(id)));
+        lc.shift = __builtin_amdgcn_ds_permute(4 * dl, b.shift);
+        lc.len = __builtin_amdgcn_ds_permute(4 * dl, L);
+        lc.orient = __builtin_amdgcn_ds_permute(4 * dl, b.orientation);
+        lc.w = __int_as_float(__builtin_amdgcn_ds_permute(4 * dl, __float_as_int(w)));
     }else{
         // more candidates than lanes: step 1 into the working list, then step 2 over it
         int n1 = 0;
@@ -1965,7 +2022,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     // 3) MSA and refinement (include/corrector.hpp:1307-1421; src/msa.cpp:532-734)
     WaveMsa<KC> m;
     int nk = n2;
-    if(!buildMsaWave<KC, AO, QM>(p, lst, a, La, arow, beg, nk, m, tile, lane)){
+    if(!buildMsaWave<KC, AO, QM>(p, lst, a, La, arow, beg, nk, m, tile, lane, inLanes, lc)){
         if(lane == 0) atomicAdd(p.error, 1);
         return;
     }
