@@ -1178,23 +1178,45 @@ struct LaneCand {
     float w;
 };
 
-// stageTile for candidates held in lanes: lane tb+q stages candidate tb+q into slot q
-// (its oriented 2-bit row, id and orientation), then the qualities. Same tile contents as
-// stageTile except the shift/length/weight arrays, which the caller reads from the lanes.
+// stageTile for candidates held in lanes: lane tb+q stages candidate tb+q's oriented 2-bit
+// row into slot q, and the quality chunks take their candidate's id and orientation from
+// its lane (ds_bpermute, every lane active), so the row and quality loads are independent.
+// Same rows and qualities as stageTile; the caller reads shift, length and weight from the
+// lanes (the tile's metadata arrays are not written).
 __device__ __forceinline__ void stageTileLanes(const MsaParams& p, const MsaTile& t, const LaneCand& c, int tb, int tn,
                                                int lane){
     const int W = p.W;
     const int q = lane - tb;
     if(q >= 0 && q < tn){
-        t.id[q] = c.id;
-        t.orient[q] = c.orient;
         const uint32_t* src = c.orient == 1 ? p.enc + p.pitch * c.id : p.rcenc + uint64_t(c.id) * uint64_t(2 * W);
         const int RW = tileRowWords(p.maxLen);
         for(int k = 0; k < RW; k++) t.rows[q * 2 * W + k] = src[k];
     }
     if(!p.useQ) return;
-    __syncthreads();
-    stageQualities(p, t, tn, lane, kWave);
+    const int qs = qStride(p.maxLen);
+    const int nc = qs >> 4;              // chunks per row; device rows are qdpitch >= qs bytes
+    const uint64_t magic = ((uint64_t(1) << 32) + uint64_t(nc) - 1) / uint64_t(nc);   // u / nc, exact here
+    const int total = tn * nc;
+    for(int base = 0; base < total; base += kWave){
+        const int u = base + lane;
+        const int j = min(int((uint64_t(u) * magic) >> 32), tn - 1);
+        const uint32_t id = uint32_t(__builtin_amdgcn_ds_bpermute(4 * (tb + j), int(c.id)));
+        const int orient = __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.orient);
+        if(u < total){
+            const int ch = u - j * nc;
+            const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * id + 16 * ch);
+            uint8_t* dst = t.q + j * qs;
+            if(orient == 1){
+                *reinterpret_cast<uint4*>(dst + 16 * ch) = v;
+            }else{
+                // byte s of the row goes to qs - 1 - s
+                uint4 r;
+                r.x = __builtin_bswap32(v.w); r.y = __builtin_bswap32(v.z);
+                r.z = __builtin_bswap32(v.y); r.w = __builtin_bswap32(v.x);
+                *reinterpret_cast<uint4*>(dst + qs - 16 * (ch + 1)) = r;
+            }
+        }
+    }
 }
 
 // Builds the MSA of the anchor and the first `nk` working candidates in LDS
