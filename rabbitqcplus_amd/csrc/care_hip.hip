@@ -407,15 +407,11 @@ __device__ __forceinline__ void gatherInsert(const CandParams& p, uint64_t a, in
 // (include/corrector.hpp:571-650; include/ordinaryminhasher.hpp:135-221,437-449).
 // Wave 0 looks the 48 signatures up; the gathered bucket values are deduplicated on
 // insertion into an LDS hash set (the first inserter of a value appends it to the
-// anchor's output area), then only the unique ids are sorted (bitonic, in LDS).
-__global__ void __launch_bounds__(kBlock)
-k_candidates(CandParams p){
-    extern __shared__ uint32_t hset[];       // hashSlots entries (reused for the sort), then mapOf
+// anchor's output area), then only the unique ids are sorted (rank sort, bitonic beyond
+// 256). hset holds hashSlots entries, then the map table. Batch anchor i.
+__device__ __forceinline__ void candidatesBlock(const CandParams& p, int i, uint32_t* hset, uint32_t* sOff, uint32_t* sPre,
+                                                int& sUnique){
     uint8_t* mapOf = reinterpret_cast<uint8_t*>(hset + p.hashSlots);
-    __shared__ uint32_t sOff[kMaxMaps];
-    __shared__ uint32_t sPre[kMaxMaps + 1];
-    __shared__ int sUnique;
-    const int i = batchAnchor(p.order, p.B);
     const uint64_t a = p.b0 + i;
     const int tid = threadIdx.x;
     if(tid < kWave){
@@ -480,6 +476,129 @@ k_candidates(CandParams p){
     if(tid == 0){
         p.uniq[i] = uint32_t(U);
         p.gath[i] = uint32_t(T);
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_candidates(CandParams p){
+    extern __shared__ uint32_t hset[];
+    __shared__ uint32_t sOff[kMaxMaps];
+    __shared__ uint32_t sPre[kMaxMaps + 1];
+    __shared__ int sUnique;
+    candidatesBlock(p, batchAnchor(p.order, p.B), hset, sOff, sPre, sUnique);
+}
+
+// One wavefront per anchor with a small hash set: most anchors have far fewer unique
+// candidates than gathered bucket values, so kSmallHash slots keep the wave's LDS near
+// 5 KB and many anchors in flight per compute unit. At most kSmallUnique distinct ids are
+// taken; an anchor with more is appended to `over` and redone by k_candidates_over (the
+// workgroup code above), so every anchor gets the same ids either way.
+constexpr int kSmallHash = 256;
+constexpr int kSmallUnique = 128;
+
+__host__ __device__ inline size_t candWaveLdsBytes(int capPerAnchor){
+    return size_t(kSmallHash) * 4 + (size_t(capPerAnchor) + 15) / 16 * 16;
+}
+
+__global__ void __launch_bounds__(kWave)
+k_candidates_wave(CandParams p, uint32_t* __restrict__ over, uint32_t* __restrict__ overCount){
+    extern __shared__ uint32_t hset[];       // kSmallHash entries (reused for the sort), then mapOf
+    uint8_t* mapOf = reinterpret_cast<uint8_t*>(hset + kSmallHash);
+    __shared__ uint32_t sOff[kMaxMaps];
+    __shared__ uint32_t sPre[kMaxMaps + 1];
+    __shared__ int sUnique;
+    __shared__ int sOver;
+    const int i = batchAnchor(p.order, p.B);
+    const uint64_t a = p.b0 + i;
+    const int lane = threadIdx.x;
+    const bool valid = p.len[a] >= p.k && !(p.excludeAmbiguous && p.ambig[a]);
+    uint32_t cnt = 0, off = 0;
+    if(valid && lane < p.M){
+        const uint2 b = p.bucketOf[a * uint64_t(p.M) + uint64_t(lane)];
+        off = b.x;
+        cnt = b.y > uint32_t(p.thr) ? 0u : b.y;
+    }
+    const int incl = waveInclusiveSum(int(cnt));
+    if(lane < kMaxMaps){ sOff[lane] = off; sPre[lane + 1] = uint32_t(incl); }
+    if(lane == 0){ sPre[0] = 0; sUnique = 0; sOver = 0; }
+    for(int q = incl - int(cnt); q < incl; q++) mapOf[q] = uint8_t(lane);
+    const int T = __builtin_amdgcn_readlane(incl, 63);
+    for(int q = lane; q < kSmallHash; q += kWave) hset[q] = 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t* out = p.raw + uint64_t(i) * p.capPerAnchor;
+    // gather in batches of kGatherBatch loads per lane, then insert them one step at a time;
+    // a step is skipped by the whole wave once the set holds too many ids, so at most
+    // kSmallUnique + kWave < kSmallHash slots are ever filled and every probe finds a slot
+    for(int base = 0; base < T; base += kWave * kGatherBatch){
+        uint32_t v[kGatherBatch];
+        #pragma unroll
+        for(int u = 0; u < kGatherBatch; u++){
+            const int idx = base + u * kWave + lane;
+            v[u] = 0xFFFFFFFFu;
+            if(idx < T){
+                const int m = mapOf[idx];
+                v[u] = p.values[uint64_t(m) * p.n + sOff[m] + uint32_t(idx - int(sPre[m]))];
+            }
+        }
+        #pragma unroll
+        for(int u = 0; u < kGatherBatch; u++){
+            if(__builtin_amdgcn_readfirstlane(*(volatile int*)&sOver)) break;
+            const uint32_t x = v[u];
+            if(x == 0xFFFFFFFFu || uint64_t(x) == a || (p.excludeAmbiguous && p.ambig[x])) continue;
+            uint32_t h = (x * 2654435761u) >> (32 - 8);
+            for(;;){
+                const uint32_t prev = atomicCAS(&hset[h], 0xFFFFFFFFu, x);
+                if(prev == 0xFFFFFFFFu){
+                    const int r = atomicAdd(&sUnique, 1);
+                    if(r < kSmallUnique) out[r] = x;
+                    else sOver = 1;
+                    break;
+                }
+                if(prev == x) break;
+                h = (h + 1) & uint32_t(kSmallHash - 1);
+            }
+        }
+        if(__builtin_amdgcn_readfirstlane(*(volatile int*)&sOver)) break;
+    }
+    __syncthreads();
+    const int U = sUnique;
+    if(sOver || U > kSmallUnique){
+        if(lane == 0) over[atomicAdd(overCount, 1u)] = uint32_t(i);
+        return;
+    }
+    // rank sort of the distinct ids, up to two per lane
+    const int U4 = (U + 3) & ~3;
+    for(int q = lane; q < U4; q += kWave) hset[q] = q < U ? out[q] : 0xFFFFFFFFu;
+    __syncthreads();
+    uint32_t w0 = lane < U ? hset[lane] : 0xFFFFFFFFu;
+    uint32_t w1 = lane + kWave < U ? hset[lane + kWave] : 0xFFFFFFFFu;
+    int r0 = 0, r1 = 0;
+    const uint4* h4 = reinterpret_cast<const uint4*>(hset);
+    for(int q = 0; q < U4 / 4; q++){
+        const uint4 w = h4[q];
+        r0 += int(w.x < w0) + int(w.y < w0) + int(w.z < w0) + int(w.w < w0);
+        r1 += int(w.x < w1) + int(w.y < w1) + int(w.z < w1) + int(w.w < w1);
+    }
+    if(lane < U) out[r0] = w0;
+    if(lane + kWave < U) out[r1] = w1;
+    if(lane == 0){
+        p.uniq[i] = uint32_t(U);
+        p.gath[i] = uint32_t(T);
+    }
+}
+
+// The anchors k_candidates_wave handed over (over[0, *overCount)), by workgroups that
+// stride through the list.
+__global__ void __launch_bounds__(kBlock)
+k_candidates_over(CandParams p, const uint32_t* __restrict__ over, const uint32_t* __restrict__ overCount){
+    extern __shared__ uint32_t hset[];
+    __shared__ uint32_t sOff[kMaxMaps];
+    __shared__ uint32_t sPre[kMaxMaps + 1];
+    __shared__ int sUnique;
+    const uint32_t count = *overCount;
+    for(uint32_t t = blockIdx.x; t < count; t += gridDim.x){
+        __syncthreads();
+        candidatesBlock(p, int(over[t]), hset, sOff, sPre, sUnique);
     }
 }
 
@@ -2741,7 +2860,7 @@ struct care_ctx {
     // batch working memory
     DBuf raw, uniq, offsets, ids, anchorOf, best, dbg, paired;
     DBuf kId, kShift, kLen, kNops, kOvl, kW, kOrient, kPaired;
-    DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp, scratch;
+    DBuf slots, recBytes, recOff, recOff64, outBuf, info, msaDbg, counters, scanTmp, scratch, candOver;
     // learned seam: classifier (bfloat16 weights then float biases) and feature output
     DBuf seam, featBuf, metaBuf;
     // pinned host words: record bytes (u64) and the error flag (i32) of the last pass
@@ -3326,9 +3445,18 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
         if(rc != CARE_OK) return rc;
     }
     cp.order = order;
+    // one wavefront per anchor; the few anchors with more distinct candidates than its
+    // small hash set takes are redone by workgroups (same ids either way)
+    HIPCHECK(ctx->candOver.ensure(size_t(B) * 4 + 16));
+    uint32_t* overCount = reinterpret_cast<uint32_t*>(ctx->candOver.as<uint8_t>() + size_t(B) * 4);
+    k_zero_u32<<<1, 64, 0, st>>>(overCount, 1);
+    const size_t waveLds = candWaveLdsBytes(capPerAnchor);
+    if(waveLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates_wave, hipFuncAttributeMaxDynamicSharedMemorySize, int(waveLds)));
+    k_candidates_wave<<<B, kWave, waveLds, st>>>(cp, ctx->candOver.as<uint32_t>(), overCount);
+    HIPCHECK(hipGetLastError());
     const size_t candLds = candLdsBytes(P, capPerAnchor);
-    if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
-    k_candidates<<<B, kBlock, candLds, st>>>(cp);
+    if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates_over, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
+    k_candidates_over<<<std::min(B, 4096), kBlock, candLds, st>>>(cp, ctx->candOver.as<uint32_t>(), overCount);
     HIPCHECK(hipGetLastError());
     k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(ctx->gath.as<uint32_t>(), B,
                                                                           ctx->counters.as<unsigned long long>());
@@ -3562,7 +3690,7 @@ void care_hip_destroy(care_ctx* ctx){
                    &ctx->capsDev, &ctx->values, &ctx->raw, &ctx->uniq, &ctx->offsets, &ctx->ids, &ctx->anchorOf,
                    &ctx->best, &ctx->dbg, &ctx->paired, &ctx->kId, &ctx->kShift, &ctx->kLen, &ctx->kNops, &ctx->kOvl,
                    &ctx->kW, &ctx->kOrient, &ctx->kPaired, &ctx->slots, &ctx->recBytes, &ctx->recOff, &ctx->recOff64,
-                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp, &ctx->scratch,
+                   &ctx->outBuf, &ctx->info, &ctx->msaDbg, &ctx->counters, &ctx->scanTmp, &ctx->scratch, &ctx->candOver,
                    &ctx->seam, &ctx->featBuf, &ctx->metaBuf, &ctx->fqText[0], &ctx->fqText[1], &ctx->fqHdr,
                    &ctx->fqHdrLen, &ctx->fqSeq, &ctx->fqQual, &ctx->mRec, &ctx->mOff, &ctx->mKeyIn, &ctx->mKeyOut,
                    &ctx->mIdxIn, &ctx->mIdxOut, &ctx->mRunBeg, &ctx->mRunEnd, &ctx->mPlan, &ctx->mSize, &ctx->mOutOff,
