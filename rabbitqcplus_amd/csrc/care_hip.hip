@@ -621,15 +621,19 @@ __global__ void k_add_gathered(const uint32_t* __restrict__ gath, int B, unsigne
     }
 }
 
+// Candidate lists of the batch's B anchors into one array (offsets from the scan of their
+// counts), a wavefront per anchor; anchorOf (the anchor of every entry) when not null.
 __global__ void __launch_bounds__(kBlock)
-k_compact(const uint32_t* raw, int capPerAnchor, const uint32_t* offsets, uint32_t* ids, uint32_t* anchorOf){
-    const int i = blockIdx.x;
+k_compact(const uint32_t* raw, int capPerAnchor, const uint32_t* offsets, int B, uint32_t* ids, uint32_t* anchorOf){
+    const int i = blockIdx.x * (kBlock / kWave) + int(threadIdx.x / kWave);
+    if(i >= B) return;
+    const uint32_t lane = threadIdx.x % kWave;
     const uint32_t beg = offsets[i];
     const uint32_t cnt = offsets[i + 1] - beg;
     const uint32_t* src = raw + uint64_t(i) * capPerAnchor;
-    for(uint32_t j = threadIdx.x; j < cnt; j += kBlock){
+    for(uint32_t j = lane; j < cnt; j += kWave){
         ids[beg + j] = src[j];
-        anchorOf[beg + j] = uint32_t(i);
+        if(anchorOf) anchorOf[beg + j] = uint32_t(i);
     }
 }
 
@@ -3479,8 +3483,12 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     HIPCHECK(ctx->kNops.ensure(T * 4)); HIPCHECK(ctx->kOvl.ensure(T * 4)); HIPCHECK(ctx->kW.ensure(T * 4));
     HIPCHECK(ctx->kOrient.ensure(T)); HIPCHECK(ctx->kPaired.ensure(T));
     if(ctx->o.keep_intermediates) HIPCHECK(ctx->dbg.ensure(T * 11 * sizeof(int)));
-    k_compact<<<B, kBlock, 0, st>>>(ctx->raw.as<uint32_t>(), capPerAnchor, ctx->offsets.as<uint32_t>(),
-                                    ctx->ids.as<uint32_t>(), ctx->anchorOf.as<uint32_t>());
+    // anchorOf is read by the wavefront-per-pair alignment and by candidate correction
+    const int wm = ctx->o.align_variant == 1 ? 0 : alignLaneWords(ctx->W);
+    const bool needAnchorOf = wm == 0 || ctx->o.correct_candidates != 0;
+    k_compact<<<(B + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, st>>>(
+        ctx->raw.as<uint32_t>(), capPerAnchor, ctx->offsets.as<uint32_t>(), B, ctx->ids.as<uint32_t>(),
+        needAnchorOf ? ctx->anchorOf.as<uint32_t>() : nullptr);
     HIPCHECK(hipGetLastError());
     if(ctx->o.paired){
         const int pairs = B / 2;
@@ -3496,7 +3504,6 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
     ap.offsets = ctx->offsets.as<uint32_t>(); ap.B = B; ap.maxLen = ctx->maxLen;
     ap.order = order;
-    const int wm = ctx->o.align_variant == 1 ? 0 : alignLaneWords(ctx->W);
     if(total){
         switch(wm){
             case 2: k_align_lanes<2><<<B, kWave, 0, st>>>(ap); break;
