@@ -72,7 +72,7 @@ typedef struct care_opts {
     int new_columns_to_correct;     /* --candidateCorrectionNewColumns, 15 */
     int msa_tile;                   /* staged engine: candidates staged into LDS per step of the
                                        one-wavefront MSA, a multiple of 4 in 4..64; 0 = automatic
-                                       (24 with quality scores, else 64) */
+                                       (20 with quality scores, else 64) */
 } care_opts;
 
 typedef struct care_ctx care_ctx;

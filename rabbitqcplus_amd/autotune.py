@@ -3,7 +3,7 @@
 Candidates are the engine and schedule options the library offers:
 - staged kernels with one lane per candidate in the alignment, or one wavefront per pair
   (the MSA's candidate tile keeps its default: on a 65k-anchor sample the tile sizes differ
-  by less than the timing noise, at full size 24 is the fastest);
+  by less than the timing noise, at full size the default is the fastest);
 - the persistent fused kernel at 4, 8 and 12 resident waves per compute unit, its pass
   captured in a HIP graph.
 Each candidate corrects the same sample of anchors (after one warm-up pass); the fastest

@@ -1231,6 +1231,15 @@ struct MsaTile {
 // one 16-byte store per chunk.
 __host__ __device__ inline int qStride(int maxLen){ return (maxLen + 15) & ~15; }
 
+// Bytes of the one-wavefront MSA's tile region: the candidate tile, which the learned seam
+// (kSeamLdsBytes) and the consensus of candidate correction (maxCols bytes) reuse.
+__host__ __device__ inline size_t waveTileRegion(int T, int W, int maxLen, bool useQ, bool seam, int maxCols){
+    size_t b = size_t(T) * (20 + 8 * size_t(W)) + (useQ ? size_t(T) * size_t(qStride(maxLen)) : 0);
+    if(seam && b < kSeamLdsBytes) b = kSeamLdsBytes;
+    if(b < size_t(maxCols)) b = size_t(maxCols);
+    return (b + 15) / 16 * 16;
+}
+
 // 2-bit words of a tile row
 __host__ __device__ inline int tileRowWords(int maxLen){ return (maxLen + 15) >> 4; }
 
@@ -1952,8 +1961,8 @@ __device__ __forceinline__ void carveWaveTile(const MsaParams& p, uint8_t* tp, c
     tile.rows = reinterpret_cast<uint32_t*>(tile.orient + T);
     tile.q = reinterpret_cast<uint8_t*>(tile.rows + T * 2 * p.W);
     tile.qw = qw;
-    const size_t tileBytes = size_t(T) * (20 + 8 * p.W) + (p.useQ ? size_t(T) * qStride(p.maxLen) : 0);
-    sSup = reinterpret_cast<float*>(tp + (max(tileBytes, kSeamLdsBytes) + 15) / 16 * 16);
+    sSup = reinterpret_cast<float*>(tp + waveTileRegion(T, p.W, p.maxLen, p.useQ != 0, p.seamMode != 0 || p.featOut,
+                                                        p.maxCols));
 }
 
 // Candidate correction of an HQ anchor (correctCandidatesClassic, include/corrector.hpp:
@@ -2467,7 +2476,8 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
 }
 
 template<int KC, bool AO, int QM>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 4 : (KC <= 12 ? 3 : 2))))
+__global__ void __launch_bounds__(kWave)
+__attribute__((amdgpu_waves_per_eu(AO && KC <= 3 ? 7 : (KC <= 8 ? 4 : 2))))
 k_msa_wave(MsaParams p){
     extern __shared__ uint8_t smem[];
     __shared__ float sQw[256];
@@ -2668,7 +2678,7 @@ __device__ __forceinline__ void alignAllWave(const AlignParams& p, uint64_t a, c
 // counter and runs candidates -> pair flags -> alignments -> filters/MSA/refinement/
 // decision -> record for them, keeping per-anchor lists in its private scratch.
 template<int KC, bool PAIRED>
-__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 8 ? 3 : 2)))
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(KC <= 3 ? 3 : 2)))
 k_fused(FusedParams fp){
     extern __shared__ uint8_t smem[];
     __shared__ float sQw[256];
@@ -3038,16 +3048,14 @@ int waveMsaAnchorSlots(int maxLen){
 }
 
 // Candidates per LDS tile of the one-wavefront MSA. With qualities (220 bytes per
-// candidate at 150 bp) 24 keeps the workgroup near 7 KB, so about 23 one-wavefront
-// workgroups fit a compute unit's LDS, while two tiles still cover a typical candidate
-// list; without qualities 64 fit in the same space.
-int defaultWaveMsaTile(bool useQ){ return useQ ? 24 : 64; }
+// candidate at 150 bp) 20 keeps the workgroup near 6 KB, so about 26 one-wavefront
+// workgroups fit a compute unit's LDS (the anchor-column kernels with at most 3 register
+// columns per lane are built for 7 waves per SIMD); without qualities 64 fit in less.
+int defaultWaveMsaTile(bool useQ){ return useQ ? 20 : 64; }
 
-// dynamic LDS of k_msa_wave: candidate tile + anchor-column supports
-size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ){
-    size_t b = size_t(tile) * (20 + 8 * size_t(W)) + (useQ ? size_t(tile) * qStride(maxLen) : 0);
-    b = std::max(b, kSeamLdsBytes);      // the seam reuses the tile region
-    return (b + 15) / 16 * 16 + size_t(maxLen) * 4 + 16;
+// dynamic LDS of k_msa_wave: tile region + anchor-column supports
+size_t waveMsaLdsBytes(int tile, int W, int maxLen, bool useQ, bool seam, int maxCols){
+    return waveTileRegion(tile, W, maxLen, useQ, seam, maxCols) + size_t(maxLen) * 4 + 16;
 }
 
 // Candidates per LDS tile: as many as fit next to the column arrays in ~40 KB, 16..64.
@@ -3246,7 +3254,8 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
     fp.units = uint32_t(units);
     fp.scratch = ctx->scratch.as<uint8_t>();
     const size_t lds = std::max({candLdsBytes(fp.c.hashSlots, fp.c.capPerAnchor), size_t(10) * (ctx->W + 1) * 8,
-                                 waveMsaLdsBytes(fp.m.tile, ctx->W, ctx->maxLen, q)});
+                                 waveMsaLdsBytes(fp.m.tile, ctx->W, ctx->maxLen, q,
+                                                 fp.m.seamMode != 0 || fp.m.featOut != nullptr, fp.m.maxCols)});
     auto kernelFor = [&]() -> const void* {
         switch(kc){
             case 2: return paired ? (const void*)k_fused<2, true> : (const void*)k_fused<2, false>;
@@ -3560,7 +3569,8 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
         // one wavefront per anchor, MSA columns in registers
         const bool q = ctx->o.use_quality_scores != 0;
         mp.tile = ctx->o.msa_tile ? ctx->o.msa_tile : defaultWaveMsaTile(q);
-        const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q);
+        const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q, mp.seamMode != 0 || mp.featOut != nullptr,
+                                           mp.maxCols);
         if(wholeMsa){
             // candidate correction and the dumps: quality mode read at run time
             switch(kc){
