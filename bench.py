@@ -147,6 +147,16 @@ def timed_passes(eng, steps, warmup, log, tag):
     return float(np.mean(step_ms)), nrec, data, eng.timings()
 
 
+# What bounds each staged kernel on MI355X (hardware counters, not HBM bandwidth)
+LIMITERS = {
+    "k_align": "vector and scalar instruction issue (about 1.06 VALU + 0.75 SALU instructions per clock per CU), "
+               "not HBM bandwidth",
+    "k_msa": "vector issue and LDS/memory latency, not HBM bandwidth",
+    "k_candidates": "latency of the random bucket-value loads, not HBM bandwidth",
+    "k_fused": "occupancy and instruction issue, not HBM bandwidth",
+}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -233,8 +243,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant),
                 "kernel": dominant, "avg_launch_ms": round(kernel_ms[dominant], 4),
                 "algorithmic_bytes_per_launch": int(per_launch[dominant]), "launches_per_step": int(batches),
-                "limiter": "vector ALU issue and LDS/memory latency, not HBM bandwidth "
-                           "(counters: profiles/pmc_staged_se_5mbp_150bp_1M.md)"}
+                "limiter": LIMITERS.get(dominant, "instruction issue and latency, not HBM bandwidth") +
+                           " (counters: profiles/pmc_staged_se_5mbp_150bp_1M.md)"}
     plain_err = evaluate.error_counts(s, rs, data)
     log(f"GPU plain: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
         f"errors {plain_err}")
