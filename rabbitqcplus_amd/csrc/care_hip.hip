@@ -1125,7 +1125,7 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
 
 // One wavefront per anchor of the batch, its candidates in chunks of 64 lanes.
 template<int WM>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(WM <= 5 ? 8 : 1)))
 k_align_lanes(AlignParams p){
     if(int(blockIdx.x) >= p.B) return;
     const int i = batchAnchor(p.order, p.B);
