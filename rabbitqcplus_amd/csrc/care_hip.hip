@@ -1159,6 +1159,7 @@ struct MsaParams {
     int isPaired; int useQ;
     float maxErrorRate; float ratio2, ratio3, ratio4; float goodCount; float pairedThreshold;
     float avgThr, minThr, minCovThr; int pct; float pctHalf; int sigThr;
+    float hqAvgT, hqMinT;           // HQ thresholds at hqPercent (see the decision)
     int maxCols; int maxLen;
     const uint32_t* hilo; int W; int tile;
     const uint32_t* rcenc;          // reverse-complement 2-bit rows, 2W words per read
@@ -1676,14 +1677,13 @@ k_msa(MsaParams p){
     const bool allGood = fgeq(avgSupport, p.avgThr) && fgeq(minSupport, p.minThr) && fgeq(float(mincov), p.minCovThr);
     bool isHQ = false;
     if(allGood){
-        int smallest = 100;
-        for(int percent = p.pct; percent >= 0; percent--){
-            const float factor = percent / 100.0f;
-            const float avgT = 1.0f - 1.0f * factor;
-            const float minT = 1.0f - 3.0f * factor;
-            if(fgeq(avgSupport, avgT) && fgeq(minSupport, minT)) smallest = percent;
-        }
-        isHQ = fgeq(float(mincov), p.minCovThr) && fleq(float(smallest), p.pctHalf);
+        // src/msa.cpp:357-372 takes the smallest percent in [0, pct] whose thresholds
+        // 1 - f and 1 - 3f (f = percent / 100) both supports meet and calls the anchor HQ
+        // when it is at most pct / 2. The thresholds fall as the percent grows and fgeq
+        // stays true as its bound falls, so that holds exactly when the supports meet the
+        // thresholds of hqPercent, the largest percent that is at most pct / 2 (the host
+        // computes both with the same float expressions).
+        isHQ = fgeq(float(mincov), p.minCovThr) && fgeq(avgSupport, p.hqAvgT) && fgeq(minSupport, p.hqMinT);
     }
     int nEdits = 0;
     for(int q = tid; q < La; q += kBlock){
@@ -2315,14 +2315,13 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     const bool allGood = fgeq(avgSupport, p.avgThr) && fgeq(minSupport, p.minThr) && fgeq(float(mincov), p.minCovThr);
     bool isHQ = false;
     if(allGood){
-        int smallest = 100;
-        for(int percent = p.pct; percent >= 0; percent--){
-            const float factor = percent / 100.0f;
-            const float avgT = 1.0f - 1.0f * factor;
-            const float minT = 1.0f - 3.0f * factor;
-            if(fgeq(avgSupport, avgT) && fgeq(minSupport, minT)) smallest = percent;
-        }
-        isHQ = fgeq(float(mincov), p.minCovThr) && fleq(float(smallest), p.pctHalf);
+        // src/msa.cpp:357-372 takes the smallest percent in [0, pct] whose thresholds
+        // 1 - f and 1 - 3f (f = percent / 100) both supports meet and calls the anchor HQ
+        // when it is at most pct / 2. The thresholds fall as the percent grows and fgeq
+        // stays true as its bound falls, so that holds exactly when the supports meet the
+        // thresholds of hqPercent, the largest percent that is at most pct / 2 (the host
+        // computes both with the same float expressions).
+        isHQ = fgeq(float(mincov), p.minCovThr) && fgeq(avgSupport, p.hqAvgT) && fgeq(minSupport, p.hqMinT);
     }
     // learned seam (include/corrector.hpp:1448-1472): in a non-HQ anchor, every anchor
     // column whose base differs from the consensus is a decision column; a logit >= ln 9
@@ -3087,6 +3086,18 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.minCovThr = ctx->o.m_coverage / 6.0f * ctx->o.estimated_coverage;
     mp.pct = int(std::ceil(e * 100.0f));
     mp.pctHalf = mp.pct * 0.5f;
+    {
+        // hqPercent: the largest percent in [0, pct] with fleq(percent, pct / 2) (feq within 1e-5)
+        auto fleqHost = [](float l, float r){
+            const float d = l - r < 0 ? -(l - r) : l - r;
+            return (l < r) || d < 1e-5f;
+        };
+        int hqPercent = 0;
+        for(int q = 0; q <= mp.pct; q++) if(fleqHost(float(q), mp.pctHalf)) hqPercent = q;
+        const float factor = hqPercent / 100.0f;
+        mp.hqAvgT = 1.0f - 1.0f * factor;
+        mp.hqMinT = 1.0f - 3.0f * factor;
+    }
     mp.sigThr = int(int(ctx->o.estimated_coverage) * 0.3f);
     mp.maxCols = ctx->maxCols; mp.maxLen = ctx->maxLen;
     mp.hilo = ctx->hilo.as<uint32_t>(); mp.W = ctx->W; mp.rcenc = ctx->rcenc.as<uint32_t>();
