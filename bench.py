@@ -149,7 +149,7 @@ def timed_passes(eng, steps, warmup, log, tag):
 
 # What bounds each staged kernel on MI355X (hardware counters, not HBM bandwidth)
 LIMITERS = {
-    "k_align": "vector and scalar instruction issue (about 1.06 VALU + 0.75 SALU instructions per clock per CU), "
+    "k_align": "vector and scalar instruction issue (about 1.1 VALU + 0.76 SALU instructions per clock per CU), "
                "not HBM bandwidth",
     "k_msa": "vector issue and LDS/memory latency, not HBM bandwidth",
     "k_candidates": "latency of the random bucket-value loads, not HBM bandwidth",
