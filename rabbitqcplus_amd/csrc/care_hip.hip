@@ -58,7 +58,8 @@ constexpr int kHid = 64;
 constexpr int kMlpParams = kHid * kFeat + kHid + kHid * kHid + kHid + kHid + 1;
 constexpr float kConfLogit = 2.1972245773f;      // ln 9: sigmoid(z) >= 0.9 <=> z >= ln 9
 // LDS of the seam: 64 feature rows, both hidden layers, 64 logits
-constexpr size_t kSeamLdsBytes = size_t(kWave) * kFeat * 4 + 2 * kHid * 4 + kWave * 4;
+// seam scratch: bfloat16 feature rows of up to 64 columns, two hidden layers, the logits
+constexpr size_t kSeamLdsBytes = size_t(kWave) * kFeat * 2 + 2 * kHid * 4 + kWave * 4;
 constexpr size_t kSeamWeightBytes = size_t(kFeat * kHid + kHid * kHid + kHid) * 2;   // 10368, 16-byte aligned
 
 struct Slot {            // one open-addressing slot of a per-map table (16 bytes)
@@ -1923,17 +1924,17 @@ __device__ __forceinline__ float bf16r(float x){
 }
 __device__ __forceinline__ float bf16f(uint16_t h){ return __uint_as_float(uint32_t(h) << 16); }
 
-// Logit of the feature row sx (already rounded to bfloat16) by the whole wave: lane j
-// computes hidden unit j of both layers, each sum starting at the bias and adding the
-// inputs in index order; the output sum runs over the hidden units in order on every
-// lane. Weights are bfloat16, sums float. Returns the wave-uniform logit.
-__device__ __forceinline__ float seamLogit(const MsaParams& p, const float* sx, float* sH1, float* sH2, int lane){
+// Logit of the feature row sx (bfloat16 bits) by the whole wave: lane j computes hidden
+// unit j of both layers, each sum starting at the bias and adding the inputs in index
+// order; the output sum runs over the hidden units in order on every lane. Weights are
+// bfloat16, sums float. Returns the wave-uniform logit.
+__device__ __forceinline__ float seamLogit(const MsaParams& p, const uint16_t* sx, float* sH1, float* sH2, int lane){
     const uint16_t* w1t = p.seamW;
     const uint16_t* w2t = p.seamW + kFeat * kHid;
     const uint16_t* w3 = w2t + kHid * kHid;
     float a = p.seamB[lane];
     #pragma unroll
-    for(int k = 0; k < kFeat; k++) a = a + bf16f(w1t[k * kHid + lane]) * sx[k];
+    for(int k = 0; k < kFeat; k++) a = a + bf16f(w1t[k * kHid + lane]) * bf16f(sx[k]);
     sH1[lane] = bf16r(a > 0.0f ? a : 0.0f);
     __syncthreads();
     a = p.seamB[kHid + lane];
@@ -2329,8 +2330,8 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
     // classic rule below. Decision columns can also be written out with their features.
     uint32_t seamTake = 0, seamKeep = 0;    // 1 bit per slot
     if((p.seamMode != 0 || p.featOut) && !isHQ){
-        float* sX = reinterpret_cast<float*>(tile.shift);       // the tile region is free now
-        float* sH1 = sX + kWave * kFeat;
+        uint16_t* sX = reinterpret_cast<uint16_t*>(tile.shift);   // the tile region is free now
+        float* sH1 = reinterpret_cast<float*>(sX + kWave * kFeat);
         float* sH2 = sH1 + kHid;
         float* sZ = sH2 + kHid;
         #pragma unroll
@@ -2378,7 +2379,7 @@ __device__ __forceinline__ void msaAnchorWave(const MsaParams& p, const AnchorLi
                 const float right = i + 1 < La ? sSup[i + 1] : 1.0f;
                 f[15] = left < right ? left : right;
                 #pragma unroll
-                for(int q = 0; q < kFeat; q++) sX[e * kFeat + q] = bf16r(f[q]);
+                for(int q = 0; q < kFeat; q++) sX[e * kFeat + q] = uint16_t(__float_as_uint(bf16r(f[q])) >> 16);
                 if(p.featOut && fbase + e < p.featCap){
                     const int classic = allGood ? cn : ((m.sup[k] > 0.90f && origCov <= 2) ? cn : ob);
                     float* fo = p.featOut + (fbase + e) * kFeat;
