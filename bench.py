@@ -100,25 +100,48 @@ def cpu_baseline_child(args):
     return None
 
 
+def baseline_anchors(sample, rs, paired):
+    n = min(sample, rs.n)
+    return n - n % 2 if paired else n
+
+
+def records_digest(data: bytes, below: int):
+    """(count, sha256) of the records of reads < below, ordered by read id."""
+    import hashlib
+    spans = []
+    p = 0
+    while p < len(data):
+        rid = int.from_bytes(data[p:p + 4], "little")
+        nb = 8 + (int.from_bytes(data[p + 4:p + 8], "little") & ((1 << 29) - 1))
+        if rid < below:
+            spans.append((rid, p, nb))
+        p += nb
+    spans.sort(key=lambda x: x[0])
+    return len(spans), hashlib.sha256(b"".join(data[q:q + nb] for _, q, nb in spans)).hexdigest()
+
+
 def cpu_baseline(rs, opts, sample, threads):
     import oracle
-    n = min(sample, rs.n)
-    if opts.get("paired"):
-        n -= n % 2
+    n = baseline_anchors(sample, rs, opts.get("paired"))
     if oracle.ref_available():
         ref = oracle.CareRef(rs, threads=threads, **opts)
         secs, recs = ref.correct_timed(threads, n)
+        # the same anchors once more, keeping the records (sorted by read id), for the
+        # comparison with the GPU's
+        _, data = ref.correct(threads, n)
         ref.close()
         kind = "reference"
     else:
         cpu = oracle.CareCpu(rs, threads=threads, **opts)
-        cpu.correct(0, n, threads=threads)
+        _, data = cpu.correct(0, n, threads=threads)
         secs = cpu.last_seconds()
         cpu.close()
         kind = "port"
+    count, digest = records_digest(data, n)
     return {"value": n / secs, "unit": "reads/s", "cores": threads, "kind": kind,
             "sample": f"first {n} anchors of the same workload (tables over all {rs.n} reads), "
-                      f"{secs:.2f} s wall on {threads} threads"}
+                      f"{secs:.2f} s wall on {threads} threads",
+            "records": count, "records_sha256": digest}
 
 
 def measured_traffic(workload, kernel):
@@ -313,6 +336,13 @@ def main():
     if args.cpu_sample > 0:
         log(f"CPU baseline, {args.cpu_sample} anchors")
         cpu = cpu_baseline_child(args)
+        if cpu is not None and "records_sha256" in cpu:
+            # the plain pass's records of the same anchors against the CPU run's
+            n = baseline_anchors(args.cpu_sample, rs, rs.paired)
+            gcount, gdigest = records_digest(data, n)
+            cpu["records_identical_to_gpu"] = bool(gcount == cpu["records"] and gdigest == cpu["records_sha256"])
+            log(f"CPU baseline records of the first {n} anchors: {cpu['records']}, identical to the GPU's: "
+                f"{cpu['records_identical_to_gpu']}")
 
     line = {
         "metric": "corrected reads/s on 1 MI355X, plain vs learned; post-correction error rate",
