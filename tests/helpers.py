@@ -42,9 +42,18 @@ CASES = {
     # reference picks for reads of 160 bp and more
     "len300_q": (dict(genome_length=60_000, read_length_min=280, read_length_max=330, coverage=30.0,
                       q_first=30.0, q_last=20.0, seed=19), dict(use_quality_scores=1, kmerlength=32)),
-    # 360..420 bp: MSA wider than the register kernel, workgroup kernel path
+    # 360..420 bp: the whole MSA is wider than the register kernel (workgroup kernel for the
+    # intermediate dumps); the plain correction's anchor-column MSA has 8 register columns
+    # per lane; alignment by the wavefront-per-pair kernel (planes wider than 11 words)
     "len400": (dict(genome_length=60_000, read_length_min=360, read_length_max=420, coverage=30.0,
                     q_first=30.0, q_last=20.0, seed=20), dict(kmerlength=32)),
+    # 600..760 bp, qualities: anchor-column MSA with 12 register columns per lane
+    "len700_q": (dict(genome_length=40_000, read_length_min=600, read_length_max=760, coverage=25.0,
+                      q_first=30.0, q_last=20.0, seed=22),
+                 dict(use_quality_scores=1, kmerlength=32, estimated_coverage=25.0)),
+    # 850..1000 bp, no qualities: anchor-column MSA with 16 register columns per lane
+    "len900": (dict(genome_length=40_000, read_length_min=850, read_length_max=1000, coverage=25.0,
+                    q_first=30.0, q_last=20.0, seed=23), dict(kmerlength=32, estimated_coverage=25.0)),
     # short reads below k and near the minimum overlap
     "short": (dict(genome_length=20_000, read_length_min=15, read_length_max=60, coverage=40.0,
                    q_first=30.0, q_last=24.0, seed=17), dict(kmerlength=20, estimated_coverage=40.0)),
