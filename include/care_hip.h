@@ -121,6 +121,21 @@ int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints
                        const int32_t* lengths, const uint8_t* quals, size_t qual_pitch,
                        uint64_t num_reads, const uint8_t* ambiguous);
 
+/* Host helper (no device work, no context): the arrays care_hip_set_reads takes, from
+ * reads as ASCII rows of seq_pitch bytes. Replaces the parser-batch encoding of
+ * constructChunkedReadStorageFromFiles (include/chunkedreadstorageconstruction.hpp:
+ * 67-106, 260-297; 2-bit encoding include/sequencehelpers.hpp:154-187): ACGT and acgt
+ * map to 0..3; every other byte inside a read becomes A, C, G, T in turn, the turn
+ * restarting at every batch of 65,536 reads, and marks the read ambiguous; bases past a
+ * read's length are 0. With quals_in, quality rows are copied into rows of
+ * quals_out_pitch bytes, zero past each read's length. threads = 0: all host threads.
+ * Returns CARE_EINVAL if a length is negative or exceeds 16 * pitch_ints, seq_pitch,
+ * or (with qualities) either quality pitch. */
+int care_hip_encode_reads(const uint8_t* seqs, size_t seq_pitch, const int32_t* lengths, uint64_t num_reads,
+                          uint32_t* enc2bit, size_t pitch_ints, uint8_t* ambiguous,
+                          const uint8_t* quals_in, size_t quals_in_pitch, uint8_t* quals_out,
+                          size_t quals_out_pitch, int threads);
+
 /* Read storage built on the device from FASTQ text (replaces
  * constructChunkedReadStorageFromFiles, include/chunkedreadstorageconstruction.hpp:29-480,
  * with its reader include/readlibraryio.hpp): one text for single-end input, two for

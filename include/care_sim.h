@@ -47,6 +47,11 @@ typedef struct care_sim_config {
     double n_rate;              /* probability of replacing a base by 'N' */
     double tandem_fraction;     /* genome fraction overwritten by tandem repeats */
     int tandem_period_max;      /* repeat unit length 1..tandem_period_max */
+    /* 0: every read is drawn from the one generator stream, in read order.
+       1: the genome comes from that stream; read i (or pair i) then draws from its own
+          stream seeded by (seed, i), so reads are simulated on all host threads and the
+          output does not depend on the thread count. */
+    int per_read_streams;
 } care_sim_config;
 
 typedef struct care_sim_result care_sim_result;
@@ -60,6 +65,11 @@ int care_sim_copy(const care_sim_result* r, uint64_t first, uint64_t n, size_t p
                   char* seqs, char* quals, char* truth, int* lengths,
                   uint64_t* positions, int* strands);
 void care_sim_free(care_sim_result* r);
+/* 4-line FASTQ text of reads first, first+step, ... (count reads) from row-major
+   buffers of `pitch` bytes per read; read i is named "@r<i>". With out == NULL the
+   size is returned; otherwise the bytes written (0 when cap is too small). */
+size_t care_sim_fastq(const char* seqs, const char* quals, const int* lengths, size_t pitch,
+                      uint64_t first, uint64_t step, uint64_t count, char* out, size_t cap);
 
 #ifdef __cplusplus
 }

@@ -47,7 +47,7 @@ def _run(cmd, cwd=None):
 def build_sim():
     os.makedirs(LIBDIR, exist_ok=True)
     out = os.path.join(LIBDIR, "libcaresim.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{INCLUDE}",
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", f"-I{INCLUDE}",
           os.path.join(CSRC, "readsim.cpp"), "-o", out])
     return out
 

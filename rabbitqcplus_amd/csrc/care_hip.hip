@@ -39,9 +39,12 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <limits>
+#include <new>
 #include <string>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -3296,9 +3299,18 @@ int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& ou
         return recordsDevicePart(ctx, B) == CARE_OK ? hipSuccess : hipErrorUnknown;
     };
     if(ctx->o.use_graphs){
-        // one executable graph per (parameters, grid, buffers); replayed while they hold
-        std::vector<uint8_t> key(sizeof(fp) + 3 * sizeof(uint64_t));
-        const uint64_t extra[3] = {G, uint64_t(lds), uint64_t(B)};
+        // one executable graph per (parameters, grid, buffers); replayed while they hold.
+        // The key holds every device or pinned address the captured work uses: those in
+        // the kernel parameters, and the record scan's input, output and workspace, which
+        // other calls on this context can reallocate.
+        const uint64_t extra[] = {G, uint64_t(lds), uint64_t(B),
+                                  uint64_t(reinterpret_cast<uintptr_t>(ctx->scanTmp.p)), uint64_t(ctx->scanTmp.bytes),
+                                  uint64_t(reinterpret_cast<uintptr_t>(ctx->recOff64.p)), uint64_t(ctx->recOff64.bytes),
+                                  uint64_t(reinterpret_cast<uintptr_t>(ctx->recOff.p)), uint64_t(ctx->recOff.bytes),
+                                  uint64_t(reinterpret_cast<uintptr_t>(ctx->recBytes.p)), uint64_t(ctx->recBytes.bytes),
+                                  uint64_t(reinterpret_cast<uintptr_t>(ctx->counters.p)),
+                                  uint64_t(reinterpret_cast<uintptr_t>(ctx->pinned))};
+        std::vector<uint8_t> key(sizeof(fp) + sizeof(extra));
         std::memcpy(key.data(), &fp, sizeof(fp));
         std::memcpy(key.data() + sizeof(fp), extra, sizeof(extra));
         if(!ctx->fusedExec || key != ctx->fusedKey){
@@ -3669,6 +3681,27 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
 
 }  // namespace
 
+// Entry points run their bodies through guarded(): no C++ exception crosses the
+// C-ABI (a failed host allocation is CARE_ENOMEM, any other exception CARE_EINVAL).
+template<class F>
+int guarded(care_ctx* h, F&& f){
+    const char* what = "unexpected C++ exception";
+    int code = CARE_EINVAL;
+    try{
+        return f();
+    }catch(const std::bad_alloc&){
+        what = "host allocation failed";
+        code = CARE_ENOMEM;
+    }catch(const std::exception& e){
+        what = e.what();
+    }catch(...){
+    }
+    if(h){
+        try{ h->err = what; }catch(...){}
+    }
+    return code;
+}
+
 extern "C" {
 
 void care_hip_default_opts(care_opts* o){
@@ -3685,7 +3718,7 @@ void care_hip_default_opts(care_opts* o){
     o->new_columns_to_correct = 15;
 }
 
-int care_hip_create(const care_opts* opts, int device, care_ctx** out){
+static int care_hip_create_body(const care_opts* opts, int device, care_ctx** out){
     if(!opts || !out) return CARE_EINVAL;
     *out = nullptr;
     if(opts->num_hash_maps < 1 || opts->num_hash_maps > kMaxMaps) return CARE_EINVAL;
@@ -3735,7 +3768,7 @@ void care_hip_destroy(care_ctx* ctx){
     delete ctx;
 }
 
-int care_hip_device_info(int device, char* name, size_t cap, int* compute_units){
+static int care_hip_device_info_body(int device, char* name, size_t cap, int* compute_units){
     int count = 0;
     if(hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return CARE_ENODEV;
     hipDeviceProp_t prop;
@@ -3803,7 +3836,72 @@ int finishReads(care_ctx* ctx, hipEvent_t done = nullptr){
     return CARE_OK;
 }
 
-int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints, const int32_t* lengths,
+static int care_hip_encode_reads_body(const uint8_t* seqs, size_t seq_pitch, const int32_t* lengths, uint64_t num_reads,
+                          uint32_t* enc2bit, size_t pitch_ints, uint8_t* ambiguous,
+                          const uint8_t* quals_in, size_t quals_in_pitch, uint8_t* quals_out,
+                          size_t quals_out_pitch, int threads){
+    if(num_reads == 0) return CARE_OK;
+    if(!seqs || !lengths || !enc2bit || !ambiguous || pitch_ints == 0) return CARE_EINVAL;
+    if((quals_in == nullptr) != (quals_out == nullptr)) return CARE_EINVAL;
+    for(uint64_t i = 0; i < num_reads; i++){
+        const int L = lengths[i];
+        if(L < 0 || size_t(L) > 16 * pitch_ints || size_t(L) > seq_pitch) return CARE_EINVAL;
+        if(quals_in && (size_t(L) > quals_in_pitch || size_t(L) > quals_out_pitch)) return CARE_EINVAL;
+    }
+    constexpr uint64_t kBatch = 65536;     // parser batch: the replacement turn restarts here
+    const uint64_t batches = (num_reads + kBatch - 1) / kBatch;
+    uint8_t code[256];
+    std::memset(code, 4, sizeof(code));
+    code['A'] = code['a'] = 0; code['C'] = code['c'] = 1; code['G'] = code['g'] = 2; code['T'] = code['t'] = 3;
+    auto encodeBatch = [&](uint64_t b){
+        uint32_t turn = 0;
+        const uint64_t r1 = std::min(num_reads, (b + 1) * kBatch);
+        for(uint64_t r = b * kBatch; r < r1; r++){
+            const uint8_t* s = seqs + r * seq_pitch;
+            uint32_t* row = enc2bit + r * pitch_ints;
+            const int L = lengths[r];
+            uint8_t amb = 0;
+            for(size_t w = 0; w < pitch_ints; w++){
+                uint32_t word = 0;
+                const int p0 = int(w) * 16;
+                for(int j = 0; j < 16; j++){
+                    const int pos = p0 + j;
+                    uint32_t c = 0;
+                    if(pos < L){
+                        c = code[s[pos]];
+                        if(c == 4){ c = turn & 3; turn++; amb = 1; }
+                    }
+                    word |= c << (30 - 2 * j);
+                }
+                row[w] = word;
+            }
+            ambiguous[r] = amb;
+            if(quals_in){
+                uint8_t* qo = quals_out + r * quals_out_pitch;
+                std::memcpy(qo, quals_in + r * quals_in_pitch, size_t(L));
+                std::memset(qo + L, 0, quals_out_pitch - size_t(L));
+            }
+        }
+    };
+    int T = threads > 0 ? threads : int(std::max(1u, std::thread::hardware_concurrency()));
+    T = int(std::min<uint64_t>(uint64_t(T), batches));
+    if(T <= 1){
+        for(uint64_t b = 0; b < batches; b++) encodeBatch(b);
+        return CARE_OK;
+    }
+    std::vector<std::thread> pool;
+    try{
+        for(int t = 0; t < T; t++)
+            pool.emplace_back([&, t](){ for(uint64_t b = uint64_t(t); b < batches; b += uint64_t(T)) encodeBatch(b); });
+    }catch(...){
+        for(auto& th : pool) th.join();
+        return CARE_ENOMEM;
+    }
+    for(auto& th : pool) th.join();
+    return CARE_OK;
+}
+
+static int care_hip_set_reads_body(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints, const int32_t* lengths,
                        const uint8_t* quals, size_t qual_pitch, uint64_t num_reads, const uint8_t* ambiguous){
     if(!ctx) return CARE_EINVAL;
     if(!enc2bit || !lengths || !ambiguous || num_reads == 0) return ctx->fail(CARE_EINVAL, "empty or null read storage");
@@ -4094,7 +4192,7 @@ static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t raw, int 
     return CARE_OK;
 }
 
-int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, const char* fastq2, size_t bytes2){
+static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size_t bytes1, const char* fastq2, size_t bytes2){
     if(!ctx || !fastq1) return CARE_EINVAL;
     const bool paired = ctx->o.paired != 0;
     if(paired != (fastq2 != nullptr)) return ctx->fail(CARE_EINVAL, paired ? "paired input needs two FASTQ texts"
@@ -4394,7 +4492,7 @@ __global__ void __launch_bounds__(256) k_merge_write(MergeParams p){
     }
 }
 
-int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file, int quality_labels,
+static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file, int quality_labels,
                          char* dst, size_t cap, size_t* written){
     if(!ctx || !written) return CARE_EINVAL;
     *written = 0;
@@ -4568,7 +4666,7 @@ __global__ void k_set_flags(const uint32_t* __restrict__ ids, uint64_t m, uint8_
     if(i < m) flags[ids[i]] = 1;
 }
 
-int care_hip_save_reads(care_ctx* ctx, const char* path){
+static int care_hip_save_reads_body(care_ctx* ctx, const char* path){
     if(!ctx || !path) return CARE_EINVAL;
     if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "no reads");
     HIPCHECK(hipSetDevice(ctx->device));
@@ -4662,14 +4760,26 @@ int care_hip_save_reads(care_ctx* ctx, const char* path){
     return CARE_OK;
 }
 
-int care_hip_load_reads(care_ctx* ctx, const char* path){
+static int care_hip_load_reads_body(care_ctx* ctx, const char* path){
     if(!ctx || !path) return CARE_EINVAL;
     std::FILE* f = std::fopen(path, "rb");
     if(!f) return ctx->fail(CARE_EINVAL, "cannot open %s", path);
     struct Closer { std::FILE* f; ~Closer(){ if(f) std::fclose(f); } } closer{f};
+    // every section size is checked against the bytes left in the file before anything
+    // is allocated for it
+    if(std::fseek(f, 0, SEEK_END) != 0) return ctx->fail(CARE_EINVAL, "%s: cannot seek", path);
+    const long fileEnd = std::ftell(f);
+    if(fileEnd < 0 || std::fseek(f, 0, SEEK_SET) != 0) return ctx->fail(CARE_EINVAL, "%s: cannot seek", path);
     bool ok = true;
     auto get = [&](void* p, size_t b){ ok = ok && (b == 0 || std::fread(p, 1, b, f) == b); return ok; };
     auto skip = [&](uint64_t b){ ok = ok && std::fseek(f, long(b), SEEK_CUR) == 0; };
+    // true when `count` items of `size` bytes fit in what is left of the file
+    auto fits = [&](uint64_t count, uint64_t size){
+        const long at = std::ftell(f);
+        if(at < 0 || at > fileEnd) return false;
+        const uint64_t left = uint64_t(fileEnd - at);
+        return size == 0 || count <= left / size;
+    };
     uint64_t nr = 0, sec[4] = {0, 0, 0, 0};
     int upper = 0, lower = 0, qbits = 0;
     uint8_t hasQ = 0;
@@ -4685,12 +4795,28 @@ int care_hip_load_reads(care_ctx* ctx, const char* path){
        || (bits == 0 && maxLen != minLen) || (bits > 0 && lwords < (uint64_t(bits) * nr + 31) / 32) || nr == 0
        || nr >= 0xFFFFFFFFull)
         return ctx->fail(CARE_EINVAL, "%s: not a preprocessed-reads file of this format", path);
+    if(!fits(lwords, 4)) return ctx->fail(CARE_EINVAL, "%s: length section larger than the file", path);
     std::vector<uint32_t> lw(lwords);
     get(lw.data(), lbytes);
+    if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
+    if(bits > 0){
+        // every length (minLen + its bits-wide value, MSB first) must lie within
+        // [minLen, maxLen]: the row pitch, the MSA bounds and the record slots come from maxLen
+        uint64_t pos = 0;
+        for(uint64_t i = 0; i < nr; i++){
+            uint32_t v = 0;
+            for(int k = 0; k < bits; k++, pos++) v = (v << 1) | ((lw[pos >> 5] >> (31 - int(pos & 31))) & 1u);
+            if(int64_t(minLen) + int64_t(v) > int64_t(maxLen))
+                return ctx->fail(CARE_EINVAL, "%s: read %llu is longer (%lld) than the header's maximum %d", path,
+                                 (unsigned long long)i, (long long)(int64_t(minLen) + int64_t(v)), maxLen);
+        }
+    }
     uint64_t sp = 0, sn = 0;
     get(&sp, 8); get(&sn, 8);
     if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
-    if(sp * 16 < uint64_t(maxLen) || sn != sp * nr) return ctx->fail(CARE_EINVAL, "%s: bad sequence section", path);
+    if(sp == 0 || sp > (uint64_t(1) << 20) || sp * 16 < uint64_t(maxLen) || sn / sp != nr || sn % sp != 0)
+        return ctx->fail(CARE_EINVAL, "%s: bad sequence section", path);
+    if(!fits(sn, 4)) return ctx->fail(CARE_EINVAL, "%s: sequence section larger than the file", path);
     std::vector<uint32_t> seq(sn);
     get(seq.data(), sn * 4);
     uint64_t qp = 0, qn = 0;
@@ -4702,16 +4828,19 @@ int care_hip_load_reads(care_ctx* ctx, const char* path){
         // include/chunkedreadstorage.hpp:201-232
         if(!hasQ) return ctx->fail(CARE_EINVAL, "%s: quality scores expected, but none are present", path);
         if(qbits != 8) return ctx->fail(CARE_EINVAL, "%s uses %d bits per quality score, 8 are required", path, qbits);
-        if(qp * 4 < uint64_t(maxLen) || qn != qp * nr) return ctx->fail(CARE_EINVAL, "%s: bad quality section", path);
+        if(qp == 0 || qp > (uint64_t(1) << 20) || qp * 4 < uint64_t(maxLen) || qn / qp != nr || qn % qp != 0)
+            return ctx->fail(CARE_EINVAL, "%s: bad quality section", path);
+        if(!fits(qn, 4)) return ctx->fail(CARE_EINVAL, "%s: quality section larger than the file", path);
         qual.resize(qn);
         get(qual.data(), qn * 4);
     }else{
+        if(!fits(qn, 4)) return ctx->fail(CARE_EINVAL, "%s: quality section larger than the file", path);
         skip(qn * 4);
     }
     uint64_t na = 0;
     get(&na, 8);
     if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
-    if(na > nr) return ctx->fail(CARE_EINVAL, "%s: bad ambiguous-read section", path);
+    if(na > nr || !fits(na, 4)) return ctx->fail(CARE_EINVAL, "%s: bad ambiguous-read section", path);
     std::vector<uint32_t> ambIds(na);
     get(ambIds.data(), na * 4);
     if(!ok) return ctx->fail(CARE_EINVAL, "%s: truncated", path);
@@ -4760,7 +4889,7 @@ int care_hip_load_reads(care_ctx* ctx, const char* path){
     return finishReads(ctx);
 }
 
-int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints){
+static int care_hip_read_layout_body(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints){
     if(!ctx) return CARE_EINVAL;
     if(num_reads) *num_reads = ctx->hasReads ? ctx->n : 0;
     if(max_length) *max_length = ctx->hasReads ? ctx->maxLen : 0;
@@ -4768,7 +4897,7 @@ int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_leng
     return CARE_OK;
 }
 
-int care_hip_inspect_reads(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* enc2bit, int32_t* lengths,
+static int care_hip_inspect_reads_body(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* enc2bit, int32_t* lengths,
                            uint8_t* quals, size_t qual_pitch, uint8_t* ambiguous){
     if(!ctx) return CARE_EINVAL;
     if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "no reads");
@@ -4824,7 +4953,7 @@ static int resolveBuckets(care_ctx* ctx){
     return CARE_OK;
 }
 
-int care_hip_build_tables(care_ctx* ctx, int* maps_built){
+static int care_hip_build_tables_body(care_ctx* ctx, int* maps_built){
     if(!ctx) return CARE_EINVAL;
     if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "set_reads first");
     HIPCHECK(hipSetDevice(ctx->device));
@@ -4861,7 +4990,7 @@ constexpr float kTableFileLoad = 0.8f;   // ProgramOptions::hashtableLoadfactor 
 struct FileSlot { uint64_t key; uint32_t offset; uint16_t count; uint16_t pad; };
 static_assert(sizeof(FileSlot) == 16, "file slot layout");
 
-int care_hip_save_tables(care_ctx* ctx, const char* path){
+static int care_hip_save_tables_body(care_ctx* ctx, const char* path){
     if(!ctx || !path) return CARE_EINVAL;
     if(!ctx->hasTables) return ctx->fail(CARE_ESTATE, "build_tables first");
     HIPCHECK(hipSetDevice(ctx->device));
@@ -4909,7 +5038,7 @@ int care_hip_save_tables(care_ctx* ctx, const char* path){
     return CARE_OK;
 }
 
-int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
+static int care_hip_load_tables_body(care_ctx* ctx, const char* path, int* maps_loaded){
     if(!ctx || !path) return CARE_EINVAL;
     if(!ctx->hasReads) return ctx->fail(CARE_ESTATE, "set_reads first");
     HIPCHECK(hipSetDevice(ctx->device));
@@ -4917,7 +5046,16 @@ int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
     std::FILE* f = std::fopen(path, "rb");
     if(!f) return ctx->fail(CARE_EINVAL, "cannot open %s", path);
     struct Closer { std::FILE* f; ~Closer(){ if(f) std::fclose(f); } } closer{f};
+    if(std::fseek(f, 0, SEEK_END) != 0) return ctx->fail(CARE_EINVAL, "%s: cannot seek", path);
+    const long fileEnd = std::ftell(f);
+    if(fileEnd < 0 || std::fseek(f, 0, SEEK_SET) != 0) return ctx->fail(CARE_EINVAL, "%s: cannot seek", path);
     auto get = [&](void* p, size_t b){ return std::fread(p, 1, b, f) == b; };
+    // true when `count` items of `size` bytes fit in what is left of the file
+    auto fits = [&](uint64_t count, uint64_t size){
+        const long at = std::ftell(f);
+        if(at < 0 || at > fileEnd) return false;
+        return size == 0 || count <= uint64_t(fileEnd - at) / size;
+    };
     int k = 0, thr = 0, maps = 0;
     float lf = 0.0f;
     if(!get(&k, 4) || !get(&thr, 4) || !get(&lf, 4) || !get(&maps, 4)) return ctx->fail(CARE_EINVAL, "%s: truncated header", path);
@@ -4940,6 +5078,7 @@ int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
         if(!get(&elements, 8)) return ctx->fail(CARE_EINVAL, "%s: truncated map %d", path, m);
         if(elements > ctx->n) return ctx->fail(CARE_EINVAL, "%s: map %d holds %llu values for %llu reads", path, m,
                                                (unsigned long long)elements, (unsigned long long)ctx->n);
+        if(!fits(elements, 4)) return ctx->fail(CARE_EINVAL, "%s: values of map %d larger than the file", path, m);
         std::vector<uint32_t> vals(elements);
         if(!get(vals.data(), elements * 4)) return ctx->fail(CARE_EINVAL, "%s: truncated values of map %d", path, m);
         for(uint32_t v : vals) if(v >= ctx->n) return ctx->fail(CARE_EINVAL, "%s: read id %u out of range", path, v);
@@ -4947,7 +5086,7 @@ int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
         uint64_t numKeys = 0, maxProbes = 0, size = 0, capacity = 0, slotCount = 0;
         if(!get(&load, 4) || !get(&numKeys, 8) || !get(&maxProbes, 8) || !get(&size, 8) || !get(&capacity, 8) ||
            !get(&slotCount, 8)) return ctx->fail(CARE_EINVAL, "%s: truncated table of map %d", path, m);
-        if(slotCount > (uint64_t(1) << 34)) return ctx->fail(CARE_EINVAL, "%s: implausible table size", path);
+        if(!fits(slotCount, sizeof(FileSlot))) return ctx->fail(CARE_EINVAL, "%s: table of map %d larger than the file", path, m);
         std::vector<FileSlot> table(slotCount);
         if(!get(table.data(), slotCount * sizeof(FileSlot))) return ctx->fail(CARE_EINVAL, "%s: truncated slots of map %d", path, m);
         std::vector<Slot> entries;
@@ -4985,7 +5124,7 @@ int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
     return CARE_OK;
 }
 
-int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_records, size_t* num_bytes){
+static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_records, size_t* num_bytes){
     if(!ctx) return CARE_EINVAL;
     if(!ctx->hasTables) return ctx->fail(CARE_ESTATE, "build_tables first");
     if(first > ctx->n || n > ctx->n - first) return ctx->fail(CARE_EINVAL, "anchor range out of bounds");
@@ -5059,14 +5198,14 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
     return CARE_OK;
 }
 
-int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap){
+static int care_hip_copy_records_body(care_ctx* ctx, uint8_t* dst, size_t cap){
     if(!ctx || (!dst && !ctx->records.empty())) return CARE_EINVAL;
     if(cap < ctx->records.size()) return ctx->fail(CARE_EINVAL, "buffer too small");
     if(!ctx->records.empty()) std::memcpy(dst, ctx->records.data(), ctx->records.size());
     return CARE_OK;
 }
 
-int care_hip_set_classifier(care_ctx* ctx, const float* params, size_t count){
+static int care_hip_set_classifier_body(care_ctx* ctx, const float* params, size_t count){
     if(!ctx) return CARE_EINVAL;
     if(!params){ ctx->hasSeam = false; return CARE_OK; }
     if(count != size_t(kMlpParams)) return ctx->fail(CARE_EINVAL, "classifier needs %d parameters, got %zu", kMlpParams, count);
@@ -5106,7 +5245,7 @@ int care_hip_set_classifier(care_ctx* ctx, const float* params, size_t count){
     return CARE_OK;
 }
 
-int care_hip_extract_features(care_ctx* ctx, uint64_t first, uint64_t n, float* features, uint32_t* meta,
+static int care_hip_extract_features_body(care_ctx* ctx, uint64_t first, uint64_t n, float* features, uint32_t* meta,
                               size_t cap, size_t* total){
     if(!ctx || !total || (cap && (!features || !meta))) return CARE_EINVAL;
     HIPCHECK(hipSetDevice(ctx->device));
@@ -5139,7 +5278,7 @@ int care_hip_extract_features(care_ctx* ctx, uint64_t first, uint64_t n, float* 
     return CARE_OK;
 }
 
-int care_hip_get_timings(const care_ctx* ctx, care_timings* out){
+static int care_hip_get_timings_body(const care_ctx* ctx, care_timings* out){
     if(!ctx || !out) return CARE_EINVAL;
     *out = ctx->t;
     out->fastq_device_ms = ctx->fastqMs;
@@ -5147,7 +5286,7 @@ int care_hip_get_timings(const care_ctx* ctx, care_timings* out){
     return CARE_OK;
 }
 
-int care_hip_inspect_signatures(care_ctx* ctx, uint64_t first, uint64_t n, uint64_t* out){
+static int care_hip_inspect_signatures_body(care_ctx* ctx, uint64_t first, uint64_t n, uint64_t* out){
     if(!ctx || !out) return CARE_EINVAL;
     if(!ctx->hasTables) return ctx->fail(CARE_ESTATE, "build_tables first");
     if(first > ctx->n || n > ctx->n - first) return ctx->fail(CARE_EINVAL, "range out of bounds");
@@ -5160,7 +5299,7 @@ int care_hip_inspect_signatures(care_ctx* ctx, uint64_t first, uint64_t n, uint6
     return CARE_OK;
 }
 
-int care_hip_inspect_batch(care_ctx* ctx, uint64_t* batch_first, uint64_t* batch_n, uint64_t* total,
+static int care_hip_inspect_batch_body(care_ctx* ctx, uint64_t* batch_first, uint64_t* batch_n, uint64_t* total,
                            uint32_t* candidate_offsets, uint32_t* candidate_ids, int32_t* alignments,
                            uint8_t* paired, int32_t* anchor_info){
     if(!ctx) return CARE_EINVAL;
@@ -5182,7 +5321,7 @@ int care_hip_inspect_batch(care_ctx* ctx, uint64_t* batch_first, uint64_t* batch
     return CARE_OK;
 }
 
-int care_hip_inspect_msa(care_ctx* ctx, uint64_t index, int cap_columns, int* num_columns, int32_t* counts,
+static int care_hip_inspect_msa_body(care_ctx* ctx, uint64_t index, int cap_columns, int* num_columns, int32_t* counts,
                          float* weights, int32_t* coverage, uint8_t* consensus, float* support){
     if(!ctx) return CARE_EINVAL;
     if(!ctx->o.keep_intermediates) return ctx->fail(CARE_ESTATE, "keep_intermediates is off");
@@ -5213,6 +5352,102 @@ int care_hip_inspect_msa(care_ctx* ctx, uint64_t index, int cap_columns, int* nu
         if(consensus) consensus[c] = dk[c];
     }
     return CARE_OK;
+}
+
+
+// Exported entry points: each runs its body under guarded().
+int care_hip_create(const care_opts* opts, int device, care_ctx** out){
+    return guarded(nullptr, [&]() { return care_hip_create_body(opts, device, out); });
+}
+
+int care_hip_device_info(int device, char* name, size_t cap, int* compute_units){
+    return guarded(nullptr, [&]() { return care_hip_device_info_body(device, name, cap, compute_units); });
+}
+
+int care_hip_encode_reads(const uint8_t* seqs, size_t seq_pitch, const int32_t* lengths, uint64_t num_reads,
+                          uint32_t* enc2bit, size_t pitch_ints, uint8_t* ambiguous,
+                          const uint8_t* quals_in, size_t quals_in_pitch, uint8_t* quals_out,
+                          size_t quals_out_pitch, int threads){
+    return guarded(nullptr, [&]() { return care_hip_encode_reads_body(seqs, seq_pitch, lengths, num_reads, enc2bit, pitch_ints, ambiguous, quals_in, quals_in_pitch, quals_out, quals_out_pitch, threads); });
+}
+
+int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints, const int32_t* lengths,
+                       const uint8_t* quals, size_t qual_pitch, uint64_t num_reads, const uint8_t* ambiguous){
+    return guarded(ctx, [&]() { return care_hip_set_reads_body(ctx, enc2bit, pitch_ints, lengths, quals, qual_pitch, num_reads, ambiguous); });
+}
+
+int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1, const char* fastq2, size_t bytes2){
+    return guarded(ctx, [&]() { return care_hip_set_reads_fastq_body(ctx, fastq1, bytes1, fastq2, bytes2); });
+}
+
+int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file, int quality_labels,
+                         char* dst, size_t cap, size_t* written){
+    return guarded(ctx, [&]() { return care_hip_write_fastq_body(ctx, records, record_bytes, file, quality_labels, dst, cap, written); });
+}
+
+int care_hip_save_reads(care_ctx* ctx, const char* path){
+    return guarded(ctx, [&]() { return care_hip_save_reads_body(ctx, path); });
+}
+
+int care_hip_load_reads(care_ctx* ctx, const char* path){
+    return guarded(ctx, [&]() { return care_hip_load_reads_body(ctx, path); });
+}
+
+int care_hip_read_layout(const care_ctx* ctx, uint64_t* num_reads, int* max_length, size_t* pitch_ints){
+    return guarded(nullptr, [&]() { return care_hip_read_layout_body(ctx, num_reads, max_length, pitch_ints); });
+}
+
+int care_hip_inspect_reads(care_ctx* ctx, uint64_t first, uint64_t n, uint32_t* enc2bit, int32_t* lengths,
+                           uint8_t* quals, size_t qual_pitch, uint8_t* ambiguous){
+    return guarded(ctx, [&]() { return care_hip_inspect_reads_body(ctx, first, n, enc2bit, lengths, quals, qual_pitch, ambiguous); });
+}
+
+int care_hip_build_tables(care_ctx* ctx, int* maps_built){
+    return guarded(ctx, [&]() { return care_hip_build_tables_body(ctx, maps_built); });
+}
+
+int care_hip_save_tables(care_ctx* ctx, const char* path){
+    return guarded(ctx, [&]() { return care_hip_save_tables_body(ctx, path); });
+}
+
+int care_hip_load_tables(care_ctx* ctx, const char* path, int* maps_loaded){
+    return guarded(ctx, [&]() { return care_hip_load_tables_body(ctx, path, maps_loaded); });
+}
+
+int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_records, size_t* num_bytes){
+    return guarded(ctx, [&]() { return care_hip_correct_body(ctx, first, n, num_records, num_bytes); });
+}
+
+int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap){
+    return guarded(ctx, [&]() { return care_hip_copy_records_body(ctx, dst, cap); });
+}
+
+int care_hip_set_classifier(care_ctx* ctx, const float* params, size_t count){
+    return guarded(ctx, [&]() { return care_hip_set_classifier_body(ctx, params, count); });
+}
+
+int care_hip_extract_features(care_ctx* ctx, uint64_t first, uint64_t n, float* features, uint32_t* meta,
+                              size_t cap, size_t* total){
+    return guarded(ctx, [&]() { return care_hip_extract_features_body(ctx, first, n, features, meta, cap, total); });
+}
+
+int care_hip_get_timings(const care_ctx* ctx, care_timings* out){
+    return guarded(nullptr, [&]() { return care_hip_get_timings_body(ctx, out); });
+}
+
+int care_hip_inspect_signatures(care_ctx* ctx, uint64_t first, uint64_t n, uint64_t* out){
+    return guarded(ctx, [&]() { return care_hip_inspect_signatures_body(ctx, first, n, out); });
+}
+
+int care_hip_inspect_batch(care_ctx* ctx, uint64_t* batch_first, uint64_t* batch_n, uint64_t* total,
+                           uint32_t* candidate_offsets, uint32_t* candidate_ids, int32_t* alignments,
+                           uint8_t* paired, int32_t* anchor_info){
+    return guarded(ctx, [&]() { return care_hip_inspect_batch_body(ctx, batch_first, batch_n, total, candidate_offsets, candidate_ids, alignments, paired, anchor_info); });
+}
+
+int care_hip_inspect_msa(care_ctx* ctx, uint64_t index, int cap_columns, int* num_columns, int32_t* counts,
+                         float* weights, int32_t* coverage, uint8_t* consensus, float* support){
+    return guarded(ctx, [&]() { return care_hip_inspect_msa_body(ctx, index, cap_columns, num_columns, counts, weights, coverage, consensus, support); });
 }
 
 }  // extern "C"

@@ -372,9 +372,30 @@ int lossGradDevice(care_mlp* h, const float* X, const float* y, int n){
 
 }  // namespace
 
+// Entry points run their bodies through guarded(): no C++ exception crosses the
+// C-ABI (a failed host allocation is CARE_ENOMEM, any other exception CARE_EINVAL).
+template<class F>
+int guarded(care_mlp* h, F&& f){
+    const char* what = "unexpected C++ exception";
+    int code = CARE_EINVAL;
+    try{
+        return f();
+    }catch(const std::bad_alloc&){
+        what = "host allocation failed";
+        code = CARE_ENOMEM;
+    }catch(const std::exception& e){
+        what = e.what();
+    }catch(...){
+    }
+    if(h){
+        try{ h->err = what; }catch(...){}
+    }
+    return code;
+}
+
 extern "C" {
 
-int care_mlp_create(int device, care_mlp** out){
+static int care_mlp_create_body(int device, care_mlp** out){
     if(!out) return CARE_EINVAL;
     *out = nullptr;
     int count = 0;
@@ -408,7 +429,7 @@ void care_mlp_destroy(care_mlp* h){
 
 const char* care_mlp_last_error(const care_mlp* h){ return h ? h->err.c_str() : "null handle"; }
 
-int care_mlp_set_params(care_mlp* h, const float* params, size_t count){
+static int care_mlp_set_params_body(care_mlp* h, const float* params, size_t count){
     if(!h || !params) return CARE_EINVAL;
     if(count != size_t(kP)) return h->fail(CARE_EINVAL, "expected %d parameters, got %zu", kP, count);
     MCHECK(hipSetDevice(h->device));
@@ -416,7 +437,7 @@ int care_mlp_set_params(care_mlp* h, const float* params, size_t count){
     return CARE_OK;
 }
 
-int care_mlp_get_params(care_mlp* h, float* params, size_t count){
+static int care_mlp_get_params_body(care_mlp* h, float* params, size_t count){
     if(!h || !params) return CARE_EINVAL;
     if(count != size_t(kP)) return h->fail(CARE_EINVAL, "expected %d parameters, got %zu", kP, count);
     MCHECK(hipSetDevice(h->device));
@@ -425,7 +446,7 @@ int care_mlp_get_params(care_mlp* h, float* params, size_t count){
     return CARE_OK;
 }
 
-int care_mlp_reset_optimizer(care_mlp* h){
+static int care_mlp_reset_optimizer_body(care_mlp* h){
     if(!h) return CARE_EINVAL;
     MCHECK(hipSetDevice(h->device));
     k_fill<<<(kP + 255) / 256, 256, 0, h->stream>>>(h->m.as<float>(), kP, 0.0f);
@@ -435,7 +456,7 @@ int care_mlp_reset_optimizer(care_mlp* h){
     return CARE_OK;
 }
 
-int care_mlp_forward(care_mlp* h, const float* X, size_t n, float* z){
+static int care_mlp_forward_body(care_mlp* h, const float* X, size_t n, float* z){
     if(!h || (n && (!X || !z))) return CARE_EINVAL;
     if(n > size_t(INT32_MAX)) return h->fail(CARE_EINVAL, "too many rows");
     MCHECK(hipSetDevice(h->device));
@@ -449,7 +470,7 @@ int care_mlp_forward(care_mlp* h, const float* X, size_t n, float* z){
     return CARE_OK;
 }
 
-int care_mlp_loss_grad(care_mlp* h, const float* X, const float* y, size_t n, float* loss, float* grad){
+static int care_mlp_loss_grad_body(care_mlp* h, const float* X, const float* y, size_t n, float* loss, float* grad){
     if(!h || (n && (!X || !y))) return CARE_EINVAL;
     if(n > size_t(INT32_MAX)) return h->fail(CARE_EINVAL, "too many rows");
     MCHECK(hipSetDevice(h->device));
@@ -471,7 +492,7 @@ int care_mlp_loss_grad(care_mlp* h, const float* X, const float* y, size_t n, fl
     return CARE_OK;
 }
 
-int care_mlp_set_data(care_mlp* h, const float* X, const float* y, size_t n){
+static int care_mlp_set_data_body(care_mlp* h, const float* X, const float* y, size_t n){
     if(!h || (n && (!X || !y))) return CARE_EINVAL;
     if(n > size_t(INT32_MAX)) return h->fail(CARE_EINVAL, "too many rows");
     MCHECK(hipSetDevice(h->device));
@@ -485,7 +506,7 @@ int care_mlp_set_data(care_mlp* h, const float* X, const float* y, size_t n){
     return CARE_OK;
 }
 
-int care_mlp_train(care_mlp* h, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss){
+static int care_mlp_train_body(care_mlp* h, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss){
     if(!h || epochs < 0 || batch <= 0) return CARE_EINVAL;
     if(h->nData == 0) return h->fail(CARE_ESTATE, "no training data (care_mlp_set_data)");
     MCHECK(hipSetDevice(h->device));
@@ -530,6 +551,40 @@ int care_mlp_train(care_mlp* h, int epochs, int batch, float lr, uint64_t seed, 
     if(epoch_loss && epochs > 0) MCHECK(hipMemcpy(epoch_loss, h->epochLoss.p, size_t(epochs) * 4, hipMemcpyDeviceToHost));
     MCHECK(hipStreamSynchronize(st));
     return CARE_OK;
+}
+
+
+// Exported entry points: each runs its body under guarded().
+int care_mlp_create(int device, care_mlp** out){
+    return guarded(nullptr, [&]() { return care_mlp_create_body(device, out); });
+}
+
+int care_mlp_set_params(care_mlp* h, const float* params, size_t count){
+    return guarded(h, [&]() { return care_mlp_set_params_body(h, params, count); });
+}
+
+int care_mlp_get_params(care_mlp* h, float* params, size_t count){
+    return guarded(h, [&]() { return care_mlp_get_params_body(h, params, count); });
+}
+
+int care_mlp_reset_optimizer(care_mlp* h){
+    return guarded(h, [&]() { return care_mlp_reset_optimizer_body(h); });
+}
+
+int care_mlp_forward(care_mlp* h, const float* X, size_t n, float* z){
+    return guarded(h, [&]() { return care_mlp_forward_body(h, X, n, z); });
+}
+
+int care_mlp_loss_grad(care_mlp* h, const float* X, const float* y, size_t n, float* loss, float* grad){
+    return guarded(h, [&]() { return care_mlp_loss_grad_body(h, X, y, n, loss, grad); });
+}
+
+int care_mlp_set_data(care_mlp* h, const float* X, const float* y, size_t n){
+    return guarded(h, [&]() { return care_mlp_set_data_body(h, X, y, n); });
+}
+
+int care_mlp_train(care_mlp* h, int epochs, int batch, float lr, uint64_t seed, float* epoch_loss){
+    return guarded(h, [&]() { return care_mlp_train_body(h, epochs, batch, lr, seed, epoch_loss); });
 }
 
 }  // extern "C"

@@ -16,6 +16,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <new>
 #include <string>
 #include <vector>
@@ -33,6 +34,12 @@ struct Rng {
     }
     explicit Rng(std::uint64_t seed){
         for(auto& w : s) w = splitmix(seed);
+    }
+    // generator of stream i under a base seed (streams are seeded through splitmix64
+    // from distinct 64-bit values)
+    static Rng stream(std::uint64_t base, std::uint64_t i){
+        std::uint64_t x = base ^ (0xd1b54a32d192ed03ull * (i + 1));
+        return Rng(splitmix(x));
     }
     static std::uint64_t rotl(std::uint64_t x, int k){ return (x << k) | (x >> (64 - k)); }
     std::uint64_t next(){
@@ -167,23 +174,29 @@ void revcompInplace(std::string& s){
     for(auto& c : s) c = complement(c);
 }
 
-struct ReadOut {
-    std::string seq, qual, truth;
-    std::uint64_t pos;
-    int strand;
+void revcompInplace(char* s, int L){
+    std::reverse(s, s + L);
+    for(int i = 0; i < L; i++) s[i] = complement(s[i]);
+}
+
+// One read's rows inside the result's flat arrays.
+struct ReadView {
+    char* seq;
+    char* qual;
+    char* truth;
+    int L;
 };
 
-void finishRead(const care_sim_config& cfg, Rng& rng, ReadOut& r){
-    const int L = int(r.truth.size());
-    r.seq = r.truth;
-    r.qual.resize(L);
-    for(int j = 0; j < L; j++){
-        r.qual[j] = drawQuality(cfg, rng, j, L);
-        static const std::vector<double> kErr = [](){
-            std::vector<double> t(128);
-            for(int q = 0; q < 128; q++) t[q] = std::pow(10.0, -double(q) / 10.0);
-            return t;
-        }();
+// Qualities and errors of a read whose truth row is filled.
+void finishRead(const care_sim_config& cfg, Rng& rng, ReadView r){
+    static const std::vector<double> kErr = [](){
+        std::vector<double> t(128);
+        for(int q = 0; q < 128; q++) t[q] = std::pow(10.0, -double(q) / 10.0);
+        return t;
+    }();
+    std::memcpy(r.seq, r.truth, std::size_t(r.L));
+    for(int j = 0; j < r.L; j++){
+        r.qual[j] = drawQuality(cfg, rng, j, r.L);
         const double perr = kErr[(r.qual[j] - 33) & 127];
         if(rng.uniform() < perr) r.seq[j] = otherBase(rng, r.seq[j]);
         if(cfg.n_rate > 0 && rng.uniform() < cfg.n_rate) r.seq[j] = 'N';
@@ -192,10 +205,34 @@ void finishRead(const care_sim_config& cfg, Rng& rng, ReadOut& r){
 
 }  // namespace
 
+// Reads as row-major arrays of `pitch` (= longest possible read) bytes, zero padded.
+// The arrays are allocated uninitialised and every row is written whole by the thread
+// that simulates its read, so first-touch page faults are spread over the threads.
 struct care_sim_result {
-    std::vector<ReadOut> reads;
+    std::size_t n = 0;
+    std::size_t pitch = 0;
+    std::unique_ptr<char[]> seq, qual, truth;
+    std::vector<int> len;
+    std::vector<std::uint64_t> pos;
+    std::vector<int> strand;
     int maxLen = 0;
     std::string genome;
+
+    ReadView view(std::size_t i, int L){
+        return ReadView{seq.get() + i * pitch, qual.get() + i * pitch, truth.get() + i * pitch, L};
+    }
+    void resize(std::size_t count, std::size_t p){
+        n = count; pitch = p;
+        seq.reset(new char[n * pitch]); qual.reset(new char[n * pitch]); truth.reset(new char[n * pitch]);
+        len.resize(n); pos.resize(n); strand.resize(n);
+    }
+    // zero padding of row i past its length
+    void pad(std::size_t i){
+        const std::size_t L = std::size_t(len[i]);
+        std::memset(seq.get() + i * pitch + L, 0, pitch - L);
+        std::memset(qual.get() + i * pitch + L, 0, pitch - L);
+        std::memset(truth.get() + i * pitch + L, 0, pitch - L);
+    }
 };
 
 extern "C" {
@@ -230,47 +267,74 @@ int care_sim_run(const care_sim_config* cfg, care_sim_result** out){
     const double meanLen = 0.5 * double(cfg->read_length_min + cfg->read_length_max);
     std::size_t numReads = std::size_t(double(G) * cfg->coverage / meanLen);
     if(cfg->num_reads_override > 0) numReads = cfg->num_reads_override;
+    // Read (or pair) i draws from `rng` in read order, or from its own stream.
+    const bool own = cfg->per_read_streams != 0;
+    const std::uint64_t streamBase = own ? rng.next() : 0;
+    const std::size_t P = std::size_t(cfg->read_length_max);
     if(cfg->paired){
         const std::size_t pairs = numReads / 2;
-        res->reads.resize(2 * pairs);
-        for(std::size_t p = 0; p < pairs; p++){
-            const int L1 = drawLength(*cfg, rng);
-            const int L2 = drawLength(*cfg, rng);
-            long long ins = std::llround(cfg->insert_mean + cfg->insert_sd * rng.normal());
+        res->resize(2 * pairs, P);
+        auto onePair = [&](Rng& r, std::size_t p){
+            const int L1 = drawLength(*cfg, r);
+            const int L2 = drawLength(*cfg, r);
+            long long ins = std::llround(cfg->insert_mean + cfg->insert_sd * r.normal());
             ins = std::max<long long>(ins, std::max(L1, L2));
             ins = std::min<long long>(ins, (long long)G - 1);
-            const std::uint64_t start = rng.below(G - std::uint64_t(ins));
+            const std::uint64_t start = r.below(G - std::uint64_t(ins));
             std::string frag = res->genome.substr(start, std::size_t(ins));
-            const int strand = int(rng.next() & 1);
+            const int strand = int(r.next() & 1);
             if(strand) revcompInplace(frag);
-            ReadOut& m1 = res->reads[2 * p];
-            ReadOut& m2 = res->reads[2 * p + 1];
-            m1.truth = frag.substr(0, L1);
-            m2.truth = frag.substr(frag.size() - L2);
-            revcompInplace(m2.truth);
-            m1.pos = start; m1.strand = strand;
-            m2.pos = start; m2.strand = !strand;
-            finishRead(*cfg, rng, m1);
-            finishRead(*cfg, rng, m2);
+            const ReadView m1 = res->view(2 * p, L1);
+            const ReadView m2 = res->view(2 * p + 1, L2);
+            std::memcpy(m1.truth, frag.data(), std::size_t(L1));
+            std::memcpy(m2.truth, frag.data() + frag.size() - std::size_t(L2), std::size_t(L2));
+            revcompInplace(m2.truth, L2);
+            res->len[2 * p] = L1; res->len[2 * p + 1] = L2;
+            res->pos[2 * p] = start; res->strand[2 * p] = strand;
+            res->pos[2 * p + 1] = start; res->strand[2 * p + 1] = !strand;
+            finishRead(*cfg, r, m1);
+            finishRead(*cfg, r, m2);
+            res->pad(2 * p);
+            res->pad(2 * p + 1);
+        };
+        if(own){
+            #pragma omp parallel for schedule(dynamic, 4096)
+            for(std::size_t p = 0; p < pairs; p++){
+                Rng r = Rng::stream(streamBase, p);
+                onePair(r, p);
+            }
+        }else{
+            for(std::size_t p = 0; p < pairs; p++) onePair(rng, p);
         }
     }else{
-        res->reads.resize(numReads);
-        for(std::size_t i = 0; i < numReads; i++){
-            ReadOut& r = res->reads[i];
-            const int L = drawLength(*cfg, rng);
-            r.pos = rng.below(G - std::uint64_t(L) + 1);
-            r.strand = int(rng.next() & 1);
-            r.truth = res->genome.substr(r.pos, L);
-            if(r.strand) revcompInplace(r.truth);
-            finishRead(*cfg, rng, r);
+        res->resize(numReads, P);
+        auto oneRead = [&](Rng& r, std::size_t i){
+            const int L = drawLength(*cfg, r);
+            const std::uint64_t at = r.below(G - std::uint64_t(L) + 1);
+            const int strand = int(r.next() & 1);
+            const ReadView rd = res->view(i, L);
+            std::memcpy(rd.truth, res->genome.data() + at, std::size_t(L));
+            if(strand) revcompInplace(rd.truth, L);
+            res->len[i] = L; res->pos[i] = at; res->strand[i] = strand;
+            finishRead(*cfg, r, rd);
+            res->pad(i);
+        };
+        if(own){
+            #pragma omp parallel for schedule(dynamic, 4096)
+            for(std::size_t i = 0; i < numReads; i++){
+                Rng r = Rng::stream(streamBase, i);
+                oneRead(r, i);
+            }
+        }else{
+            for(std::size_t i = 0; i < numReads; i++) oneRead(rng, i);
         }
     }
-    for(const auto& r : res->reads) res->maxLen = std::max(res->maxLen, int(r.seq.size()));
+    for(const int L : res->len) res->maxLen = std::max(res->maxLen, L);
     *out = res;
     return 0;
 }
 
-std::uint64_t care_sim_num_reads(const care_sim_result* r){ return r->reads.size(); }
+std::uint64_t care_sim_num_reads(const care_sim_result* r){ return r->n; }
 int care_sim_max_length(const care_sim_result* r){ return r->maxLen; }
 std::uint64_t care_sim_genome_length(const care_sim_result* r){ return r->genome.size(); }
 
@@ -278,21 +342,50 @@ std::uint64_t care_sim_genome_length(const care_sim_result* r){ return r->genome
 // unused bytes are zero.  Any output pointer may be NULL.
 int care_sim_copy(const care_sim_result* r, std::uint64_t first, std::uint64_t n, std::size_t pitch,
                   char* seqs, char* quals, char* truth, int* lengths, std::uint64_t* positions, int* strands){
-    if(first + n > r->reads.size()) return 1;
+    if(first + n > r->n) return 1;
     if(pitch < std::size_t(r->maxLen)) return 2;
+    const std::size_t w = std::min(pitch, r->pitch);
+    #pragma omp parallel for schedule(static, 16384)
     for(std::uint64_t i = 0; i < n; i++){
-        const ReadOut& rd = r->reads[first + i];
-        const std::size_t L = rd.seq.size();
-        if(seqs){ std::memset(seqs + i * pitch, 0, pitch); std::memcpy(seqs + i * pitch, rd.seq.data(), L); }
-        if(quals){ std::memset(quals + i * pitch, 0, pitch); std::memcpy(quals + i * pitch, rd.qual.data(), L); }
-        if(truth){ std::memset(truth + i * pitch, 0, pitch); std::memcpy(truth + i * pitch, rd.truth.data(), L); }
-        if(lengths) lengths[i] = int(L);
-        if(positions) positions[i] = rd.pos;
-        if(strands) strands[i] = rd.strand;
+        const std::size_t src = (first + i) * r->pitch;
+        if(seqs){ std::memcpy(seqs + i * pitch, r->seq.get() + src, w); std::memset(seqs + i * pitch + w, 0, pitch - w); }
+        if(quals){ std::memcpy(quals + i * pitch, r->qual.get() + src, w); std::memset(quals + i * pitch + w, 0, pitch - w); }
+        if(truth){ std::memcpy(truth + i * pitch, r->truth.get() + src, w); std::memset(truth + i * pitch + w, 0, pitch - w); }
+        if(lengths) lengths[i] = r->len[first + i];
+        if(positions) positions[i] = r->pos[first + i];
+        if(strands) strands[i] = r->strand[first + i];
     }
     return 0;
 }
 
 void care_sim_free(care_sim_result* r){ delete r; }
+
+std::size_t care_sim_fastq(const char* seqs, const char* quals, const int* lengths, std::size_t pitch,
+                           std::uint64_t first, std::uint64_t step, std::uint64_t count, char* out, std::size_t cap){
+    // record r: "@r" id "\n" seq "\n+\n" qual "\n"
+    std::vector<std::size_t> off(count + 1, 0);
+    for(std::uint64_t r = 0; r < count; r++){
+        const std::uint64_t id = first + r * step;
+        const std::string name = std::to_string(id);
+        off[r + 1] = off[r] + 2 + name.size() + 1 + 2 * std::size_t(lengths[id]) + 4;
+    }
+    if(!out) return off[count];
+    if(cap < off[count]) return 0;
+    #pragma omp parallel for schedule(static, 16384)
+    for(std::uint64_t r = 0; r < count; r++){
+        const std::uint64_t id = first + r * step;
+        const std::string name = std::to_string(id);
+        const std::size_t L = std::size_t(lengths[id]);
+        char* p = out + off[r];
+        *p++ = '@'; *p++ = 'r';
+        std::memcpy(p, name.data(), name.size()); p += name.size();
+        *p++ = '\n';
+        std::memcpy(p, seqs + id * pitch, L); p += L;
+        *p++ = '\n'; *p++ = '+'; *p++ = '\n';
+        std::memcpy(p, quals + id * pitch, L); p += L;
+        *p++ = '\n';
+    }
+    return off[count];
+}
 
 }  // extern "C"

@@ -57,7 +57,7 @@ class CareTimings(ctypes.Structure):
 
 EXPORTS = [
     "care_hip_default_opts", "care_hip_create", "care_hip_destroy", "care_hip_last_error",
-    "care_hip_set_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
+    "care_hip_set_reads", "care_hip_encode_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
     "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
     "care_hip_save_tables", "care_hip_load_tables", "care_hip_set_reads_fastq", "care_hip_read_layout",
@@ -87,6 +87,7 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_last_error.restype = ctypes.c_char_p
     lib.care_hip_set_reads.argtypes = [vp, vp, sz, vp, vp, sz, u64, vp]
     lib.care_hip_set_reads_fastq.argtypes = [vp, vp, sz, vp, sz]
+    lib.care_hip_encode_reads.argtypes = [vp, sz, vp, u64, vp, sz, vp, vp, sz, vp, sz, i32]
     lib.care_hip_read_layout.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(i32), ctypes.POINTER(sz)]
     lib.care_hip_inspect_reads.argtypes = [vp, u64, u64, vp, vp, vp, sz, vp]
     lib.care_hip_write_fastq.argtypes = [vp, vp, sz, i32, i32, vp, sz, ctypes.POINTER(sz)]
@@ -199,6 +200,8 @@ class CareEngine:
         """Builds the read storage on the device from FASTQ files (paths or bytes);
         two files for paired input. Returns the number of reads."""
         def text(f):
+            if isinstance(f, np.ndarray):
+                return np.ascontiguousarray(f.reshape(-1).view(np.uint8))
             if isinstance(f, (bytes, bytearray, memoryview)):
                 return np.frombuffer(f, np.uint8)
             return np.fromfile(f, np.uint8)

@@ -69,31 +69,47 @@ def replace_ambiguous(seqs: np.ndarray, lengths: np.ndarray) -> tuple[np.ndarray
     return codes, ambiguous
 
 
-def encode_codes(codes: np.ndarray, lengths: np.ndarray) -> np.ndarray:
+def encode_codes(codes: np.ndarray, lengths: np.ndarray, words: int | None = None) -> np.ndarray:
+    """2-bit rows of base codes (0..3, zero past each read): four codes per byte, first
+    code in the top bits, four bytes per u32 word read most significant byte first."""
     n, pitch = codes.shape
-    words = ints_per_row(int(lengths.max()) if n else 0)
-    padded = np.zeros((n, words * 16), np.uint32)
+    if words is None:
+        words = ints_per_row(int(lengths.max()) if n else 0)
+    padded = np.zeros((n, words * 16), np.uint8)
     w = min(pitch, words * 16)
     padded[:, :w] = codes[:, :w]
-    padded = padded.reshape(n, words, 16)
-    shifts = (30 - 2 * np.arange(16, dtype=np.uint32)).astype(np.uint32)
-    return np.bitwise_or.reduce(padded << shifts[None, None, :], axis=2).astype(np.uint32)
+    q = padded.reshape(n, words * 4, 4)
+    packed = (q[:, :, 0] << 6) | (q[:, :, 1] << 4) | (q[:, :, 2] << 2) | q[:, :, 3]
+    return np.ascontiguousarray(packed).view(">u4").astype(np.uint32).reshape(n, words)
 
 
 def make_storage(seqs: np.ndarray, lengths: np.ndarray, quals: np.ndarray | None = None,
-                 paired: bool = False) -> ReadStorage:
+                 paired: bool = False, threads: int = 0) -> ReadStorage:
+    """Read storage from ASCII rows, encoded on all host threads by the library's
+    ``care_hip_encode_reads`` (the rule of ``replace_ambiguous`` + ``encode_codes``)."""
+    import ctypes
+    from . import engine
+    lib = engine.load_library()
     lengths = np.ascontiguousarray(lengths, np.int32)
-    codes, ambiguous = replace_ambiguous(np.ascontiguousarray(seqs, np.uint8), lengths)
-    enc = np.ascontiguousarray(encode_codes(codes, lengths))
+    seqs = np.ascontiguousarray(seqs, np.uint8)
+    n = lengths.shape[0]
+    maxlen = int(lengths.max()) if n else 0
+    words = ints_per_row(maxlen)
+    enc = np.empty((n, max(words, 1)), np.uint32)
+    ambiguous = np.empty(n, np.uint8)
     q = None
+    qin = None
     if quals is not None:
-        maxlen = int(lengths.max()) if lengths.size else 0
-        q = np.zeros((lengths.shape[0], maxlen), np.uint8)
-        q[:, :min(maxlen, quals.shape[1])] = quals[:, :maxlen]
-        mask = np.arange(maxlen)[None, :] >= lengths[:, None]
-        q[mask] = 0
-        q = np.ascontiguousarray(q)
-    return ReadStorage(enc, lengths, q, ambiguous, paired)
+        qin = np.ascontiguousarray(quals, np.uint8)
+        q = np.empty((n, maxlen), np.uint8)
+    rc = lib.care_hip_encode_reads(
+        seqs.ctypes.data, seqs.shape[1] if seqs.ndim == 2 else 0, lengths.ctypes.data, n,
+        enc.ctypes.data, enc.shape[1], ambiguous.ctypes.data,
+        None if qin is None else qin.ctypes.data, 0 if qin is None else qin.shape[1],
+        None if q is None else q.ctypes.data, 0 if q is None else maxlen, ctypes.c_int(threads))
+    if rc != 0:
+        raise ValueError(f"care_hip_encode_reads failed ({rc}): lengths exceed the given rows")
+    return ReadStorage(enc[:, :words] if words else enc[:, :0], lengths, q, ambiguous, paired)
 
 
 def decode_row(row: np.ndarray, length: int) -> bytes:

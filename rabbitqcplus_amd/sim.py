@@ -45,6 +45,7 @@ class SimConfig(ctypes.Structure):
         ("n_rate", ctypes.c_double),
         ("tandem_fraction", ctypes.c_double),
         ("tandem_period_max", ctypes.c_int),
+        ("per_read_streams", ctypes.c_int),
     ]
 
 
@@ -67,6 +68,10 @@ def _lib():
                                       ctypes.c_void_p, ctypes.c_void_p]
         lib.care_sim_copy.restype = ctypes.c_int
         lib.care_sim_free.argtypes = [ctypes.c_void_p]
+        lib.care_sim_fastq.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_void_p,
+                                       ctypes.c_size_t]
+        lib.care_sim_fastq.restype = ctypes.c_size_t
         _LIB = lib
     return _LIB
 
@@ -107,9 +112,9 @@ def simulate(**kw) -> SimReads:
     try:
         n = lib.care_sim_num_reads(handle)
         maxlen = lib.care_sim_max_length(handle)
-        seqs = np.zeros((n, maxlen), np.uint8)
-        quals = np.zeros((n, maxlen), np.uint8)
-        truth = np.zeros((n, maxlen), np.uint8)
+        seqs = np.empty((n, maxlen), np.uint8)        # every byte is written by care_sim_copy
+        quals = np.empty((n, maxlen), np.uint8)
+        truth = np.empty((n, maxlen), np.uint8)
         lengths = np.zeros(n, np.int32)
         positions = np.zeros(n, np.uint64)
         strands = np.zeros(n, np.int32)
@@ -123,7 +128,8 @@ def simulate(**kw) -> SimReads:
 
 
 # Correction workloads (sizes as named). Qualities fall linearly from the first to
-# the last base; substitutions occur with probability 10^(-Q/10).
+# the last base; substitutions occur with probability 10^(-Q/10). The three large
+# workloads draw each read from its own generator stream (simulated on all host threads).
 WORKLOADS = {
     # plain SE: 200 kbp uniform genome, 150 bp reads at 30x (~40k reads)
     "se_200kbp_150bp_30x": dict(genome_length=200_000, read_length_min=150, read_length_max=150,
@@ -133,18 +139,19 @@ WORKLOADS = {
     "se_5mbp_150bp_30x_dup": dict(genome_length=5_000_000, gc_content=0.5, dup_fraction=0.02,
                                   dup_segment_length=1000, dup_copies=2, dup_divergence=0.005,
                                   read_length_min=150, read_length_max=150, coverage=30.0,
-                                  q_first=36.0, q_last=28.0),
+                                  q_first=36.0, q_last=28.0, per_read_streams=1),
     # batched PE: 20 Mbp, 10% in 20-copy 2 kbp repeat families at 1% divergence;
     # 2x150 bp pairs, insert N(350,50), 30x; steeper 3' quality drop (Q24 at the end)
     "pe_20mbp_2x150bp_30x_rep": dict(genome_length=20_000_000, repeat_fraction=0.10,
                                      repeat_unit_length=2000, repeat_copies=20, repeat_divergence=0.01,
                                      read_length_min=150, read_length_max=150, coverage=30.0, paired=1,
-                                     insert_mean=350.0, insert_sd=50.0, q_first=36.0, q_last=24.0),
+                                     insert_mean=350.0, insert_sd=50.0, q_first=36.0, q_last=24.0,
+                                     per_read_streams=1),
     # learned inference at scale: 50 Mbp, 8% in 10-copy repeats; lengths U[100,250] at 30x
     "se_50mbp_100to250bp_30x_rep": dict(genome_length=50_000_000, repeat_fraction=0.08,
                                         repeat_unit_length=2000, repeat_copies=10, repeat_divergence=0.01,
                                         read_length_min=100, read_length_max=250, coverage=30.0,
-                                        q_first=36.0, q_last=28.0),
+                                        q_first=36.0, q_last=28.0, per_read_streams=1),
 }
 
 
@@ -155,16 +162,23 @@ def workload(name: str, seed: int = 12345, **overrides) -> SimReads:
     return simulate(**kw)
 
 
-def fastq_texts(s: SimReads) -> list[bytes]:
-    """4-line FASTQ text of simulated reads: one text, or R1/R2 texts for pairs
-    (reads 2i and 2i+1 are mates)."""
+def fastq_texts(s: SimReads) -> list[np.ndarray]:
+    """4-line FASTQ text of simulated reads (uint8 arrays): one text, or R1/R2 texts for
+    pairs (reads 2i and 2i+1 are mates). Read i is written as ``@r<i>``."""
+    lib = _lib()
     n = int(s.lengths.shape[0])
-    parts = [range(n)] if not s.paired else [range(0, n, 2), range(1, n, 2)]
+    seqs = np.ascontiguousarray(s.seqs)
+    quals = np.ascontiguousarray(s.quals)
+    lengths = np.ascontiguousarray(s.lengths, np.int32)
+    parts = [(0, 1, n)] if not s.paired else [(0, 2, n // 2), (1, 2, n // 2)]
     out = []
-    for part in parts:
-        chunks = []
-        for i in part:
-            L = int(s.lengths[i])
-            chunks.append(b"@r%d\n%s\n+\n%s\n" % (i, s.seqs[i, :L].tobytes(), s.quals[i, :L].tobytes()))
-        out.append(b"".join(chunks))
+    for first, step, count in parts:
+        size = lib.care_sim_fastq(seqs.ctypes.data, quals.ctypes.data, lengths.ctypes.data, seqs.shape[1],
+                                  first, step, count, None, 0)
+        buf = np.empty(max(1, size), np.uint8)
+        got = lib.care_sim_fastq(seqs.ctypes.data, quals.ctypes.data, lengths.ctypes.data, seqs.shape[1],
+                                 first, step, count, buf.ctypes.data, size)
+        if got != size:
+            raise RuntimeError("care_sim_fastq failed")
+        out.append(buf[:size])
     return out

@@ -60,6 +60,53 @@ CASES = {
 }
 
 
+# Workloads checked at their full size (the benchmark's sizes; inputs from sim.WORKLOADS
+# with the default seed): workload -> engine/oracle options. The records of the compiled
+# reference are pinned per read-id slice in tests/golden/full_<workload>.json.
+FULL_WORKLOADS = {
+    # 1.0M x 150 bp SE, 5 Mbp with duplicated segments
+    "se_5mbp_150bp_30x_dup": dict(use_quality_scores=1),
+    # 2.0M pairs 2x150 bp, 20 Mbp with 20-copy repeat families
+    "pe_20mbp_2x150bp_30x_rep": dict(use_quality_scores=1, paired=1),
+    # 8.57M SE reads of 100..250 bp, 50 Mbp with 10-copy repeat families; k = 20
+    "se_50mbp_100to250bp_30x_rep": dict(use_quality_scores=1, kmerlength=20),
+}
+FULL_SLICES = 16
+
+
+def full_case(workload: str):
+    s = sim.workload(workload)
+    rs = reads.storage_from_sim(s, use_qualities=True)
+    return s, rs, dict(FULL_WORKLOADS[workload])
+
+
+def slice_digests(data: bytes, n: int, slices: int = FULL_SLICES):
+    """Records (in read-id order, at most one per read) cut into `slices` equal read-id
+    ranges of [0, n): per range [first, end, record count, SHA-256 of its record bytes]."""
+    import hashlib
+    bounds = [n * i // slices for i in range(slices + 1)]
+    out = []
+    p = 0
+    last = -1
+    for i in range(slices):
+        lo, hi = bounds[i], bounds[i + 1]
+        h = hashlib.sha256()
+        count = 0
+        start = p
+        while p < len(data):
+            rid = int.from_bytes(data[p:p + 4], "little")
+            if rid >= hi:
+                break
+            assert rid > last, "records must be in ascending read-id order"
+            last = rid
+            p += 8 + (int.from_bytes(data[p + 4:p + 8], "little") & ((1 << 29) - 1))
+            count += 1
+        h.update(data[start:p])
+        out.append([lo, hi, count, h.hexdigest()])
+    assert p == len(data)
+    return out
+
+
 @functools.lru_cache(maxsize=None)
 def case(name):
     simkw, opts = CASES[name]

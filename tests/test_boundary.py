@@ -99,7 +99,8 @@ def test_ambiguous_bases_replaced_cyclically_per_parser_batch(monkeypatch):
     seqs = np.zeros((len(rows), 4), np.uint8)
     for i, r in enumerate(rows):
         seqs[i, :len(r)] = np.frombuffer(r, np.uint8)
-    rs = reads.make_storage(seqs, L)
+    codes, amb = reads.replace_ambiguous(seqs, L)
+    rs = reads.ReadStorage(reads.encode_codes(codes, L), L, None, amb)
     expect = []
     counter = 0
     for i, r in enumerate(rows):
@@ -116,6 +117,36 @@ def test_ambiguous_bases_replaced_cyclically_per_parser_batch(monkeypatch):
         expect.append(bytes(s))
     assert [rs.decoded(i) for i in range(len(rows))] == expect
     assert rs.ambiguous.tolist() == [1, 0, 1, 1, 1, 1]
+
+
+def test_native_encoder_equals_the_numpy_rule_across_parser_batches():
+    """care_hip_encode_reads (the library's threaded encoder behind make_storage) gives
+    the rows, ambiguity flags and quality rows of the numpy restatement of the rule,
+    on 140,000 reads with N, IUPAC codes and lower case across two 65,536-read parser
+    batches, for 1 and 8 threads."""
+    rng = np.random.default_rng(5)
+    n, pitch = 140_000, 40
+    lens = rng.integers(0, pitch + 1, size=n).astype(np.int32)
+    alphabet = np.frombuffer(b"ACGTACGTACGTacgtNRYN", np.uint8)
+    seqs = rng.choice(alphabet, size=(n, pitch)).astype(np.uint8)
+    clean = rng.random(n) < 0.9
+    seqs[clean] = rng.choice(np.frombuffer(b"ACGT", np.uint8), size=(int(clean.sum()), pitch))
+    quals = rng.integers(33, 75, size=(n, pitch)).astype(np.uint8)
+    codes, amb = reads.replace_ambiguous(seqs, lens)
+    want = reads.encode_codes(codes, lens)
+    for threads in (1, 8):
+        rs = reads.make_storage(seqs, lens, quals, threads=threads)
+        assert np.array_equal(rs.enc, want)
+        assert np.array_equal(rs.ambiguous, amb)
+        mask = np.arange(pitch)[None, :] < lens[:, None]
+        assert np.array_equal(rs.quals, np.where(mask, quals, 0))
+    assert amb.sum() > 1000
+
+
+def test_native_encoder_rejects_lengths_beyond_the_rows():
+    seqs = np.zeros((2, 8), np.uint8)
+    with pytest.raises(ValueError):
+        reads.make_storage(seqs, np.array([4, 9], np.int32))
 
 
 def test_storage_rejects_nothing_silently():

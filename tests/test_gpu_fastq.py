@@ -209,7 +209,52 @@ def test_reads_file_errors(tmp_path):
         for cut in (1000, 30000):
             short = str(tmp_path / "short.bin")
             open(short, "wb").write(open(withq, "rb").read()[:cut])
-            with pytest.raises(engine.CareError, match="truncated"):
+            with pytest.raises(engine.CareError, match="truncated|larger than the file"):
                 eng.load_reads(short)
+    finally:
+        eng.close()
+
+
+def _reads_file_header(raw: bytes):
+    """Offsets of the length section of a preprocessed-reads file: bits per length at 57,
+    minimum and maximum length at 61 and 65, word count at 81, the words from 97."""
+    bits, min_len, max_len = (int.from_bytes(raw[o:o + 4], "little", signed=True) for o in (57, 61, 65))
+    lwords = int.from_bytes(raw[81:89], "little")
+    return bits, min_len, max_len, lwords
+
+
+def test_reads_file_with_bad_sizes_is_rejected_before_any_allocation(tmp_path):
+    """A length stream whose values exceed the header's maximum length, and sections whose
+    sizes exceed the file, are CARE_EINVAL with a message (the longest read sizes the rows,
+    the MSA bounds and the record slots; sizes are checked before allocating)."""
+    data = fastq_data.make(seed=31, n_reads=800, paired=False)
+    eng = engine.CareEngine(0, paired=0, use_quality_scores=1)
+    try:
+        eng.set_reads_fastq(_texts(data, False)[0])
+        good = str(tmp_path / "good.bin")
+        eng.save_reads(good)
+        raw = open(good, "rb").read()
+        bits, min_len, max_len, lwords = _reads_file_header(raw)
+        assert bits > 0 and min_len + (1 << bits) - 1 > max_len, "the workload needs variable lengths"
+        cases = {
+            # every length value all ones: min + 2^bits - 1 > max
+            "longer": raw[:97] + b"\xff" * (4 * lwords) + raw[97 + 4 * lwords:],
+            # word count (and byte count) far beyond the file
+            "larger than the file": raw[:81] + (1 << 40).to_bytes(8, "little") + (4 << 40).to_bytes(8, "little")
+                                    + raw[97:],
+        }
+        # sequence section: pitch kept, row count claimed 2^30 times larger
+        sp_at = 97 + 4 * lwords
+        sp = int.from_bytes(raw[sp_at:sp_at + 8], "little")
+        n = int.from_bytes(raw[0:8], "little")
+        cases["bad sequence section|larger than the file"] = (raw[:sp_at + 8] + (sp * n * (1 << 30)).to_bytes(8, "little")
+                                                             + raw[sp_at + 16:])
+        for i, (match, blob) in enumerate(cases.items()):
+            path = str(tmp_path / f"bad{i}.bin")
+            open(path, "wb").write(blob)
+            with pytest.raises(engine.CareError, match=match):
+                eng.load_reads(path)
+        # the context still works after the rejections
+        assert eng.load_reads(good) == n
     finally:
         eng.close()

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from rabbitqcplus_amd import engine
+from rabbitqcplus_amd import sim, engine
 from tests import helpers
 
 pytestmark = pytest.mark.gpu
@@ -212,6 +212,33 @@ def test_fused_graph_replays_match_eager(name):
     assert g.correct()[1] == want
     g.close()
     eager.close()
+
+
+def test_fused_graph_recaptured_after_the_output_merge_reallocates():
+    """Same context: a graph-captured fused pass, then the output merge (its radix sort
+    can grow the context's scan workspace, which the captured record scan uses), then the
+    pass again. The second pass is recaptured when a buffer it uses moved, and its records
+    equal the first pass's and the eager engine's."""
+    s, rs, o = helpers.case("se_higherr_q")
+    eager = engine.CareEngine(0, engine=1, **o)
+    eager.set_reads(rs)
+    eager.build_tables()
+    want = eager.correct()[1]
+    eager.close()
+    g = engine.CareEngine(0, engine=2, use_graphs=1, **o)
+    try:
+        g.set_reads_fastq(*sim.fastq_texts(s))
+        g.build_tables()
+        first = g.correct()[1]
+        assert first == want
+        built = g.timings()["graphs_built"]
+        out = g.write_fastq(first, 0)
+        assert len(out) > 0
+        for _ in range(2):
+            assert g.correct()[1] == want
+        assert g.timings()["graphs_built"] in (built, built + 1)
+    finally:
+        g.close()
 
 
 def test_autotuner_choice_is_exact(tmp_path):
