@@ -73,6 +73,15 @@ typedef struct care_opts {
     int msa_tile;                   /* staged engine: candidates staged into LDS per step of the
                                        one-wavefront MSA, a multiple of 4 in 4..64; 0 = automatic
                                        (20 with quality scores, else 64) */
+    int seeded_alignment;           /* staged engine, reads of at most 352 bases; not in the reference:
+                                       1 = each candidate is aligned only at the strand and shift that
+                                       line up the k-mer of the first minhash map it shares with the
+                                       anchor (its seed), instead of the scan of every shift in both
+                                       orientations (include/corrector.hpp:925-979); the shift is kept
+                                       when the reference's overlap, mismatch and orientation bounds
+                                       hold. Records then differ from the reference's where the scan
+                                       would choose another shift or strand; used by the fast learned
+                                       form, whose bar is the error rate */
 } care_opts;
 
 typedef struct care_ctx care_ctx;

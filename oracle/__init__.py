@@ -102,6 +102,12 @@ def cpu_lib():
         lib.carecpu_set_classifier.argtypes = [vp, vp, i32]
         lib.carecpu_set_candidate_correction.argtypes = [vp, i32, i32]
         lib.carecpu_set_candidate_correction.restype = None
+        lib.carecpu_set_seeded_alignment.argtypes = [vp, i32]
+        lib.carecpu_set_seeded_alignment.restype = None
+        lib.carecpu_seeds.argtypes = [vp, u64, u64, vp]
+        lib.carecpu_seeds.restype = None
+        lib.carecpu_seeded_alignment.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp]
+        lib.carecpu_seeded_alignment.restype = None
         lib.carecpu_set_classifier.restype = i32
         lib.carecpu_mlp_num_params.restype = i32
         lib.carecpu_features.argtypes = [vp, u64, u64, i32, vp, vp, sz]
@@ -269,6 +275,22 @@ class CareCpu:
 
     def set_candidate_correction(self, on=True, new_columns=15):
         self.lib.carecpu_set_candidate_correction(self.h, int(on), int(new_columns))
+
+    def set_seeded_alignment(self, on=True):
+        """Seeded alignment (the fast learned form's alignment) instead of the shift scan."""
+        self.lib.carecpu_set_seeded_alignment(self.h, int(on))
+
+    def seeds(self, first=0, n=None):
+        n = self.rs.n - first if n is None else n
+        out = np.zeros((n, self.opts.num_hash_maps), np.uint16)
+        self.lib.carecpu_seeds(self.h, first, n, out.ctypes.data)
+        return out
+
+    def seeded_alignment(self, anchor, candidate):
+        """(map or -1, flag 1/2/3, shift, overlap, mismatches) of the seeded alignment."""
+        out = np.zeros(5, np.int32)
+        self.lib.carecpu_seeded_alignment(self.h, int(anchor), int(candidate), out.ctypes.data)
+        return tuple(int(x) for x in out)
 
     def set_classifier(self, params):
         """MLP parameters (float32, MLP_PARAMS values) for the learned seam; None = classic."""

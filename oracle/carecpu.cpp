@@ -58,10 +58,15 @@ inline uint64_t fmix64(uint64_t x){
 // as 2k-bit integers, include/sequencehelpers.hpp:844-923) and map i, take
 // min fmix64(kmer + i); mask to 2k bits after the min
 // (include/cpusequencehasher.hpp:88-125). Reads shorter than k give all-ones & mask.
-void signature(const uint32_t* row, int L, int k, int nmaps, uint64_t* out){
+//
+// With `seeds` (an addition for the seeded alignment; the reference keeps no positions):
+// per map, the start position of the first k-mer attaining the minimum times 2, plus 1
+// when its canonical form is the reverse complement (0 for reads shorter than k).
+void signature(const uint32_t* row, int L, int k, int nmaps, uint64_t* out, uint16_t* seeds = nullptr){
     const uint64_t mask = std::numeric_limits<uint64_t>::max() >> ((32 - k) * 2);
     uint64_t mins[kMaxMaps];
-    for(int m = 0; m < nmaps; m++) mins[m] = std::numeric_limits<uint64_t>::max();
+    uint16_t seed[kMaxMaps];
+    for(int m = 0; m < nmaps; m++){ mins[m] = std::numeric_limits<uint64_t>::max(); seed[m] = 0; }
     if(L >= k){
         uint64_t fwd = 0, rc = 0;
         const int topShift = 2 * k - 2;
@@ -72,12 +77,17 @@ void signature(const uint32_t* row, int L, int k, int nmaps, uint64_t* out){
             if(i >= k - 1){
                 const uint64_t canon = std::min(fwd, rc);
                 for(int m = 0; m < nmaps; m++){
-                    mins[m] = std::min(mins[m], fmix64(canon + uint64_t(m)));
+                    const uint64_t h = fmix64(canon + uint64_t(m));
+                    if(h < mins[m]){
+                        mins[m] = h;
+                        seed[m] = uint16_t(((i - k + 1) << 1) | (rc < fwd ? 1 : 0));
+                    }
                 }
             }
         }
     }
     for(int m = 0; m < nmaps; m++) out[m] = mins[m] & mask;
+    if(seeds) for(int m = 0; m < nmaps; m++) seeds[m] = seed[m];
 }
 
 // calculateResultsPerMapThreshold(int coverage) (include/minhasherlimit.hpp:10-16),
@@ -539,6 +549,8 @@ struct carecpu_ctx {
     std::vector<char> quals;
     std::vector<uint8_t> ambig;
     std::vector<uint64_t> sigs;     // [read][map]
+    std::vector<uint16_t> seeds;    // [read][map], seeds of the signatures
+    bool seeded = false;            // seeded alignment instead of the shift scan
     std::vector<Table> tables;
     std::vector<uint8_t> records;
     std::vector<float> mlp;         // classifier parameters (empty: classic decision)
@@ -590,6 +602,51 @@ struct carecpu_ctx {
             ids.erase(std::remove_if(ids.begin(), ids.end(), [&](uint32_t r){ return ambig[r] != 0; }), ids.end());
         }
         return ids;
+    }
+
+    // Seeded alignment of candidate c against anchor a (an addition, not in the
+    // reference): the first map whose kept bucket (1..threshold values) holds both reads
+    // places their shared canonical k-mer; equal seed strands make the candidate forward
+    // at shift pa - pc, different strands its reverse complement at pa - (Lc - k - pc).
+    // The alignment at that shift is kept (flag 1 or 2) when overlap >= the minimum
+    // overlap, mismatches < int(float(overlap) * maxErrorRate) and
+    // mismatches / overlap < estimatedErrorrate * 4 (the reference's bounds,
+    // src/cpu_alignment.cpp:32-497, include/alignmentorientation.hpp:14-92); flag 3
+    // otherwise. `map` is the map used (-1: none).
+    int seededAlignment(uint32_t a, uint32_t c, const std::vector<uint8_t>& anchor, const std::vector<uint8_t>& fwd,
+                        const std::vector<uint8_t>& rc, Alignment& out, int& map) const {
+        const int M = o.num_hash_maps;
+        const int La = lens[a], Lc = lens[c];
+        map = -1;
+        for(int m = 0; m < M; m++){
+            const uint64_t key = sigs[size_t(a) * M + m];
+            const uint32_t* v;
+            const int cnt = bucket(m, key, &v);
+            if(cnt <= 0 || cnt > threshold) continue;
+            if(sigs[size_t(c) * M + m] == key){ map = m; break; }
+        }
+        out = Alignment{};
+        if(map < 0) return 3;
+        const int sa = seeds[size_t(a) * M + map], sc = seeds[size_t(c) * M + map];
+        const int pa = sa >> 1, pc = sc >> 1;
+        const bool same = (sa & 1) == (sc & 1);
+        const int shift = same ? pa - pc : pa - (Lc - o.kmerlength - pc);
+        const std::vector<uint8_t>& C = same ? fwd : rc;
+        const int cb = std::max(-shift, 0);
+        const int ce = std::min(Lc, La - shift);
+        const int ov = ce - cb;
+        int mis = 0;
+        for(int j = cb; j < ce; j++) mis += (C[j] != anchor[j + shift]);
+        out.shift = shift;
+        out.overlap = ov;
+        out.nOps = mis;
+        out.score = mis + La + Lc - 2 * ov;
+        out.valid = true;
+        const int mo = std::max(o.min_overlap, int(float(La) * o.min_overlap_ratio));
+        if(ov > 0 && ov >= mo && mis < int(float(ov) * o.max_error_rate)
+           && float(mis) / float(ov) < o.estimated_errorrate * 4.0f)
+            return same ? 1 : 2;
+        return 3;
     }
 
     // One anchor through CpuErrorCorrector::process (include/corrector.hpp:145-312),
@@ -680,9 +737,17 @@ bool carecpu_ctx::processAnchor(uint32_t a, const std::vector<uint32_t>& cids, c
         fwd = decodeBases(id);
         rc.resize(Lc);
         for(int j = 0; j < Lc; j++) rc[j] = uint8_t(3 - fwd[Lc - 1 - j]);
-        const Alignment af = align(anchor.data(), La, fwd.data(), Lc, o.min_overlap, o.max_error_rate, o.min_overlap_ratio);
-        const Alignment ar = align(anchor.data(), La, rc.data(), Lc, o.min_overlap, o.max_error_rate, o.min_overlap_ratio);
-        const int flag = orientation(af, ar, La, o.min_overlap_ratio, o.min_overlap, o.estimated_errorrate);
+        Alignment af, ar;
+        int flag;
+        if(seeded){
+            int map;
+            flag = seededAlignment(a, id, anchor, fwd, rc, af, map);
+            ar = af;
+        }else{
+            af = align(anchor.data(), La, fwd.data(), Lc, o.min_overlap, o.max_error_rate, o.min_overlap_ratio);
+            ar = align(anchor.data(), La, rc.data(), Lc, o.min_overlap, o.max_error_rate, o.min_overlap_ratio);
+            flag = orientation(af, ar, La, o.min_overlap_ratio, o.min_overlap, o.estimated_errorrate);
+        }
         if(tr && i < tr->cap_candidates){
             auto put = [](int* dst, const Alignment& x){
                 dst[0] = x.score; dst[1] = x.overlap; dst[2] = x.shift; dst[3] = x.nOps; dst[4] = x.valid ? 1 : 0;
@@ -904,10 +969,12 @@ carecpu_ctx* carecpu_create(const carecpu_opts* opts, uint64_t num_reads, const 
 
     const int M = opts->num_hash_maps;
     c->sigs.resize(num_reads * M);
+    c->seeds.resize(num_reads * M);
     if(threads > 0) omp_set_num_threads(threads);
     #pragma omp parallel for schedule(static)
     for(long long r = 0; r < (long long)num_reads; r++){
-        signature(c->row(uint32_t(r)), c->lens[r], opts->kmerlength, M, c->sigs.data() + size_t(r) * M);
+        signature(c->row(uint32_t(r)), c->lens[r], opts->kmerlength, M, c->sigs.data() + size_t(r) * M,
+                  c->seeds.data() + size_t(r) * M);
     }
 
     // Tables: group (signature, read id) pairs by key with read ids ascending, and keep
@@ -1073,6 +1140,25 @@ size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads
         ctx->records.insert(ctx->records.end(), e.second->begin(), e.second->end());
     }
     return count;
+}
+
+void carecpu_set_seeded_alignment(carecpu_ctx* ctx, int on){ ctx->seeded = on != 0; }
+
+void carecpu_seeds(carecpu_ctx* ctx, uint64_t first, uint64_t n, uint16_t* out){
+    const int M = ctx->o.num_hash_maps;
+    std::memcpy(out, ctx->seeds.data() + first * M, n * M * sizeof(uint16_t));
+}
+
+// Seeded alignment of candidate c against anchor a: out = {map, flag, shift, overlap, nOps}.
+void carecpu_seeded_alignment(carecpu_ctx* ctx, uint32_t a, uint32_t c, int* out){
+    const std::vector<uint8_t> anchor = ctx->decodeBases(a);
+    const std::vector<uint8_t> fwd = ctx->decodeBases(c);
+    std::vector<uint8_t> rc(fwd.size());
+    for(size_t j = 0; j < fwd.size(); j++) rc[j] = uint8_t(3 - fwd[fwd.size() - 1 - j]);
+    Alignment al;
+    int map;
+    const int flag = ctx->seededAlignment(a, c, anchor, fwd, rc, al, map);
+    out[0] = map; out[1] = flag; out[2] = al.shift; out[3] = al.overlap; out[4] = al.nOps;
 }
 
 void carecpu_set_candidate_correction(carecpu_ctx* ctx, int on, int new_columns){

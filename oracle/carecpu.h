@@ -108,6 +108,15 @@ size_t carecpu_correct(carecpu_ctx* ctx, uint64_t first, uint64_t n, int threads
 
 /* Candidate correction (--candidateCorrection, include/options.hpp:41,52): records of
  * corrected candidates of HQ anchors, in a one-thread run's order. */
+/* Seeded alignment (an addition for the fast learned form; not in the reference):
+ * every candidate aligned only at the strand and shift that line up the k-mer of the
+ * first minhash map it shares with the anchor. */
+void carecpu_set_seeded_alignment(carecpu_ctx* ctx, int on);
+/* Seeds of reads [first, first+n), [read][map]: start of the first minimizing k-mer * 2,
+ * plus 1 when its canonical form is the reverse complement. */
+void carecpu_seeds(carecpu_ctx* ctx, uint64_t first, uint64_t n, uint16_t* out);
+/* out = {map (-1: none), flag (1 fwd, 2 revcomp, 3 none), shift, overlap, mismatches}. */
+void carecpu_seeded_alignment(carecpu_ctx* ctx, uint32_t a, uint32_t c, int* out);
 void carecpu_set_candidate_correction(carecpu_ctx* ctx, int on, int new_columns);
 size_t carecpu_records_bytes(carecpu_ctx* ctx);
 const uint8_t* carecpu_records_data(carecpu_ctx* ctx);

@@ -180,15 +180,20 @@ struct OpOr  { __device__ int operator()(int a, int b) const { return a | b; } }
 // Minhash signatures: one thread per read, all maps; output map-major [map][read].
 // Canonical k-mer = min(forward, reverse complement) (include/sequencehelpers.hpp:844-923);
 // sig[m] = (min over k-mers of fmix64(kmer + m)) & mask (include/cpusequencehasher.hpp:88-125).
+// With `seeds`, also the seed of every map, read-major [read][map]: the start position of
+// the first k-mer that attains the minimum, times 2, plus 1 when its canonical form is the
+// reverse complement (0 for reads shorter than k). Seeds place a shared k-mer in both reads
+// of a candidate pair (seeded alignment).
 __global__ void __launch_bounds__(256)
 k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n,
-             int k, int M, uint64_t* __restrict__ sig){
+             int k, int M, uint64_t* __restrict__ sig, uint16_t* __restrict__ seeds){
     const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if(r >= n) return;
     const uint64_t mask = ~0ull >> ((32 - k) * 2);
     uint64_t mins[kMaxMaps];
+    uint32_t seed[kMaxMaps];
     #pragma unroll
-    for(int m = 0; m < kMaxMaps; m++) mins[m] = ~0ull;
+    for(int m = 0; m < kMaxMaps; m++){ mins[m] = ~0ull; seed[m] = 0; }
     const int L = len[r];
     const uint32_t* row = enc + pitch * r;
     if(L >= k){
@@ -202,11 +207,12 @@ k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restri
             rc = (rc >> 2) | ((3ull - b) << top);
             if(i >= k - 1){
                 const uint64_t canon = fwd < rc ? fwd : rc;
+                const uint32_t here = (uint32_t(i - k + 1) << 1) | uint32_t(rc < fwd);
                 #pragma unroll
                 for(int m = 0; m < kMaxMaps; m++){
                     if(m < M){
                         const uint64_t h = fmix64(canon + uint64_t(m));
-                        mins[m] = h < mins[m] ? h : mins[m];
+                        if(h < mins[m]){ mins[m] = h; seed[m] = here; }
                     }
                 }
             }
@@ -214,7 +220,10 @@ k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restri
     }
     #pragma unroll
     for(int m = 0; m < kMaxMaps; m++){
-        if(m < M) sig[uint64_t(m) * n + r] = mins[m] & mask;
+        if(m < M){
+            sig[uint64_t(m) * n + r] = mins[m] & mask;
+            if(seeds) seeds[r * uint64_t(M) + uint64_t(m)] = uint16_t(seed[m]);
+        }
     }
 }
 
@@ -704,6 +713,8 @@ struct AlignParams {
     uint32_t eMul; int ePre; int eExact;
     AlignBest* best; int* dbg;   // dbg: [total][11] or null
     const uint32_t* order;       // processing order of the batch's anchors, or null
+    // seeded alignment: every read's bucket (offset, count) and seed per map, read-major
+    const uint2* bucketOf; const uint16_t* seeds; int M; int k; int thr;
 };
 
 struct ShdResult { int score, overlap, shift, nOps, valid; };
@@ -1138,6 +1149,122 @@ k_align_lanes(AlignParams p){
     if(n == 0) return;
     alignLanes<WM>(p, p.b0 + i, p.ids + beg, n, p.best + beg, p.dbg ? p.dbg + uint64_t(beg) * 11 : nullptr,
                    threadIdx.x);
+}
+
+// Seeded alignment (an addition; the reference has no counterpart): one shift per
+// candidate instead of the scan of both orientations. A candidate was found through a
+// bucket it shares with the anchor, i.e. through a canonical k-mer both reads contain;
+// the first map (lowest index) whose kept bucket holds both reads gives that k-mer's
+// seed in each read (start position p and whether the canonical form is the reverse
+// complement, s). Equal s: the candidate is forward and starts at anchor position
+// pa - pc; different s: it is the reverse complement, starting at pa - (Lc - k - pc).
+// At that shift the candidate gets the reference's tests: overlap >= the minimum
+// overlap, mismatches < int(float(overlap) * maxErrorRate), and mismatches / overlap
+// below the orientation bound (include/alignmentorientation.hpp:14-92); otherwise it is
+// dropped (orientation 3). The chosen alignment is written as the scan's would be.
+//
+// One wavefront per anchor, one lane per candidate. Mismatches are counted on the HiLo
+// planes in LDS: the anchor's, and per lane its candidate's planes of the chosen strand,
+// each zero padded to 2*WM+2 words so that a shifted read of WM words never leaves them.
+template<int WM>
+__global__ void __launch_bounds__(kWave)
+k_align_seeded(AlignParams p){
+    constexpr int P = 2 * WM + 2;
+    __shared__ uint32_t aPl[2][P];
+    __shared__ uint32_t cPl[2][kWave][P];
+    __shared__ uint2 aBucket[kMaxMaps];
+    __shared__ uint16_t aSeed[kMaxMaps];
+    if(int(blockIdx.x) >= p.B) return;
+    const int i = batchAnchor(p.order, p.B);
+    const uint32_t beg = p.offsets[i];
+    const int n = int(p.offsets[i + 1] - beg);
+    if(n == 0) return;
+    const int lane = threadIdx.x;
+    const uint64_t a = p.b0 + i;
+    const int W = p.W;
+    const int La = p.len[a];
+    const uint32_t* arow = p.hilo + a * uint64_t(4 * W);
+    for(int q = lane; q < 2 * P; q += kWave){
+        const int pl = q / P, w = q % P;
+        aPl[pl][w] = w < W ? arow[pl * W + w] : 0u;
+    }
+    if(lane < p.M){
+        const uint2 b = p.bucketOf[a * uint64_t(p.M) + uint64_t(lane)];
+        aBucket[lane] = (b.y > 0 && b.y <= uint32_t(p.thr)) ? b : make_uint2(0u, 0u);
+        aSeed[lane] = p.seeds[a * uint64_t(p.M) + uint64_t(lane)];
+    }
+    __syncthreads();
+    const int mo = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
+    for(int c0 = 0; c0 < n; c0 += kWave){
+        const int c = c0 + lane;
+        const bool valid = c < n;
+        uint32_t id = 0;
+        int Lc = 0, shift = 0, orient = 3;
+        if(valid){
+            id = p.ids[beg + c];
+            Lc = p.len[id];
+            // first map whose kept bucket holds both reads (equal offset and count)
+            const uint2* cb = p.bucketOf + uint64_t(id) * uint64_t(p.M);
+            int m = 0;
+            for(; m < p.M; m++){
+                const uint2 ab = aBucket[m];
+                if(ab.y == 0) continue;
+                const uint2 x = cb[m];
+                if(x.x == ab.x && x.y == ab.y) break;
+            }
+            if(m < p.M){
+                const int sa = aSeed[m], sc = p.seeds[uint64_t(id) * uint64_t(p.M) + uint64_t(m)];
+                const int pa = sa >> 1, pc = sc >> 1;
+                if((sa & 1) == (sc & 1)){ orient = 1; shift = pa - pc; }
+                else{ orient = 2; shift = pa - (Lc - p.k - pc); }
+            }
+        }
+        // the candidate's planes of the chosen strand into this lane's LDS rows
+        const uint32_t* crow = p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 2 ? 2 * W : 0);
+        #pragma unroll
+        for(int w = 0; w < P; w++){
+            const bool in = valid && orient != 3 && w < W;
+            cPl[0][lane][w] = in ? crow[w] : 0u;
+            cPl[1][lane][w] = in ? crow[W + w] : 0u;
+        }
+        __syncthreads();
+        if(valid){
+            AlignBest b;
+            b.orientation = 3;
+            b.shift = shift;
+            b.overlap = 0;
+            b.nOps = 0;
+            if(orient != 3){
+                const int ov = min(Lc, La - shift) - max(-shift, 0);
+                // X[d + j] against Y[j] for j < ov: X is the read that starts first
+                const uint32_t* Xh = shift >= 0 ? aPl[0] : cPl[0][lane];
+                const uint32_t* Xl = shift >= 0 ? aPl[1] : cPl[1][lane];
+                const uint32_t* Yh = shift >= 0 ? cPl[0][lane] : aPl[0];
+                const uint32_t* Yl = shift >= 0 ? cPl[1][lane] : aPl[1];
+                const int d = shift >= 0 ? shift : -shift;
+                const int q = d >> 5, r = d & 31;
+                int mis = 0;
+                if(ov > 0 && q < WM + 1){
+                    #pragma unroll
+                    for(int j = 0; j < WM; j++){
+                        if(32 * j < ov){
+                            const uint32_t xh = uint32_t((((uint64_t(Xh[q + j]) << 32) | Xh[q + j + 1]) << r) >> 32);
+                            const uint32_t xl = uint32_t((((uint64_t(Xl[q + j]) << 32) | Xl[q + j + 1]) << r) >> 32);
+                            const int left = ov - 32 * j;
+                            const uint32_t mask = left >= 32 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> left);
+                            mis += __popc(((xh ^ Yh[j]) | (xl ^ Yl[j])) & mask);
+                        }
+                    }
+                }
+                b.overlap = ov;
+                b.nOps = mis;
+                if(ov > 0 && ov >= mo && mis < int(float(ov) * p.maxErrorRate) && float(mis) / float(ov) < p.orientErrorRate)
+                    b.orientation = orient;
+            }
+            p.best[beg + c] = b;
+        }
+        __syncthreads();
+    }
 }
 
 // Words per plane handled by the lane kernel for reads of W words (0: too long).
@@ -2870,6 +2997,7 @@ struct care_ctx {
     bool hasTables = false;
 
     DBuf enc, len, qual, ambig, sig, hilo, rcenc, qw, bucketOf, gath, order, orderFlags, batchOrd;
+    DBuf seeds;             // per (read, map): seed of the map's minimum k-mer (seeded alignment)
     std::vector<DBuf> slotBufs;
     std::vector<uint32_t> caps;
     DBuf slotPtrs, capsDev, values;
@@ -3160,6 +3288,8 @@ void fillAlignCommon(const care_ctx* ctx, AlignParams& ap){
     ap.eExact = errorBoundAsInteger(ap.maxErrorRate, ctx->maxLen, ap.eMul, ap.ePre) ? 1 : 0;
     if(!ap.eExact){ ap.eMul = 0; ap.ePre = 0; }
     ap.order = nullptr;
+    ap.bucketOf = ctx->bucketOf.as<uint2>(); ap.seeds = ctx->seeds.as<uint16_t>();
+    ap.M = ctx->M; ap.k = ctx->k; ap.thr = ctx->thr;
 }
 
 int recordSlotBytes(int maxLen){
@@ -3537,7 +3667,17 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
     ap.offsets = ctx->offsets.as<uint32_t>(); ap.B = B; ap.maxLen = ctx->maxLen;
     ap.order = order;
-    if(total){
+    if(total && ctx->o.seeded_alignment){
+        switch(alignLaneWords(ctx->W)){
+            case 2: k_align_seeded<2><<<B, kWave, 0, st>>>(ap); break;
+            case 4: k_align_seeded<4><<<B, kWave, 0, st>>>(ap); break;
+            case 5: k_align_seeded<5><<<B, kWave, 0, st>>>(ap); break;
+            case 8: k_align_seeded<8><<<B, kWave, 0, st>>>(ap); break;
+            case 11: k_align_seeded<11><<<B, kWave, 0, st>>>(ap); break;
+            default: return ctx->fail(CARE_EINVAL, "seeded alignment handles reads of at most 352 bases");
+        }
+        HIPCHECK(hipGetLastError());
+    }else if(total){
         switch(wm){
             case 2: k_align_lanes<2><<<B, kWave, 0, st>>>(ap); break;
             case 4: k_align_lanes<4><<<B, kWave, 0, st>>>(ap); break;
@@ -3726,6 +3866,7 @@ static int care_hip_create_body(const care_opts* opts, int device, care_ctx** ou
     if(!(opts->estimated_coverage > 0.0f)) return CARE_EINVAL;
     if(opts->new_columns_to_correct < 0) return CARE_EINVAL;
     if(opts->msa_tile != 0 && (opts->msa_tile < 4 || opts->msa_tile > 64 || (opts->msa_tile & 3))) return CARE_EINVAL;
+    if(opts->seeded_alignment != 0 && opts->seeded_alignment != 1) return CARE_EINVAL;
     int count = 0;
     if(hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return CARE_ENODEV;
     auto* ctx = new (std::nothrow) care_ctx;
@@ -4959,8 +5100,9 @@ static int care_hip_build_tables_body(care_ctx* ctx, int* maps_built){
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     HIPCHECK(ctx->sig.ensure(ctx->n * ctx->M * 8));
+    HIPCHECK(ctx->seeds.ensure(std::max<uint64_t>(1, ctx->n * ctx->M) * 2));
     k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
-                                                                   ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>());
+                                                                   ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>(), ctx->seeds.as<uint16_t>());
     HIPCHECK(hipGetLastError());
     int rc = buildTables(ctx);
     if(rc != CARE_OK) return rc;
@@ -5065,8 +5207,9 @@ static int care_hip_load_tables_body(care_ctx* ctx, const char* path, int* maps_
     const int M = std::min(maps, ctx->o.num_hash_maps);
     ctx->M = M;
     HIPCHECK(ctx->sig.ensure(ctx->n * M * 8));
+    HIPCHECK(ctx->seeds.ensure(std::max<uint64_t>(1, ctx->n * M) * 2));
     k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
-                                                                   ctx->n, ctx->k, M, ctx->sig.as<uint64_t>());
+                                                                   ctx->n, ctx->k, M, ctx->sig.as<uint64_t>(), ctx->seeds.as<uint16_t>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(ctx->values.ensure(std::max<uint64_t>(1, ctx->n) * M * sizeof(uint32_t)));
     ctx->slotBufs.resize(M);
@@ -5164,6 +5307,9 @@ static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size
     if(cands && waveMsaColumnsPerLane(ctx->maxCols) == 0)
         return ctx->fail(CARE_EINVAL, "candidate correction needs reads of at most 340 bases");
     if(cands && ctx->o.engine == 2) return ctx->fail(CARE_EINVAL, "candidate correction runs on the staged engine");
+    if(ctx->o.seeded_alignment && ctx->o.engine == 2) return ctx->fail(CARE_EINVAL, "seeded alignment runs on the staged engine");
+    if(ctx->o.seeded_alignment && alignLaneWords(ctx->W) == 0)
+        return ctx->fail(CARE_EINVAL, "seeded alignment handles reads of at most 352 bases");
     if(cands){
         HIPCHECK(ctx->hqAnchor.ensure(ctx->n));
         ctx->callFirst = first;

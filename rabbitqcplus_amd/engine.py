@@ -39,7 +39,8 @@ class CareOpts(ctypes.Structure):
                 ("engine", ctypes.c_int), ("waves_per_cu", ctypes.c_int),
                 ("correction_type", ctypes.c_int), ("align_variant", ctypes.c_int),
                 ("use_graphs", ctypes.c_int), ("correct_candidates", ctypes.c_int),
-                ("new_columns_to_correct", ctypes.c_int), ("msa_tile", ctypes.c_int)]
+                ("new_columns_to_correct", ctypes.c_int), ("msa_tile", ctypes.c_int),
+                ("seeded_alignment", ctypes.c_int)]
 
 
 class CareTimings(ctypes.Structure):
