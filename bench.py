@@ -4,26 +4,39 @@
 A step is one pass of the correction stage over every read of the workload
 (candidate lookup -> alignment -> MSA/refinement/decision -> serialized records on
 the host), with the read storage and the minhash tables already resident in HBM.
-Table construction is timed once and reported separately.
+Table construction is timed once and reported separately, and so is one end-to-end
+run (reads uploaded, tables built, one pass).
 
-Prints one JSON line (see the keys below). Options:
-  --gpus N        only 1 is supported (one GPU per box); N > 1 exits with a message
-  --steps K       timed passes (default 3)
-  --warmup W      untimed passes (default 1)
-  --workload      simulator workload name (default se_5mbp_150bp_30x_dup)
-  --no-quality    correct without quality scores
-  --cpu-sample S  anchors for the CPU baseline leg (default 300000; 0 disables)
-  --cpu-threads T threads for the CPU baseline (default min(16, cores))
-  --no-learned    skip the learned leg (the classifier seam with the shipped classifier)
-  --no-candidates skip the candidate-correction leg
-  --no-autotune   skip the autotuner (which times the schedules on 65,536 anchors first)
+The default workload is the largest one: 8.57M single-end reads of 100..250 bp at 30x
+from a 50 Mbp genome with 10-copy repeat families (k = 20, 48 maps, qualities on).
+
+Prints one JSON line. Legs beside the headline (plain path, bit-exact to the reference):
+  learned               the learned form: the MLP classifier seam on seeded alignment
+  learned_exact         the classifier seam on the reference's shift scan
+  pe_workload           2.0M read pairs with 20-copy repeats: plain and learned
+  fastq_io              FASTQ text -> device read storage, records -> corrected FASTQ text
+  candidate_correction  --candidateCorrection passes
+  cpu_baseline          the reference engine compiled from source, on every host core,
+                        over a bounded sample of the same workload's anchors
+Options:
+  --gpus N          only 1 is supported (one GPU per box); N > 1 exits with a message
+  --steps K         timed passes (default 3)
+  --warmup W        untimed passes (default 1)
+  --workload        simulator workload name (default se_50mbp_100to250bp_30x_rep)
+  --no-quality      correct without quality scores
+  --cpu-sample S    anchors for the CPU baseline leg (default 300000; 0 disables)
+  --cpu-threads T   threads for the CPU baseline (default: every core this process may use)
+  --pe-workload     paired workload of the PE leg ('' disables)
+  --no-learned / --no-candidates / --no-fastq / --no-autotune   skip those legs
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -34,32 +47,60 @@ sys.path.insert(0, ROOT)
 from rabbitqcplus_amd import autotune, engine, evaluate, learned, reads, sim  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+MAPS_SLOT_BYTES = 16      # one 16-byte AoS table slot per map and anchor
 
 
-def algorithmic_bytes(t: dict, rs, W: int, M: int):
-    """Bytes each kernel must move at minimum, per launch (one launch per batch)."""
-    batches = max(1, t["batches"])
+def kmer_length(rs):
+    """k = 20 for every workload here (the reference picks 32 for reads of 160 bp and
+    more, src/dispatch_care_correct_cpu.cpp:283-297; the workloads fix k = 20)."""
+    return 20
+
+
+def algorithmic_bytes(t: dict, rs, use_q: bool, M: int):
+    """Algorithmic bytes of the correction stage, by the per-anchor terms of the path:
+    own 2-bit row + length (4*pitch + 4), one 16-byte table slot per map, every gathered
+    bucket value (4 B), every candidate's row + length (4*pitch + 4), qualities of the
+    anchor and its candidates (L each; the candidates kept after the filters are at most
+    the aligned ones, which are counted), and the serialized records. Returns the bytes
+    of one pass split by the kernel that moves each term, and the pass total."""
     A, C, G = t["anchors"], t["candidates"], t["bucket_values"]
-    pitch_b = 4 * rs.pitch_ints
-    # k_candidates: length+ambiguity flag, M signatures (8 B) and M table slots (16 B) per
-    # anchor, every gathered bucket value (4 B), unique candidate ids written (4 B)
-    cand = A * (4 + 1 + 24 * M) + 4 * G + 4 * C
-    # k_align (lane per candidate): per candidate its id and length (8 B), fwd+rc HiLo planes
-    # (16W B) and the chosen alignment written (16 B); per anchor its fwd planes (8W B), its
-    # length and candidate offsets (12 B)
-    align = C * (24 + 16 * W) + A * (8 * W + 12)
-    # k_msa per candidate: chosen alignment + id (20 B), its 2-bit row once (pitch) and
-    # qualities once when enabled; per anchor: its row, offsets, record slot bytes
-    qbytes = rs.quals.shape[1] if rs.quals is not None else 0
-    msa = C * (20 + pitch_b + qbytes) + A * (pitch_b + 8 + qbytes) + t["record_bytes"]
-    # k_fused (one launch per pass) moves the same inputs once, with the candidate lists and
-    # alignments kept in wave scratch: per anchor length+flag, signatures, table slots, its
-    # planes (8W), row, qualities; every gathered value; per candidate its length (4 B),
-    # fwd+rc planes (16W), 2-bit row and qualities; the record bytes
-    fused = (A * (4 + 1 + 24 * M + 8 * W + pitch_b + qbytes) + 4 * G
-             + C * (4 + 16 * W + pitch_b + qbytes) + t["record_bytes"])
-    return {"k_candidates": cand / batches, "k_align": align / batches, "k_msa": msa / batches,
-            "k_fused": fused / batches}
+    row = 4 * rs.pitch_ints + 4
+    mean_len = float(rs.lengths.mean())
+    cand = A * M * MAPS_SLOT_BYTES + 4 * G
+    align = row * (A + C)
+    msa = (mean_len * (A + C) if use_q else 0.0) + t["record_bytes"]
+    return {"k_candidates": cand, "k_align": align, "k_msa": msa, "pass": cand + align + msa}
+
+
+def host_info(threads):
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "threads_used": threads}
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, p = fh.read().split()
+            info["cgroup_cpus"] = None if q == "max" else round(int(q) / int(p), 2)
+    except (OSError, ValueError):
+        info["cgroup_cpus"] = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                info["cpu_model"] = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def default_threads():
+    """Every core this process may run on (its affinity set, bounded by a cgroup quota)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, p = fh.read().split()
+            if q != "max":
+                n = min(n, max(1, int(int(q) // int(p))))
+    except (OSError, ValueError):
+        pass
+    return n
 
 
 def calibration_copy(nbytes: int):
@@ -77,27 +118,6 @@ def calibration_copy(nbytes: int):
     hip.hipDeviceSynchronize()
     hip.hipFree(src)
     hip.hipFree(dst)
-
-
-def cpu_baseline_child(args):
-    """CPU-baseline leg in a child process (regenerates the same seeded workload), under
-    a time limit and retried once, so that a stalled CPU leg cannot hold the GPU line."""
-    import subprocess
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", args.workload,
-           "--seed", str(args.seed), "--cpu-sample", str(args.cpu_sample), "--cpu-threads", str(args.cpu_threads)]
-    if args.no_quality:
-        cmd.append("--no-quality")
-    for attempt in range(2):
-        try:
-            r = subprocess.run(cmd, capture_output=True, text=True, timeout=150)
-        except subprocess.TimeoutExpired:
-            print(f"[bench] CPU baseline attempt {attempt + 1} timed out", file=sys.stderr, flush=True)
-            continue
-        lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
-        if r.returncode == 0 and lines:
-            return json.loads(lines[-1])
-        print(f"[bench] CPU baseline attempt {attempt + 1} failed: {r.stderr[-500:]}", file=sys.stderr, flush=True)
-    return None
 
 
 def baseline_anchors(sample, rs, paired):
@@ -120,40 +140,77 @@ def records_digest(data: bytes, below: int):
     return len(spans), hashlib.sha256(b"".join(data[q:q + nb] for _, q, nb in spans)).hexdigest()
 
 
-def cpu_baseline(rs, opts, sample, threads):
+def cpu_baseline(rs, opts, sample, threads, tables):
+    """The reference engine compiled from source (kind "reference"), its tables read by its
+    own loader from the engine's table file (the reference's format), timed over the
+    first `sample` anchors on `threads` threads; the restatement's parallel table build on
+    the same threads is timed beside it (the reference builds its tables only sequentially
+    without its thread pool)."""
     import oracle
     n = baseline_anchors(sample, rs, opts.get("paired"))
+    out = {"unit": "reads/s", "cores": threads}
+    t0 = time.time()
+    cpu = oracle.CareCpu(rs, threads=threads, **opts)
+    out["table_build_s"] = round(time.time() - t0, 2)
+    out["table_build_kind"] = "port (the restatement's tables equal the reference's)"
+    cpu.close()
+    del cpu
     if oracle.ref_available():
-        ref = oracle.CareRef(rs, threads=threads, **opts)
-        secs, recs = ref.correct_timed(threads, n)
+        ref = oracle.CareRef(rs, threads=threads, tables=tables, **opts)
+        out["tables_loaded_s"] = round(ref.load_seconds, 2)
+        secs, _ = ref.correct_timed(threads, n)
         # the same anchors once more, keeping the records (sorted by read id), for the
         # comparison with the GPU's
         _, data = ref.correct(threads, n)
         ref.close()
-        kind = "reference"
+        out["kind"] = "reference"
     else:
         cpu = oracle.CareCpu(rs, threads=threads, **opts)
         _, data = cpu.correct(0, n, threads=threads)
         secs = cpu.last_seconds()
         cpu.close()
-        kind = "port"
+        out["kind"] = "port"
     count, digest = records_digest(data, n)
-    return {"value": n / secs, "unit": "reads/s", "cores": threads, "kind": kind,
-            "sample": f"first {n} anchors of the same workload (tables over all {rs.n} reads), "
-                      f"{secs:.2f} s wall on {threads} threads",
-            "records": count, "records_sha256": digest}
+    out.update({"value": round(n / secs, 1), "seconds": round(secs, 3),
+                "sample": f"first {n} anchors of the same workload (tables over all {rs.n} reads), "
+                          f"{secs:.2f} s wall on {threads} threads",
+                "records": count, "records_sha256": digest})
+    return out
 
 
-def measured_traffic(workload, kernel):
-    """HBM bytes per launch of `kernel` on `workload` from the profiler byte counters
-    (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE, each scaled by its reading for a plain
-    copy of known size), as recorded in profiles/traffic.json; None when not recorded."""
+def cpu_baseline_child(args, tables, threads, log):
+    """CPU-baseline leg in a child process (it regenerates the same seeded workload) under
+    a time limit; a failure or time-out is recorded, not retried."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-only", "--workload", args.workload,
+           "--seed", str(args.seed), "--cpu-sample", str(args.cpu_sample), "--cpu-threads", str(threads),
+           "--tables", tables]
+    if args.no_quality:
+        cmd.append("--no-quality")
+    t0 = time.time()
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=args.cpu_timeout)
+    except subprocess.TimeoutExpired:
+        log(f"CPU baseline timed out after {args.cpu_timeout} s")
+        return {"error": f"timed out after {args.cpu_timeout} s (not retried)"}
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("{")]
+    if r.returncode != 0 or not lines:
+        log(f"CPU baseline failed: {r.stderr[-800:]}")
+        return {"error": f"child exited with {r.returncode}: {r.stderr[-300:]}"}
+    res = json.loads(lines[-1])
+    res["child_wall_s"] = round(time.time() - t0, 1)
+    return res
+
+
+def measured(workload, kernel, what="traffic"):
+    """Profiler figures recorded in profiles/traffic.json for `kernel` on `workload`."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         rec = json.load(fh).get(workload, {}).get(kernel)
-    return None if rec is None else int(rec["bytes_per_launch"])
+    if rec is None:
+        return None
+    return int(rec["bytes_per_launch"]) if what == "traffic" else rec
 
 
 def timed_passes(eng, steps, warmup, log, tag):
@@ -170,14 +227,30 @@ def timed_passes(eng, steps, warmup, log, tag):
     return float(np.mean(step_ms)), nrec, data, eng.timings()
 
 
-# What bounds each staged kernel on MI355X (hardware counters, not HBM bandwidth)
-LIMITERS = {
-    "k_align": "vector and scalar instruction issue (about 1.1 VALU + 0.76 SALU instructions per clock per CU), "
-               "not HBM bandwidth",
-    "k_msa": "vector issue and LDS/memory latency, not HBM bandwidth",
-    "k_candidates": "latency of the random bucket-value loads, not HBM bandwidth",
-    "k_fused": "occupancy and instruction issue, not HBM bandwidth",
-}
+def engine_opts(rs, use_q, args):
+    o = dict(use_quality_scores=int(use_q), paired=int(rs.paired), batch_size=args.batch_size,
+             engine={"auto": 0, "staged": 1, "fused": 2}[args.engine], waves_per_cu=args.waves_per_cu,
+             kmerlength=kmer_length(rs))
+    return o
+
+
+def run_leg(rs, opts, steps, warmup, log, tag, classifier=None):
+    eng = engine.CareEngine(0, **opts)
+    try:
+        eng.set_reads(rs)
+        eng.build_tables()
+        if classifier is not None:
+            eng.set_classifier(classifier)
+        return timed_passes(eng, steps, warmup, log, tag)
+    finally:
+        eng.close()
+
+
+def error_line(s, rs, data):
+    e = evaluate.error_counts(s, rs, data)
+    return {"errors_before": e["before"], "errors_after": e["after"], "bases": e["bases"],
+            "error_rate_before": e["before"] / e["bases"], "error_rate_after": e["after"] / e["bases"],
+            "fixed": e["fixed"], "miscorrected": e["miscorrected"]}
 
 
 def main():
@@ -185,34 +258,37 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="se_5mbp_150bp_30x_dup")
+    ap.add_argument("--workload", default="se_50mbp_100to250bp_30x_rep")
+    ap.add_argument("--pe-workload", default="pe_20mbp_2x150bp_30x_rep")
     ap.add_argument("--seed", type=int, default=12345)
     ap.add_argument("--no-quality", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=300000)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-timeout", type=int, default=600)
     ap.add_argument("--batch-size", type=int, default=0)
     ap.add_argument("--engine", choices=["auto", "staged", "fused"], default="auto",
-                    help="auto = staged kernels; fused = one persistent kernel")
+                    help="auto = the autotuner's choice; staged = per-stage kernels; fused = one persistent kernel")
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-candidates", action="store_true",
                     help="skip the candidate-correction leg (--candidateCorrection)")
+    ap.add_argument("--no-fastq", action="store_true", help="skip the FASTQ parse / merge leg")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --engine/--waves-per-cu as given instead of the autotuner's choice")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (profiler byte-counter calibration)")
     ap.add_argument("--cpu-baseline-only", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--tables", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
 
     if args.cpu_baseline_only:
         s = sim.workload(args.workload, seed=args.seed)
         use_q = not args.no_quality
         rs = reads.storage_from_sim(s, use_qualities=use_q)
-        copts = dict(use_quality_scores=int(use_q), paired=int(rs.paired))
-        if rs.max_length >= 160:
-            copts["kmerlength"] = 20
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        print(json.dumps(cpu_baseline(rs, copts, args.cpu_sample, threads)), flush=True)
+        del s
+        copts = dict(use_quality_scores=int(use_q), paired=int(rs.paired), kmerlength=kmer_length(rs))
+        threads = args.cpu_threads or default_threads()
+        print(json.dumps(cpu_baseline(rs, copts, args.cpu_sample, threads, args.tables)), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the correction stage shards "
@@ -226,11 +302,9 @@ def main():
     s = sim.workload(args.workload, seed=args.seed)
     use_q = not args.no_quality
     rs = reads.storage_from_sim(s, use_qualities=use_q)
-    opts = dict(use_quality_scores=int(use_q), paired=int(rs.paired), batch_size=args.batch_size,
-                engine={"auto": 0, "staged": 1, "fused": 2}[args.engine], waves_per_cu=args.waves_per_cu)
-    if rs.max_length >= 160:
-        opts["kmerlength"] = 20
-    log(f"workload {args.workload}: {rs.n} reads, max length {rs.max_length}, simulated in {time.time() - t0:.1f} s")
+    opts = engine_opts(rs, use_q, args)
+    log(f"workload {args.workload}: {rs.n} reads, lengths {int(rs.lengths.min())}..{rs.max_length}, "
+        f"simulated in {time.time() - t0:.1f} s")
     tuned = None
     if not args.no_autotune and args.engine == "auto":
         base = {k: v for k, v in opts.items() if k not in ("engine", "waves_per_cu")}
@@ -238,110 +312,160 @@ def main():
         opts.update(tuned["choice"])
         log(f"autotune choice {tuned['choice']}")
 
-    eng = engine.CareEngine(0, **opts)
     if args.calib_copy_mb > 0:
         calibration_copy(args.calib_copy_mb << 20)
         log(f"calibration copy of {args.calib_copy_mb} MiB done")
-    t1 = time.time()
+
+    # end to end once: reads to the device, tables (signatures, sort-based build, every
+    # read's own bucket), one pass
+    eng = engine.CareEngine(0, **opts)
+    t1 = time.perf_counter()
     eng.set_reads(rs)
-    t2 = time.time()
+    t2 = time.perf_counter()
     maps = eng.build_tables()
-    t3 = time.time()
-    log(f"reads to device {1e3 * (t2 - t1):.1f} ms, tables ({maps} maps) {1e3 * (t3 - t2):.1f} ms")
+    t3 = time.perf_counter()
+    eng.correct()
+    t4 = time.perf_counter()
+    e2e = {"upload_ms": round(1e3 * (t2 - t1), 2), "tables_ms": round(1e3 * (t3 - t2), 2),
+           "first_pass_ms": round(1e3 * (t4 - t3), 2), "total_ms": round(1e3 * (t4 - t1), 2),
+           "reads_per_s": round(rs.n / (t4 - t1), 1)}
+    log(f"end to end: {json.dumps(e2e)}")
     ms_per_step, nrec, data, t = timed_passes(eng, args.steps, args.warmup, log, "plain")
+    M = int(eng.opts.num_hash_maps)
+    cpu_tables = None
+    if args.cpu_sample > 0:
+        fd, cpu_tables = tempfile.mkstemp(prefix="care_tables_", suffix=".bin")
+        os.close(fd)
+        eng.save_tables(cpu_tables)
     eng.close()
     value = rs.n / (ms_per_step / 1e3)
 
-    W = max(1, (rs.max_length + 31) // 32)
-    per_launch = algorithmic_bytes(t, rs, W, eng.opts.num_hash_maps)
     batches = max(1, t["batches"])
     if t["fused_kernel_ms"] > 0:
         kernel_ms = {"k_fused": t["fused_kernel_ms"] / batches}
     else:
         kernel_ms = {"k_candidates": t["cand_kernel_ms"] / batches, "k_align": t["align_kernel_ms"] / batches,
                      "k_msa": t["msa_kernel_ms"] / batches}
+    ab = algorithmic_bytes(t, rs, use_q, M)
     dominant = max(kernel_ms, key=kernel_ms.get)
-    achieved = per_launch[dominant] / (kernel_ms[dominant] / 1e3) / 1e9
+    per_launch = (ab["pass"] if dominant == "k_fused" else ab[dominant]) / batches
+    achieved = per_launch / (kernel_ms[dominant] / 1e3) / 1e9
+    counters = measured(args.workload, dominant, "record") or {}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload, dominant),
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured(args.workload, dominant),
                 "kernel": dominant, "avg_launch_ms": round(kernel_ms[dominant], 4),
-                "algorithmic_bytes_per_launch": int(per_launch[dominant]), "launches_per_step": int(batches),
-                "limiter": LIMITERS.get(dominant, "instruction issue and latency, not HBM bandwidth") +
-                           " (counters: profiles/pmc_staged_se_5mbp_150bp_1M.md)"}
-    plain_err = evaluate.error_counts(s, rs, data)
+                "algorithmic_bytes_per_launch": int(per_launch), "launches_per_step": int(batches),
+                "bytes_model": "per anchor: own row+length, 48 x 16 B slots, 4 B per bucket value, row+length per "
+                               "candidate, qualities of anchor and candidates, record bytes (split by kernel)",
+                "issue": counters.get("issue"), "counters_source": counters.get("source")}
+    pass_frac = ab["pass"] / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS
+    path_roofline = {"algorithmic_bytes_per_step": int(ab["pass"]),
+                     "achieved": round(ab["pass"] / (ms_per_step / 1e3) / 1e9, 2), "unit": "GB/s",
+                     "frac": round(pass_frac, 5)}
+    plain_err = error_line(s, rs, data)
     log(f"GPU plain: {value:.0f} reads/s, {ms_per_step:.1f} ms/pass; kernel avg ms {json.dumps(kernel_ms)}; "
-        f"errors {plain_err}")
+        f"errors {plain_err['errors_after']}")
 
-    # FASTQ text -> device read storage, and records -> corrected FASTQ text (the stages
-    # either side of the correction pass; each timed once)
-    texts = sim.fastq_texts(s)
-    text_bytes = sum(len(t) for t in texts)
-    eng = engine.CareEngine(0, **opts)
-    ta = time.time()
-    eng.set_reads_fastq(*texts)
-    tb = time.time()
-    outs = [eng.write_fastq(data, f) for f in range(len(texts))]
-    tc = time.time()
-    same_storage = bool(np.array_equal(eng.inspect_reads()["enc"], rs.enc))
-    eng.write_fastq(data, 0)
-    tio = eng.timings()
-    eng.close()
-    out_bytes = sum(len(o) for o in outs)
-    io_line = {
-        "fastq_bytes": text_bytes, "storage_equal_to_array_path": same_storage,
-        "parse_device_ms": round(tio["fastq_device_ms"], 3), "parse_wall_ms": round(1e3 * (tb - ta), 2),
-        "parse_device_gbps": round(text_bytes / (tio["fastq_device_ms"] / 1e3) / 1e9, 2),
-        "merge_output_bytes": out_bytes, "merge_device_ms_file0": round(tio["merge_device_ms"], 3),
-        "merge_wall_ms_all_files": round(1e3 * (tc - tb), 2),
-    }
-    log(f"FASTQ parse {text_bytes / 1e6:.1f} MB: device {tio['fastq_device_ms']:.2f} ms, wall {1e3 * (tb - ta):.1f} ms; "
-        f"merge device {tio['merge_device_ms']:.2f} ms (file 0), wall {1e3 * (tc - tb):.1f} ms")
-    del texts, outs
-
-    learned_line = None
+    learned_line = exact_line = None
+    params = None
     if not args.no_learned and os.path.exists(learned.DEFAULT_CLASSIFIER) and rs.max_length <= 340:
         params, _ = learned.load_classifier()
-        eng = engine.CareEngine(0, correction_type=1, **opts)
-        eng.set_reads(rs)
-        eng.build_tables()
-        eng.set_classifier(params)
-        lms, lrec, ldata, lt = timed_passes(eng, args.steps, args.warmup, log, "learned")
-        eng.close()
-        lerr = evaluate.error_counts(s, rs, ldata)
+        lo = dict(opts, correction_type=1, seeded_alignment=1, engine=1)
+        lo.pop("use_graphs", None)
+        lms, lrec, ldata, lt = run_leg(rs, lo, args.steps, args.warmup, log, "learned", params)
         lvalue = rs.n / (lms / 1e3)
         learned_line = {"value": round(lvalue, 1), "unit": "reads/s", "ms_per_step": round(lms, 3),
                         "vs_plain": round(lvalue / value, 4), "records": lrec,
-                        "error_rate_after": lerr["after"] / lerr["bases"], "errors_after": lerr["after"],
-                        "fixed": lerr["fixed"], "miscorrected": lerr["miscorrected"],
+                        "form": "MLP classifier seam (bf16, fallback below confidence 0.9) on seeded alignment",
+                        "kernel_avg_ms": {"k_candidates": round(lt["cand_kernel_ms"] / max(1, lt["batches"]), 4),
+                                          "k_align_seeded": round(lt["align_kernel_ms"] / max(1, lt["batches"]), 4),
+                                          "k_msa": round(lt["msa_kernel_ms"] / max(1, lt["batches"]), 4)},
                         "classifier": os.path.relpath(learned.DEFAULT_CLASSIFIER, ROOT)}
-        log(f"GPU learned: {lvalue:.0f} reads/s, {lms:.1f} ms/pass; errors {lerr}")
+        learned_line.update(error_line(s, rs, ldata))
+        log(f"GPU learned: {lvalue:.0f} reads/s ({lvalue / value:.3f}x plain), errors {learned_line['errors_after']}")
+        eo = dict(opts, correction_type=1)
+        ems, erec, edata, _ = run_leg(rs, eo, args.steps, args.warmup, log, "learned_exact", params)
+        evalue = rs.n / (ems / 1e3)
+        exact_line = {"value": round(evalue, 1), "ms_per_step": round(ems, 3), "vs_plain": round(evalue / value, 4),
+                      "form": "MLP classifier seam on the reference's shift scan"}
+        exact_line.update(error_line(s, rs, edata))
+        del ldata, edata
 
     # candidate correction (opt-in in the reference): the same passes with HQ anchors also
     # correcting their candidates; records then hold several results per read
     cand_line = None
     if not args.no_candidates and rs.max_length <= 340:
         copts = {k: v for k, v in opts.items() if k not in ("engine", "waves_per_cu", "use_graphs")}
-        eng = engine.CareEngine(0, correct_candidates=1, engine=1, **copts)
-        eng.set_reads(rs)
-        eng.build_tables()
-        cms, crec, cdata, ct = timed_passes(eng, args.steps, args.warmup, log, "candidates")
-        eng.close()
+        cms, crec, cdata, ct = run_leg(rs, dict(copts, correct_candidates=1, engine=1), args.steps, args.warmup,
+                                       log, "candidates")
         cvalue = rs.n / (cms / 1e3)
         cand_line = {"value": round(cvalue, 1), "unit": "reads/s", "ms_per_step": round(cms, 3),
                      "vs_plain": round(cvalue / value, 4), "records": crec, "record_bytes": len(cdata)}
-        log(f"GPU candidates: {cvalue:.0f} reads/s, {cms:.1f} ms/pass, {crec} records")
         del cdata
 
+    # FASTQ text -> device read storage, and records -> corrected FASTQ text (the stages
+    # either side of the correction pass; each timed once)
+    io_line = None
+    if not args.no_fastq:
+        texts = sim.fastq_texts(s)
+        text_bytes = int(sum(len(x) for x in texts))
+        eng = engine.CareEngine(0, **opts)
+        try:
+            ta = time.perf_counter()
+            eng.set_reads_fastq(*texts)
+            tb = time.perf_counter()
+            outs = [eng.write_fastq(data, f) for f in range(len(texts))]
+            tc = time.perf_counter()
+            same_storage = bool(np.array_equal(eng.inspect_reads()["enc"], rs.enc))
+            tio = eng.timings()
+        finally:
+            eng.close()
+        out_bytes = sum(len(o) for o in outs)
+        io_line = {
+            "fastq_bytes": text_bytes, "storage_equal_to_array_path": same_storage,
+            "parse_device_ms": round(tio["fastq_device_ms"], 3), "parse_wall_ms": round(1e3 * (tb - ta), 2),
+            "parse_device_gbps": round(text_bytes / (tio["fastq_device_ms"] / 1e3) / 1e9, 2),
+            "merge_output_bytes": out_bytes, "merge_device_ms_last_file": round(tio["merge_device_ms"], 3),
+            "merge_wall_ms_all_files": round(1e3 * (tc - tb), 2),
+        }
+        log(f"FASTQ parse {text_bytes / 1e6:.1f} MB: device {tio['fastq_device_ms']:.2f} ms; "
+            f"merge wall {1e3 * (tc - tb):.1f} ms")
+        del texts, outs
+
+    # the paired workload: plain and learned passes
+    pe_line = None
+    if args.pe_workload:
+        sp = sim.workload(args.pe_workload, seed=args.seed)
+        rp = reads.storage_from_sim(sp, use_qualities=use_q)
+        po = dict(use_quality_scores=int(use_q), paired=1, kmerlength=kmer_length(rp), engine=1)
+        pms, prec, pdata, pt = run_leg(rp, po, args.steps, args.warmup, log, "pe plain")
+        pe_line = {"workload": args.pe_workload, "reads": rp.n, "plain": {
+            "value": round(rp.n / (pms / 1e3), 1), "ms_per_step": round(pms, 3), "records": prec,
+            "pairs_aligned": int(pt["candidates"]), "bucket_values": int(pt["bucket_values"])}}
+        pe_line["plain"].update(error_line(sp, rp, pdata))
+        if params is not None:
+            lms, _, ldata, _ = run_leg(rp, dict(po, correction_type=1, seeded_alignment=1), args.steps, args.warmup,
+                                       log, "pe learned", params)
+            pe_line["learned"] = {"value": round(rp.n / (lms / 1e3), 1), "ms_per_step": round(lms, 3),
+                                  "vs_plain": round(pms / lms, 4)}
+            pe_line["learned"].update(error_line(sp, rp, ldata))
+        del sp, rp
+
     cpu = None
+    threads = args.cpu_threads or default_threads()
     if args.cpu_sample > 0:
-        log(f"CPU baseline, {args.cpu_sample} anchors")
-        cpu = cpu_baseline_child(args)
-        if cpu is not None and "records_sha256" in cpu:
+        log(f"CPU baseline, {args.cpu_sample} anchors on {threads} threads")
+        try:
+            cpu = cpu_baseline_child(args, cpu_tables, threads, log)
+        finally:
+            os.unlink(cpu_tables)
+        cpu.update(host_info(threads))
+        if "records_sha256" in cpu:
             # the plain pass's records of the same anchors against the CPU run's
             n = baseline_anchors(args.cpu_sample, rs, rs.paired)
             gcount, gdigest = records_digest(data, n)
             cpu["records_identical_to_gpu"] = bool(gcount == cpu["records"] and gdigest == cpu["records_sha256"])
-            log(f"CPU baseline records of the first {n} anchors: {cpu['records']}, identical to the GPU's: "
+            log(f"CPU baseline {cpu['value']:.0f} reads/s; records of the first {n} anchors identical to the GPU's: "
                 f"{cpu['records_identical_to_gpu']}")
 
     line = {
@@ -358,26 +482,33 @@ def main():
         "dtype": "int32+fp32",
         "data": "synthetic (seeded simulator)",
         "config": {"workload": args.workload, "reads": rs.n, "read_length": [int(rs.lengths.min()), rs.max_length],
-                   "coverage": 30, "k": int(eng.opts.kmerlength) or (20 if rs.max_length < 160 else 32),
-                   "maps": int(eng.opts.num_hash_maps), "quality_scores": use_q, "paired": bool(rs.paired),
-                   "path": "plain", "engine": "fused" if "k_fused" in kernel_ms else "staged",
-                   "schedule": {k: opts[k] for k in ("engine", "waves_per_cu", "align_variant", "use_graphs") if k in opts}},
-        "autotune": tuned["timings_ms"] if tuned else None,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-        "error_rate_before": plain_err["before"] / plain_err["bases"],
-        "error_rate_after": plain_err["after"] / plain_err["bases"],
-        "errors": plain_err,
+                   "coverage": 30, "k": kmer_length(rs), "maps": M, "quality_scores": use_q, "paired": bool(rs.paired),
+                   "path": "plain (bit-exact to the reference)",
+                   "engine": "fused" if "k_fused" in kernel_ms else "staged",
+                   "schedule": {k: opts[k] for k in ("engine", "waves_per_cu", "align_variant", "use_graphs", "msa_tile")
+                                if k in opts}},
         "learned": learned_line,
+        "learned_exact": exact_line,
+        "roofline": roofline,
+        "path_roofline": path_roofline,
+        "cpu_baseline": cpu,
+        "end_to_end": e2e,
+        "table_build_ms": e2e["tables_ms"],
+        "error_rate_before": plain_err["error_rate_before"],
+        "error_rate_after": plain_err["error_rate_after"],
+        "errors": plain_err,
+        "pe_workload": pe_line,
         "candidate_correction": cand_line,
-        "records": nrec,
         "fastq_io": io_line,
-        "table_build_ms": round(1e3 * (t3 - t2), 2),
+        "autotune": tuned["timings_ms"] if tuned else None,
+        "records": nrec,
         "stage_ms": {"candidates": round(t["candidates_ms"], 3), "align": round(t["align_ms"], 3),
                      "msa": round(t["msa_ms"], 3), "output": round(t["output_ms"], 3)},
         "kernel_avg_ms": {k: round(v, 4) for k, v in kernel_ms.items()},
         "pairs_aligned": int(t["candidates"]),
-        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "bucket_values": int(t["bucket_values"]),
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu and "value" in cpu else None,
+        "gpu_end_to_end_vs_cpu": round(e2e["reads_per_s"] / cpu["value"], 1) if cpu and "value" in cpu else None,
     }
     print(json.dumps(line), flush=True)
 

@@ -131,6 +131,9 @@ def ref_lib():
         vp, sz, i32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int
         lib.ref_create.argtypes = [ctypes.POINTER(RefOpts), sz, vp, sz, vp, vp, sz, vp]
         lib.ref_create.restype = vp
+        lib.ref_create_loaded.argtypes = [ctypes.POINTER(RefOpts), sz, vp, sz, vp, vp, sz, vp, ctypes.c_char_p,
+                                          ctypes.POINTER(ctypes.c_double)]
+        lib.ref_create_loaded.restype = vp
         lib.ref_destroy.argtypes = [vp]
         lib.ref_num_maps.argtypes = [vp]
         lib.ref_correct.argtypes = [vp, i32, ctypes.c_longlong]
@@ -314,7 +317,9 @@ class CareCpu:
 class CareRef:
     """The reference engine (compiled from source) over the same ReadStorage."""
 
-    def __init__(self, rs, threads: int = 8, **opts):
+    def __init__(self, rs, threads: int = 8, tables: str | None = None, **opts):
+        """tables: a minhasher file in the reference's format, read by the reference's own
+        loader instead of building the tables (self.load_seconds = its wall time)."""
         self.lib = ref_lib()
         self.rs = rs
         opts.setdefault("paired", int(rs.paired))
@@ -323,9 +328,15 @@ class CareRef:
         opts.setdefault("fixed_number_of_reads", 0)
         self.opts = _opts(RefOpts, **opts)
         q = rs.quals
-        self.h = self.lib.ref_create(ctypes.byref(self.opts), rs.n, rs.enc.ctypes.data, rs.pitch_ints,
-                                     rs.lengths.ctypes.data, _p(q), 0 if q is None else q.shape[1],
-                                     rs.ambiguous.ctypes.data)
+        args = (ctypes.byref(self.opts), rs.n, rs.enc.ctypes.data, rs.pitch_ints, rs.lengths.ctypes.data, _p(q),
+                0 if q is None else q.shape[1], rs.ambiguous.ctypes.data)
+        self.load_seconds = None
+        if tables is None:
+            self.h = self.lib.ref_create(*args)
+        else:
+            secs = ctypes.c_double(0.0)
+            self.h = self.lib.ref_create_loaded(*args, os.fsencode(tables), ctypes.byref(secs))
+            self.load_seconds = secs.value
         if not self.h:
             raise RuntimeError("ref_create failed")
 

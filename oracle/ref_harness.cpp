@@ -214,8 +214,39 @@ static void fill_program_options(care::ProgramOptions& po, const ref_opts* o, in
 
 // Creates the read storage and the reference minhasher (tables built by the
 // reference's own construction routine).
+static ref_ctx* ref_storage(const ref_opts* o, std::size_t n, const unsigned int* enc, std::size_t pitchInts,
+                            const int* lens, const char* quals, std::size_t qualPitch, const std::uint8_t* ambig);
+
 ref_ctx* ref_create(const ref_opts* o, std::size_t n, const unsigned int* enc, std::size_t pitchInts,
                     const int* lens, const char* quals, std::size_t qualPitch, const std::uint8_t* ambig){
+    auto* c = ref_storage(o, n, enc, pitchInts, lens, quals, qualPitch, ambig);
+    c->mh = buildMinhasherSequential(c->po, c->rs);
+    return c;
+}
+
+// The same storage with the minhasher read from a table file by the reference's own
+// loader (OrdinaryCpuMinhasher::loadFromStream, include/ordinaryminhasher.hpp:318-339)
+// instead of being built; *seconds = the load's wall time. Null when the file does not
+// give every map.
+ref_ctx* ref_create_loaded(const ref_opts* o, std::size_t n, const unsigned int* enc, std::size_t pitchInts,
+                           const int* lens, const char* quals, std::size_t qualPitch, const std::uint8_t* ambig,
+                           const char* tablesPath, double* seconds){
+    auto* c = ref_storage(o, n, enc, pitchInts, lens, quals, qualPitch, ambig);
+    const auto t0 = std::chrono::steady_clock::now();
+    std::ifstream is(tablesPath, std::ios::binary);
+    if(!is){ delete c; return nullptr; }
+    auto mh = std::make_unique<care::OrdinaryCpuMinhasher>(c->rs.getNumberOfReads(),
+                                                           calculateResultsPerMapThreshold(c->po.estimatedCoverage),
+                                                           c->po.kmerlength, c->po.hashtableLoadfactor);
+    const int loaded = mh->loadFromStream(is, c->po.numHashFunctions);
+    if(loaded != c->po.numHashFunctions){ delete c; return nullptr; }
+    c->mh = std::move(mh);
+    if(seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return c;
+}
+
+static ref_ctx* ref_storage(const ref_opts* o, std::size_t n, const unsigned int* enc, std::size_t pitchInts,
+                            const int* lens, const char* quals, std::size_t qualPitch, const std::uint8_t* ambig){
     auto* c = new ref_ctx;
     c->rs.n = n;
     c->rs.pitchInts = pitchInts;
@@ -237,7 +268,6 @@ ref_ctx* ref_create(const ref_opts* o, std::size_t n, const unsigned int* enc, s
         }
     }
     fill_program_options(c->po, o, c->rs.maxLen);
-    c->mh = buildMinhasherSequential(c->po, c->rs);
     return c;
 }
 

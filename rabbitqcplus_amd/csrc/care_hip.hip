@@ -1163,17 +1163,22 @@ k_align_lanes(AlignParams p){
 // below the orientation bound (include/alignmentorientation.hpp:14-92); otherwise it is
 // dropped (orientation 3). The chosen alignment is written as the scan's would be.
 //
-// One wavefront per anchor, one lane per candidate. Mismatches are counted on the HiLo
-// planes in LDS: the anchor's, and per lane its candidate's planes of the chosen strand,
-// each zero padded to 2*WM+2 words so that a shifted read of WM words never leaves them.
+// One wavefront per anchor, one lane per candidate. The anchor's HiLo planes sit in LDS
+// between zero words, stored as (word i, word i+1) pairs so that one 64-bit read feeds
+// the funnel shift (a word with any overlap position has its anchor bit offset in
+// (-32, La), so pairs -1..W-1 suffice); a candidate word j (its bases 32j..32j+31 in the
+// chosen strand, loaded into registers) is compared with the anchor bits at offset
+// 32j + shift, of either sign, under the mask of the overlap
+// [max(0, -shift), min(Lc, La - shift)). The map search reads two maps' buckets per load
+// and stops at the first hit (a true overlap shares most maps).
 template<int WM>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(8)))
 k_align_seeded(AlignParams p){
-    constexpr int P = 2 * WM + 2;
-    __shared__ uint32_t aPl[2][P];
-    __shared__ uint32_t cPl[2][kWave][P];
-    __shared__ uint2 aBucket[kMaxMaps];
-    __shared__ uint16_t aSeed[kMaxMaps];
+    constexpr int PAD = 1;
+    constexpr int P = WM + 2;
+    __shared__ uint2 aPl[2][P];
+    __shared__ uint4 aBucket[kMaxMaps / 2];     // (offset, count) of maps 2q, 2q+1; count 0 = not kept
+    __shared__ int aSeed[kMaxMaps];
     if(int(blockIdx.x) >= p.B) return;
     const int i = batchAnchor(p.order, p.B);
     const uint32_t beg = p.offsets[i];
@@ -1182,88 +1187,77 @@ k_align_seeded(AlignParams p){
     const int lane = threadIdx.x;
     const uint64_t a = p.b0 + i;
     const int W = p.W;
+    const int M = p.M;
     const int La = p.len[a];
     const uint32_t* arow = p.hilo + a * uint64_t(4 * W);
     for(int q = lane; q < 2 * P; q += kWave){
-        const int pl = q / P, w = q % P;
-        aPl[pl][w] = w < W ? arow[pl * W + w] : 0u;
+        const int pl = q / P, x = q % P;
+        const int w0 = x - PAD, w1 = x + 1 - PAD;
+        const uint32_t lo = (w0 >= 0 && w0 < W) ? arow[pl * W + w0] : 0u;
+        const uint32_t hi = (w1 >= 0 && w1 < W) ? arow[pl * W + w1] : 0u;
+        aPl[pl][x] = make_uint2(hi, lo);        // .y = word x - 1, .x = word x
     }
-    if(lane < p.M){
-        const uint2 b = p.bucketOf[a * uint64_t(p.M) + uint64_t(lane)];
-        aBucket[lane] = (b.y > 0 && b.y <= uint32_t(p.thr)) ? b : make_uint2(0u, 0u);
-        aSeed[lane] = p.seeds[a * uint64_t(p.M) + uint64_t(lane)];
+    if(lane < kMaxMaps){
+        uint2 b = make_uint2(0u, 0u);
+        int sd = 0;
+        if(lane < M){
+            b = p.bucketOf[a * uint64_t(M) + uint64_t(lane)];
+            if(b.y == 0 || b.y > uint32_t(p.thr)) b = make_uint2(0u, 0u);
+            sd = p.seeds[a * uint64_t(M) + uint64_t(lane)];
+        }
+        reinterpret_cast<uint2*>(aBucket)[lane] = b;
+        aSeed[lane] = sd;
     }
     __syncthreads();
     const int mo = max(p.minOverlap, int(float(La) * p.minOverlapRatio));
     for(int c0 = 0; c0 < n; c0 += kWave){
         const int c = c0 + lane;
-        const bool valid = c < n;
-        uint32_t id = 0;
-        int Lc = 0, shift = 0, orient = 3;
-        if(valid){
-            id = p.ids[beg + c];
-            Lc = p.len[id];
-            // first map whose kept bucket holds both reads (equal offset and count)
-            const uint2* cb = p.bucketOf + uint64_t(id) * uint64_t(p.M);
-            int m = 0;
-            for(; m < p.M; m++){
-                const uint2 ab = aBucket[m];
-                if(ab.y == 0) continue;
-                const uint2 x = cb[m];
-                if(x.x == ab.x && x.y == ab.y) break;
-            }
-            if(m < p.M){
-                const int sa = aSeed[m], sc = p.seeds[uint64_t(id) * uint64_t(p.M) + uint64_t(m)];
-                const int pa = sa >> 1, pc = sc >> 1;
-                if((sa & 1) == (sc & 1)){ orient = 1; shift = pa - pc; }
-                else{ orient = 2; shift = pa - (Lc - p.k - pc); }
-            }
+        if(c >= n) break;
+        const uint32_t id = p.ids[beg + c];
+        const int Lc = p.len[id];
+        // first map whose kept bucket holds both reads (equal offset and count); rows of
+        // M (even) buckets are 16-byte aligned
+        const uint2* cbk = p.bucketOf + uint64_t(id) * uint64_t(M);
+        int m = M;
+        for(int q = 0; 2 * q < M; q++){
+            const uint4 ab = aBucket[q];
+            uint4 x;
+            if(2 * q + 1 < M) x = *reinterpret_cast<const uint4*>(cbk + 2 * q);
+            else { const uint2 y = cbk[2 * q]; x = make_uint4(y.x, y.y, 0u, 0u); }
+            if(ab.y != 0 && x.x == ab.x && x.y == ab.y){ m = 2 * q; break; }
+            if(ab.w != 0 && x.z == ab.z && x.w == ab.w){ m = 2 * q + 1; break; }
         }
-        // the candidate's planes of the chosen strand into this lane's LDS rows
-        const uint32_t* crow = p.hilo + uint64_t(id) * uint64_t(4 * W) + (orient == 2 ? 2 * W : 0);
-        #pragma unroll
-        for(int w = 0; w < P; w++){
-            const bool in = valid && orient != 3 && w < W;
-            cPl[0][lane][w] = in ? crow[w] : 0u;
-            cPl[1][lane][w] = in ? crow[W + w] : 0u;
-        }
-        __syncthreads();
-        if(valid){
-            AlignBest b;
-            b.orientation = 3;
-            b.shift = shift;
-            b.overlap = 0;
-            b.nOps = 0;
-            if(orient != 3){
-                const int ov = min(Lc, La - shift) - max(-shift, 0);
-                // X[d + j] against Y[j] for j < ov: X is the read that starts first
-                const uint32_t* Xh = shift >= 0 ? aPl[0] : cPl[0][lane];
-                const uint32_t* Xl = shift >= 0 ? aPl[1] : cPl[1][lane];
-                const uint32_t* Yh = shift >= 0 ? cPl[0][lane] : aPl[0];
-                const uint32_t* Yl = shift >= 0 ? cPl[1][lane] : aPl[1];
-                const int d = shift >= 0 ? shift : -shift;
-                const int q = d >> 5, r = d & 31;
-                int mis = 0;
-                if(ov > 0 && q < WM + 1){
-                    #pragma unroll
-                    for(int j = 0; j < WM; j++){
-                        if(32 * j < ov){
-                            const uint32_t xh = uint32_t((((uint64_t(Xh[q + j]) << 32) | Xh[q + j + 1]) << r) >> 32);
-                            const uint32_t xl = uint32_t((((uint64_t(Xl[q + j]) << 32) | Xl[q + j + 1]) << r) >> 32);
-                            const int left = ov - 32 * j;
-                            const uint32_t mask = left >= 32 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> left);
-                            mis += __popc(((xh ^ Yh[j]) | (xl ^ Yl[j])) & mask);
-                        }
-                    }
+        AlignBest b;
+        b.orientation = 3; b.shift = 0; b.overlap = 0; b.nOps = 0;
+        if(m < M){
+            const int sa = aSeed[m], sc = p.seeds[uint64_t(id) * uint64_t(M) + uint64_t(m)];
+            const int pa = sa >> 1, pc = sc >> 1;
+            const bool same = (sa & 1) == (sc & 1);
+            const int shift = same ? pa - pc : pa - (Lc - p.k - pc);
+            const uint32_t* crow = p.hilo + uint64_t(id) * uint64_t(4 * W) + (same ? 0 : 2 * W);
+            const int cbeg = max(-shift, 0), cend = min(Lc, La - shift);
+            const int ov = cend - cbeg;
+            int mis = 0;
+            #pragma unroll
+            for(int j = 0; j < WM; j++){
+                const int lo = min(max(cbeg - 32 * j, 0), 32), hi = min(max(cend - 32 * j, 0), 32);
+                if(hi > lo){
+                    const uint32_t mask = uint32_t((0xFFFFFFFFull >> lo) & ~(0xFFFFFFFFull >> hi));
+                    const int o = 32 * j + shift;                 // anchor bit offset of word j
+                    const int x = (o >> 5) + PAD, r = o & 31;
+                    const uint2 vh = aPl[0][x], vl = aPl[1][x];
+                    const uint32_t ah = uint32_t((((uint64_t(vh.y) << 32) | vh.x) << r) >> 32);
+                    const uint32_t al = uint32_t((((uint64_t(vl.y) << 32) | vl.x) << r) >> 32);
+                    mis += __popc(((ah ^ crow[j]) | (al ^ crow[W + j])) & mask);
                 }
-                b.overlap = ov;
-                b.nOps = mis;
-                if(ov > 0 && ov >= mo && mis < int(float(ov) * p.maxErrorRate) && float(mis) / float(ov) < p.orientErrorRate)
-                    b.orientation = orient;
             }
-            p.best[beg + c] = b;
+            b.shift = shift;
+            b.overlap = ov;
+            b.nOps = mis;
+            if(ov > 0 && ov >= mo && mis < int(float(ov) * p.maxErrorRate) && float(mis) / float(ov) < p.orientErrorRate)
+                b.orientation = same ? 1 : 2;
         }
-        __syncthreads();
+        p.best[beg + c] = b;
     }
 }
 
@@ -4157,16 +4151,16 @@ __global__ void __launch_bounds__(kNlBlock) k_newline_count(const char* __restri
 }
 
 __global__ void __launch_bounds__(kNlBlock) k_newline_write(const char* __restrict__ t, uint64_t bytes,
-                                                            const uint32_t* __restrict__ blockOff,
-                                                            uint32_t* __restrict__ nl){
+                                                            const uint64_t* __restrict__ blockOff,
+                                                            uint64_t* __restrict__ nl){
     __shared__ int sWave[kNlBlock / 64];
     const uint64_t base = uint64_t(blockIdx.x) * kNlPerBlock + 16ull * threadIdx.x;
     uint32_t mask = base < bytes ? newlineMask(t, base, bytes) : 0u;
     int total;
-    uint32_t o = blockOff[blockIdx.x] + uint32_t(blockExclusiveSum(__popc(mask), sWave, &total));
+    uint64_t o = blockOff[blockIdx.x] + uint64_t(blockExclusiveSum(__popc(mask), sWave, &total));
     while(mask){
         const int k = __ffs(mask) - 1;
-        nl[o++] = uint32_t(base + k);
+        nl[o++] = base + uint64_t(k);
         mask &= mask - 1;
     }
 }
@@ -4201,17 +4195,17 @@ constexpr int kFqBlock = 256;        // 16 records (or reads) per block, one 16-
 // One 16-lane row per record: every lane finds the lines, the lanes count the sequence's
 // non-ACGT bytes over 16-byte slices, and lane 0 writes the record.
 __global__ void __launch_bounds__(kFqBlock) k_fastq_records(
-        const char* __restrict__ t, uint64_t bytes, const uint32_t* __restrict__ nl, uint64_t numNl, uint64_t records,
+        const char* __restrict__ t, uint64_t bytes, const uint64_t* __restrict__ nl, uint64_t numNl, uint64_t records,
         int stride, int slot, uint64_t* __restrict__ hdrStart, uint32_t* __restrict__ hdrLen,
         uint64_t* __restrict__ seqStart, uint64_t* __restrict__ qualStart, int* __restrict__ len,
         uint64_t* __restrict__ ncount, int* __restrict__ err){
     const uint64_t rec = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4;
     const int sub = threadIdx.x & 15;
     if(rec >= records) return;                   // whole rows leave together
-    auto lineStart = [&](uint64_t i){ return i ? uint64_t(nl[i - 1]) + 1 : uint64_t(0); };
+    auto lineStart = [&](uint64_t i){ return i ? nl[i - 1] + 1 : uint64_t(0); };
     // end of the content starting at s on line i
     auto contentEnd = [&](uint64_t i, uint64_t s){
-        uint64_t e = i < numNl ? uint64_t(nl[i]) : bytes;
+        uint64_t e = i < numNl ? nl[i] : bytes;
         if(e > s + 1 && t[e - 1] == '\r') e--;
         return e;
     };
@@ -4308,27 +4302,27 @@ static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t raw, int 
     if(bytes == 0){ f.numNl = 0; f.records = 0; return CARE_OK; }
     const uint32_t nb = uint32_t((bytes + kNlPerBlock - 1) / kNlPerBlock);
     HIPCHECK(f.blockCnt.ensure((size_t(nb) + 1) * 4));
-    HIPCHECK(f.blockOff.ensure((size_t(nb) + 1) * 4));
+    HIPCHECK(f.blockOff.ensure((size_t(nb) + 1) * 8));
     k_newline_count<<<nb, kNlBlock, 0, st>>>(f.text->as<char>(), bytes, f.blockCnt.as<uint32_t>());
     HIPCHECK(hipGetLastError());
     k_fill_u32<<<1, 64, 0, st>>>(f.blockCnt.as<uint32_t>() + nb, 1, 0u);
     size_t tb = 0;
-    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, f.blockCnt.as<uint32_t>(), f.blockOff.as<uint32_t>(),
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, f.blockCnt.as<uint32_t>(), f.blockOff.as<uint64_t>(),
                                               int(nb + 1), st));
     HIPCHECK(ctx->scanTmp.ensure(std::max<size_t>(tb, 256)));
     tb = ctx->scanTmp.bytes;
-    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, f.blockCnt.as<uint32_t>(), f.blockOff.as<uint32_t>(),
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, f.blockCnt.as<uint32_t>(), f.blockOff.as<uint64_t>(),
                                               int(nb + 1), st));
-    uint32_t numNl = 0;
-    HIPCHECK(hipMemcpyAsync(&numNl, f.blockOff.as<uint32_t>() + nb, 4, hipMemcpyDeviceToHost, st));
+    uint64_t numNl = 0;
+    HIPCHECK(hipMemcpyAsync(&numNl, f.blockOff.as<uint64_t>() + nb, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     f.numNl = numNl;
     const uint64_t lines = numNl + 1;
     if(lines % 4) return ctx->fail(CARE_EINVAL, "FASTQ text of %llu lines is not 4-line records",
                                    (unsigned long long)lines);
     f.records = lines / 4;
-    HIPCHECK(f.nl.ensure(std::max<size_t>(size_t(numNl) * 4, 4)));
-    k_newline_write<<<nb, kNlBlock, 0, st>>>(f.text->as<char>(), bytes, f.blockOff.as<uint32_t>(), f.nl.as<uint32_t>());
+    HIPCHECK(f.nl.ensure(std::max<size_t>(size_t(numNl) * 8, 8)));
+    k_newline_write<<<nb, kNlBlock, 0, st>>>(f.text->as<char>(), bytes, f.blockOff.as<uint64_t>(), f.nl.as<uint64_t>());
     HIPCHECK(hipGetLastError());
     return CARE_OK;
 }
@@ -4343,7 +4337,8 @@ static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size
     ctx->hasFastq = false;
     for(int slot = 0; slot < (paired ? 2 : 1); slot++){
         const size_t raw = slot ? bytes2 : bytes1;
-        if(raw > size_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "FASTQ text above 2 GiB in one call");
+        // newline blocks of 4 KiB are counted in 32 bits and read indices are 32-bit
+        if(raw / kNlPerBlock >= size_t(INT_MAX) - 1) return ctx->fail(CARE_EINVAL, "FASTQ text too large for one call");
         ctx->fqRaw[slot] = raw;
         HIPCHECK(ctx->fqText[slot].ensure((raw + 15) / 16 * 16 + 16));
         if(raw) HIPCHECK(hipMemcpyAsync(ctx->fqText[slot].p, slot ? fastq2 : fastq1, raw, hipMemcpyHostToDevice, st));
@@ -4372,7 +4367,7 @@ static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size
     for(int fi = 0; fi < (paired ? 2 : 1); fi++){
         ParsedFile& f = fi ? f2 : f1;
         k_fastq_records<<<dim3(uint32_t((f.records * 16 + kFqBlock - 1) / kFqBlock)), kFqBlock, 0, st>>>(
-            f.text->as<char>(), f.bytes, f.nl.as<uint32_t>(), f.numNl, f.records, stride, fi,
+            f.text->as<char>(), f.bytes, f.nl.as<uint64_t>(), f.numNl, f.records, stride, fi,
             ctx->fqHdr.as<uint64_t>(), ctx->fqHdrLen.as<uint32_t>(), ctx->fqSeq.as<uint64_t>(), ctx->fqQual.as<uint64_t>(),
             tmp.lens.as<int>(), tmp.ncount.as<uint64_t>(), tmp.err.as<int>());
         HIPCHECK(hipGetLastError());

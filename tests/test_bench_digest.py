@@ -27,3 +27,19 @@ def test_records_digest_agrees_across_runs():
     flipped = bytearray(d_all)
     flipped[12] ^= 1                     # first payload byte of the first record (a read below n)
     assert bench.records_digest(bytes(flipped), n) != a
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+def test_reference_with_loaded_tables_gives_the_same_records(tmp_path):
+    """The CPU baseline's reference reads its tables with its own loader from a table file
+    (the format the engine writes): the records equal those of the reference's own build."""
+    s, rs, o = helpers.case("se_q")
+    ref = oracle.CareRef(rs, threads=8, **o)
+    path = str(tmp_path / "tables.bin")
+    assert ref.lib.ref_save_tables(ref.h, path.encode()) == 0
+    want = ref.correct(8, 5000)
+    ref.close()
+    ref = oracle.CareRef(rs, threads=8, tables=path, **o)
+    assert ref.load_seconds is not None
+    assert ref.correct(8, 5000) == want
+    ref.close()
