@@ -266,6 +266,13 @@ int care_mlp_get_params(care_mlp* h, float* params, size_t count);
 int care_mlp_reset_optimizer(care_mlp* h);              /* Adam moments and step count to 0 */
 /* Logits z[n] of rows X[n][16] (float). */
 int care_mlp_forward(care_mlp* h, const float* X, size_t n, float* z);
+/* care_mlp_forward without any allocation: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The device copies of X and z live in a fixed scratch
+ * array of the library (4.25 MiB per GPU, held until the process ends), so n is at most
+ * 65536 rows per call (CARE_EINVAL beyond). Calls that overlap on one GPU share that
+ * array and race; hidden activations are not kept (no backward after this call). */
+int care_mlp_forward_static(care_mlp* h, const float* X, size_t n, float* z);
 /* Mean binary cross-entropy of sigmoid(z) against y[n] (0/1) and its gradient grad[CARE_MLP_PARAMS]. */
 int care_mlp_loss_grad(care_mlp* h, const float* X, const float* y, size_t n, float* loss, float* grad);
 /* Training rows, copied to the device. */

@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <string>
@@ -58,7 +59,8 @@ struct DevBuf {
 
 // ---------------------------------------------------------------- kernels
 
-// Forward of one tile: z[r], and the post-ReLU activations H1, H2 kept for backward.
+// Forward of one tile: z[r], and the post-ReLU activations H1, H2 kept for backward
+// (not written when H1out is null: inference only).
 __global__ void __launch_bounds__(kThreads)
 k_forward(const float* __restrict__ P, const float* __restrict__ X, int n, float* __restrict__ z,
           float* __restrict__ H1out, float* __restrict__ H2out){
@@ -104,7 +106,7 @@ k_forward(const float* __restrict__ P, const float* __restrict__ X, int n, float
     #pragma unroll
     for(int u = 0; u < 16; u++) sH2[r][j0 + u] = acc[u] > 0.0f ? acc[u] : 0.0f;
     __syncthreads();
-    if(r0 + r < n){
+    if(H1out && r0 + r < n){
         float* h1 = H1out + size_t(r0 + r) * kH + j0;
         float* h2 = H2out + size_t(r0 + r) * kH + j0;
         #pragma unroll
@@ -295,6 +297,35 @@ __global__ void k_fill(float* v, int n, float x){
 
 }  // namespace
 
+// Device scratch of care_mlp_forward_static, the allocation-free inference call: the
+// input rows and the logits of one call (at most kStaticRows rows) live in this
+// file-scope array, which every GPU the library runs on holds until the process ends.
+// Host code never passes the array's name to a kernel or a copy (in host code it is a
+// stand-in, not a device address): the device address of each GPU's copy is taken once
+// per device, at care_mlp_create with that device current, and kept in gScratch.
+constexpr size_t kStaticRows = 65536;
+constexpr size_t kStaticXBytes = kStaticRows * kIn * sizeof(float);
+constexpr size_t kStaticBytes = kStaticXBytes + kStaticRows * sizeof(float);
+static __device__ __align__(256) uint8_t scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex gScratchMutex;
+void* gScratch[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current), resolved once.
+hipError_t scratchAddress(int device, void** out){
+    if(device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if(!gScratch[device]){
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if(e != hipSuccess) return e;
+        gScratch[device] = ptr;
+    }
+    *out = gScratch[device];
+    return hipSuccess;
+}
+
 // The C++ forward/backward Function: forward(X) -> z saves what backward needs;
 // backward(dz) -> dL/dparams given dL/dz. Device pointers, one stream.
 class MlpFunction {
@@ -335,6 +366,7 @@ struct care_mlp {
     DevBuf params, grad, m, v, z, dz, lossPart, lossOut, X, y, Xall, yall, Xb, yb, perm, epochLoss;
     size_t nData = 0;
     uint64_t step = 0;
+    void* scratch = nullptr;       // this device's address of the static scratch array
 
     int fail(int code, const char* fmt, ...){
         char buf[512];
@@ -415,6 +447,8 @@ static int care_mlp_create_body(int device, care_mlp** out){
     k_fill<<<(kP + 255) / 256, 256, 0, h->stream>>>(h->m.as<float>(), kP, 0.0f);
     k_fill<<<(kP + 255) / 256, 256, 0, h->stream>>>(h->v.as<float>(), kP, 0.0f);
     if(hipStreamSynchronize(h->stream) != hipSuccess){ delete h; return CARE_EHIP; }
+    // the device address of the static scratch of care_mlp_forward_static (device current)
+    if(scratchAddress(device, &h->scratch) != hipSuccess){ care_mlp_destroy(h); return CARE_EHIP; }
     *out = h;
     return CARE_OK;
 }
@@ -466,6 +500,24 @@ static int care_mlp_forward_body(care_mlp* h, const float* X, size_t n, float* z
     if(n) MCHECK(hipMemcpyAsync(h->X.p, X, n * kIn * 4, hipMemcpyHostToDevice, st));
     MCHECK(h->fn.forward(h->params.as<float>(), h->X.as<float>(), int(n), h->z.as<float>(), st));
     if(n) MCHECK(hipMemcpyAsync(z, h->z.p, n * 4, hipMemcpyDeviceToHost, st));
+    MCHECK(hipStreamSynchronize(st));
+    return CARE_OK;
+}
+
+static int care_mlp_forward_static_body(care_mlp* h, const float* X, size_t n, float* z){
+    if(!h || (n && (!X || !z))) return CARE_EINVAL;
+    if(n > kStaticRows) return h->fail(CARE_EINVAL, "care_mlp_forward_static takes at most %zu rows per call",
+                                       kStaticRows);
+    if(!h->scratch) return h->fail(CARE_ESTATE, "static scratch not resolved for device %d", h->device);
+    if(n == 0) return CARE_OK;
+    MCHECK(hipSetDevice(h->device));
+    hipStream_t st = h->stream;
+    float* sx = static_cast<float*>(h->scratch);
+    float* sz = reinterpret_cast<float*>(static_cast<uint8_t*>(h->scratch) + kStaticXBytes);
+    MCHECK(hipMemcpyAsync(sx, X, n * kIn * 4, hipMemcpyHostToDevice, st));
+    k_forward<<<int((n + kRows - 1) / kRows), kThreads, 0, st>>>(h->params.as<float>(), sx, int(n), sz, nullptr, nullptr);
+    MCHECK(hipGetLastError());
+    MCHECK(hipMemcpyAsync(z, sz, n * 4, hipMemcpyDeviceToHost, st));
     MCHECK(hipStreamSynchronize(st));
     return CARE_OK;
 }
@@ -577,6 +629,10 @@ int care_mlp_forward(care_mlp* h, const float* X, size_t n, float* z){
 
 int care_mlp_loss_grad(care_mlp* h, const float* X, const float* y, size_t n, float* loss, float* grad){
     return guarded(h, [&]() { return care_mlp_loss_grad_body(h, X, y, n, loss, grad); });
+}
+
+int care_mlp_forward_static(care_mlp* h, const float* X, size_t n, float* z){
+    return guarded(h, [&]() { return care_mlp_forward_static_body(h, X, n, z); });
 }
 
 int care_mlp_set_data(care_mlp* h, const float* X, const float* y, size_t n){

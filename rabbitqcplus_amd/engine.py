@@ -64,7 +64,8 @@ EXPORTS = [
     "care_hip_save_tables", "care_hip_load_tables", "care_hip_set_reads_fastq", "care_hip_read_layout",
     "care_hip_inspect_reads", "care_hip_write_fastq", "care_hip_save_reads", "care_hip_load_reads",
     "care_mlp_create", "care_mlp_destroy", "care_mlp_last_error", "care_mlp_set_params", "care_mlp_get_params",
-    "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_loss_grad", "care_mlp_set_data", "care_mlp_train",
+    "care_mlp_reset_optimizer", "care_mlp_forward", "care_mlp_forward_static", "care_mlp_loss_grad",
+    "care_mlp_set_data", "care_mlp_train",
 ]
 
 MLP_PARAMS = 5313   # CARE_MLP_PARAMS: W1[64][16], b1[64], W2[64][64], b2[64], W3[64], b3
@@ -116,6 +117,7 @@ def load_library(path: str = LIB_PATH):
     lib.care_mlp_get_params.argtypes = [vp, vp, sz]
     lib.care_mlp_reset_optimizer.argtypes = [vp]
     lib.care_mlp_forward.argtypes = [vp, vp, sz, vp]
+    lib.care_mlp_forward_static.argtypes = [vp, vp, sz, vp]
     lib.care_mlp_loss_grad.argtypes = [vp, vp, vp, sz, vp, vp]
     lib.care_mlp_set_data.argtypes = [vp, vp, vp, sz]
     lib.care_mlp_train.argtypes = [vp, i32, i32, ctypes.c_float, u64, vp]
