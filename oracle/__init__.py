@@ -314,6 +314,54 @@ class CareCpu:
         return feats[:total], meta[:total]
 
 
+DROPIN_LIB = os.path.join(HERE, "_ref", "libcaredropin.so")
+_dropin = None
+
+
+def dropin_available() -> bool:
+    return os.path.exists(DROPIN_LIB)
+
+
+def dropin_correct(rs, device: int = 0, **opts):
+    """The reference-side drop-in care::mi355x::correct (rabbitqcplus_amd/dropin/, compiled
+    against the reference headers, linked to libcare_hip.so) over the harness's
+    CpuReadStorage: (record count, the bytes of its SerializedObjectStorage in order)."""
+    global _dropin
+    if _dropin is None:
+        lib = ctypes.CDLL(DROPIN_LIB)
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        lib.ref_storage.argtypes = [ctypes.POINTER(RefOpts), sz, vp, sz, vp, vp, sz, vp]
+        lib.ref_storage.restype = vp
+        lib.ref_destroy.argtypes = [vp]
+        lib.dropin_correct.argtypes = [vp, ctypes.c_int, ctypes.c_char_p]
+        lib.dropin_correct.restype = ctypes.c_longlong
+        lib.ref_records_bytes.argtypes = [vp]
+        lib.ref_records_bytes.restype = sz
+        lib.ref_records_data.argtypes = [vp]
+        lib.ref_records_data.restype = vp
+        _dropin = lib
+    lib = _dropin
+    opts.setdefault("paired", int(rs.paired))
+    opts.setdefault("use_quality_scores", int(rs.quals is not None))
+    opts.setdefault("threads", 8)
+    opts.setdefault("fixed_number_of_reads", 0)
+    o = _opts(RefOpts, **opts)
+    q = rs.quals
+    h = lib.ref_storage(ctypes.byref(o), rs.n, rs.enc.ctypes.data, rs.pitch_ints, rs.lengths.ctypes.data, _p(q),
+                        0 if q is None else q.shape[1], rs.ambiguous.ctypes.data)
+    if not h:
+        raise RuntimeError("ref_storage failed")
+    try:
+        err = ctypes.create_string_buffer(256)
+        n = lib.dropin_correct(h, device, err)
+        if n < 0:
+            raise RuntimeError(err.value.decode(errors="replace"))
+        nb = lib.ref_records_bytes(h)
+        return int(n), (ctypes.string_at(lib.ref_records_data(h), nb) if nb else b"")
+    finally:
+        lib.ref_destroy(h)
+
+
 class CareRef:
     """The reference engine (compiled from source) over the same ReadStorage."""
 

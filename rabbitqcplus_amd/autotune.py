@@ -1,9 +1,10 @@
 """Autotuner: picks the correction schedule for a read set on this GPU by timing.
 
 Candidates are the engine and schedule options the library offers:
-- staged kernels with one lane per candidate in the alignment, or one wavefront per pair
-  (the MSA's candidate tile keeps its default: on a 65k-anchor sample the tile sizes differ
-  by less than the timing noise, at full size the default is the fastest);
+- staged kernels with one lane per candidate in the alignment, or one wavefront per pair;
+- for the staged kernels, the MSA's candidate tile (candidates staged into LDS per step):
+  its LDS, with the 16-byte-aligned quality rows of long reads, sets how many one-wavefront
+  MSA workgroups a compute unit holds;
 - the persistent fused kernel at 4, 8 and 12 resident waves per compute unit, its pass
   captured in a HIP graph.
 Each candidate corrects the same sample of anchors (after one warm-up pass); the fastest
@@ -26,6 +27,15 @@ CANDIDATES = [
     {"engine": 2, "waves_per_cu": 8, "use_graphs": 1},
     {"engine": 2, "waves_per_cu": 12, "use_graphs": 1},
 ]
+# MSA tiles swept for the staged kernels (multiples of 4; the default is 20 with quality
+# scores, 64 without)
+TILES_Q = (8, 12, 16, 24, 32)
+TILES_NOQ = (16, 32, 48)
+
+
+def candidates(opts) -> list[dict]:
+    tiles = TILES_Q if opts.get("use_quality_scores") else TILES_NOQ
+    return CANDIDATES + [{"engine": 1, "align_variant": 0, "msa_tile": t} for t in tiles]
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "rabbitqcplus_amd", "autotune.json")
 
@@ -34,7 +44,7 @@ def shape_key(storage, device: int = 0, **opts) -> str:
     name, cus = engine.device_info(device)
     maxlen = int(storage.max_length)
     length_class = 32 * ((maxlen + 31) // 32)
-    return (f"{name}|cu{cus}|len{length_class}|q{int(opts.get('use_quality_scores', 0))}|"
+    return (f"{name}|cu{cus}|len{length_class}|q{int(opts.get('use_quality_scores', 0))}|v2|"
             f"pe{int(opts.get('paired', 0))}|maps{int(opts.get('num_hash_maps', 48))}|k{int(opts.get('kmerlength', 0))}")
 
 
@@ -63,7 +73,7 @@ def tune(storage, device: int = 0, sample: int = 65536, reps: int = 3, cache: st
     if opts.get("paired"):
         n -= n % 2
     results = []
-    for cand in CANDIDATES:
+    for cand in candidates(opts):
         if not _fits(cand, storage):
             continue
         eng = engine.CareEngine(device, **{**opts, **cand})

@@ -70,6 +70,9 @@ def build_oracle(with_reference: bool | None = None):
         with_reference = os.path.isdir("/root/reference/src")
     if with_reference:
         _run(["make", "-j", jobs, "-C", odir, "ref"])
+        # the reference-side drop-in, linked to libcare_hip.so (built before this step)
+        if os.path.exists(os.path.join(LIBDIR, "libcare_hip.so")):
+            _run(["make", "-j", jobs, "-C", odir, "dropin"])
 
 
 def build_all():

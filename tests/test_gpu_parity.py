@@ -248,8 +248,12 @@ def test_autotuner_choice_is_exact(tmp_path):
     s, rs, o = helpers.case("se_higherr_q")
     cache = str(tmp_path / "tune.json")
     r = autotune.tune(rs, sample=4096, reps=2, cache=cache, **o)
-    assert not r["cached"] and r["choice"] in autotune.CANDIDATES
-    assert len(r["timings_ms"]) == len(autotune.CANDIDATES)
+    cands = autotune.candidates(o)
+    assert not r["cached"] and r["choice"] in cands
+    assert len(r["timings_ms"]) == len(cands)
+    # every schedule, the tiles swept included, gives the oracle's records
+    for c in cands[::3]:
+        assert engine.correct_gpu(rs, **{**o, **c})[1] == engine.correct_gpu(rs, **o)[1], c
     assert autotune.tune(rs, sample=4096, reps=2, cache=cache, **o)["cached"]
     _, d = engine.correct_gpu(rs, **{**o, **r["choice"]})
     cpu = oracle.CareCpu(rs, threads=8, **o)
