@@ -1078,7 +1078,7 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
     uint32_t AH[2 * WM + 2], AL[2 * WM + 2], AV[2 * WM + 2];
     #pragma unroll
     for(int i = 0; i < 2 * WM + 2; i++){
-        const bool in = i >= 1 && i <= W;
+        const bool in = i >= 1 && i <= WM && i <= W;
         AH[i] = in ? arow[i - 1] : 0u;
         AL[i] = in ? arow[W + i - 1] : 0u;
         AV[i] = in ? overlapMask(La - 32 * (i - 1)) : 0u;
