@@ -1276,6 +1276,7 @@ int alignLaneWords(int W){
 struct MsaParams {
     const uint32_t* enc; size_t pitch; const int* len; const uint8_t* qual; size_t qpitch; const uint8_t* ambig;
     size_t qdpitch;                 // device quality row pitch (qpitch rounded up to 16 bytes)
+    int varLen;                     // reads of different lengths: stage only what each candidate covers
     const float* qweight;
     uint64_t b0;
     const uint32_t* offsets; const uint32_t* ids; const AlignBest* best; const uint8_t* paired;
@@ -1384,6 +1385,34 @@ __device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile
 // byte-reversed in registers and stored mirrored in the slot, so that position pos of the
 // reversed row sits at rcQualityOffset + pos. Caller synchronizes before (tile free) and
 // after (tile complete).
+// An oriented 2-bit row into a tile slot. Reads of one length copy the row's RW words in
+// a wave-uniform loop. Reads of different lengths copy only the words under the
+// candidate's length (the MSA reads positions < len), as 16-byte loads and stores where
+// vec holds (W even, RW and the enc pitch multiples of 4 words: the enc rows, the
+// reverse-complement rows and the tile slots start on 16-byte boundaries) and the group of
+// four words lies inside the row's RW words.
+__device__ __forceinline__ void stageRow(const uint32_t* __restrict__ src, uint32_t* dst, int len, int RW, bool varLen,
+                                         bool vec){
+    if(!varLen){
+        for(int k = 0; k < RW; k++) dst[k] = src[k];
+        return;
+    }
+    const int nw = min((len + 15) >> 4, RW);
+    int k = 0;
+    if(vec){
+        for(; k < nw && k + 4 <= RW; k += 4)
+            *reinterpret_cast<uint4*>(dst + k) = *reinterpret_cast<const uint4*>(src + k);
+    }
+    for(; k < nw; k++) dst[k] = src[k];
+}
+
+// Whether 16-byte chunk c of a candidate's quality row holds a byte the MSA reads: a
+// forward candidate reads positions [0, len); a reverse-complement candidate reads the
+// row reversed at the storage width qpitch, i.e. bytes [qpitch - len, qpitch).
+__device__ __forceinline__ bool qualityChunkUsed(int c, int len, bool rc, int qpitch){
+    return rc ? (16 * c + 16 > qpitch - len && 16 * c < qpitch) : 16 * c < len;
+}
+
 __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists& lst, const MsaTile& t, int beg,
                                           int tb, int tn, int tid, int nthreads){
     const int W = p.W;
@@ -1396,10 +1425,10 @@ __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists&
         t.w[q] = lst.kW[d];
         t.id[q] = id;
         t.orient[q] = orient;
-        // words [0, RW) of the oriented row (the enc pitch holds at least RW words)
+        // the oriented row (the enc pitch holds at least RW words)
         const uint32_t* src = orient == 1 ? p.enc + p.pitch * id : p.rcenc + uint64_t(id) * uint64_t(2 * W);
         const int RW = tileRowWords(p.maxLen);
-        for(int k = 0; k < RW; k++) t.rows[q * 2 * W + k] = src[k];
+        stageRow(src, t.rows + q * 2 * W, t.len[q], RW, p.varLen != 0, (W & 1) == 0 && (RW & 3) == 0 && (p.pitch & 3) == 0);
     }
     if(!p.useQ) return;
     __syncthreads();
@@ -1415,6 +1444,7 @@ __device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile
     for(int u = tid; u < tn * nc; u += nthreads){
         const int j = int((uint64_t(u) * magic) >> 32);
         const int c = u - j * nc;
+        if(p.varLen && !qualityChunkUsed(c, t.len[j], t.orient[j] != 1, int(p.qpitch))) continue;
         const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * t.id[j] + 16 * c);
         uint8_t* dst = t.q + j * qs;
         if(t.orient[j] == 1){
@@ -1448,7 +1478,7 @@ __device__ __forceinline__ void stageTileLanes(const MsaParams& p, const MsaTile
     if(q >= 0 && q < tn){
         const uint32_t* src = c.orient == 1 ? p.enc + p.pitch * c.id : p.rcenc + uint64_t(c.id) * uint64_t(2 * W);
         const int RW = tileRowWords(p.maxLen);
-        for(int k = 0; k < RW; k++) t.rows[q * 2 * W + k] = src[k];
+        stageRow(src, t.rows + q * 2 * W, c.len, RW, p.varLen != 0, (W & 1) == 0 && (RW & 3) == 0 && (p.pitch & 3) == 0);
     }
     if(!p.useQ) return;
     const int qs = qStride(p.maxLen);
@@ -1460,7 +1490,9 @@ __device__ __forceinline__ void stageTileLanes(const MsaParams& p, const MsaTile
         const int j = min(int((uint64_t(u) * magic) >> 32), tn - 1);
         const uint32_t id = uint32_t(__builtin_amdgcn_ds_bpermute(4 * (tb + j), int(c.id)));
         const int orient = __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.orient);
-        if(u < total){
+        // reads of one length use every chunk (fixed length: no test, no third permute)
+        const int len = p.varLen ? __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.len) : int(p.qpitch);
+        if(u < total && (!p.varLen || qualityChunkUsed(u - j * nc, len, orient != 1, int(p.qpitch)))){
             const int ch = u - j * nc;
             const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * id + 16 * ch);
             uint8_t* dst = t.q + j * qs;
@@ -2980,6 +3012,7 @@ struct care_ctx {
 
     uint64_t n = 0;
     int maxLen = 0;
+    bool varLen = false;    // reads of different lengths
     size_t pitch = 0;
     size_t qpitch = 0;
     size_t qdpitch = 0;     // device quality row pitch: qpitch rounded up to 16 bytes
@@ -3199,6 +3232,7 @@ void fillMsaCommon(const care_ctx* ctx, MsaParams& mp){
     mp.enc = ctx->enc.as<uint32_t>(); mp.pitch = ctx->pitch; mp.len = ctx->len.as<int>();
     mp.qual = ctx->qual.as<uint8_t>(); mp.qpitch = ctx->qpitch; mp.ambig = ctx->ambig.as<uint8_t>();
     mp.qdpitch = ctx->qdpitch;
+    mp.varLen = ctx->varLen ? 1 : 0;
     mp.qweight = ctx->qw.as<float>();
     mp.isPaired = ctx->o.paired; mp.useQ = ctx->o.use_quality_scores;
     mp.maxErrorRate = ctx->o.max_error_rate;
@@ -3920,7 +3954,7 @@ const char* care_hip_last_error(const care_ctx* ctx){
 }
 
 // Shapes and options of a read set (shared by the array and the FASTQ entry points).
-int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_ints){
+int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_ints, int minLen){
     if(num_reads == 0) return ctx->fail(CARE_EINVAL, "empty read storage");
     if(num_reads >= 0xFFFFFFFFull) return ctx->fail(CARE_EINVAL, "read ids are 32-bit");
     if(ctx->o.paired && (num_reads & 1)) return ctx->fail(CARE_EINVAL, "paired input needs an even number of reads");
@@ -3928,6 +3962,7 @@ int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_i
     if(pitch_ints * 16 < size_t(maxLen)) return ctx->fail(CARE_EINVAL, "pitch_ints too small for the longest read");
     ctx->n = num_reads;
     ctx->maxLen = maxLen;
+    ctx->varLen = minLen != maxLen;
     ctx->pitch = pitch_ints;
     ctx->W = std::max(1, (maxLen + 31) / 32);
     ctx->k = ctx->o.kmerlength ? ctx->o.kmerlength : (maxLen < 160 ? 20 : 32);
@@ -4040,10 +4075,11 @@ static int care_hip_set_reads_body(care_ctx* ctx, const uint32_t* enc2bit, size_
                        const uint8_t* quals, size_t qual_pitch, uint64_t num_reads, const uint8_t* ambiguous){
     if(!ctx) return CARE_EINVAL;
     if(!enc2bit || !lengths || !ambiguous || num_reads == 0) return ctx->fail(CARE_EINVAL, "empty or null read storage");
-    int maxLen = 0;
+    int maxLen = 0, minLen = INT_MAX;
     for(uint64_t i = 0; i < num_reads; i++){
         if(lengths[i] < 0) return ctx->fail(CARE_EINVAL, "negative length at read %llu", (unsigned long long)i);
         maxLen = std::max(maxLen, int(lengths[i]));
+        minLen = std::min(minLen, int(lengths[i]));
     }
     if(ctx->o.use_quality_scores){
         if(!quals) return ctx->fail(CARE_EINVAL, "use_quality_scores set but no qualities given");
@@ -4051,7 +4087,7 @@ static int care_hip_set_reads_body(care_ctx* ctx, const uint32_t* enc2bit, size_
     }
     HIPCHECK(hipSetDevice(ctx->device));
     {
-        const int rc = configureReads(ctx, num_reads, maxLen, pitch_ints);
+        const int rc = configureReads(ctx, num_reads, maxLen, pitch_ints, minLen);
         if(rc != CARE_OK) return rc;
     }
     hipStream_t st = ctx->stream;
@@ -4380,7 +4416,7 @@ static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size
     if(err) return ctx->fail(CARE_EINVAL, "malformed FASTQ record (header '@', separator '+', quality length)");
     const int maxLen = *std::max_element(lens.begin(), lens.end());
     const size_t pitch = size_t(std::max(1, (maxLen + 15) / 16));
-    rc = configureReads(ctx, n, maxLen, pitch);
+    rc = configureReads(ctx, n, maxLen, pitch, *std::min_element(lens.begin(), lens.end()));
     if(rc != CARE_OK) return rc;
     size_t tb = 0;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, tmp.ncount.as<uint64_t>(), tmp.prefix.as<uint64_t>(), int(n), st));
@@ -4984,7 +5020,7 @@ static int care_hip_load_reads_body(care_ctx* ctx, const char* path){
     HIPCHECK(hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     {
-        const int rc = configureReads(ctx, nr, maxLen, size_t(sp));
+        const int rc = configureReads(ctx, nr, maxLen, size_t(sp), minLen);
         if(rc != CARE_OK) return rc;
     }
     const uint64_t n = nr;
