@@ -5312,11 +5312,12 @@ static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size
     int B = ctx->o.batch_size > 0 ? ctx->o.batch_size : 0;
     if(B <= 0){
         // one batch per pass where HBM allows: the per-anchor candidate staging area
-        // (maps * threshold ids) and record slot may take up to 16 GiB or a quarter of
-        // the free device memory; fewer batches mean fewer launch tails and host syncs
+        // (maps * threshold ids) and record slot may take up to 96 GiB or a third of the
+        // free device memory (288 GB of HBM: 2 batches at 8.6M reads); fewer batches mean
+        // fewer launch tails and host syncs
         size_t freeB = 0, totalB = 0;
         HIPCHECK(hipMemGetInfo(&freeB, &totalB));
-        const size_t budget = std::min<size_t>(size_t(16) << 30, freeB / 4);
+        const size_t budget = std::min<size_t>(size_t(96) << 30, freeB / 3);
         size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + size_t(recordSlotBytes(ctx->maxLen)) + 512;
         // candidate correction: a record slot per working-list entry (about threshold per anchor)
         if(ctx->o.correct_candidates) perAnchor += size_t(ctx->thr) * (candidateSlotBytes(ctx->maxLen) + 36);
