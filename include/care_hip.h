@@ -64,8 +64,9 @@ typedef struct care_opts {
                                        classic column rule where its confidence is below 0.9 */
     int align_variant;              /* staged engine: 0 = one lane per candidate where reads have at
                                        most 352 bases (else 1), 1 = one wavefront per pair */
-    int use_graphs;                 /* fused engine: 1 = capture a pass's device work into a HIP graph
-                                       and replay it while the pass's arguments and buffers hold */
+    int use_graphs;                 /* 1 = capture device work into HIP graphs, replayed while their
+                                       arguments and buffers hold: the fused engine's pass, or each
+                                       pipelined staged batch (see sync_batches) */
     int correct_candidates;         /* --candidateCorrection (staged engine, reads of at most 340 bases):
                                        HQ anchors also correct their candidates (include/corrector.hpp:
                                        1504-1512, src/msa.cpp:457-522) */
@@ -82,6 +83,17 @@ typedef struct care_opts {
                                        hold. Records then differ from the reference's where the scan
                                        would choose another shift or strand; used by the fast learned
                                        form, whose bar is the error rate */
+    int sync_batches;               /* staged engine: 0 = pipelined batches where the lane-per-candidate
+                                       or seeded alignment runs and no candidate correction, dump or
+                                       feature output is asked for (no host synchronization inside a
+                                       batch: candidate arrays sized from the candidates per anchor
+                                       seen, a batch with more is redone; the next batch runs while a
+                                       batch's records are copied out); 1 = a host synchronization
+                                       per batch to size its candidate arrays exactly */
+    int pipeline_capacity;          /* pipelined staged batches: candidate pairs per anchor their arrays
+                                       hold before the context has counted any, 0 = 4 x coverage + 16;
+                                       later batches hold the most seen per anchor plus a quarter.
+                                       Records do not depend on it */
 } care_opts;
 
 typedef struct care_ctx care_ctx;
@@ -107,6 +119,10 @@ typedef struct care_timings {
     double fastq_device_ms;   /* last care_hip_set_reads_fastq: device span after the text upload */
     double merge_device_ms;   /* last care_hip_write_fastq: device span from the record upload to the text
                                  download (both excluded) */
+    double graph_ms;          /* pipelined staged batches with use_graphs: device span of the batches
+                                 (per-stage fields stay zero) */
+    uint64_t batches_redone;  /* pipelined staged batches redone with exact sizes (more candidates
+                                 than their arrays held) */
 } care_timings;
 
 void care_hip_default_opts(care_opts* o);
@@ -223,6 +239,13 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
 
 /* Copies the records of the last care_hip_correct call (cap must be >= num_bytes). */
 int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap);
+
+/* The records of the last care_hip_correct call without a copy: *data points into the
+ * context's pinned host buffer and stays valid until the next care_hip_correct,
+ * care_hip_extract_features or care_hip_destroy on the context. Same bytes as
+ * care_hip_copy_records (the reference reads them out of its SerializedObjectStorage,
+ * include/serializedobjectstorage.hpp, to sort and merge them). */
+int care_hip_records_view(const care_ctx* ctx, const uint8_t** data, size_t* bytes);
 
 int care_hip_get_timings(const care_ctx* ctx, care_timings* out);
 

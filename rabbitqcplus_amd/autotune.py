@@ -2,6 +2,9 @@
 
 Candidates are the engine and schedule options the library offers:
 - staged kernels with one lane per candidate in the alignment, or one wavefront per pair;
+- for the lane-per-candidate staged kernels: pipelined batches (no host synchronization
+  inside a batch), the same with each batch captured in a HIP graph, or a host
+  synchronization per batch;
 - for the staged kernels, the MSA's candidate tile (candidates staged into LDS per step):
   its LDS, with the 16-byte-aligned quality rows of long reads, sets how many one-wavefront
   MSA workgroups a compute unit holds;
@@ -22,6 +25,8 @@ from . import engine
 
 CANDIDATES = [
     {"engine": 1, "align_variant": 0},
+    {"engine": 1, "align_variant": 0, "use_graphs": 1},
+    {"engine": 1, "align_variant": 0, "sync_batches": 1},
     {"engine": 1, "align_variant": 1},
     {"engine": 2, "waves_per_cu": 4, "use_graphs": 1},
     {"engine": 2, "waves_per_cu": 8, "use_graphs": 1},
@@ -44,7 +49,7 @@ def shape_key(storage, device: int = 0, **opts) -> str:
     name, cus = engine.device_info(device)
     maxlen = int(storage.max_length)
     length_class = 32 * ((maxlen + 31) // 32)
-    return (f"{name}|cu{cus}|len{length_class}|q{int(opts.get('use_quality_scores', 0))}|v2|"
+    return (f"{name}|cu{cus}|len{length_class}|q{int(opts.get('use_quality_scores', 0))}|v3|"
             f"pe{int(opts.get('paired', 0))}|maps{int(opts.get('num_hash_maps', 48))}|k{int(opts.get('kmerlength', 0))}")
 
 

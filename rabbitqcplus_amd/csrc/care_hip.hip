@@ -41,6 +41,7 @@
 #include <cstring>
 #include <exception>
 #include <limits>
+#include <map>
 #include <new>
 #include <string>
 #include <numeric>
@@ -650,53 +651,27 @@ k_compact(const uint32_t* raw, int capPerAnchor, const uint32_t* offsets, int B,
     }
 }
 
-// Pair flags of mates 2p, 2p+1: merge of the two sorted candidate lists
-// (include/util.hpp:1016-1141 via findPositionsOfPairedReadIds :1153-1200).
-__global__ void k_pair_flags(const uint32_t* offsets, const uint32_t* ids, uint8_t* flags, int numPairs){
-    const int pidx = blockIdx.x * blockDim.x + threadIdx.x;
+// Pair flags of mates 2p, 2p+1 (include/util.hpp:1016-1141 via findPositionsOfPairedReadIds
+// :1153-1200): an entry of one mate's sorted candidate list is flagged when the other
+// mate's list holds its mate (id ^ 1) -- the entries the reference's merge of the two
+// lists outputs. One wavefront per pair: lanes over both lists' entries, each a binary
+// search in the other list; byte flags written coalesced.
+__global__ void __launch_bounds__(kBlock) k_pair_flags(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ ids,
+                                                       uint8_t* __restrict__ flags, int numPairs){
+    const int pidx = blockIdx.x * (kBlock / kWave) + int(threadIdx.x / kWave);
+    const int lane = threadIdx.x & (kWave - 1);
     if(pidx >= numPairs) return;
     const uint32_t b1 = offsets[2 * pidx], e1 = offsets[2 * pidx + 1], e2 = offsets[2 * pidx + 2];
-    const uint32_t* c1 = ids + b1;
-    const uint32_t* c2 = ids + e1;
-    uint8_t* f1 = flags + b1;
-    uint8_t* f2 = flags + e1;
-    const uint32_t n1 = e1 - b1, n2 = e2 - e1;
-    for(uint32_t q = 0; q < n1; q++) f1[q] = 0;
-    for(uint32_t q = 0; q < n2; q++) f2[q] = 0;
-    uint32_t i = 0, j = 0;
-    while(i < n1 && j < n2){
-        const int x1 = (i + 1 < n1 && c1[i] / 2 == c1[i + 1] / 2) ? 2 : 1;
-        const int x2 = (j + 1 < n2 && c2[j] / 2 == c2[j + 1] / 2) ? 2 : 1;
-        const bool same = c1[i] / 2 == c2[j] / 2;
-        if(x1 == 1 && x2 == 1){
-            if(same){
-                if(c1[i] != c2[j]){ f1[i] = 1; f2[j] = 1; }
-                i++; j++;
-            }else{
-                if(c1[i] < c2[j]) i++; else j++;
-            }
-        }else if(x1 == 2 && x2 == 2){
-            if(same){
-                f1[i] = 1; f2[j] = 1; f1[i + 1] = 1; f2[j + 1] = 1;
-                i += 2; j += 2;
-            }else{
-                if(c1[i] < c2[j]) i += 2; else j += 2;
-            }
-        }else if(x1 == 2 && x2 == 1){
-            if(same){
-                if(c1[i] == c2[j]){ i++; f1[i] = 1; f2[j] = 1; i++; j++; }
-                else { f1[i] = 1; f2[j] = 1; i += 2; j++; }
-            }else{
-                if(c1[i] < c2[j]) i += 2; else j++;
-            }
-        }else{
-            if(same){
-                if(c1[i] == c2[j]){ j++; f1[i] = 1; f2[j] = 1; i++; j++; }
-                else { f1[i] = 1; f2[j] = 1; i++; j += 2; }
-            }else{
-                if(c1[i] < c2[j]) i++; else j += 2;
-            }
+    for(uint32_t q = b1 + lane; q < e2; q += kWave){
+        const bool inFirst = q < e1;
+        const uint32_t want = ids[q] ^ 1u;
+        const uint32_t end = inFirst ? e2 : e1;
+        uint32_t lo = inFirst ? e1 : b1, hi = end;
+        while(lo < hi){
+            const uint32_t mid = (lo + hi) >> 1;
+            if(ids[mid] < want) lo = mid + 1; else hi = mid;
         }
+        flags[q] = (lo < end && ids[lo] == want) ? 1 : 0;
     }
 }
 
@@ -2913,6 +2888,24 @@ __global__ void k_zero_u32(uint32_t* v, int n){
     if(i < n) v[i] = 0u;
 }
 
+// Candidate capacity check of a batch whose candidate arrays were sized before its
+// candidates were counted: status[0] = 1 when the batch has more than `cap`, status[1] =
+// how many it has.
+__global__ void k_cap_status(const uint32_t* __restrict__ offsets, int B, uint64_t cap, uint32_t* __restrict__ status){
+    if(blockIdx.x == 0 && threadIdx.x == 0){
+        const uint32_t t = offsets[B];
+        status[0] = uint64_t(t) > cap ? 1u : 0u;
+        status[1] = t;
+    }
+}
+
+// After k_cap_status: a batch over capacity gets all-zero offsets, so that the rest of its
+// kernels see no candidates and write nothing past the arrays (the batch is redone).
+__global__ void k_cap_clear(uint32_t* __restrict__ offsets, int B, const uint32_t* __restrict__ status){
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i <= B && status[0]) offsets[i] = 0u;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_gather(const uint8_t* slots, int slotBytes, const uint32_t* recBytes, const uint64_t* outOff, uint8_t* out, int B){
     const int i = blockIdx.x;
@@ -2926,6 +2919,27 @@ k_gather(const uint8_t* slots, int slotBytes, const uint32_t* recBytes, const ui
 __global__ void k_widen(const uint32_t* in, uint64_t* out, int n){
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if(i < n) out[i] = in[i];
+}
+
+// k_widen over kWidenPerBlock sizes per 256-thread workgroup, and the number of nonzero
+// sizes (records) added to *count: wave ballots summed in LDS, one global atomic per
+// workgroup.
+constexpr int kWidenPerBlock = 1024;
+__global__ void __launch_bounds__(256) k_widen_count(const uint32_t* __restrict__ in, uint64_t* __restrict__ out, int n,
+                                                     unsigned long long* __restrict__ count){
+    __shared__ unsigned blockCount;
+    if(threadIdx.x == 0) blockCount = 0;
+    __syncthreads();
+    unsigned c = 0;
+    for(int k = 0; k < kWidenPerBlock / 256; k++){
+        const int i = blockIdx.x * kWidenPerBlock + k * 256 + int(threadIdx.x);
+        const uint32_t v = i < n ? in[i] : 0u;
+        if(i < n) out[i] = v;
+        c += unsigned(__popcll(__ballot(v != 0u)));
+    }
+    if((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(&blockCount, c);
+    __syncthreads();
+    if(threadIdx.x == 0 && blockCount) atomicAdd(count, (unsigned long long)blockCount);
 }
 
 // Candidate records of reads corrected as HQ anchors earlier in the same call are not
@@ -3002,6 +3016,34 @@ struct DBuf {
     template<class T> T* as() const { return static_cast<T*>(p); }
 };
 
+// Records of a context on the host: pinned memory, so that device-to-host copies run at
+// DMA speed; grown by doubling; bytes past size() are not initialised.
+struct HostRecords {
+    uint8_t* p = nullptr;
+    size_t n = 0, cap = 0;
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+    uint8_t* data() { return p; }
+    const uint8_t* data() const { return p; }
+    void clear(){ n = 0; }
+    // size `bytes`, keeping the first min(size(), bytes) bytes
+    hipError_t resize(size_t bytes){
+        if(bytes > cap){
+            const size_t nc = std::max({bytes, cap * 2, size_t(1) << 20});
+            void* q = nullptr;
+            const hipError_t e = hipHostMalloc(&q, nc, hipHostMallocDefault);
+            if(e != hipSuccess) return e;
+            if(n) std::memcpy(q, p, n);
+            if(p) (void)hipHostFree(p);
+            p = static_cast<uint8_t*>(q);
+            cap = nc;
+        }
+        n = bytes;
+        return hipSuccess;
+    }
+    void release(){ if(p) (void)hipHostFree(p); p = nullptr; n = cap = 0; }
+};
+
 }  // namespace
 
 struct care_ctx {
@@ -3040,11 +3082,22 @@ struct care_ctx {
     // fused engine with use_graphs: the instantiated pass and the parameter bytes it was captured with
     hipGraphExec_t fusedExec = nullptr;
     std::vector<uint8_t> fusedKey;
+    // staged engine, pipelined batches: two batches in flight, each with its own record
+    // output buffer, pinned status words and stage events; a copy stream for the records;
+    // the most candidates per anchor seen (sizes the next pass's candidate arrays)
+    DBuf pipeOut[2];
+    hipEvent_t pipeEv[2][6] = {};
+    hipStream_t copyStream = nullptr;
+    double candPerAnchor = 0;
+    double pipePerAnchor = 0;   // candidate pairs per anchor the arrays were last sized for
+    // pipelined batches with use_graphs: one instantiated graph per batch (key: the batch's
+    // parameter bytes and every address its work uses)
+    std::map<std::vector<uint8_t>, hipGraphExec_t> stagedGraphs;
     uint64_t graphsBuilt = 0;
     bool hasSeam = false;
     bool featActive = false;
     unsigned long long featCap = 0;
-    std::vector<uint8_t> records;
+    HostRecords records;
     size_t numRecords = 0;
     // candidate correction: record slots and sizes per working-list entry, the insertion
     // order of a batch's records, per read the HQ-anchor flag; first anchor of the call
@@ -3350,7 +3403,7 @@ int ensureRecordScan(care_ctx* ctx, int B){
 
 // Record compaction, host part (after recordsDevicePart and a stream synchronization):
 // checks the error flag, gathers the records into one buffer and appends it to `out`.
-int recordsFinish(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out, size_t& outRecords){
+int recordsFinish(care_ctx* ctx, int B, int slotBytes, HostRecords& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     uint64_t outBytes = 0;
     int errFlag = 0;
@@ -3362,7 +3415,7 @@ int recordsFinish(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out
                                    ctx->recOff.as<uint64_t>(), ctx->outBuf.as<uint8_t>(), B);
     HIPCHECK(hipGetLastError());
     const size_t old = out.size();
-    out.resize(old + outBytes);
+    HIPCHECK(out.resize(old + outBytes));
     if(outBytes) HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->outBuf.p, outBytes, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     size_t q = old;
@@ -3376,7 +3429,7 @@ int recordsFinish(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out
 }
 
 // Compacts the per-anchor record slots [0, B) into one buffer and appends it to `out`.
-int gatherRecords(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out, size_t& outRecords){
+int gatherRecords(care_ctx* ctx, int B, int slotBytes, HostRecords& out, size_t& outRecords){
     int rc = ensureRecordScan(ctx, B);
     if(rc != CARE_OK) return rc;
     rc = recordsDevicePart(ctx, B);
@@ -3386,7 +3439,7 @@ int gatherRecords(care_ctx* ctx, int B, int slotBytes, std::vector<uint8_t>& out
 }
 
 // Fused pass over anchors [first, first+n): one persistent kernel, then record compaction.
-int runFused(care_ctx* ctx, uint64_t first, uint64_t n, std::vector<uint8_t>& out, size_t& outRecords){
+int runFused(care_ctx* ctx, uint64_t first, uint64_t n, HostRecords& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     const bool paired = ctx->o.paired != 0;
     const uint64_t units = paired ? n / 2 : n;
@@ -3518,7 +3571,7 @@ int candidateSlotBytes(int maxLen){
 // gathers the batch's anchor and candidate records in the reference's insertion order
 // and appends them to `out` (after the MSA kernel of the batch).
 int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int slotBytes, int candSlotBytes,
-                         std::vector<uint8_t>& out, size_t& outRecords){
+                         HostRecords& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     if(total){
         k_cand_filter<<<(total + 255) / 256, 256, 0, st>>>(ctx->candBytes.as<uint32_t>(), ctx->kId.as<uint32_t>(),
@@ -3555,7 +3608,7 @@ int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int 
         entries);
     HIPCHECK(hipGetLastError());
     const size_t old = out.size();
-    out.resize(old + outBytes);
+    HIPCHECK(out.resize(old + outBytes));
     if(outBytes) HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->outBuf.p, outBytes, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[4], st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -3612,7 +3665,65 @@ int batchOrder(care_ctx* ctx, uint64_t b0, int B, const uint32_t** out){
     return CARE_OK;
 }
 
-int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_t& outRecords){
+// The alignment kernel of a staged batch: the seeded kernel (seeded alignment), the
+// lane-per-candidate kernel (wm > 0), or the wavefront-per-pair kernel over `total` pairs.
+hipError_t launchAlign(const care_ctx* ctx, const AlignParams& ap, int B, uint32_t total, int wm, hipStream_t st){
+    if(ctx->o.seeded_alignment){
+        switch(alignLaneWords(ctx->W)){
+            case 2: k_align_seeded<2><<<B, kWave, 0, st>>>(ap); break;
+            case 4: k_align_seeded<4><<<B, kWave, 0, st>>>(ap); break;
+            case 5: k_align_seeded<5><<<B, kWave, 0, st>>>(ap); break;
+            case 8: k_align_seeded<8><<<B, kWave, 0, st>>>(ap); break;
+            case 11: k_align_seeded<11><<<B, kWave, 0, st>>>(ap); break;
+            default: return hipErrorInvalidValue;      // rejected before any batch runs
+        }
+    }else{
+        switch(wm){
+            case 2: k_align_lanes<2><<<B, kWave, 0, st>>>(ap); break;
+            case 4: k_align_lanes<4><<<B, kWave, 0, st>>>(ap); break;
+            case 5: k_align_lanes<5><<<B, kWave, 0, st>>>(ap); break;
+            case 8: k_align_lanes<8><<<B, kWave, 0, st>>>(ap); break;
+            case 11: k_align_lanes<11><<<B, kWave, 0, st>>>(ap); break;
+            default: k_align<<<(total + kAlignWaves - 1) / kAlignWaves, kBlock, 0, st>>>(ap); break;
+        }
+    }
+    return hipGetLastError();
+}
+
+// The one-wavefront MSA kernel of a staged batch with kc register columns per lane: the
+// whole MSA with the quality mode read at run time (candidate correction, dumps), or the
+// anchor's columns with the quality mode fixed at compile time.
+hipError_t launchWaveMsa(const MsaParams& mp, int B, int kc, bool wholeMsa, bool q, size_t lds, hipStream_t st){
+    if(wholeMsa){
+        switch(kc){
+            case 4: k_msa_wave<4, false, 2><<<B, kWave, lds, st>>>(mp); break;
+            case 8: k_msa_wave<8, false, 2><<<B, kWave, lds, st>>>(mp); break;
+            case 12: k_msa_wave<12, false, 2><<<B, kWave, lds, st>>>(mp); break;
+            default: k_msa_wave<16, false, 2><<<B, kWave, lds, st>>>(mp); break;
+        }
+    }else if(q){
+        switch(kc){
+            case 2: k_msa_wave<2, true, 1><<<B, kWave, lds, st>>>(mp); break;
+            case 3: k_msa_wave<3, true, 1><<<B, kWave, lds, st>>>(mp); break;
+            case 4: k_msa_wave<4, true, 1><<<B, kWave, lds, st>>>(mp); break;
+            case 8: k_msa_wave<8, true, 1><<<B, kWave, lds, st>>>(mp); break;
+            case 12: k_msa_wave<12, true, 1><<<B, kWave, lds, st>>>(mp); break;
+            default: k_msa_wave<16, true, 1><<<B, kWave, lds, st>>>(mp); break;
+        }
+    }else{
+        switch(kc){
+            case 2: k_msa_wave<2, true, 0><<<B, kWave, lds, st>>>(mp); break;
+            case 3: k_msa_wave<3, true, 0><<<B, kWave, lds, st>>>(mp); break;
+            case 4: k_msa_wave<4, true, 0><<<B, kWave, lds, st>>>(mp); break;
+            case 8: k_msa_wave<8, true, 0><<<B, kWave, lds, st>>>(mp); break;
+            case 12: k_msa_wave<12, true, 0><<<B, kWave, lds, st>>>(mp); break;
+            default: k_msa_wave<16, true, 0><<<B, kWave, lds, st>>>(mp); break;
+        }
+    }
+    return hipGetLastError();
+}
+
+int runBatch(care_ctx* ctx, uint64_t b0, int B, HostRecords& out, size_t& outRecords){
     hipStream_t st = ctx->stream;
     const int capPerAnchor = ctx->M * ctx->thr;
     int P = 64;
@@ -3683,7 +3794,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     HIPCHECK(hipGetLastError());
     if(ctx->o.paired){
         const int pairs = B / 2;
-        k_pair_flags<<<(pairs + 127) / 128, 128, 0, st>>>(ctx->offsets.as<uint32_t>(), ctx->ids.as<uint32_t>(),
+        k_pair_flags<<<(pairs + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, st>>>(ctx->offsets.as<uint32_t>(), ctx->ids.as<uint32_t>(),
                                                           ctx->paired.as<uint8_t>(), pairs);
         HIPCHECK(hipGetLastError());
     }
@@ -3695,27 +3806,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     ap.best = ctx->best.as<AlignBest>(); ap.dbg = ctx->o.keep_intermediates ? ctx->dbg.as<int>() : nullptr;
     ap.offsets = ctx->offsets.as<uint32_t>(); ap.B = B; ap.maxLen = ctx->maxLen;
     ap.order = order;
-    if(total && ctx->o.seeded_alignment){
-        switch(alignLaneWords(ctx->W)){
-            case 2: k_align_seeded<2><<<B, kWave, 0, st>>>(ap); break;
-            case 4: k_align_seeded<4><<<B, kWave, 0, st>>>(ap); break;
-            case 5: k_align_seeded<5><<<B, kWave, 0, st>>>(ap); break;
-            case 8: k_align_seeded<8><<<B, kWave, 0, st>>>(ap); break;
-            case 11: k_align_seeded<11><<<B, kWave, 0, st>>>(ap); break;
-            default: return ctx->fail(CARE_EINVAL, "seeded alignment handles reads of at most 352 bases");
-        }
-        HIPCHECK(hipGetLastError());
-    }else if(total){
-        switch(wm){
-            case 2: k_align_lanes<2><<<B, kWave, 0, st>>>(ap); break;
-            case 4: k_align_lanes<4><<<B, kWave, 0, st>>>(ap); break;
-            case 5: k_align_lanes<5><<<B, kWave, 0, st>>>(ap); break;
-            case 8: k_align_lanes<8><<<B, kWave, 0, st>>>(ap); break;
-            case 11: k_align_lanes<11><<<B, kWave, 0, st>>>(ap); break;
-            default: k_align<<<(total + kAlignWaves - 1) / kAlignWaves, kBlock, 0, st>>>(ap); break;
-        }
-        HIPCHECK(hipGetLastError());
-    }
+    if(total) HIPCHECK(launchAlign(ctx, ap, B, total, wm, st));
     HIPCHECK(hipEventRecord(ctx->ev[2], st));
 
     const int maxCols = ctx->maxCols;
@@ -3763,39 +3854,13 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
         mp.tile = ctx->o.msa_tile ? ctx->o.msa_tile : defaultWaveMsaTile(q);
         const size_t lds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, q, mp.seamMode != 0 || mp.featOut != nullptr,
                                            mp.maxCols);
-        if(wholeMsa){
-            // candidate correction and the dumps: quality mode read at run time
-            switch(kc){
-                case 4: k_msa_wave<4, false, 2><<<B, kWave, lds, st>>>(mp); break;
-                case 8: k_msa_wave<8, false, 2><<<B, kWave, lds, st>>>(mp); break;
-                case 12: k_msa_wave<12, false, 2><<<B, kWave, lds, st>>>(mp); break;
-                default: k_msa_wave<16, false, 2><<<B, kWave, lds, st>>>(mp); break;
-            }
-        }else if(q){
-            switch(kc){
-                case 2: k_msa_wave<2, true, 1><<<B, kWave, lds, st>>>(mp); break;
-                case 3: k_msa_wave<3, true, 1><<<B, kWave, lds, st>>>(mp); break;
-                case 4: k_msa_wave<4, true, 1><<<B, kWave, lds, st>>>(mp); break;
-                case 8: k_msa_wave<8, true, 1><<<B, kWave, lds, st>>>(mp); break;
-                case 12: k_msa_wave<12, true, 1><<<B, kWave, lds, st>>>(mp); break;
-                default: k_msa_wave<16, true, 1><<<B, kWave, lds, st>>>(mp); break;
-            }
-        }else{
-            switch(kc){
-                case 2: k_msa_wave<2, true, 0><<<B, kWave, lds, st>>>(mp); break;
-                case 3: k_msa_wave<3, true, 0><<<B, kWave, lds, st>>>(mp); break;
-                case 4: k_msa_wave<4, true, 0><<<B, kWave, lds, st>>>(mp); break;
-                case 8: k_msa_wave<8, true, 0><<<B, kWave, lds, st>>>(mp); break;
-                case 12: k_msa_wave<12, true, 0><<<B, kWave, lds, st>>>(mp); break;
-                default: k_msa_wave<16, true, 0><<<B, kWave, lds, st>>>(mp); break;
-            }
-        }
+        HIPCHECK(launchWaveMsa(mp, B, kc, wholeMsa, q, lds, st));
     }else{
         const size_t msaLds = msaLdsBytes(maxCols, ctx->maxLen, ctx->W, mp.tile, ctx->o.use_quality_scores != 0);
         if(msaLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_msa, hipFuncAttributeMaxDynamicSharedMemorySize, int(msaLds)));
         k_msa<<<B, kBlock, msaLds, st>>>(mp);
+        HIPCHECK(hipGetLastError());
     }
-    HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[3], st));
 
     if(cands){
@@ -3823,7 +3888,7 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
                                    ctx->recOff.as<uint64_t>(), ctx->outBuf.as<uint8_t>(), B);
     HIPCHECK(hipGetLastError());
     const size_t old = out.size();
-    out.resize(old + outBytes);
+    HIPCHECK(out.resize(old + outBytes));
     if(outBytes) HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->outBuf.p, outBytes, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(ctx->ev[4], st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -3845,6 +3910,340 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, std::vector<uint8_t>& out, size_
     ctx->t.candidates += total;
     ctx->lastB0 = b0; ctx->lastB = uint64_t(B); ctx->lastTotal = total;
     return CARE_OK;
+}
+
+// Pipelined staged batches: no host synchronization inside a batch. The candidate arrays
+// are sized before a batch's candidates are counted (`cap` pairs per batch); a device
+// check flags a batch with more, whose kernels then see no candidates, and the host redoes
+// that batch alone with runBatch (exact sizes, same records). Batch i+1 is enqueued before
+// batch i's records are copied out: each batch slot (i % 2) has its own record buffer,
+// pinned status words and stage events, and the copy runs on the copy stream. With
+// use_graphs each batch's device work is one captured HIP graph (no memset node), replayed
+// while the batch's parameters and buffers hold.
+struct StagedLaunch {
+    CandParams cp;
+    AlignParams ap;
+    MsaParams mp;
+    uint64_t cap;           // candidate pairs the batch's arrays hold
+    int B, slot, wm, kc, slotBytes, capPerAnchor;
+    int wholeMsa, q;
+    size_t waveLds, candLds, msaLds;
+};
+
+// Status words of batch slot s: in `counters` at [64 + 32 s): over-capacity flag (u32),
+// candidate pairs (u32), bucket values gathered (u64), records (u64); copied after the
+// record bytes (u64) and the error flag (i32) to the pinned words at [64 + 64 s): bytes 0,
+// 8, then 16 on.
+uint32_t* slotStatus(care_ctx* ctx, int s){ return ctx->counters.as<uint32_t>() + 16 + 8 * s; }
+uint8_t* slotPinned(care_ctx* ctx, int s){ return reinterpret_cast<uint8_t*>(ctx->pinned) + 64 + 64 * s; }
+
+// The device work of one pipelined batch, after its processing order (batchOrder).
+// Stage events only when not captured.
+hipError_t enqueueStaged(care_ctx* ctx, const StagedLaunch& L, bool events){
+    hipStream_t st = ctx->stream;
+    const int B = L.B;
+    hipEvent_t* ev = ctx->pipeEv[L.slot];
+    uint32_t* status = slotStatus(ctx, L.slot);
+    hipError_t e = hipSuccess;
+#define ENQ(launch) do { launch; e = hipGetLastError(); if(e != hipSuccess) return e; } while(0)
+#define ENQE(expr) do { e = (expr); if(e != hipSuccess) return e; } while(0)
+    uint32_t* overCount = reinterpret_cast<uint32_t*>(ctx->candOver.as<uint8_t>() + size_t(B) * 4);
+    ENQ((k_fill_u32<<<dim3(uint32_t((B + 1 + 255) / 256)), 256, 0, st>>>(L.cp.uniq, uint64_t(B + 1), 0u)));
+    ENQ((k_zero_u32<<<1, 64, 0, st>>>(overCount, 1)));
+    ENQ((k_zero_u32<<<1, 64, 0, st>>>(status, 8)));
+    ENQ((k_candidates_wave<<<B, kWave, L.waveLds, st>>>(L.cp, ctx->candOver.as<uint32_t>(), overCount)));
+    ENQ((k_candidates_over<<<std::min(B, 4096), kBlock, L.candLds, st>>>(L.cp, ctx->candOver.as<uint32_t>(), overCount)));
+    ENQ((k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(
+        ctx->gath.as<uint32_t>(), B, reinterpret_cast<unsigned long long*>(status + 2))));
+    if(events) ENQE(hipEventRecord(ev[5], st));
+    size_t tb = ctx->scanTmp.bytes;
+    ENQE(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), B + 1, st));
+    ENQ((k_cap_status<<<1, 64, 0, st>>>(ctx->offsets.as<uint32_t>(), B, L.cap, status)));
+    ENQ((k_cap_clear<<<(B + 1 + 255) / 256, 256, 0, st>>>(ctx->offsets.as<uint32_t>(), B, status)));
+    ENQ((k_compact<<<(B + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, st>>>(
+        ctx->raw.as<uint32_t>(), L.capPerAnchor, ctx->offsets.as<uint32_t>(), B, ctx->ids.as<uint32_t>(), nullptr)));
+    if(ctx->o.paired){
+        const int pairs = B / 2;
+        ENQ((k_pair_flags<<<(pairs + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, st>>>(ctx->offsets.as<uint32_t>(), ctx->ids.as<uint32_t>(),
+                                                               ctx->paired.as<uint8_t>(), pairs)));
+    }
+    if(events) ENQE(hipEventRecord(ev[1], st));
+    ENQE(launchAlign(ctx, L.ap, B, 0, L.wm, st));
+    if(events) ENQE(hipEventRecord(ev[2], st));
+    ENQ((k_fill_u32<<<dim3(uint32_t((B + 1 + 255) / 256)), 256, 0, st>>>(ctx->recBytes.as<uint32_t>(), uint64_t(B + 1), 0u)));
+    if(L.kc > 0){
+        ENQE(launchWaveMsa(L.mp, B, L.kc, L.wholeMsa != 0, L.q != 0, L.msaLds, st));
+    }else{
+        ENQ((k_msa<<<B, kBlock, L.msaLds, st>>>(L.mp)));
+    }
+    if(events) ENQE(hipEventRecord(ev[3], st));
+    ENQ((k_widen_count<<<(B + 1 + kWidenPerBlock - 1) / kWidenPerBlock, 256, 0, st>>>(ctx->recBytes.as<uint32_t>(), ctx->recOff64.as<uint64_t>(), B + 1,
+                                                            reinterpret_cast<unsigned long long*>(status + 4))));
+    tb = ctx->scanTmp.bytes;
+    ENQE(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), B + 1, st));
+    ENQ((k_gather<<<B, kBlock, 0, st>>>(ctx->slots.as<uint8_t>(), L.slotBytes, ctx->recBytes.as<uint32_t>(),
+                                        ctx->recOff.as<uint64_t>(), ctx->pipeOut[L.slot].as<uint8_t>(), B)));
+    uint8_t* pw = slotPinned(ctx, L.slot);
+    ENQE(hipMemcpyAsync(pw, ctx->recOff.as<uint64_t>() + B, 8, hipMemcpyDeviceToHost, st));
+    ENQE(hipMemcpyAsync(pw + 8, ctx->counters.as<int>() + 2, 4, hipMemcpyDeviceToHost, st));
+    ENQE(hipMemcpyAsync(pw + 16, status, 32, hipMemcpyDeviceToHost, st));
+#undef ENQ
+#undef ENQE
+    return hipSuccess;
+}
+
+// Buffers of pipelined batches of at most Bmax anchors and `cap` candidate pairs, sized
+// before any batch is enqueued (growing one frees the old allocation, which waits for the
+// device), and the launch constants shared by the batches.
+int prepareStaged(care_ctx* ctx, int Bmax, uint64_t cap, StagedLaunch& L){
+    hipStream_t st = ctx->stream;
+    std::memset(&L, 0, sizeof(L));          // padding included: the bytes key the batch's graph
+    L.capPerAnchor = ctx->M * ctx->thr;
+    int P = 64;
+    while(P <= L.capPerAnchor) P <<= 1;
+    L.cap = cap;
+    L.slotBytes = recordSlotBytes(ctx->maxLen);
+    L.wm = alignLaneWords(ctx->W);
+    const size_t T = size_t(std::max<uint64_t>(cap, 1));
+    HIPCHECK(ctx->raw.ensure(size_t(Bmax) * L.capPerAnchor * sizeof(uint32_t)));
+    HIPCHECK(ctx->uniq.ensure(size_t(Bmax + 1) * 4));
+    HIPCHECK(ctx->offsets.ensure(size_t(Bmax + 1) * 4));
+    HIPCHECK(ctx->gath.ensure(size_t(Bmax) * 4));
+    HIPCHECK(ctx->candOver.ensure(size_t(Bmax) * 4 + 16));
+    HIPCHECK(ctx->ids.ensure(T * 4));
+    HIPCHECK(ctx->best.ensure(T * sizeof(AlignBest)));
+    if(ctx->o.paired) HIPCHECK(ctx->paired.ensure(T));
+    HIPCHECK(ctx->kId.ensure(T * 4)); HIPCHECK(ctx->kShift.ensure(T * 4)); HIPCHECK(ctx->kLen.ensure(T * 4));
+    HIPCHECK(ctx->kNops.ensure(T * 4)); HIPCHECK(ctx->kOvl.ensure(T * 4)); HIPCHECK(ctx->kW.ensure(T * 4));
+    HIPCHECK(ctx->kOrient.ensure(T)); HIPCHECK(ctx->kPaired.ensure(T));
+    HIPCHECK(ctx->slots.ensure(size_t(Bmax) * L.slotBytes));
+    HIPCHECK(ctx->recBytes.ensure(size_t(Bmax + 1) * 4));
+    HIPCHECK(ctx->recOff64.ensure(size_t(Bmax + 1) * 8));
+    HIPCHECK(ctx->recOff.ensure(size_t(Bmax + 1) * 8));
+    for(auto& b : ctx->pipeOut) HIPCHECK(b.ensure(size_t(Bmax) * L.slotBytes));
+    HIPCHECK(ctx->orderFlags.ensure(ctx->n + 16));
+    HIPCHECK(ctx->batchOrd.ensure(size_t(Bmax) * 4 + 16));
+    // one scan workspace for the batch order selection and both scans (sizes grow with
+    // the item count)
+    size_t t1 = 0, t2 = 0, t3 = 0;
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t1, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), Bmax + 1, st));
+    HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, t2, ctx->recOff64.as<uint64_t>(), ctx->recOff.as<uint64_t>(), Bmax + 1, st));
+    HIPCHECK(hipcub::DeviceSelect::Flagged(nullptr, t3, ctx->order.as<uint32_t>(), ctx->orderFlags.as<uint8_t>(),
+                                           ctx->batchOrd.as<uint32_t>(), ctx->batchOrd.as<uint32_t>(), int(ctx->n), st));
+    HIPCHECK(ctx->scanTmp.ensure(std::max({t1, t2, t3, size_t(256)})));
+
+    CandParams& cp = L.cp;
+    cp.sig = ctx->sig.as<uint64_t>(); cp.n = ctx->n; cp.M = ctx->M; cp.k = ctx->k; cp.thr = ctx->thr;
+    cp.bucketOf = ctx->bucketOf.as<uint2>();
+    cp.excludeAmbiguous = ctx->o.exclude_ambiguous; cp.len = ctx->len.as<int>(); cp.ambig = ctx->ambig.as<uint8_t>();
+    cp.slots = ctx->slotPtrs.as<const Slot*>(); cp.caps = ctx->capsDev.as<uint32_t>(); cp.values = ctx->values.as<uint32_t>();
+    cp.capPerAnchor = L.capPerAnchor; cp.hashSlots = P; cp.raw = ctx->raw.as<uint32_t>(); cp.uniq = ctx->uniq.as<uint32_t>();
+    cp.gathered = nullptr; cp.gath = ctx->gath.as<uint32_t>();
+    L.waveLds = candWaveLdsBytes(L.capPerAnchor);
+    L.candLds = candLdsBytes(P, L.capPerAnchor);
+    if(L.waveLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates_wave, hipFuncAttributeMaxDynamicSharedMemorySize, int(L.waveLds)));
+    if(L.candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates_over, hipFuncAttributeMaxDynamicSharedMemorySize, int(L.candLds)));
+
+    AlignParams& ap = L.ap;
+    fillAlignCommon(ctx, ap);
+    ap.ids = ctx->ids.as<uint32_t>(); ap.anchorOf = nullptr; ap.total = 0;
+    ap.best = ctx->best.as<AlignBest>(); ap.dbg = nullptr;
+    ap.offsets = ctx->offsets.as<uint32_t>(); ap.maxLen = ctx->maxLen;
+
+    MsaParams& mp = L.mp;
+    fillMsaCommon(ctx, mp);
+    mp.offsets = ctx->offsets.as<uint32_t>(); mp.ids = ctx->ids.as<uint32_t>(); mp.best = ctx->best.as<AlignBest>();
+    mp.paired = ctx->o.paired ? ctx->paired.as<uint8_t>() : nullptr;
+    mp.kId = ctx->kId.as<uint32_t>(); mp.kShift = ctx->kShift.as<int>(); mp.kLen = ctx->kLen.as<int>();
+    mp.kNops = ctx->kNops.as<int>(); mp.kOvl = ctx->kOvl.as<int>(); mp.kW = ctx->kW.as<float>();
+    mp.kOrient = ctx->kOrient.as<uint8_t>(); mp.kPaired = ctx->kPaired.as<uint8_t>();
+    mp.maxCols = ctx->maxCols;
+    mp.slots = ctx->slots.as<uint8_t>(); mp.slotBytes = L.slotBytes; mp.recBytes = ctx->recBytes.as<uint32_t>();
+    mp.info = nullptr; mp.msaDbg = nullptr; mp.dbgStride = ctx->dbgStride;
+    mp.error = ctx->counters.as<int>() + 2;
+    L.q = ctx->o.use_quality_scores != 0;
+    L.wholeMsa = 0;
+    L.kc = waveMsaAnchorSlots(ctx->maxLen);
+    if(L.kc > 0){
+        mp.tile = ctx->o.msa_tile ? ctx->o.msa_tile : defaultWaveMsaTile(L.q != 0);
+        L.msaLds = waveMsaLdsBytes(mp.tile, ctx->W, ctx->maxLen, L.q != 0, mp.seamMode != 0 || mp.featOut != nullptr, mp.maxCols);
+    }else{
+        mp.tile = msaTile(ctx->maxCols, ctx->maxLen, ctx->W, L.q != 0);
+        L.msaLds = msaLdsBytes(ctx->maxCols, ctx->maxLen, ctx->W, mp.tile, L.q != 0);
+        if(L.msaLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_msa, hipFuncAttributeMaxDynamicSharedMemorySize, int(L.msaLds)));
+    }
+    return CARE_OK;
+}
+
+// Enqueues batch [b0, b0+B) into slot `slot`: its processing order, then its device work,
+// eagerly or as a graph.
+int launchStaged(care_ctx* ctx, StagedLaunch& L, uint64_t b0, int B, int slot){
+    hipStream_t st = ctx->stream;
+    hipEvent_t* ev = ctx->pipeEv[slot];
+    L.B = B; L.slot = slot;
+    HIPCHECK(hipEventRecord(ev[0], st));
+    const uint32_t* order = nullptr;
+    {
+        const int rc = batchOrder(ctx, b0, B, &order);
+        if(rc != CARE_OK) return rc;
+    }
+    L.cp.b0 = b0; L.cp.B = B; L.cp.order = order;
+    L.ap.b0 = b0; L.ap.B = B; L.ap.order = order;
+    L.mp.b0 = b0; L.mp.B = B; L.mp.order = order;
+    if(!ctx->o.use_graphs){
+        const hipError_t e = enqueueStaged(ctx, L, true);
+        if(e != hipSuccess) return ctx->fail(CARE_EHIP, "staged batch launch failed: %s", hipGetErrorString(e));
+        HIPCHECK(hipEventRecord(ev[4], st));
+        return CARE_OK;
+    }
+    // the key: the launch constants and parameters, and every address the captured work
+    // uses beyond them (scan workspace, status words, record buffers, pinned words)
+    const uint64_t extra[] = {uint64_t(reinterpret_cast<uintptr_t>(ctx->scanTmp.p)), uint64_t(ctx->scanTmp.bytes),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->offsets.p)),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->candOver.p)),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->counters.p)),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->recOff64.p)),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->recOff.p)),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->pipeOut[slot].p)),
+                              uint64_t(reinterpret_cast<uintptr_t>(ctx->pinned))};
+    std::vector<uint8_t> key(sizeof(L) + sizeof(extra));
+    std::memcpy(key.data(), &L, sizeof(L));
+    std::memcpy(key.data() + sizeof(L), extra, sizeof(extra));
+    auto it = ctx->stagedGraphs.find(key);
+    if(it == ctx->stagedGraphs.end()){
+        if(ctx->stagedGraphs.size() >= 16){
+            // a pass has at most a few batch shapes; drop graphs of earlier ones (none is
+            // in flight: the previous pass ended with a synchronization)
+            HIPCHECK(hipStreamSynchronize(st));
+            for(auto& g : ctx->stagedGraphs) (void)hipGraphExecDestroy(g.second);
+            ctx->stagedGraphs.clear();
+        }
+        HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const hipError_t e = enqueueStaged(ctx, L, false);
+        hipGraph_t g = nullptr;
+        const hipError_t ee = hipStreamEndCapture(st, &g);
+        if(e != hipSuccess || ee != hipSuccess){
+            if(g) (void)hipGraphDestroy(g);
+            return ctx->fail(CARE_EHIP, "capturing a staged batch failed: %s", hipGetErrorString(e != hipSuccess ? e : ee));
+        }
+        hipGraphExec_t x = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if(ie != hipSuccess) return ctx->fail(CARE_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(ie));
+        it = ctx->stagedGraphs.emplace(std::move(key), x).first;
+        ctx->graphsBuilt += 1;
+    }
+    HIPCHECK(hipGraphLaunch(it->second, st));
+    HIPCHECK(hipEventRecord(ev[4], st));
+    return CARE_OK;
+}
+
+// Waits for batch [b0, b0+B) of slot `slot`, appends its records to `out` (or redoes the
+// batch with runBatch when it had more candidates than its arrays hold) and adds its
+// timings. `redone` is set when the batch was redone.
+int finishStaged(care_ctx* ctx, uint64_t b0, int B, int slot, HostRecords& out, size_t& outRecords,
+                 unsigned long long& gathered, bool& redone){
+    hipEvent_t* ev = ctx->pipeEv[slot];
+    HIPCHECK(hipEventSynchronize(ev[4]));
+    const uint8_t* pw = slotPinned(ctx, slot);
+    uint64_t outBytes = 0, gath = 0, recs = 0;
+    int errFlag = 0;
+    uint32_t over = 0, total = 0;
+    std::memcpy(&outBytes, pw, 8);
+    std::memcpy(&errFlag, pw + 8, 4);
+    std::memcpy(&over, pw + 16, 4);
+    std::memcpy(&total, pw + 20, 4);
+    std::memcpy(&gath, pw + 24, 8);
+    std::memcpy(&recs, pw + 32, 8);
+    if(errFlag) return ctx->fail(CARE_EINVAL, "MSA column bound exceeded in batch at %llu", (unsigned long long)b0);
+    ctx->candPerAnchor = std::max(ctx->candPerAnchor, double(total) / double(std::max(B, 1)));
+    float ms = 0;
+    if(ctx->o.use_graphs){
+        HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[4])); ctx->t.graph_ms += ms;
+    }else{
+        HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[1])); ctx->t.candidates_ms += ms;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[0], ev[5])); ctx->t.cand_kernel_ms += ms;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[1], ev[2])); ctx->t.align_ms += ms; ctx->t.align_kernel_ms += ms;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[2], ev[3])); ctx->t.msa_ms += ms; ctx->t.msa_kernel_ms += ms;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[3], ev[4])); ctx->t.output_ms += ms;
+    }
+    redone = over != 0;
+    if(redone){
+        // the next batch may be in flight and uses the arrays runBatch may reallocate:
+        // wait for it first
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        ctx->t.batches_redone += 1;
+        return runBatch(ctx, b0, B, out, outRecords);
+    }
+    gathered += gath;
+    ctx->t.candidates += total;
+    const size_t old = out.size();
+    HIPCHECK(out.resize(old + outBytes));
+    if(outBytes){
+        HIPCHECK(hipMemcpyAsync(out.data() + old, ctx->pipeOut[slot].p, outBytes, hipMemcpyDeviceToHost, ctx->copyStream));
+        HIPCHECK(hipStreamSynchronize(ctx->copyStream));
+    }
+    outRecords += recs;
+    ctx->lastB0 = b0; ctx->lastB = uint64_t(B); ctx->lastTotal = total;
+    return CARE_OK;
+}
+
+// Candidate pairs the pipelined arrays of a batch are sized for: the most per anchor seen
+// so far on this context with a quarter of headroom; before any batch was counted, the
+// option's count per anchor, else four per unit of coverage. Never below what the context
+// used before, so that repeated passes keep their arrays and graphs.
+uint64_t stagedCapacity(care_ctx* ctx, int Bmax){
+    double per;
+    if(ctx->candPerAnchor <= 0 && ctx->o.pipeline_capacity > 0){
+        per = ctx->o.pipeline_capacity;
+    }else{
+        const int capPerAnchor = ctx->M * ctx->thr;
+        per = ctx->candPerAnchor > 0 ? ctx->candPerAnchor * 1.25 + 8 : 4.0 * ctx->o.estimated_coverage + 16;
+        per = std::min<double>(std::max(per, 32.0), capPerAnchor);
+    }
+    per = std::ceil(std::max(per, ctx->pipePerAnchor));
+    ctx->pipePerAnchor = per;
+    return std::min<uint64_t>(uint64_t(per) * uint64_t(Bmax), uint64_t(UINT32_MAX));
+}
+
+// Anchors [first, first+n) in pipelined batches of at most Bmax anchors.
+int runPipelined(care_ctx* ctx, uint64_t first, uint64_t n, int Bmax, HostRecords& out, size_t& outRecords,
+                 unsigned long long& gathered){
+    std::vector<std::pair<uint64_t, int>> batches;
+    for(uint64_t b0 = first; b0 < first + n;){
+        const uint64_t e = std::min<uint64_t>(b0 + uint64_t(Bmax), first + n);
+        batches.emplace_back(b0, int(e - b0));
+        b0 = e;
+    }
+    const int Bfirst = batches.empty() ? 0 : batches[0].second;
+    StagedLaunch L;
+    int rc = prepareStaged(ctx, Bfirst, stagedCapacity(ctx, Bfirst), L);
+    if(rc != CARE_OK) return rc;
+    const size_t nb = batches.size();
+    for(size_t i = 0; i < nb; i++){
+        rc = launchStaged(ctx, L, batches[i].first, batches[i].second, int(i & 1));
+        if(rc != CARE_OK) break;
+        if(i == 0) continue;
+        bool redone = false;
+        rc = finishStaged(ctx, batches[i - 1].first, batches[i - 1].second, int((i - 1) & 1), out, outRecords, gathered, redone);
+        if(rc != CARE_OK) break;
+        ctx->t.batches++;
+        if(redone){
+            // runBatch may have reallocated the arrays (batch i, enqueued before it, has
+            // ended): the batches after i get the current arrays, sized from what has been
+            // seen; batch i may be redone as well
+            HIPCHECK(hipStreamSynchronize(ctx->stream));
+            rc = prepareStaged(ctx, Bfirst, std::max(L.cap, stagedCapacity(ctx, Bfirst)), L);
+            if(rc != CARE_OK) break;
+        }
+    }
+    if(rc == CARE_OK && nb){
+        bool redone = false;
+        rc = finishStaged(ctx, batches[nb - 1].first, batches[nb - 1].second, int((nb - 1) & 1), out, outRecords, gathered, redone);
+        if(rc == CARE_OK) ctx->t.batches++;
+    }
+    if(rc != CARE_OK) (void)hipStreamSynchronize(ctx->stream);    // nothing left in flight on failure
+    return rc;
 }
 
 }  // namespace
@@ -3895,6 +4294,8 @@ static int care_hip_create_body(const care_opts* opts, int device, care_ctx** ou
     if(opts->new_columns_to_correct < 0) return CARE_EINVAL;
     if(opts->msa_tile != 0 && (opts->msa_tile < 4 || opts->msa_tile > 64 || (opts->msa_tile & 3))) return CARE_EINVAL;
     if(opts->seeded_alignment != 0 && opts->seeded_alignment != 1) return CARE_EINVAL;
+    if(opts->sync_batches != 0 && opts->sync_batches != 1) return CARE_EINVAL;
+    if(opts->pipeline_capacity < 0) return CARE_EINVAL;
     int count = 0;
     if(hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return CARE_ENODEV;
     auto* ctx = new (std::nothrow) care_ctx;
@@ -3905,9 +4306,16 @@ static int care_hip_create_body(const care_opts* opts, int device, care_ctx** ou
         delete ctx;
         return CARE_EHIP;
     }
-    if(hipHostMalloc(&ctx->pinned, 64, hipHostMallocDefault) != hipSuccess){ care_hip_destroy(ctx); return CARE_ENOMEM; }
+    // pinned words: [0, 64) the last pass, [64 + 64 s, 128 + 64 s) pipelined batch slot s
+    if(hipHostMalloc(&ctx->pinned, 256, hipHostMallocDefault) != hipSuccess){ care_hip_destroy(ctx); return CARE_ENOMEM; }
+    if(hipStreamCreateWithFlags(&ctx->copyStream, hipStreamNonBlocking) != hipSuccess){ care_hip_destroy(ctx); return CARE_EHIP; }
     for(auto& e : ctx->ev){
-        if(hipEventCreate(&e) != hipSuccess){ delete ctx; return CARE_EHIP; }
+        if(hipEventCreate(&e) != hipSuccess){ care_hip_destroy(ctx); return CARE_EHIP; }
+    }
+    for(auto& row : ctx->pipeEv){
+        for(auto& e : row){
+            if(hipEventCreate(&e) != hipSuccess){ care_hip_destroy(ctx); return CARE_EHIP; }
+        }
     }
     *out = ctx;
     return CARE_OK;
@@ -3926,13 +4334,17 @@ void care_hip_destroy(care_ctx* ctx){
                    &ctx->fqHdrLen, &ctx->fqSeq, &ctx->fqQual, &ctx->mRec, &ctx->mOff, &ctx->mKeyIn, &ctx->mKeyOut,
                    &ctx->mIdxIn, &ctx->mIdxOut, &ctx->mRunBeg, &ctx->mRunEnd, &ctx->mPlan, &ctx->mSize, &ctx->mOutOff,
                    &ctx->mText, &ctx->mErr, &ctx->candSlots, &ctx->candBytes, &ctx->insSize, &ctx->insOff,
-                   &ctx->insSrc, &ctx->hqAnchor}){
+                   &ctx->insSrc, &ctx->hqAnchor, &ctx->seeds, &ctx->pipeOut[0], &ctx->pipeOut[1]}){
         b->release();
     }
     if(ctx->fusedExec) (void)hipGraphExecDestroy(ctx->fusedExec);
+    for(auto& g : ctx->stagedGraphs) (void)hipGraphExecDestroy(g.second);
     if(ctx->pinned) (void)hipHostFree(ctx->pinned);
+    ctx->records.release();
     for(auto& b : ctx->slotBufs) b.release();
     for(auto& e : ctx->ev) if(e) (void)hipEventDestroy(e);
+    for(auto& row : ctx->pipeEv) for(auto& e : row) if(e) (void)hipEventDestroy(e);
+    if(ctx->copyStream) (void)hipStreamDestroy(ctx->copyStream);
     if(ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -5307,8 +5719,8 @@ static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size
     ctx->t = care_timings{};
     ctx->records.clear();
     ctx->numRecords = 0;
-    HIPCHECK(ctx->counters.ensure(64));
-    HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, 64, ctx->stream));
+    HIPCHECK(ctx->counters.ensure(128));
+    HIPCHECK(hipMemsetAsync(ctx->counters.p, 0, 128, ctx->stream));
     int B = ctx->o.batch_size > 0 ? ctx->o.batch_size : 0;
     if(B <= 0){
         // one batch per pass where HBM allows: the per-anchor candidate staging area
@@ -5346,9 +5758,18 @@ static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size
         HIPCHECK(ctx->hqAnchor.ensure(ctx->n));
         ctx->callFirst = first;
     }
+    // pipelined batches (no host synchronization inside a batch) for the plain and learned
+    // correction with the lane-per-candidate or seeded alignment; candidate correction,
+    // intermediate dumps and feature output synchronize per batch
+    const bool pipelined = !fused && !cands && !ctx->o.keep_intermediates && !ctx->featActive && !ctx->o.sync_batches &&
+                           (ctx->o.seeded_alignment || (ctx->o.align_variant != 1 && alignLaneWords(ctx->W) > 0));
+    unsigned long long pipeGathered = 0;
     const auto t0 = std::chrono::steady_clock::now();
     if(fused){
         const int rc = runFused(ctx, first, n, ctx->records, ctx->numRecords);
+        if(rc != CARE_OK) return rc;
+    }else if(pipelined){
+        const int rc = runPipelined(ctx, first, n, B, ctx->records, ctx->numRecords, pipeGathered);
         if(rc != CARE_OK) return rc;
     }else{
         for(uint64_t b0 = first; b0 < first + n;){
@@ -5362,7 +5783,7 @@ static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size
     }
     unsigned long long cnt[4] = {0, 0, 0, 0};
     HIPCHECK(hipMemcpy(cnt, ctx->counters.p, 32, hipMemcpyDeviceToHost));
-    const unsigned long long gathered = cnt[0];
+    const unsigned long long gathered = cnt[0] + pipeGathered;
     if(fused) ctx->t.candidates = cnt[3];
     const auto t1 = std::chrono::steady_clock::now();
     ctx->t.total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
@@ -5380,6 +5801,13 @@ static int care_hip_copy_records_body(care_ctx* ctx, uint8_t* dst, size_t cap){
     if(!ctx || (!dst && !ctx->records.empty())) return CARE_EINVAL;
     if(cap < ctx->records.size()) return ctx->fail(CARE_EINVAL, "buffer too small");
     if(!ctx->records.empty()) std::memcpy(dst, ctx->records.data(), ctx->records.size());
+    return CARE_OK;
+}
+
+static int care_hip_records_view_body(const care_ctx* ctx, const uint8_t** data, size_t* bytes){
+    if(!ctx || !data || !bytes) return CARE_EINVAL;
+    *data = ctx->records.data();
+    *bytes = ctx->records.size();
     return CARE_OK;
 }
 
@@ -5598,6 +6026,10 @@ int care_hip_correct(care_ctx* ctx, uint64_t first, uint64_t n, size_t* num_reco
 
 int care_hip_copy_records(care_ctx* ctx, uint8_t* dst, size_t cap){
     return guarded(ctx, [&]() { return care_hip_copy_records_body(ctx, dst, cap); });
+}
+
+int care_hip_records_view(const care_ctx* ctx, const uint8_t** data, size_t* bytes){
+    return guarded(nullptr, [&]() { return care_hip_records_view_body(ctx, data, bytes); });
 }
 
 int care_hip_set_classifier(care_ctx* ctx, const float* params, size_t count){

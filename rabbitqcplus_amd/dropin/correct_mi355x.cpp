@@ -89,8 +89,11 @@ SerializedObjectStorage correct(const ProgramOptions& po, const CpuReadStorage& 
     e.check(care_hip_build_tables(e.ctx, &maps), "care_hip_build_tables");
     std::size_t numRecords = 0, numBytes = 0;
     e.check(care_hip_correct(e.ctx, 0, n, &numRecords, &numBytes), "care_hip_correct");
-    std::vector<std::uint8_t> bytes(numBytes);
-    e.check(care_hip_copy_records(e.ctx, bytes.data(), bytes.size()), "care_hip_copy_records");
+    // the records in the engine's host buffer, read in place (valid while the context lives)
+    const std::uint8_t* rec = nullptr;
+    std::size_t recBytes = 0;
+    e.check(care_hip_records_view(e.ctx, &rec, &recBytes), "care_hip_records_view");
+    if(recBytes != numBytes) throw std::runtime_error("MI355X correction: record byte count mismatch");
 
     // the storage correct_cpu returns, sized as it sizes its own (src/correct_cpu.cpp:50-95):
     // host memory left after the read storage (no CPU minhasher here), minus 2 GiB; 3/4
@@ -100,13 +103,13 @@ SerializedObjectStorage correct(const ProgramOptions& po, const CpuReadStorage& 
     available = available > storageBytes ? available - storageBytes : 0;
     const std::size_t partial = available > (std::size_t(2) << 30) ? available - (std::size_t(2) << 30) : 0;
     SerializedObjectStorage out(std::size_t(partial * 0.75), std::size_t(partial * 0.25), po.tempdirectory + "/");
-    for(std::size_t p = 0; p < bytes.size();){
+    for(std::size_t p = 0; p < recBytes;){
         // one record: read id (u32), flags (u32: body size in the low 29 bits), body
         std::uint32_t flags;
-        std::memcpy(&flags, bytes.data() + p + 4, 4);
+        std::memcpy(&flags, rec + p + 4, 4);
         const std::size_t len = 8 + (flags & ((1u << 29) - 1));
-        if(p + len > bytes.size()) throw std::runtime_error("MI355X correction: truncated record stream");
-        out.insert(bytes.data() + p, bytes.data() + p + len);
+        if(p + len > recBytes) throw std::runtime_error("MI355X correction: truncated record stream");
+        out.insert(rec + p, rec + p + len);
         p += len;
     }
     return out;

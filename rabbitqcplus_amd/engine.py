@@ -40,7 +40,8 @@ class CareOpts(ctypes.Structure):
                 ("correction_type", ctypes.c_int), ("align_variant", ctypes.c_int),
                 ("use_graphs", ctypes.c_int), ("correct_candidates", ctypes.c_int),
                 ("new_columns_to_correct", ctypes.c_int), ("msa_tile", ctypes.c_int),
-                ("seeded_alignment", ctypes.c_int)]
+                ("seeded_alignment", ctypes.c_int), ("sync_batches", ctypes.c_int),
+                ("pipeline_capacity", ctypes.c_int)]
 
 
 class CareTimings(ctypes.Structure):
@@ -53,12 +54,14 @@ class CareTimings(ctypes.Structure):
                 ("records", ctypes.c_uint64), ("record_bytes", ctypes.c_uint64),
                 ("bucket_values", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("graphs_built", ctypes.c_uint64), ("fastq_device_ms", ctypes.c_double),
-                ("merge_device_ms", ctypes.c_double)]
+                ("merge_device_ms", ctypes.c_double), ("graph_ms", ctypes.c_double),
+                ("batches_redone", ctypes.c_uint64)]
 
 
 EXPORTS = [
     "care_hip_default_opts", "care_hip_create", "care_hip_destroy", "care_hip_last_error",
     "care_hip_set_reads", "care_hip_encode_reads", "care_hip_build_tables", "care_hip_correct", "care_hip_copy_records",
+    "care_hip_records_view",
     "care_hip_get_timings", "care_hip_inspect_signatures", "care_hip_inspect_batch",
     "care_hip_inspect_msa", "care_hip_set_classifier", "care_hip_extract_features", "care_hip_device_info",
     "care_hip_save_tables", "care_hip_load_tables", "care_hip_set_reads_fastq", "care_hip_read_layout",
@@ -98,6 +101,7 @@ def load_library(path: str = LIB_PATH):
     lib.care_hip_build_tables.argtypes = [vp, ctypes.POINTER(i32)]
     lib.care_hip_correct.argtypes = [vp, u64, u64, ctypes.POINTER(sz), ctypes.POINTER(sz)]
     lib.care_hip_copy_records.argtypes = [vp, vp, sz]
+    lib.care_hip_records_view.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(sz)]
     lib.care_hip_get_timings.argtypes = [vp, ctypes.POINTER(CareTimings)]
     lib.care_hip_inspect_signatures.argtypes = [vp, u64, u64, vp]
     lib.care_hip_inspect_batch.argtypes = [vp, ctypes.POINTER(u64), ctypes.POINTER(u64), ctypes.POINTER(u64),
@@ -292,9 +296,10 @@ class CareEngine:
         nrec, nbytes = ctypes.c_size_t(0), ctypes.c_size_t(0)
         self._check(self.lib.care_hip_correct(self.h, first, n, ctypes.byref(nrec), ctypes.byref(nbytes)),
                     "care_hip_correct")
-        buf = ctypes.create_string_buffer(max(1, nbytes.value))
-        self._check(self.lib.care_hip_copy_records(self.h, buf, nbytes.value), "care_hip_copy_records")
-        return nrec.value, buf.raw[:nbytes.value]
+        # one copy, from the context's pinned record buffer into the returned bytes
+        ptr, size = ctypes.c_void_p(), ctypes.c_size_t(0)
+        self._check(self.lib.care_hip_records_view(self.h, ctypes.byref(ptr), ctypes.byref(size)), "care_hip_records_view")
+        return nrec.value, ctypes.string_at(ptr.value, size.value) if size.value else b""
 
     def set_classifier(self, params) -> None:
         """MLP parameters (MLP_PARAMS float32 values) for correction_type=1; None unloads."""
