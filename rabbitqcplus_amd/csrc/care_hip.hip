@@ -950,6 +950,13 @@ __device__ __forceinline__ uint32_t overlapMask(int k){
     return uint32_t(0xFFFFFFFF00000000ull >> c);
 }
 
+// Word counts (from a scan's first word) after which the lane scan tests whether any lane
+// can still stay below its mismatch bound at the current shift.
+template<int WM>
+__device__ __forceinline__ constexpr bool abandonCheck(int w){
+    return WM >= 8 ? (w == 2 || w == 3) : (w == 2);
+}
+
 // The two scan directions of one lane (src/cpu_alignment.cpp:32-497). UNI: every valid
 // lane's candidate has the anchor's length, so the overlap of a shift, its max_errors
 // (the host-checked integer form, on the scalar unit) and the overlap masks (anchor
@@ -958,9 +965,17 @@ __device__ __forceinline__ uint32_t overlapMask(int k){
 template<int WM, bool UNI>
 __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t* AH, const uint32_t* AL, const uint32_t* AV,
                                            const uint32_t* FH, const uint32_t* FL, const uint32_t* RH, const uint32_t* RL,
-                                           const uint32_t* CV, int La, int Lc, int lastF, int mo, uint64_t start,
+                                           const uint32_t* CV, int La, int Lc, int LcMax, int lastF, int mo, uint64_t start,
                                            int& gF, int& sF, int& gR, int& sR){
     gF = 0; sF = -Lc; gR = 0; sR = -Lc;
+    // Reads of more than 160 bases (kLongWords): a shift's overlap lies in candidate
+    // positions [0, min(La - sft, Lc)) (anchor shifted) or [t, min(t + La, Lc)) (candidate
+    // shifted), so words from the wave-uniform bound on (LcMax: the longest candidate of
+    // the wave) add nothing and are skipped. Mismatch counts only grow with the words: at
+    // the words abandonCheck names, a shift is abandoned once no live lane can still stay
+    // below its bound (short reads: after two words, where random sequence already
+    // exceeds the bound of the longest overlap; long reads need a few more).
+    constexpr bool kLongWords = WM >= 8;
     // anchor shifted: shifts 0..lastF. A position counts when it lies inside both the
     // candidate (CV) and the shifted anchor (shifted AV): exactly the overlap. The
     // lanes whose scan in an orientation is still running are kept as wave masks
@@ -980,13 +995,13 @@ __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t*
             // lane is live, which rarely happens inside a block)
             liveF &= __builtin_amdgcn_ballot_w64(mF > 0);
             liveR &= __builtin_amdgcn_ballot_w64(mR > 0);
+            const int nw = kLongWords ? (min(La - sft, LcMax) + 31) >> 5 : WM;
             int dF = 0, dR = 0;
             #pragma unroll
             for(int w = 0; w < WM; w++){
-                // once no lane can still stay below its bound, the remaining words
-                // cannot make any shift admissible (mismatch counts only grow)
-                if(w == 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
-                              (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
+                if(kLongWords && w >= nw) break;
+                if(abandonCheck<WM>(w) && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                           (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
                 const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
                 const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
                 const uint32_t xv = uint32_t((((uint64_t(AV[w + ps + 1]) << 32) | AV[w + ps + 2]) << r) >> 32);
@@ -1021,11 +1036,13 @@ __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t*
             const uint64_t inRange = UNI ? ~0ull : __builtin_amdgcn_ballot_w64(t <= lastB);
             liveF &= inRange & __builtin_amdgcn_ballot_w64(mF > 0);
             liveR &= inRange & __builtin_amdgcn_ballot_w64(mR > 0);
+            const int nw = kLongWords ? (min(t + La, LcMax) + 31) >> 5 : WM;
             int dF = 0, dR = 0;
             #pragma unroll
             for(int w = pt; w < WM; w++){
-                if(w == pt + 2 && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
-                                   (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
+                if(kLongWords && w >= nw) break;
+                if(abandonCheck<WM>(w - pt) && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                                (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
                 const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
                 const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
                 const uint32_t xv = uint32_t(((uint64_t(AV[w - pt]) << 32) | AV[w - pt + 1]) >> r);
@@ -1084,10 +1101,15 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
         // gF, gR: best score minus total (score = total - 2 * overlap + mismatches)
         int gF, sF, gR, sR;
         const uint64_t start = __builtin_amdgcn_ballot_w64(valid);
+        // longest candidate of the chunk (wave-uniform)
+        int LcMax = Lc;
+        #pragma unroll
+        for(int o = 1; o < kWave; o <<= 1) LcMax = max(LcMax, __shfl_xor(LcMax, o));
+        LcMax = __builtin_amdgcn_readfirstlane(LcMax);
         if(p.eExact && __builtin_amdgcn_ballot_w64(valid && Lc != La) == 0)
-            scanShifts<WM, true>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, lastF, mo, start, gF, sF, gR, sR);
+            scanShifts<WM, true>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, LcMax, lastF, mo, start, gF, sF, gR, sR);
         else
-            scanShifts<WM, false>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, lastF, mo, start, gF, sF, gR, sR);
+            scanShifts<WM, false>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, LcMax, lastF, mo, start, gF, sF, gR, sR);
         const int bF = gF + total, bR = gR + total;
         if(!valid) continue;
         ShdResult f, rr;
