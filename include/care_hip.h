@@ -167,8 +167,8 @@ int care_hip_encode_reads(const uint8_t* seqs, size_t seq_pitch, const int32_t* 
  * paired input (records of the two texts interleaved R1, R2, R1, ...). Records are
  * 4-line FASTQ ("@header", sequence, "+...", quality; a '\r' ending a header, sequence
  * or quality of more than one character is dropped, as the reference's reader does,
- * include/kseqpp/kseqpp.hpp:286-288; blank lines at the end are ignored; texts of any
- * size that fits the device). Lower-case acgt become upper case; every other sequence character becomes
+ * include/kseqpp/kseqpp.hpp:286-288; blank lines at the end are ignored; texts of up to
+ * 64 GiB each that fit the device). Lower-case acgt become upper case; every other sequence character becomes
  * A, C, G, T in turn, the turn restarting at every batch of 65,536 reads, and marks the
  * read ambiguous. Qualities are kept as given (8 bits) when use_quality_scores is set.
  * Pitch = ceil(longest/16). The text stays on the device for care_hip_write_fastq. */

@@ -52,6 +52,16 @@ namespace {
 
 constexpr int kMaxMaps = 48;         // include/config.hpp:30
 constexpr int kBlock = 256;          // threads per workgroup of the per-anchor kernels
+// One-dimensional launches: the dispatch packet carries a grid's size in work items as a
+// 32-bit number, so element-wise kernels take a capped grid (gridFor) and stride over
+// [0, n); kernels with one wavefront or one 16-lane row per item stride by rows.
+constexpr uint64_t kMaxGridBlocks = uint64_t(1) << 20;    // x at most 1024 threads < 2^32 work items
+#define FOR_GRID(i, n) for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < uint64_t(n); \
+                            i += uint64_t(gridDim.x) * blockDim.x)
+inline dim3 gridFor(uint64_t items, uint64_t block){
+    const uint64_t b = (items + block - 1) / block;
+    return dim3(uint32_t(b < 1 ? 1 : (b > kMaxGridBlocks ? kMaxGridBlocks : b)));
+}
 constexpr int kWave = 64;
 constexpr int kMaxHiLoWords = 32;    // reads up to 1024 bases
 constexpr int kMaxSortEntries = 32768;  // per-anchor gathered bucket values held in LDS
@@ -188,9 +198,8 @@ struct OpOr  { __device__ int operator()(int a, int b) const { return a | b; } }
 __global__ void __launch_bounds__(256)
 k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n,
              int k, int M, uint64_t* __restrict__ sig, uint16_t* __restrict__ seeds){
-    const uint64_t r = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(r >= n) return;
     const uint64_t mask = ~0ull >> ((32 - k) * 2);
+    FOR_GRID(r, n){
     uint64_t mins[kMaxMaps];
     uint32_t seed[kMaxMaps];
     #pragma unroll
@@ -226,6 +235,7 @@ k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restri
             if(seeds) seeds[r * uint64_t(M) + uint64_t(m)] = uint16_t(seed[m]);
         }
     }
+    }
 }
 
 // HiLo planes (include/sequencehelpers.hpp:646-689): bit 31-(i%32) of word i/32 holds
@@ -235,8 +245,7 @@ k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restri
 __global__ void __launch_bounds__(256)
 k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n, int W,
        uint32_t* __restrict__ hilo, uint32_t* __restrict__ rcenc){
-    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(t >= n * uint64_t(W)) return;
+    FOR_GRID(t, n * uint64_t(W)){
     const uint64_t r = t / W;
     const int w = int(t % W);
     const int L = len[r];
@@ -263,21 +272,21 @@ k_hilo(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ l
     uint32_t* rout = rcenc + r * uint64_t(2 * W);
     rout[2 * w] = r0;
     rout[2 * w + 1] = r1;
+    }
 }
 
 // Rows of `sp` bytes to rows of `dp` bytes: truncated, or zero padded past sp.
 __global__ void k_repack_rows(const uint8_t* __restrict__ src, size_t sp, uint8_t* __restrict__ dst, size_t dp,
                               uint64_t total){
-    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(t >= total) return;
-    const uint64_t r = t / dp;
-    const uint64_t c = t - r * dp;
-    dst[t] = c < sp ? src[r * sp + c] : uint8_t(0);
+    FOR_GRID(t, total){
+        const uint64_t r = t / dp;
+        const uint64_t c = t - r * dp;
+        dst[t] = c < sp ? src[r * sp + c] : uint8_t(0);
+    }
 }
 
 __global__ void k_iota(uint32_t* v, uint64_t n){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i < n) v[i] = uint32_t(i);
+    FOR_GRID(i, n) v[i] = uint32_t(i);
 }
 
 // Buckets that survive the 2..threshold rule (include/groupbykey.hpp:176-215).
@@ -350,13 +359,13 @@ __device__ __forceinline__ int batchAnchor(const uint32_t* order, int B){
 // pass's 48 signature loads and hash probes per anchor into one 384-byte read.
 __global__ void k_bucket_of(const uint64_t* __restrict__ sig, uint64_t n, int M, const Slot* const* slots,
                             const uint32_t* __restrict__ caps, uint2* __restrict__ out){
-    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(t >= n * uint64_t(M)) return;
-    const int m = int(t / n);
-    const uint64_t a = t - uint64_t(m) * n;
-    uint32_t off = 0;
-    const uint32_t cnt = lookup(slots[m], caps[m], sig[t], &off);
-    out[a * uint64_t(M) + uint64_t(m)] = make_uint2(cnt ? off : 0u, cnt);
+    FOR_GRID(t, n * uint64_t(M)){
+        const int m = int(t / n);
+        const uint64_t a = t - uint64_t(m) * n;
+        uint32_t off = 0;
+        const uint32_t cnt = lookup(slots[m], caps[m], sig[t], &off);
+        out[a * uint64_t(M) + uint64_t(m)] = make_uint2(cnt ? off : 0u, cnt);
+    }
 }
 
 // Inclusive prefix sum over the wave (DPP row shifts, then row broadcasts).
@@ -884,7 +893,10 @@ k_align(AlignParams p){
     __shared__ uint2 planes[kAlignWaves][6][kMaxHiLoWords + 1];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
-    const uint32_t c = blockIdx.x * kAlignWaves + wave;
+    // workgroups stride over the pairs, kAlignWaves at a time (the same trip count for the
+    // whole workgroup, which meets the barriers every time)
+    for(uint64_t base = uint64_t(blockIdx.x) * kAlignWaves; base < p.total; base += uint64_t(gridDim.x) * kAlignWaves){
+    const uint32_t c = uint32_t(base) + uint32_t(wave);
     const bool active = c < p.total;
     uint32_t id = 0;
     uint64_t a = 0;
@@ -904,7 +916,7 @@ k_align(AlignParams p){
         }
     }
     __syncthreads();
-    if(!active) return;
+    if(active){
     const int La = p.len[a];
     const int Lc = p.len[id];
     ShdResult f, r;
@@ -925,6 +937,9 @@ k_align(AlignParams p){
             d[5] = r.score; d[6] = r.overlap; d[7] = r.shift; d[8] = r.nOps; d[9] = r.valid;
             d[10] = flag;
         }
+    }
+    }
+    __syncthreads();            // the planes are staged again for the next pairs
     }
 }
 
@@ -2901,8 +2916,7 @@ k_fused(FusedParams fp){
 }
 
 __global__ void k_fill_u32(uint32_t* __restrict__ v, uint64_t n, uint32_t x){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i < n) v[i] = x;
+    FOR_GRID(i, n) v[i] = x;
 }
 
 __global__ void k_zero_u32(uint32_t* v, int n){
@@ -2988,7 +3002,7 @@ __global__ void k_insertion_order(const uint32_t* __restrict__ recBytes, const u
                                   uint64_t b0, int B, uint32_t total, const uint8_t* slots, int slotBytes,
                                   const uint8_t* candSlots, int candSlotBytes, uint64_t* __restrict__ size,
                                   uint64_t* __restrict__ src){
-    const uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    FOR_GRID(e, uint64_t(B) + total){
     if(e < uint64_t(B)){
         const int i = int(e);
         const uint64_t a = b0 + uint64_t(i);
@@ -3006,19 +3020,21 @@ __global__ void k_insertion_order(const uint32_t* __restrict__ recBytes, const u
         size[pos] = candBytes[q];
         src[pos] = reinterpret_cast<uint64_t>(candSlots + size_t(q) * candSlotBytes);
     }
+    }
 }
 
 // One wavefront per entry: copies its record to its offset in the batch's stream.
 __global__ void __launch_bounds__(256) k_gather_entries(const uint64_t* __restrict__ size, const uint64_t* __restrict__ src,
                                                         const uint64_t* __restrict__ off, uint8_t* __restrict__ out,
                                                         uint64_t entries){
-    const uint64_t e = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if(e >= entries) return;
-    const uint64_t nb = size[e];
-    const uint8_t* s = reinterpret_cast<const uint8_t*>(src[e]);
-    uint8_t* d = out + off[e];
-    for(uint64_t q = lane; q < nb; q += 64) d[q] = s[q];
+    for(uint64_t e = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; e < entries;
+        e += (uint64_t(gridDim.x) * blockDim.x) >> 6){
+        const uint64_t nb = size[e];
+        const uint8_t* s = reinterpret_cast<const uint8_t*>(src[e]);
+        uint8_t* d = out + off[e];
+        for(uint64_t q = lane; q < nb; q += 64) d[q] = s[q];
+    }
 }
 
 // ---------------------------------------------------------------- host side
@@ -3188,7 +3204,7 @@ int buildTables(care_ctx* ctx){
     HIPCHECK(runOffsets.ensure(std::max<uint64_t>(1, n) * sizeof(uint32_t)));
     HIPCHECK(numRuns.ensure(sizeof(int)));
     HIPCHECK(kept.ensure(sizeof(uint32_t)));
-    if(n) k_iota<<<dim3((n + 255) / 256), dim3(256), 0, st>>>(iota.as<uint32_t>(), n);
+    if(n) k_iota<<<gridFor(n, 256), dim3(256), 0, st>>>(iota.as<uint32_t>(), n);
     HIPCHECK(hipGetLastError());
 
     size_t tb1 = 0, tb2 = 0, tb3 = 0;
@@ -3606,7 +3622,7 @@ int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int 
     HIPCHECK(ctx->insOff.ensure((entries + 1) * 8));
     HIPCHECK(ctx->insSrc.ensure(entries * 8));
     k_fill_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(ctx->insSize.as<uint64_t>() + entries), 2, 0u);
-    k_insertion_order<<<dim3(uint32_t((entries + 255) / 256)), 256, 0, st>>>(
+    k_insertion_order<<<gridFor(entries, 256), 256, 0, st>>>(
         ctx->recBytes.as<uint32_t>(), ctx->candBytes.as<uint32_t>(), ctx->offsets.as<uint32_t>(), ctx->anchorOf.as<uint32_t>(),
         b0, B, total, ctx->slots.as<uint8_t>(), slotBytes, ctx->candSlots.as<uint8_t>(), candSlotBytes,
         ctx->insSize.as<uint64_t>(), ctx->insSrc.as<uint64_t>());
@@ -3625,7 +3641,7 @@ int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int 
     HIPCHECK(hipStreamSynchronize(st));
     if(errFlag) return ctx->fail(CARE_EINVAL, "MSA column bound exceeded in batch at %llu", (unsigned long long)b0);
     HIPCHECK(ctx->outBuf.ensure(std::max<uint64_t>(outBytes, 1)));
-    k_gather_entries<<<dim3(uint32_t((entries * 64 + 255) / 256)), 256, 0, st>>>(
+    k_gather_entries<<<gridFor(entries * 64, 256), 256, 0, st>>>(
         ctx->insSize.as<uint64_t>(), ctx->insSrc.as<uint64_t>(), ctx->insOff.as<uint64_t>(), ctx->outBuf.as<uint8_t>(),
         entries);
     HIPCHECK(hipGetLastError());
@@ -3655,8 +3671,7 @@ int gatherInsertionOrder(care_ctx* ctx, uint64_t b0, int B, uint32_t total, int 
 // the batch, as batch indices.
 __global__ void k_order_flags(const uint32_t* __restrict__ order, uint64_t n, uint64_t b0, uint64_t B,
                               uint8_t* __restrict__ flags){
-    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(t < n) flags[t] = order[t] >= b0 && order[t] < b0 + B;
+    FOR_GRID(t, n) flags[t] = order[t] >= b0 && order[t] < b0 + B;
 }
 
 __global__ void k_order_rebase(uint32_t* __restrict__ v, uint64_t m, uint32_t b0){
@@ -3670,7 +3685,7 @@ int batchOrder(care_ctx* ctx, uint64_t b0, int B, const uint32_t** out){
     const uint64_t n = ctx->n;
     HIPCHECK(ctx->orderFlags.ensure(n + 16));
     HIPCHECK(ctx->batchOrd.ensure(size_t(B) * 4 + 16));
-    k_order_flags<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(ctx->order.as<uint32_t>(), n, b0, uint64_t(B),
+    k_order_flags<<<gridFor(n, 256), 256, 0, st>>>(ctx->order.as<uint32_t>(), n, b0, uint64_t(B),
                                                                  ctx->orderFlags.as<uint8_t>());
     HIPCHECK(hipGetLastError());
     uint32_t* count = reinterpret_cast<uint32_t*>(ctx->batchOrd.as<uint8_t>() + size_t(B) * 4);
@@ -3706,7 +3721,7 @@ hipError_t launchAlign(const care_ctx* ctx, const AlignParams& ap, int B, uint32
             case 5: k_align_lanes<5><<<B, kWave, 0, st>>>(ap); break;
             case 8: k_align_lanes<8><<<B, kWave, 0, st>>>(ap); break;
             case 11: k_align_lanes<11><<<B, kWave, 0, st>>>(ap); break;
-            default: k_align<<<(total + kAlignWaves - 1) / kAlignWaves, kBlock, 0, st>>>(ap); break;
+            default: k_align<<<gridFor(uint64_t(total) * kWave, kBlock), kBlock, 0, st>>>(ap); break;
         }
     }
     return hipGetLastError();
@@ -4429,7 +4444,7 @@ int finishReads(care_ctx* ctx, hipEvent_t done = nullptr){
     HIPCHECK(ctx->hilo.ensure(num_reads * 4 * ctx->W * 4 + 64));
     HIPCHECK(ctx->rcenc.ensure(num_reads * 2 * ctx->W * 4 + 64));
     const uint64_t hiloThreads = num_reads * uint64_t(ctx->W);
-    k_hilo<<<dim3((hiloThreads + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
+    k_hilo<<<gridFor(hiloThreads, 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
                                                                   num_reads, ctx->W, ctx->hilo.as<uint32_t>(),
                                                                   ctx->rcenc.as<uint32_t>());
     HIPCHECK(hipGetLastError());
@@ -4546,7 +4561,7 @@ static int care_hip_set_reads_body(care_ctx* ctx, const uint32_t* enc2bit, size_
             HIPCHECK(staging.ensure(num_reads * qual_pitch));
             HIPCHECK(hipMemcpyAsync(staging.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
             const uint64_t total = num_reads * ctx->qdpitch;
-            k_repack_rows<<<dim3(uint32_t((total + 255) / 256)), 256, 0, st>>>(staging.as<uint8_t>(), qual_pitch,
+            k_repack_rows<<<gridFor(total, 256), 256, 0, st>>>(staging.as<uint8_t>(), qual_pitch,
                                                                               ctx->qual.as<uint8_t>(), ctx->qdpitch, total);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipStreamSynchronize(st));
@@ -4669,9 +4684,10 @@ __global__ void __launch_bounds__(kFqBlock) k_fastq_records(
         int stride, int slot, uint64_t* __restrict__ hdrStart, uint32_t* __restrict__ hdrLen,
         uint64_t* __restrict__ seqStart, uint64_t* __restrict__ qualStart, int* __restrict__ len,
         uint64_t* __restrict__ ncount, int* __restrict__ err){
-    const uint64_t rec = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4;
     const int sub = threadIdx.x & 15;
-    if(rec >= records) return;                   // whole rows leave together
+    // rows stride over the records; the 16 lanes of a row always hold the same record
+    for(uint64_t rec = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4; rec < records;
+        rec += (uint64_t(gridDim.x) * kFqBlock) >> 4){
     auto lineStart = [&](uint64_t i){ return i ? nl[i - 1] + 1 : uint64_t(0); };
     // end of the content starting at s on line i
     auto contentEnd = [&](uint64_t i, uint64_t s){
@@ -4689,7 +4705,7 @@ __global__ void __launch_bounds__(kFqBlock) k_fastq_records(
         for(int k = 0; k < m; k++) nc += fastqCode(t[p + k]) < 0;
     }
     nc = rowBroadcastLast(rowInclusiveSum(nc));
-    if(sub != 0) return;
+    if(sub != 0) continue;
     const uint64_t r = uint64_t(stride) * rec + uint64_t(slot);
     const bool ok = h < bytes && t[h] == '@' && ps < bytes && t[ps] == '+' && qe - qs == se - ss;
     const uint64_t he = ok ? contentEnd(l0, h + 1) : h;
@@ -4700,6 +4716,7 @@ __global__ void __launch_bounds__(kFqBlock) k_fastq_records(
     len[r] = int(se - ss);
     ncount[r] = uint64_t(nc);
     if(!ok) atomicOr(err, 1);
+    }
 }
 
 // One 16-lane row per read: each lane encodes a run of the read's 2-bit words (MSB first,
@@ -4711,9 +4728,8 @@ __global__ void __launch_bounds__(kFqBlock) k_fastq_encode(
         const uint64_t* __restrict__ qualStart, const int* __restrict__ len, const uint64_t* __restrict__ prefix,
         uint64_t n, size_t pitch, uint32_t* __restrict__ enc, uint8_t* __restrict__ ambig,
         uint8_t* __restrict__ qual, size_t qdpitch){
-    const uint64_t r = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4;
     const int sub = threadIdx.x & 15;
-    if(r >= n) return;
+    for(uint64_t r = (uint64_t(blockIdx.x) * kFqBlock + threadIdx.x) >> 4; r < n; r += (uint64_t(gridDim.x) * kFqBlock) >> 4){
     const char* t = (paired && (r & 1)) ? t2 : t1;
     const int L = len[r];
     const char* sq = t + seqStart[r];
@@ -4751,6 +4767,7 @@ __global__ void __launch_bounds__(kFqBlock) k_fastq_encode(
             *reinterpret_cast<uint4*>(qrow + 16 * c) = make_uint4(w[0], w[1], w[2], w[3]);
         }
     }
+    }
 }
 
 namespace {
@@ -4770,6 +4787,9 @@ static int uploadAndSplitLines(care_ctx* ctx, const char* data, size_t raw, int 
     while(bytes && (data[bytes - 1] == '\n' || data[bytes - 1] == '\r')) bytes--;
     f.bytes = bytes;
     if(bytes == 0){ f.numNl = 0; f.records = 0; return CARE_OK; }
+    // one work item per 16 bytes in one launch: the grid's work items must fit 32 bits
+    if(bytes / 16 >= (uint64_t(1) << 32) - kNlBlock)
+        return ctx->fail(CARE_EINVAL, "a FASTQ text of %zu bytes: at most 64 GiB per text", bytes);
     const uint32_t nb = uint32_t((bytes + kNlPerBlock - 1) / kNlPerBlock);
     HIPCHECK(f.blockCnt.ensure((size_t(nb) + 1) * 4));
     HIPCHECK(f.blockOff.ensure((size_t(nb) + 1) * 8));
@@ -4836,7 +4856,7 @@ static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size
     const int stride = paired ? 2 : 1;
     for(int fi = 0; fi < (paired ? 2 : 1); fi++){
         ParsedFile& f = fi ? f2 : f1;
-        k_fastq_records<<<dim3(uint32_t((f.records * 16 + kFqBlock - 1) / kFqBlock)), kFqBlock, 0, st>>>(
+        k_fastq_records<<<gridFor(f.records * 16, kFqBlock), kFqBlock, 0, st>>>(
             f.text->as<char>(), f.bytes, f.nl.as<uint64_t>(), f.numNl, f.records, stride, fi,
             ctx->fqHdr.as<uint64_t>(), ctx->fqHdrLen.as<uint32_t>(), ctx->fqSeq.as<uint64_t>(), ctx->fqQual.as<uint64_t>(),
             tmp.lens.as<int>(), tmp.ncount.as<uint64_t>(), tmp.err.as<int>());
@@ -4868,7 +4888,7 @@ static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size
         HIPCHECK(ctx->qual.ensure(n * ctx->qdpitch));
         qual = ctx->qual.as<uint8_t>();
     }
-    k_fastq_encode<<<dim3(uint32_t((n * 16 + kFqBlock - 1) / kFqBlock)), kFqBlock, 0, st>>>(
+    k_fastq_encode<<<gridFor(n * 16, kFqBlock), kFqBlock, 0, st>>>(
         f1.text->as<char>(), paired ? f2.text->as<char>() : nullptr, paired, ctx->fqSeq.as<uint64_t>(),
         ctx->fqQual.as<uint64_t>(), tmp.lens.as<int>(), tmp.prefix.as<uint64_t>(), n, pitch, ctx->enc.as<uint32_t>(),
         ctx->ambig.as<uint8_t>(), qual, ctx->qdpitch);
@@ -5012,8 +5032,7 @@ constexpr int kLabelBytes = 9;     // " care:q=N"
 // One thread per read of the file: the combine decision (record index or -1, label) and
 // the size of the read's output text.
 __global__ void k_merge_plan(MergeParams p){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i >= p.count) return;
+    FOR_GRID(i, p.count){
     const uint64_t r = uint64_t(p.stride) * i + uint64_t(p.file);
     const uint32_t b = p.runBeg[r], e = p.runEnd[r];
     const int L = p.len[r];
@@ -5047,13 +5066,14 @@ __global__ void k_merge_plan(MergeParams p){
     }
     p.plan[i] = make_int2(choice, label | (labelled ? 0x100 : 0));
     p.sizes[i] = 1 + uint64_t(p.hdrLen[r]) + (labelled ? kLabelBytes : 0) + 1 + uint64_t(seqLen) + 3 + uint64_t(L) + 1;
+    }
 }
 
 // One wavefront per read: lanes write consecutive bytes of the read's output text.
 __global__ void __launch_bounds__(256) k_merge_write(MergeParams p){
-    const uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if(i >= p.count) return;
+    for(uint64_t i = (uint64_t(blockIdx.x) * blockDim.x + threadIdx.x) >> 6; i < p.count;
+        i += (uint64_t(gridDim.x) * blockDim.x) >> 6){
     const uint64_t r = uint64_t(p.stride) * i + uint64_t(p.file);
     const int2 pl = p.plan[i];
     const int hl = int(p.hdrLen[r]);
@@ -5095,6 +5115,7 @@ __global__ void __launch_bounds__(256) k_merge_write(MergeParams p){
             else ch = '\n';
         }
         out[k] = char(ch);
+    }
     }
 }
 
@@ -5156,9 +5177,8 @@ static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size
     HIPCHECK(hipcub::DeviceRadixSort::SortPairs(ctx->scanTmp.p, tb, ctx->mKeyIn.as<uint32_t>(), ctx->mKeyOut.as<uint32_t>(),
                                                 ctx->mIdxIn.as<uint32_t>(), ctx->mIdxOut.as<uint32_t>(), int(m), 0, endBit, st));
     HIPCHECK(ctx->mRunBeg.ensure(n * 4)); HIPCHECK(ctx->mRunEnd.ensure(n * 4));
-    const uint32_t nb = uint32_t((n + 255) / 256);
-    k_fill_u32<<<nb, 256, 0, st>>>(ctx->mRunBeg.as<uint32_t>(), n, 0u);
-    k_fill_u32<<<nb, 256, 0, st>>>(ctx->mRunEnd.as<uint32_t>(), n, 0u);
+    k_fill_u32<<<gridFor(n, 256), 256, 0, st>>>(ctx->mRunBeg.as<uint32_t>(), n, 0u);
+    k_fill_u32<<<gridFor(n, 256), 256, 0, st>>>(ctx->mRunEnd.as<uint32_t>(), n, 0u);
     int err = 0;
     HIPCHECK(hipMemcpyAsync(&err, ctx->mErr.p, 4, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -5177,8 +5197,7 @@ static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size
     mp.seq = ctx->fqSeq.as<uint64_t>(); mp.qual = ctx->fqQual.as<uint64_t>(); mp.len = ctx->len.as<int>();
     mp.n = n; mp.count = count; mp.stride = paired ? 2 : 1; mp.file = file; mp.labels = quality_labels != 0;
     mp.plan = ctx->mPlan.as<int2>(); mp.sizes = ctx->mSize.as<uint64_t>(); mp.outOff = ctx->mOutOff.as<uint64_t>();
-    const uint32_t cb = uint32_t((count + 255) / 256);
-    k_merge_plan<<<cb, 256, 0, st>>>(mp);
+    k_merge_plan<<<gridFor(count, 256), 256, 0, st>>>(mp);
     HIPCHECK(hipGetLastError());
     tb = 0;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, ctx->mSize.as<uint64_t>(), ctx->mOutOff.as<uint64_t>(),
@@ -5197,8 +5216,7 @@ static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size
     if(cap < total) return ctx->fail(CARE_EINVAL, "output needs %llu bytes, cap is %zu", (unsigned long long)total, cap);
     HIPCHECK(ctx->mText.ensure(std::max<uint64_t>(total, 1)));
     mp.out = ctx->mText.as<char>();
-    const uint64_t wb = (count * 64 + 255) / 256;
-    k_merge_write<<<dim3(uint32_t(wb)), 256, 0, st>>>(mp);
+    k_merge_write<<<gridFor(count * 64, 256), 256, 0, st>>>(mp);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[7], st));
     HIPCHECK(hipMemcpyAsync(dst, ctx->mText.p, total, hipMemcpyDeviceToHost, st));
@@ -5228,8 +5246,7 @@ static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size
 // One thread per output word of the length bit stream.
 __global__ void k_pack_lengths(const int* __restrict__ len, uint64_t n, int minLen, int bits,
                                uint32_t* __restrict__ words, uint64_t nwords){
-    const uint64_t w = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(w >= nwords) return;
+    FOR_GRID(w, nwords){
     const uint64_t b0 = 32 * w, b1 = b0 + 32;
     const uint64_t i0 = b0 / uint64_t(bits);
     const uint64_t i1 = min(n, (b1 + uint64_t(bits) - 1) / uint64_t(bits));
@@ -5242,34 +5259,34 @@ __global__ void k_pack_lengths(const int* __restrict__ len, uint64_t n, int minL
         }
     }
     words[w] = word;
+    }
 }
 
 // One thread per read: its length from the bit stream.
 __global__ void k_unpack_lengths(const uint32_t* __restrict__ words, uint64_t n, int minLen, int bits,
                                  int* __restrict__ len){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i >= n) return;
-    uint32_t v = 0;
-    for(int k = 0; k < bits; k++){
-        const uint64_t pos = i * uint64_t(bits) + uint64_t(k);
-        v = (v << 1) | ((words[pos >> 5] >> (31 - int(pos & 31))) & 1u);
+    FOR_GRID(i, n){
+        uint32_t v = 0;
+        for(int k = 0; k < bits; k++){
+            const uint64_t pos = i * uint64_t(bits) + uint64_t(k);
+            v = (v << 1) | ((words[pos >> 5] >> (31 - int(pos & 31))) & 1u);
+        }
+        len[i] = minLen + int(v);
     }
-    len[i] = minLen + int(v);
 }
 
 // Quality rows of `sp` bytes to rows of `dp` bytes, zero past each read's length.
 __global__ void k_quality_rows(const uint8_t* __restrict__ src, size_t sp, const int* __restrict__ len,
                                uint8_t* __restrict__ dst, size_t dp, uint64_t total){
-    const uint64_t t = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(t >= total) return;
-    const uint64_t r = t / dp;
-    const uint64_t c = t - r * dp;
-    dst[t] = (c < sp && c < uint64_t(len[r])) ? src[r * sp + c] : uint8_t(0);
+    FOR_GRID(t, total){
+        const uint64_t r = t / dp;
+        const uint64_t c = t - r * dp;
+        dst[t] = (c < sp && c < uint64_t(len[r])) ? src[r * sp + c] : uint8_t(0);
+    }
 }
 
 __global__ void k_set_flags(const uint32_t* __restrict__ ids, uint64_t m, uint8_t* __restrict__ flags){
-    const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if(i < m) flags[ids[i]] = 1;
+    FOR_GRID(i, m) flags[ids[i]] = 1;
 }
 
 static int care_hip_save_reads_body(care_ctx* ctx, const char* path){
@@ -5308,7 +5325,7 @@ static int care_hip_save_reads_body(care_ctx* ctx, const char* path){
         struct Tmp { DBuf words, rows; ~Tmp(){ words.release(); rows.release(); } } tmp;
         if(lwords){
             HIPCHECK(tmp.words.ensure(lwords * 4));
-            k_pack_lengths<<<dim3(uint32_t((lwords + 255) / 256)), 256, 0, st>>>(ctx->len.as<int>(), n, minLen, bits,
+            k_pack_lengths<<<gridFor(lwords, 256), 256, 0, st>>>(ctx->len.as<int>(), n, minLen, bits,
                                                                                tmp.words.as<uint32_t>(), lwords);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(lw.data(), tmp.words.p, lwords * 4, hipMemcpyDeviceToHost, st));
@@ -5318,7 +5335,7 @@ static int care_hip_save_reads_body(care_ctx* ctx, const char* path){
             HIPCHECK(hipMemcpyAsync(seq.data(), ctx->enc.p, seqBytes, hipMemcpyDeviceToHost, st));
         }else{
             HIPCHECK(tmp.rows.ensure(seqBytes));
-            k_repack_rows<<<dim3(uint32_t((seqBytes + 255) / 256)), 256, 0, st>>>(ctx->enc.as<uint8_t>(), ctx->pitch * 4,
+            k_repack_rows<<<gridFor(seqBytes, 256), 256, 0, st>>>(ctx->enc.as<uint8_t>(), ctx->pitch * 4,
                                                                                 tmp.rows.as<uint8_t>(), refPitch * 4, seqBytes);
             HIPCHECK(hipGetLastError());
             HIPCHECK(hipMemcpyAsync(seq.data(), tmp.rows.p, seqBytes, hipMemcpyDeviceToHost, st));
@@ -5327,7 +5344,7 @@ static int care_hip_save_reads_body(care_ctx* ctx, const char* path){
         if(q){
             const uint64_t qBytes = n * qPitch * 4;
             HIPCHECK(tmp.rows.ensure(qBytes));
-            k_quality_rows<<<dim3(uint32_t((qBytes + 255) / 256)), 256, 0, st>>>(ctx->qual.as<uint8_t>(), ctx->qdpitch,
+            k_quality_rows<<<gridFor(qBytes, 256), 256, 0, st>>>(ctx->qual.as<uint8_t>(), ctx->qdpitch,
                                                                                 ctx->len.as<int>(), tmp.rows.as<uint8_t>(),
                                                                                 qPitch * 4, qBytes);
             HIPCHECK(hipGetLastError());
@@ -5463,10 +5480,10 @@ static int care_hip_load_reads_body(care_ctx* ctx, const char* path){
     if(bits){
         HIPCHECK(tmp.words.ensure(lwords * 4));
         HIPCHECK(hipMemcpyAsync(tmp.words.p, lw.data(), lwords * 4, hipMemcpyHostToDevice, st));
-        k_unpack_lengths<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(tmp.words.as<uint32_t>(), n, minLen, bits,
+        k_unpack_lengths<<<gridFor(n, 256), 256, 0, st>>>(tmp.words.as<uint32_t>(), n, minLen, bits,
                                                                          ctx->len.as<int>());
     }else{
-        k_fill_u32<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(ctx->len.as<uint32_t>(), n, uint32_t(minLen));
+        k_fill_u32<<<gridFor(n, 256), 256, 0, st>>>(ctx->len.as<uint32_t>(), n, uint32_t(minLen));
     }
     HIPCHECK(hipGetLastError());
     HIPCHECK(ctx->enc.ensure(sn * 4));
@@ -5476,7 +5493,7 @@ static int care_hip_load_reads_body(care_ctx* ctx, const char* path){
     if(na){
         HIPCHECK(tmp.ids.ensure(na * 4));
         HIPCHECK(hipMemcpyAsync(tmp.ids.p, ambIds.data(), na * 4, hipMemcpyHostToDevice, st));
-        k_set_flags<<<dim3(uint32_t((na + 255) / 256)), 256, 0, st>>>(tmp.ids.as<uint32_t>(), na, ctx->ambig.as<uint8_t>());
+        k_set_flags<<<gridFor(na, 256), 256, 0, st>>>(tmp.ids.as<uint32_t>(), na, ctx->ambig.as<uint8_t>());
         HIPCHECK(hipGetLastError());
     }
     if(useQ){
@@ -5486,7 +5503,7 @@ static int care_hip_load_reads_body(care_ctx* ctx, const char* path){
         HIPCHECK(tmp.rows.ensure(qn * 4));
         HIPCHECK(hipMemcpyAsync(tmp.rows.p, qual.data(), qn * 4, hipMemcpyHostToDevice, st));
         const uint64_t total = n * ctx->qdpitch;
-        k_quality_rows<<<dim3(uint32_t((total + 255) / 256)), 256, 0, st>>>(tmp.rows.as<uint8_t>(), qp * 4,
+        k_quality_rows<<<gridFor(total, 256), 256, 0, st>>>(tmp.rows.as<uint8_t>(), qp * 4,
                                                                            ctx->len.as<int>(), ctx->qual.as<uint8_t>(),
                                                                            ctx->qdpitch, total);
         HIPCHECK(hipGetLastError());
@@ -5520,7 +5537,7 @@ static int care_hip_inspect_reads_body(care_ctx* ctx, uint64_t first, uint64_t n
         struct Staging { DBuf b; ~Staging(){ b.release(); } } hold;
         HIPCHECK(hold.b.ensure(n * qual_pitch));
         const uint64_t total = n * qual_pitch;
-        k_repack_rows<<<dim3(uint32_t((total + 255) / 256)), 256, 0, ctx->stream>>>(
+        k_repack_rows<<<gridFor(total, 256), 256, 0, ctx->stream>>>(
             ctx->qual.as<uint8_t>() + ctx->qdpitch * first, ctx->qdpitch, hold.b.as<uint8_t>(), qual_pitch, total);
         HIPCHECK(hipGetLastError());
         HIPCHECK(hipMemcpyAsync(quals, hold.b.p, total, hipMemcpyDeviceToHost, ctx->stream));
@@ -5534,7 +5551,7 @@ static int resolveBuckets(care_ctx* ctx){
     hipStream_t st = ctx->stream;
     const uint64_t total = ctx->n * uint64_t(ctx->M);
     HIPCHECK(ctx->bucketOf.ensure(total * sizeof(uint2)));
-    k_bucket_of<<<dim3(uint32_t((total + 255) / 256)), 256, 0, st>>>(ctx->sig.as<uint64_t>(), ctx->n, ctx->M,
+    k_bucket_of<<<gridFor(total, 256), 256, 0, st>>>(ctx->sig.as<uint64_t>(), ctx->n, ctx->M,
                                                                     ctx->slotPtrs.as<const Slot*>(),
                                                                     ctx->capsDev.as<uint32_t>(), ctx->bucketOf.as<uint2>());
     HIPCHECK(hipGetLastError());
@@ -5545,7 +5562,7 @@ static int resolveBuckets(care_ctx* ctx){
         HIPCHECK(ctx->order.ensure(n * 4));
         HIPCHECK(tmp.keys.ensure(n * 8));
         HIPCHECK(tmp.ids.ensure(n * 4));
-        k_iota<<<dim3(uint32_t((n + 255) / 256)), 256, 0, st>>>(tmp.ids.as<uint32_t>(), n);
+        k_iota<<<gridFor(n, 256), 256, 0, st>>>(tmp.ids.as<uint32_t>(), n);
         HIPCHECK(hipGetLastError());
         size_t tb = 0;
         HIPCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, ctx->sig.as<uint64_t>(), tmp.keys.as<uint64_t>(),
@@ -5566,7 +5583,7 @@ static int care_hip_build_tables_body(care_ctx* ctx, int* maps_built){
     hipStream_t st = ctx->stream;
     HIPCHECK(ctx->sig.ensure(ctx->n * ctx->M * 8));
     HIPCHECK(ctx->seeds.ensure(std::max<uint64_t>(1, ctx->n * ctx->M) * 2));
-    k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
+    k_signatures<<<gridFor(ctx->n, 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
                                                                    ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>(), ctx->seeds.as<uint16_t>());
     HIPCHECK(hipGetLastError());
     int rc = buildTables(ctx);
@@ -5673,7 +5690,7 @@ static int care_hip_load_tables_body(care_ctx* ctx, const char* path, int* maps_
     ctx->M = M;
     HIPCHECK(ctx->sig.ensure(ctx->n * M * 8));
     HIPCHECK(ctx->seeds.ensure(std::max<uint64_t>(1, ctx->n * M) * 2));
-    k_signatures<<<dim3((ctx->n + 255) / 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
+    k_signatures<<<gridFor(ctx->n, 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
                                                                    ctx->n, ctx->k, M, ctx->sig.as<uint64_t>(), ctx->seeds.as<uint16_t>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(ctx->values.ensure(std::max<uint64_t>(1, ctx->n) * M * sizeof(uint32_t)));

@@ -83,6 +83,40 @@ def test_full_size_batch_and_range_invariance(w):
         eng.close()
 
 
+def _record_count(data):
+    mv = memoryview(data)
+    p = n = 0
+    while p < len(data):
+        p += 8 + (int.from_bytes(mv[p + 4:p + 8], "little") & ((1 << 29) - 1))
+        n += 1
+    assert p == len(data)
+    return n
+
+
+def test_full_size_candidate_correction_batch_invariance():
+    """Candidate correction over the 8.57M-read workload at the automatic batch size (about
+    2.7M anchors and 165M working-list entries per batch: the record gather strides over
+    more than 2^32 work items' worth of entries) gives the same record stream as batches of
+    2^20 anchors, and the reported record count is the number of records in it."""
+    w = "se_50mbp_100to250bp_30x_rep"
+    s, rs, o = _case(w)
+    outs = []
+    for bs in (0, 1 << 20):
+        eng = engine.CareEngine(0, correct_candidates=1, batch_size=bs, **o)
+        try:
+            eng.set_reads(rs)
+            eng.build_tables()
+            n, data = eng.correct()
+            outs.append((n, data, eng.timings()["batches"]))
+        finally:
+            eng.close()
+        print(f"[full {w}] candidate correction, batch size {bs}: {outs[-1][0]} records in {outs[-1][2]} batches",
+              flush=True)
+    assert outs[0][2] < outs[1][2]
+    assert outs[0][1] == outs[1][1]
+    assert outs[0][0] == outs[1][0] == _record_count(outs[0][1])
+
+
 @pytest.mark.parametrize("w", ["pe_20mbp_2x150bp_30x_rep", "se_50mbp_100to250bp_30x_rep"])
 def test_full_size_fast_learned_form_not_worse_than_plain(w):
     s, rs, o = _case(w)
