@@ -2925,13 +2925,14 @@ __global__ void k_zero_u32(uint32_t* v, int n){
 }
 
 // Candidate capacity check of a batch whose candidate arrays were sized before its
-// candidates were counted: status[0] = 1 when the batch has more than `cap`, status[1] =
-// how many it has.
-__global__ void k_cap_status(const uint32_t* __restrict__ offsets, int B, uint64_t cap, uint32_t* __restrict__ status){
+// candidates were counted (cap < 2^32): from the batch's pair count in 64 bits at
+// status[6..7], status[0] = 1 when the batch has more than `cap`, status[1] = how many
+// it has (saturated at 2^32 - 1).
+__global__ void k_cap_status(uint64_t cap, uint32_t* __restrict__ status){
     if(blockIdx.x == 0 && threadIdx.x == 0){
-        const uint32_t t = offsets[B];
-        status[0] = uint64_t(t) > cap ? 1u : 0u;
-        status[1] = t;
+        const uint64_t t = *reinterpret_cast<const unsigned long long*>(status + 6);
+        status[0] = t > cap ? 1u : 0u;
+        status[1] = uint32_t(t < uint64_t(UINT32_MAX) ? t : uint64_t(UINT32_MAX));
     }
 }
 
@@ -3801,8 +3802,10 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, HostRecords& out, size_t& outRec
     if(candLds > 65536) HIPCHECK(hipFuncSetAttribute((const void*)k_candidates_over, hipFuncAttributeMaxDynamicSharedMemorySize, int(candLds)));
     k_candidates_over<<<std::min(B, 4096), kBlock, candLds, st>>>(cp, ctx->candOver.as<uint32_t>(), overCount);
     HIPCHECK(hipGetLastError());
-    k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(ctx->gath.as<uint32_t>(), B,
-                                                                          ctx->counters.as<unsigned long long>());
+    // the batch's candidate pairs in 64 bits: list offsets and indices are 32-bit
+    unsigned long long* pairs64 = ctx->counters.as<unsigned long long>() + 5;
+    k_zero_u32<<<1, 64, 0, st>>>(reinterpret_cast<uint32_t*>(pairs64), 2);
+    k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(ctx->uniq.as<uint32_t>(), B, pairs64);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[5], st));
     size_t tb = 0;
@@ -3811,8 +3814,26 @@ int runBatch(care_ctx* ctx, uint64_t b0, int B, HostRecords& out, size_t& outRec
     tb = ctx->scanTmp.bytes;
     HIPCHECK(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), B + 1, st));
     uint32_t total = 0;
+    unsigned long long total64 = 0;
     HIPCHECK(hipMemcpyAsync(&total, ctx->offsets.as<uint32_t>() + B, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipMemcpyAsync(&total64, pairs64, 8, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    if(total64 > uint64_t(UINT32_MAX)){
+        // more pairs than 32-bit list offsets address: the two halves one after the other
+        // (records keep their order; halves end on 16-read groups for candidate correction
+        // and on mate pairs)
+        int B1 = B / 2;
+        if(ctx->o.correct_candidates) B1 &= ~15;
+        if(ctx->o.paired) B1 &= ~1;
+        if(B1 <= 0 || B1 >= B) return ctx->fail(CARE_EINVAL, "more than 2^32 candidate pairs in a batch of %d anchors", B);
+        const int rc = runBatch(ctx, b0, B1, out, outRecords);
+        if(rc != CARE_OK) return rc;
+        return runBatch(ctx, b0 + uint64_t(B1), B - B1, out, outRecords);
+    }
+    // the batch's gathered bucket values into the pass statistic (once the batch is kept)
+    k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(ctx->gath.as<uint32_t>(), B,
+                                                                          ctx->counters.as<unsigned long long>());
+    HIPCHECK(hipGetLastError());
     const size_t T = std::max<size_t>(total, 1);
     HIPCHECK(ctx->ids.ensure(T * 4));
     HIPCHECK(ctx->anchorOf.ensure(T * 4));
@@ -3968,9 +3989,9 @@ struct StagedLaunch {
 };
 
 // Status words of batch slot s: in `counters` at [64 + 32 s): over-capacity flag (u32),
-// candidate pairs (u32), bucket values gathered (u64), records (u64); copied after the
-// record bytes (u64) and the error flag (i32) to the pinned words at [64 + 64 s): bytes 0,
-// 8, then 16 on.
+// candidate pairs (u32, saturated), bucket values gathered (u64), records (u64), candidate
+// pairs (u64); copied after the record bytes (u64) and the error flag (i32) to the pinned
+// words at [64 + 64 s): bytes 0, 8, then 16 on.
 uint32_t* slotStatus(care_ctx* ctx, int s){ return ctx->counters.as<uint32_t>() + 16 + 8 * s; }
 uint8_t* slotPinned(care_ctx* ctx, int s){ return reinterpret_cast<uint8_t*>(ctx->pinned) + 64 + 64 * s; }
 
@@ -3992,10 +4013,12 @@ hipError_t enqueueStaged(care_ctx* ctx, const StagedLaunch& L, bool events){
     ENQ((k_candidates_over<<<std::min(B, 4096), kBlock, L.candLds, st>>>(L.cp, ctx->candOver.as<uint32_t>(), overCount)));
     ENQ((k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(
         ctx->gath.as<uint32_t>(), B, reinterpret_cast<unsigned long long*>(status + 2))));
+    ENQ((k_add_gathered<<<(B + kAddPerBlock - 1) / kAddPerBlock, kBlock, 0, st>>>(
+        L.cp.uniq, B, reinterpret_cast<unsigned long long*>(status + 6))));
     if(events) ENQE(hipEventRecord(ev[5], st));
     size_t tb = ctx->scanTmp.bytes;
     ENQE(hipcub::DeviceScan::ExclusiveSum(ctx->scanTmp.p, tb, ctx->uniq.as<uint32_t>(), ctx->offsets.as<uint32_t>(), B + 1, st));
-    ENQ((k_cap_status<<<1, 64, 0, st>>>(ctx->offsets.as<uint32_t>(), B, L.cap, status)));
+    ENQ((k_cap_status<<<1, 64, 0, st>>>(L.cap, status)));
     ENQ((k_cap_clear<<<(B + 1 + 255) / 256, 256, 0, st>>>(ctx->offsets.as<uint32_t>(), B, status)));
     ENQ((k_compact<<<(B + kBlock / kWave - 1) / (kBlock / kWave), kBlock, 0, st>>>(
         ctx->raw.as<uint32_t>(), L.capPerAnchor, ctx->offsets.as<uint32_t>(), B, ctx->ids.as<uint32_t>(), nullptr)));
@@ -4240,7 +4263,12 @@ uint64_t stagedCapacity(care_ctx* ctx, int Bmax){
     }
     per = std::ceil(std::max(per, ctx->pipePerAnchor));
     ctx->pipePerAnchor = per;
-    return std::min<uint64_t>(uint64_t(per) * uint64_t(Bmax), uint64_t(UINT32_MAX));
+    // at most a quarter of the free device memory at about 48 bytes per pair (batches with
+    // more pairs are redone with exact sizes)
+    size_t freeB = 0, totalB = 0;
+    uint64_t memCap = uint64_t(UINT32_MAX);
+    if(hipMemGetInfo(&freeB, &totalB) == hipSuccess) memCap = std::max<uint64_t>(uint64_t(freeB) / 4 / 48, uint64_t(Bmax) * 32);
+    return std::min<uint64_t>({uint64_t(per) * uint64_t(Bmax), uint64_t(UINT32_MAX), memCap});
 }
 
 // Anchors [first, first+n) in pipelined batches of at most Bmax anchors.
