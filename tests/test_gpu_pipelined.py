@@ -143,3 +143,22 @@ def test_records_view_equals_copy():
         assert eng.lib.care_hip_copy_records(eng.h, buf, len(data) - 1) != 0
     finally:
         eng.close()
+
+
+@pytest.mark.parametrize("extra", [{}, {"use_graphs": 1}, {"sync_batches": 1}, {"engine": 2}])
+def test_empty_and_small_anchor_ranges(extra):
+    """Ranges of no anchor, one anchor and a few anchors: the records of exactly those
+    anchors, as the whole pass gives them (records come in read-id order)."""
+    s, rs, o = helpers.case("se_q")
+    eng = _engine(rs, o, **extra)
+    try:
+        whole = helpers.parse_records(eng.correct()[1])
+        assert eng.correct(0, 0) == (0, b"")
+        assert eng.correct(rs.n, 0) == (0, b"")
+        for first, n in ((rs.n - 1, 1), (0, 1), (17, 5), (1000, 63), (2000, 65)):
+            nrec, data = eng.correct(first, n)
+            got = helpers.parse_records(data)
+            want = [r for r in whole if first <= r[0] < first + n]
+            assert got == want and nrec == len(want), (first, n)
+    finally:
+        eng.close()
