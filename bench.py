@@ -28,6 +28,7 @@ Options:
   --cpu-threads T   threads for the CPU baseline (default: every core this process may use)
   --pe-workload     paired workload of the PE leg ('' disables)
   --no-learned / --no-candidates / --no-fastq / --no-autotune   skip those legs
+  --engine / --waves-per-cu / --batch-size / --graphs   a fixed schedule (with --no-autotune)
 """
 from __future__ import annotations
 
@@ -231,6 +232,8 @@ def engine_opts(rs, use_q, args):
     o = dict(use_quality_scores=int(use_q), paired=int(rs.paired), batch_size=args.batch_size,
              engine={"auto": 0, "staged": 1, "fused": 2}[args.engine], waves_per_cu=args.waves_per_cu,
              kmerlength=kmer_length(rs))
+    if args.graphs:
+        o["use_graphs"] = 1
     return o
 
 
@@ -269,6 +272,8 @@ def main():
     ap.add_argument("--engine", choices=["auto", "staged", "fused"], default="auto",
                     help="auto = the autotuner's choice; staged = per-stage kernels; fused = one persistent kernel")
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--graphs", action="store_true",
+                    help="capture each pass's batches in HIP graphs (with --no-autotune or --engine)")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-candidates", action="store_true",
                     help="skip the candidate-correction leg (--candidateCorrection)")
