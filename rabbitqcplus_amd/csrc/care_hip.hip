@@ -5792,15 +5792,22 @@ static int care_hip_correct_body(care_ctx* ctx, uint64_t first, uint64_t n, size
     if(B <= 0){
         // one batch per pass where HBM allows: the per-anchor candidate staging area
         // (maps * threshold ids) and record slot may take up to 96 GiB or a third of the
-        // free device memory (288 GB of HBM: 2 batches at 8.6M reads); fewer batches mean
-        // fewer launch tails and host syncs
+        // free device memory (288 GB of HBM), at most 2^23 anchors; fewer batches mean
+        // fewer launch tails
         size_t freeB = 0, totalB = 0;
         HIPCHECK(hipMemGetInfo(&freeB, &totalB));
         const size_t budget = std::min<size_t>(size_t(96) << 30, freeB / 3);
         size_t perAnchor = size_t(ctx->M) * ctx->thr * 4 + size_t(recordSlotBytes(ctx->maxLen)) + 512;
         // candidate correction: a record slot per working-list entry (about threshold per anchor)
         if(ctx->o.correct_candidates) perAnchor += size_t(ctx->thr) * (candidateSlotBytes(ctx->maxLen) + 36);
-        B = int(std::min<size_t>(size_t(1) << 22, std::max<size_t>(1024, budget / perAnchor)));
+        B = int(std::min<size_t>(size_t(1) << 23, std::max<size_t>(1024, budget / perAnchor)));
+        // batches of equal size (no short last batch): 2 of 4.29M at 8.57M reads
+        if(n > 0){
+            const uint64_t nb = (n + uint64_t(B) - 1) / uint64_t(B);
+            uint64_t even = (n + nb - 1) / nb;
+            if(ctx->o.correct_candidates) even = (even + 15) & ~uint64_t(15);
+            B = int(std::min<uint64_t>(uint64_t(B), std::max<uint64_t>(even, 16)));
+        }
     }
     if(ctx->o.paired && (B & 1)) B++;
     const bool cands = ctx->o.correct_candidates != 0;
