@@ -117,6 +117,84 @@ def test_full_size_candidate_correction_batch_invariance():
     assert outs[0][0] == outs[1][0] == _record_count(outs[0][1])
 
 
+def _record_index(data):
+    """(read ids, byte offsets) of the records of a stream, in stream order."""
+    import struct
+    ids, offs = [], []
+    p, end = 0, len(data)
+    while p < end:
+        rid, flags = struct.unpack_from("<II", data, p)
+        ids.append(rid)
+        offs.append(p)
+        p += 8 + (flags & ((1 << 29) - 1))
+    assert p == end
+    return np.asarray(ids, np.int64), np.asarray(offs, np.int64)
+
+
+def _lines(buf, nl, i):
+    """The 4 lines of read i of 4-line FASTQ text (newline positions nl)."""
+    starts = [0 if 4 * i == 0 else int(nl[4 * i - 1]) + 1] + [int(nl[4 * i + j]) + 1 for j in range(3)]
+    return [bytes(buf[a:int(nl[4 * i + j])]) for j, a in enumerate(starts)]
+
+
+def test_full_size_fastq_text_to_corrected_text():
+    """FASTQ text of the 8.57M-read workload (about 3 GB) -> device read storage ->
+    correction -> corrected FASTQ text, with and without quality labels. The merge writes
+    with one wavefront per read, so its grid-stride loop takes several strides (8.57M reads
+    x 64 lanes > 2^28 work items of the largest grid).
+
+    Checks: the records from the text equal the reference's (the golden digests); without
+    labels the output is the input with bytes changed only inside sequence lines of reads
+    that have a record; 20,000 random reads and the reads either side of each grid stride
+    equal the oracle's combine rule applied to their records, with and without labels."""
+    from oracle import output_merge
+    from rabbitqcplus_amd import sim
+    w = "se_50mbp_100to250bp_30x_rep"
+    s, rs, o = _case(w)
+    text = sim.fastq_texts(s)[0]
+    print(f"[full {w}] FASTQ text {text.size / 1e9:.2f} GB", flush=True)
+    eng = engine.CareEngine(0, **o)
+    try:
+        assert eng.set_reads_fastq(text) == rs.n
+        eng.build_tables()
+        n, data = eng.correct()
+        outs = [np.frombuffer(eng.write_fastq(data, 0, lab), np.uint8) for lab in (False, True)]
+    finally:
+        eng.close()
+    g = _golden(w)
+    assert n == g["records"] and helpers.slice_digests(data, rs.n) == g["slices"]
+    ids, offs = _record_index(data)
+    assert np.all(np.diff(ids) >= 0)
+    nl = np.flatnonzero(text == 10)
+    assert nl.size == 4 * rs.n
+    plain, labelled = outs
+    # equal-length corrections: the unlabelled output lines up with the input byte for byte
+    assert plain.size == text.size
+    diff = np.flatnonzero(plain != text)
+    line = np.searchsorted(nl, diff)
+    assert np.all(line % 4 == 1) and np.all(text[diff] != 10) and np.all(plain[diff] != 10)
+    assert np.isin(np.unique(line // 4), ids).all()
+    print(f"[full {w}] {diff.size} bytes corrected in {np.unique(line // 4).size} reads", flush=True)
+    nll = np.flatnonzero(labelled == 10)
+    assert nll.size == 4 * rs.n
+    per_stride = (1 << 28) // 64                       # reads per stride of the merge's grid
+    edge = [k * per_stride + d for k in range(1, rs.n // per_stride + 1) for d in (-1, 0)]
+    sample = np.unique(np.concatenate([np.random.default_rng(5).choice(rs.n, 20000, replace=False),
+                                       [0, rs.n - 1], [e for e in edge if e < rs.n]]))
+    last = int(ids[-1])
+    tag = {0: b" care:q=0", 1: b" care:q=1", 2: b" care:q=2", 3: b" care:q=3"}
+    for i in sample:
+        i = int(i)
+        h, seq, plus, q = _lines(text, nl, i)
+        a, b = np.searchsorted(ids, i), np.searchsorted(ids, i, side="right")
+        recs = [output_merge.decode_records(data[int(offs[k]):int(offs[k + 1]) if k + 1 < len(offs) else len(data)])[0]
+                for k in range(a, b)]
+        want, lab = output_merge.combine(recs, seq)
+        assert _lines(plain, nl, i) == [h, want, plus, q], i
+        hl = h + tag[lab] if i <= last else h
+        assert _lines(labelled, nll, i) == [hl, want, plus, q], i
+
+
 @pytest.mark.parametrize("w", ["pe_20mbp_2x150bp_30x_rep", "se_50mbp_100to250bp_30x_rep"])
 def test_full_size_fast_learned_form_not_worse_than_plain(w):
     s, rs, o = _case(w)
