@@ -1025,11 +1025,12 @@ __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t*
                 dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                 dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
             }
-            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
-                gF = dF - 2 * ov; sF = sft;
-            }
-            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
-                gR = dR - 2 * ov; sR = sft;
+            const uint64_t iF = __builtin_amdgcn_ballot_w64(dF < mF) & liveF;
+            const uint64_t iR = __builtin_amdgcn_ballot_w64(dR < mR) & liveR;
+            if(__builtin_expect((iF | iR) != 0, 0)){
+                __asm__ volatile("" ::: "memory");          // keeps the rare update a branch
+                if(__builtin_amdgcn_inverse_ballot_w64(iF)){ gF = dF - 2 * ov; sF = sft; }
+                if(__builtin_amdgcn_inverse_ballot_w64(iR)){ gR = dR - 2 * ov; sR = sft; }
             }
         }
     }
@@ -1065,11 +1066,127 @@ __device__ __forceinline__ void scanShifts(const AlignParams& p, const uint32_t*
                 dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
                 dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
             }
-            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dF < mF) & liveF)){
-                gF = dF - 2 * ov; sF = -t;
+            const uint64_t iF = __builtin_amdgcn_ballot_w64(dF < mF) & liveF;
+            const uint64_t iR = __builtin_amdgcn_ballot_w64(dR < mR) & liveR;
+            if(__builtin_expect((iF | iR) != 0, 0)){
+                __asm__ volatile("" ::: "memory");
+                if(__builtin_amdgcn_inverse_ballot_w64(iF)){ gF = dF - 2 * ov; sF = -t; }
+                if(__builtin_amdgcn_inverse_ballot_w64(iR)){ gR = dR - 2 * ov; sR = -t; }
             }
-            if(__builtin_amdgcn_inverse_ballot_w64(__builtin_amdgcn_ballot_w64(dR < mR) & liveR)){
-                gR = dR - 2 * ov; sR = -t;
+        }
+    }
+}
+
+// int(float(x) * maxErrorRate) for 0 <= x <= maxLen in the host-checked integer form
+// (valid when p.eExact)
+__device__ __forceinline__ int errBoundInt(const AlignParams& p, int x){
+    return int(__umulhi(uint32_t(x) << p.ePre, p.eMul));
+}
+
+// The scan of scanShifts for candidates of differing lengths when the error bound has the
+// integer form (p.eExact) and the minimum overlap mo is at least 1. The overlap of a
+// shift is ov = min(a, b) of a wave-uniform term a and a lane term b (anchor shifted:
+// a = La - sft, b = Lc; candidate shifted: a = La, b = Lc - t), both in [1, maxLen] while
+// the lane's scan runs. e() is non-decreasing, so the bound min(e(ov), g + 2 ov) equals
+// min(e(a), g + 2a, e(b), g + 2b): the wave-uniform parts are scalar, and the lane parts
+// that do not move with the shift change only with the lane's best g, whose update runs
+// in a wave-uniform branch taken at the few shifts where some lane improves. Results are
+// those of scanShifts<WM, false>.
+template<int WM>
+__device__ __forceinline__ void scanShiftsVar(const AlignParams& p, const uint32_t* AH, const uint32_t* AL,
+                                              const uint32_t* AV, const uint32_t* FH, const uint32_t* FL,
+                                              const uint32_t* RH, const uint32_t* RL, const uint32_t* CV, int La, int Lc,
+                                              int LcMax, int lastF, int mo, uint64_t start,
+                                              int& gF, int& sF, int& gR, int& sR){
+    gF = 0; sF = -Lc; gR = 0; sR = -Lc;
+    constexpr bool kLongWords = WM >= 8;
+    const int eC = errBoundInt(p, Lc);
+    // anchor shifted: shifts 0..lastF, ov = min(La - sft, Lc); lane part min(e(Lc), g + 2 Lc)
+    int cF = min(eC, gF + 2 * Lc), cR = min(eC, gR + 2 * Lc);
+    uint64_t liveF = start, liveR = start;
+    #pragma unroll
+    for(int ps = 0; ps <= WM; ps++){
+        if(32 * ps > lastF || (liveF | liveR) == 0) break;
+        const int rEnd = min(31, lastF - 32 * ps);
+        for(int r = 0; r <= rEnd; r++){
+            const int sft = 32 * ps + r;
+            const int a = La - sft;
+            const int eA = errBoundInt(p, a);
+            const int mF = min(min(eA, cF), gF + 2 * a);
+            const int mR = min(min(eA, cR), gR + 2 * a);
+            liveF &= __builtin_amdgcn_ballot_w64(mF > 0);
+            liveR &= __builtin_amdgcn_ballot_w64(mR > 0);
+            const int nw = kLongWords ? (min(a, LcMax) + 31) >> 5 : WM;
+            int dF = 0, dR = 0;
+            #pragma unroll
+            for(int w = 0; w < WM; w++){
+                if(kLongWords && w >= nw) break;
+                if(abandonCheck<WM>(w) && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                           (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
+                const uint32_t xh = uint32_t((((uint64_t(AH[w + ps + 1]) << 32) | AH[w + ps + 2]) << r) >> 32);
+                const uint32_t xl = uint32_t((((uint64_t(AL[w + ps + 1]) << 32) | AL[w + ps + 2]) << r) >> 32);
+                const uint32_t xv = uint32_t((((uint64_t(AV[w + ps + 1]) << 32) | AV[w + ps + 2]) << r) >> 32);
+                const uint32_t m = xv & CV[w];
+                dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
+                dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
+            }
+            const uint64_t iF = __builtin_amdgcn_ballot_w64(dF < mF) & liveF;
+            const uint64_t iR = __builtin_amdgcn_ballot_w64(dR < mR) & liveR;
+            if(__builtin_expect((iF | iR) != 0, 0)){
+                __asm__ volatile("" ::: "memory");          // keeps the rare update a branch
+                const int ov = min(a, Lc);
+                if(__builtin_amdgcn_inverse_ballot_w64(iF)){ gF = dF - 2 * ov; sF = sft; cF = min(eC, gF + 2 * Lc); }
+                if(__builtin_amdgcn_inverse_ballot_w64(iR)){ gR = dR - 2 * ov; sR = sft; cR = min(eC, gR + 2 * Lc); }
+            }
+        }
+    }
+    // candidate shifted: t = 1..Lc-mo, ov = min(La, Lc - t). Lane parts: min(e(La), g + 2 La)
+    // and g + 2 Lc (less 2t per shift); e(Lc - t) from (Lc << ePre) - (t << ePre), exact
+    // while t <= Lc - mo < Lc.
+    const int eLa = errBoundInt(p, La);
+    const uint32_t LcP = uint32_t(Lc) << p.ePre;
+    int bF = min(eLa, gF + 2 * La), bR = min(eLa, gR + 2 * La);
+    int hF = gF + 2 * Lc, hR = gR + 2 * Lc;
+    const int lastB = Lc - mo;
+    const int lastBw = p.maxLen - mo;
+    liveF = start; liveR = start;
+    #pragma unroll
+    for(int pt = 0; pt <= WM; pt++){
+        if(32 * pt > lastBw || (liveF | liveR) == 0) break;
+        const int rEnd = min(31, lastBw - 32 * pt);
+        for(int r = (pt == 0 ? 1 : 0); r <= rEnd; r++){
+            const int t = 32 * pt + r;
+            const int eB = int(__umulhi(LcP - (uint32_t(t) << p.ePre), p.eMul));
+            const int mF = min(min(eB, bF), hF - 2 * t);
+            const int mR = min(min(eB, bR), hR - 2 * t);
+            const uint64_t inRange = __builtin_amdgcn_ballot_w64(t <= lastB);
+            liveF &= inRange & __builtin_amdgcn_ballot_w64(mF > 0);
+            liveR &= inRange & __builtin_amdgcn_ballot_w64(mR > 0);
+            const int nw = kLongWords ? (min(t + La, LcMax) + 31) >> 5 : WM;
+            int dF = 0, dR = 0;
+            #pragma unroll
+            for(int w = pt; w < WM; w++){
+                if(kLongWords && w >= nw) break;
+                if(abandonCheck<WM>(w - pt) && ((__builtin_amdgcn_ballot_w64(dF < mF) & liveF) |
+                                                (__builtin_amdgcn_ballot_w64(dR < mR) & liveR)) == 0) break;
+                const uint32_t xh = uint32_t(((uint64_t(AH[w - pt]) << 32) | AH[w - pt + 1]) >> r);
+                const uint32_t xl = uint32_t(((uint64_t(AL[w - pt]) << 32) | AL[w - pt + 1]) >> r);
+                const uint32_t xv = uint32_t(((uint64_t(AV[w - pt]) << 32) | AV[w - pt + 1]) >> r);
+                const uint32_t m = xv & CV[w];
+                dF += __popc(((xh ^ FH[w]) | (xl ^ FL[w])) & m);
+                dR += __popc(((xh ^ RH[w]) | (xl ^ RL[w])) & m);
+            }
+            const uint64_t iF = __builtin_amdgcn_ballot_w64(dF < mF) & liveF;
+            const uint64_t iR = __builtin_amdgcn_ballot_w64(dR < mR) & liveR;
+            if(__builtin_expect((iF | iR) != 0, 0)){
+                __asm__ volatile("" ::: "memory");
+                const int ov = min(La, Lc - t);
+                if(__builtin_amdgcn_inverse_ballot_w64(iF)){
+                    gF = dF - 2 * ov; sF = -t; bF = min(eLa, gF + 2 * La); hF = gF + 2 * Lc;
+                }
+                if(__builtin_amdgcn_inverse_ballot_w64(iR)){
+                    gR = dR - 2 * ov; sR = -t; bR = min(eLa, gR + 2 * La); hR = gR + 2 * Lc;
+                }
             }
         }
     }
@@ -1123,6 +1240,8 @@ __device__ __forceinline__ void alignLanes(const AlignParams& p, uint64_t a, con
         LcMax = __builtin_amdgcn_readfirstlane(LcMax);
         if(p.eExact && __builtin_amdgcn_ballot_w64(valid && Lc != La) == 0)
             scanShifts<WM, true>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, LcMax, lastF, mo, start, gF, sF, gR, sR);
+        else if(p.eExact && mo >= 1)
+            scanShiftsVar<WM>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, LcMax, lastF, mo, start, gF, sF, gR, sR);
         else
             scanShifts<WM, false>(p, AH, AL, AV, FH, FL, RH, RL, CV, La, Lc, LcMax, lastF, mo, start, gF, sF, gR, sR);
         const int bF = gF + total, bR = gR + total;
