@@ -3204,6 +3204,12 @@ struct HostRecords {
 
 }  // namespace
 
+// Uploads of at least kUpMin bytes go through kUpWorkers host threads with pinned
+// double buffers of kUpChunk bytes each (64 MiB of pinned memory per context).
+constexpr int kUpWorkers = 8;
+constexpr size_t kUpChunk = size_t(4) << 20;
+constexpr size_t kUpMin = size_t(64) << 20;
+
 struct care_ctx {
     care_opts o{};
     int device = 0;
@@ -3246,6 +3252,11 @@ struct care_ctx {
     DBuf pipeOut[2];
     hipEvent_t pipeEv[2][6] = {};
     hipStream_t copyStream = nullptr;
+    // large host-to-device uploads: per worker a stream, two pinned chunk buffers and their
+    // events (created on the first large upload)
+    hipStream_t upStream[kUpWorkers] = {};
+    void* upBuf[kUpWorkers][2] = {};
+    hipEvent_t upEv[kUpWorkers][2] = {};
     double candPerAnchor = 0;
     double pipePerAnchor = 0;   // candidate pairs per anchor the arrays were last sized for
     // pipelined batches with use_graphs: one instantiated graph per batch (key: the batch's
@@ -4529,6 +4540,13 @@ void care_hip_destroy(care_ctx* ctx){
     for(auto& e : ctx->ev) if(e) (void)hipEventDestroy(e);
     for(auto& row : ctx->pipeEv) for(auto& e : row) if(e) (void)hipEventDestroy(e);
     if(ctx->copyStream) (void)hipStreamDestroy(ctx->copyStream);
+    for(int t = 0; t < kUpWorkers; t++){
+        for(int b = 0; b < 2; b++){
+            if(ctx->upEv[t][b]) (void)hipEventDestroy(ctx->upEv[t][b]);
+            if(ctx->upBuf[t][b]) (void)hipHostFree(ctx->upBuf[t][b]);
+        }
+        if(ctx->upStream[t]) (void)hipStreamDestroy(ctx->upStream[t]);
+    }
     if(ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -4550,6 +4568,86 @@ const char* care_hip_last_error(const care_ctx* ctx){
 }
 
 // Shapes and options of a read set (shared by the array and the FASTQ entry points).
+// Host-to-device copy of `rows` rows of a (pageable) host buffer, row pitch `spitch`,
+// into device rows of pitch `dpitch` >= spitch whose last dpitch - spitch bytes are
+// zeroed; complete on the device when it returns. Large copies are cut into one slice of
+// rows per worker thread; a worker packs its rows into its two pinned chunk buffers and
+// transfers them on its own stream, so the host copies of the workers and their transfers
+// overlap. Work already enqueued on `st` finishes first (it may still read the
+// destination). A linear copy is one row of `bytes` with both pitches equal to it.
+int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size_t spitch, uint64_t rows,
+                  hipStream_t st){
+    HIPCHECK(hipStreamSynchronize(st));
+    const bool linear = rows == 1 && dpitch == spitch;
+    const size_t bytes = size_t(rows) * dpitch;
+    const int T = int(std::min<unsigned>(kUpWorkers, std::max(1u, std::thread::hardware_concurrency())));
+    for(int t = 0; t < T; t++){
+        if(!ctx->upStream[t]) HIPCHECK(hipStreamCreateWithFlags(&ctx->upStream[t], hipStreamNonBlocking));
+        for(int b = 0; b < 2; b++){
+            if(!ctx->upEv[t][b]) HIPCHECK(hipEventCreateWithFlags(&ctx->upEv[t][b], hipEventDisableTiming));
+            if(!ctx->upBuf[t][b]) HIPCHECK(hipHostMalloc(&ctx->upBuf[t][b], kUpChunk, hipHostMallocDefault));
+        }
+    }
+    // work units: bytes of a linear copy, rows of a pitched one (whole rows per chunk)
+    const uint64_t units = linear ? bytes : rows;
+    const uint64_t perChunk = linear ? kUpChunk : kUpChunk / dpitch;
+    const uint64_t slice = (units + T - 1) / T;
+    std::vector<hipError_t> errs(T, hipSuccess);
+    auto work = [&](int t){
+        hipError_t e = hipSetDevice(ctx->device);
+        const uint64_t beg = std::min(units, uint64_t(t) * slice), end = std::min(units, beg + slice);
+        int k = 0;
+        for(uint64_t u = beg; e == hipSuccess && u < end; u += perChunk, k++){
+            const int b = k & 1;
+            const uint64_t n = std::min(perChunk, end - u);
+            if(k >= 2) e = hipEventSynchronize(ctx->upEv[t][b]);     // that buffer's last transfer
+            if(e != hipSuccess) break;
+            char* buf = static_cast<char*>(ctx->upBuf[t][b]);
+            const char* s = static_cast<const char*>(src);
+            size_t len, off;
+            if(linear){
+                std::memcpy(buf, s + u, n);
+                len = n; off = u;
+            }else{
+                for(uint64_t r = 0; r < n; r++){
+                    std::memcpy(buf + r * dpitch, s + (u + r) * spitch, spitch);
+                    if(dpitch > spitch) std::memset(buf + r * dpitch + spitch, 0, dpitch - spitch);
+                }
+                len = n * dpitch; off = u * dpitch;
+            }
+            e = hipMemcpyAsync(static_cast<char*>(dst) + off, buf, len, hipMemcpyHostToDevice, ctx->upStream[t]);
+            if(e == hipSuccess) e = hipEventRecord(ctx->upEv[t][b], ctx->upStream[t]);
+        }
+        const hipError_t e2 = hipStreamSynchronize(ctx->upStream[t]);
+        errs[t] = e != hipSuccess ? e : e2;
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(T);
+    try{
+        for(int t = 1; t < T; t++) pool.emplace_back(work, t);
+    }catch(...){
+        for(auto& th : pool) th.join();
+        return ctx->fail(CARE_ENOMEM, "could not start the upload threads");
+    }
+    work(0);
+    for(auto& th : pool) th.join();
+    for(int t = 0; t < T; t++){
+        if(errs[t] != hipSuccess) return ctx->fail(CARE_EHIP, "upload failed: %s", hipGetErrorString(errs[t]));
+    }
+    return CARE_OK;
+}
+
+// Linear host-to-device copy, complete on the device when it returns: small ones as one
+// copy on `st`, large ones through the upload workers.
+int uploadH2D(care_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st){
+    if(bytes < kUpMin){
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return CARE_OK;
+    }
+    return uploadRowsH2D(ctx, dst, bytes, src, bytes, 1, st);
+}
+
 int configureReads(care_ctx* ctx, uint64_t num_reads, int maxLen, size_t pitch_ints, int minLen){
     if(num_reads == 0) return ctx->fail(CARE_EINVAL, "empty read storage");
     if(num_reads >= 0xFFFFFFFFull) return ctx->fail(CARE_EINVAL, "read ids are 32-bit");
@@ -4688,7 +4786,10 @@ static int care_hip_set_reads_body(care_ctx* ctx, const uint32_t* enc2bit, size_
     }
     hipStream_t st = ctx->stream;
     HIPCHECK(ctx->enc.ensure(num_reads * pitch_ints * 4));
-    HIPCHECK(hipMemcpyAsync(ctx->enc.p, enc2bit, num_reads * pitch_ints * 4, hipMemcpyHostToDevice, st));
+    {
+        const int rc = uploadH2D(ctx, ctx->enc.p, enc2bit, num_reads * pitch_ints * 4, st);
+        if(rc != CARE_OK) return rc;
+    }
     HIPCHECK(ctx->len.ensure(num_reads * 4));
     HIPCHECK(hipMemcpyAsync(ctx->len.p, lengths, num_reads * 4, hipMemcpyHostToDevice, st));
     HIPCHECK(ctx->ambig.ensure(num_reads));
@@ -4700,13 +4801,19 @@ static int care_hip_set_reads_body(care_ctx* ctx, const uint32_t* enc2bit, size_
         ctx->qdpitch = (qual_pitch + 15) / 16 * 16;
         HIPCHECK(ctx->qual.ensure(num_reads * ctx->qdpitch));
         if(ctx->qdpitch == qual_pitch){
-            HIPCHECK(hipMemcpyAsync(ctx->qual.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
+            const int rc = uploadH2D(ctx, ctx->qual.p, quals, num_reads * qual_pitch, st);
+            if(rc != CARE_OK) return rc;
+        }else if(num_reads * qual_pitch >= kUpMin && ctx->qdpitch <= kUpChunk){
+            // rows repacked to the 16-byte pitch by the upload workers
+            const int rc = uploadRowsH2D(ctx, ctx->qual.p, ctx->qdpitch, quals, qual_pitch, num_reads, st);
+            if(rc != CARE_OK) return rc;
         }else{
             // one linear copy, then rows repacked to the 16-byte pitch on the device
             struct Staging { DBuf b; ~Staging(){ b.release(); } } hold;
             DBuf& staging = hold.b;
             HIPCHECK(staging.ensure(num_reads * qual_pitch));
-            HIPCHECK(hipMemcpyAsync(staging.p, quals, num_reads * qual_pitch, hipMemcpyHostToDevice, st));
+            const int rc = uploadH2D(ctx, staging.p, quals, num_reads * qual_pitch, st);
+            if(rc != CARE_OK) return rc;
             const uint64_t total = num_reads * ctx->qdpitch;
             k_repack_rows<<<gridFor(total, 256), 256, 0, st>>>(staging.as<uint8_t>(), qual_pitch,
                                                                               ctx->qual.as<uint8_t>(), ctx->qdpitch, total);
@@ -4978,7 +5085,10 @@ static int care_hip_set_reads_fastq_body(care_ctx* ctx, const char* fastq1, size
         if(raw / kNlPerBlock >= size_t(INT_MAX) - 1) return ctx->fail(CARE_EINVAL, "FASTQ text too large for one call");
         ctx->fqRaw[slot] = raw;
         HIPCHECK(ctx->fqText[slot].ensure((raw + 15) / 16 * 16 + 16));
-        if(raw) HIPCHECK(hipMemcpyAsync(ctx->fqText[slot].p, slot ? fastq2 : fastq1, raw, hipMemcpyHostToDevice, st));
+        if(raw){
+            const int rc = uploadH2D(ctx, ctx->fqText[slot].p, slot ? fastq2 : fastq1, raw, st);
+            if(rc != CARE_OK) return rc;
+        }
     }
     HIPCHECK(hipEventRecord(ctx->ev[6], st));
     ParsedFile f1, f2;
