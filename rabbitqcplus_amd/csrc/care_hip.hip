@@ -3255,7 +3255,7 @@ struct care_ctx {
     // large host-to-device uploads: per worker a stream, two pinned chunk buffers and their
     // events (created on the first large upload)
     hipStream_t upStream[kUpWorkers] = {};
-    void* upBuf[kUpWorkers][2] = {};
+    void* upPinned = nullptr;                   // kUpWorkers * 2 chunk buffers
     hipEvent_t upEv[kUpWorkers][2] = {};
     double candPerAnchor = 0;
     double pipePerAnchor = 0;   // candidate pairs per anchor the arrays were last sized for
@@ -4541,12 +4541,10 @@ void care_hip_destroy(care_ctx* ctx){
     for(auto& row : ctx->pipeEv) for(auto& e : row) if(e) (void)hipEventDestroy(e);
     if(ctx->copyStream) (void)hipStreamDestroy(ctx->copyStream);
     for(int t = 0; t < kUpWorkers; t++){
-        for(int b = 0; b < 2; b++){
-            if(ctx->upEv[t][b]) (void)hipEventDestroy(ctx->upEv[t][b]);
-            if(ctx->upBuf[t][b]) (void)hipHostFree(ctx->upBuf[t][b]);
-        }
+        for(int b = 0; b < 2; b++) if(ctx->upEv[t][b]) (void)hipEventDestroy(ctx->upEv[t][b]);
         if(ctx->upStream[t]) (void)hipStreamDestroy(ctx->upStream[t]);
     }
+    if(ctx->upPinned) (void)hipHostFree(ctx->upPinned);
     if(ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -4581,11 +4579,11 @@ int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size
     const bool linear = rows == 1 && dpitch == spitch;
     const size_t bytes = size_t(rows) * dpitch;
     const int T = int(std::min<unsigned>(kUpWorkers, std::max(1u, std::thread::hardware_concurrency())));
+    if(!ctx->upPinned) HIPCHECK(hipHostMalloc(&ctx->upPinned, size_t(kUpWorkers) * 2 * kUpChunk, hipHostMallocDefault));
     for(int t = 0; t < T; t++){
         if(!ctx->upStream[t]) HIPCHECK(hipStreamCreateWithFlags(&ctx->upStream[t], hipStreamNonBlocking));
         for(int b = 0; b < 2; b++){
             if(!ctx->upEv[t][b]) HIPCHECK(hipEventCreateWithFlags(&ctx->upEv[t][b], hipEventDisableTiming));
-            if(!ctx->upBuf[t][b]) HIPCHECK(hipHostMalloc(&ctx->upBuf[t][b], kUpChunk, hipHostMallocDefault));
         }
     }
     // work units: bytes of a linear copy, rows of a pitched one (whole rows per chunk)
@@ -4602,7 +4600,7 @@ int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size
             const uint64_t n = std::min(perChunk, end - u);
             if(k >= 2) e = hipEventSynchronize(ctx->upEv[t][b]);     // that buffer's last transfer
             if(e != hipSuccess) break;
-            char* buf = static_cast<char*>(ctx->upBuf[t][b]);
+            char* buf = static_cast<char*>(ctx->upPinned) + (size_t(t) * 2 + size_t(b)) * kUpChunk;
             const char* s = static_cast<const char*>(src);
             size_t len, off;
             if(linear){
