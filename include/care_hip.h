@@ -141,7 +141,9 @@ const char* care_hip_last_error(const care_ctx* ctx);
  * reverse-complement candidates take the row of that width reversed, as the reference
  * does with its storage pitch, src/correct_cpu.cpp:141, include/corrector.hpp:1227-1239;
  * the reference's storage holds zeros there), and one ambiguity flag per read
- * (1 = the read contained non-ACGT bases before replacement). Copied to the device. */
+ * (1 = the read contained non-ACGT bases before replacement). Copied to the device before
+ * the call returns; copies of 64 MiB or more run on up to 8 host threads of the library
+ * through the context's 64 MiB of pinned buffers (the caller's arrays are only read). */
 int care_hip_set_reads(care_ctx* ctx, const uint32_t* enc2bit, size_t pitch_ints,
                        const int32_t* lengths, const uint8_t* quals, size_t qual_pitch,
                        uint64_t num_reads, const uint8_t* ambiguous);
