@@ -13,7 +13,8 @@ from a 50 Mbp genome with 10-copy repeat families (k = 20, 48 maps, qualities on
 Prints one JSON line. Legs beside the headline (plain path, bit-exact to the reference):
   learned               the learned form: the MLP classifier seam on seeded alignment
   learned_exact         the classifier seam on the reference's shift scan
-  pe_workload           2.0M read pairs with 20-copy repeats: plain and learned
+  pe_workload           2.0M read pairs with 20-copy repeats: its own autotuner sweep, plain
+                        and learned
   fastq_io              FASTQ text -> device read storage, records -> corrected FASTQ text
   candidate_correction  --candidateCorrection passes
   cpu_baseline          the reference engine compiled from source, on every host core,
@@ -459,8 +460,18 @@ def main():
         sp = sim.workload(args.pe_workload, seed=args.seed)
         rp = reads.storage_from_sim(sp, use_qualities=use_q)
         po = dict(use_quality_scores=int(use_q), paired=1, kmerlength=kmer_length(rp), engine=1)
+        ptuned = None
+        if not args.no_autotune and args.engine == "auto":
+            # the paired workload's own sweep (its shape key differs: pairs, 150 bp, k)
+            ptuned = autotune.tune(rp, sample=1 << 19, log=log, cache=None,
+                                   **{k: v for k, v in po.items() if k != "engine"})
+            po.update(ptuned["choice"])
+            log(f"pe autotune choice {ptuned['choice']}")
         pms, prec, pdata, pt = run_leg(rp, po, args.steps, args.warmup, log, "pe plain")
-        pe_line = {"workload": args.pe_workload, "reads": rp.n, "plain": {
+        pe_line = {"workload": args.pe_workload, "reads": rp.n,
+                   "schedule": {k: po[k] for k in ("engine", "waves_per_cu", "align_variant", "msa_tile", "use_graphs",
+                                                   "sync_batches") if k in po},
+                   "autotune": ptuned["timings_ms"] if ptuned else None, "plain": {
             "value": round(rp.n / (pms / 1e3), 1), "ms_per_step": round(pms, 3), "records": prec,
             "pairs_aligned": int(pt["candidates"]), "bucket_values": int(pt["bucket_values"])}}
         pe_line["plain"].update(error_line(sp, rp, pdata))
