@@ -436,7 +436,7 @@ def main():
             ta = time.perf_counter()
             eng.set_reads_fastq(*texts)
             tb = time.perf_counter()
-            outs = [eng.write_fastq(data, f) for f in range(len(texts))]
+            outs = [eng.write_fastq(data, f, as_array=True) for f in range(len(texts))]
             tc = time.perf_counter()
             same_storage = bool(np.array_equal(eng.inspect_reads()["enc"], rs.enc))
             tio = eng.timings()

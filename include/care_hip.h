@@ -189,7 +189,9 @@ int care_hip_set_reads_fastq(care_ctx* ctx, const char* fastq1, size_t bytes1,
  * (outputCorrectionQualityLabels) to the headers of reads up to the last corrected one.
  * With no records the input text is returned unchanged (the reference then hands QC
  * nothing and QC reads the input itself, src/seqc.cpp:662-672). *written = size of the
- * text; dst may be null to ask for the size; otherwise cap must be at least the size. */
+ * text; dst may be null to ask for the size; otherwise cap must be at least the size.
+ * Texts of 64 MiB or more are copied into dst by up to 8 library threads (the
+ * context's pinned transfer buffers, as in care_hip_set_reads). */
 int care_hip_write_fastq(care_ctx* ctx, const uint8_t* records, size_t record_bytes, int file,
                          int quality_labels, char* dst, size_t cap, size_t* written);
 

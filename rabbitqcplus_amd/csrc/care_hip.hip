@@ -4573,18 +4573,47 @@ const char* care_hip_last_error(const care_ctx* ctx){
 // transfers them on its own stream, so the host copies of the workers and their transfers
 // overlap. Work already enqueued on `st` finishes first (it may still read the
 // destination). A linear copy is one row of `bytes` with both pitches equal to it.
-int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size_t spitch, uint64_t rows,
-                  hipStream_t st){
-    HIPCHECK(hipStreamSynchronize(st));
-    const bool linear = rows == 1 && dpitch == spitch;
-    const size_t bytes = size_t(rows) * dpitch;
-    const int T = int(std::min<unsigned>(kUpWorkers, std::max(1u, std::thread::hardware_concurrency())));
+// The transfer workers' streams, events and pinned chunk buffers (made on first use);
+// *T = the number of workers.
+static int transferWorkers(care_ctx* ctx, int* T){
+    *T = int(std::min<unsigned>(kUpWorkers, std::max(1u, std::thread::hardware_concurrency())));
     if(!ctx->upPinned) HIPCHECK(hipHostMalloc(&ctx->upPinned, size_t(kUpWorkers) * 2 * kUpChunk, hipHostMallocDefault));
-    for(int t = 0; t < T; t++){
+    for(int t = 0; t < *T; t++){
         if(!ctx->upStream[t]) HIPCHECK(hipStreamCreateWithFlags(&ctx->upStream[t], hipStreamNonBlocking));
         for(int b = 0; b < 2; b++){
             if(!ctx->upEv[t][b]) HIPCHECK(hipEventCreateWithFlags(&ctx->upEv[t][b], hipEventDisableTiming));
         }
+    }
+    return CARE_OK;
+}
+
+// Runs work(t) for t in [0, T) on T host threads (t = 0 on the calling thread).
+extern "C++" {
+template<class F>
+static int onTransferThreads(care_ctx* ctx, int T, F&& work){
+    std::vector<std::thread> pool;
+    pool.reserve(T);
+    try{
+        for(int t = 1; t < T; t++) pool.emplace_back(work, t);
+    }catch(...){
+        for(auto& th : pool) th.join();
+        return ctx->fail(CARE_ENOMEM, "could not start the transfer threads");
+    }
+    work(0);
+    for(auto& th : pool) th.join();
+    return CARE_OK;
+}
+}  // extern "C++"
+
+static int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size_t spitch, uint64_t rows,
+                  hipStream_t st){
+    HIPCHECK(hipStreamSynchronize(st));
+    const bool linear = rows == 1 && dpitch == spitch;
+    const size_t bytes = size_t(rows) * dpitch;
+    int T = 0;
+    {
+        const int rc = transferWorkers(ctx, &T);
+        if(rc != CARE_OK) return rc;
     }
     // work units: bytes of a linear copy, rows of a pitched one (whole rows per chunk)
     const uint64_t units = linear ? bytes : rows;
@@ -4619,25 +4648,76 @@ int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size
         const hipError_t e2 = hipStreamSynchronize(ctx->upStream[t]);
         errs[t] = e != hipSuccess ? e : e2;
     };
-    std::vector<std::thread> pool;
-    pool.reserve(T);
-    try{
-        for(int t = 1; t < T; t++) pool.emplace_back(work, t);
-    }catch(...){
-        for(auto& th : pool) th.join();
-        return ctx->fail(CARE_ENOMEM, "could not start the upload threads");
+    {
+        const int rc = onTransferThreads(ctx, T, work);
+        if(rc != CARE_OK) return rc;
     }
-    work(0);
-    for(auto& th : pool) th.join();
     for(int t = 0; t < T; t++){
         if(errs[t] != hipSuccess) return ctx->fail(CARE_EHIP, "upload failed: %s", hipGetErrorString(errs[t]));
     }
     return CARE_OK;
 }
 
+// Device-to-host copy into a (pageable) host buffer, complete when it returns. Large
+// copies are cut into one slice per transfer worker; a worker moves its slice through its
+// two pinned chunk buffers on its own stream, copying one chunk out while the next is in
+// flight. Work enqueued on `st` before the call (which may write the source) finishes first.
+static int downloadD2H(care_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st){
+    if(bytes < kUpMin){
+        HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return CARE_OK;
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    int T = 0;
+    {
+        const int rc = transferWorkers(ctx, &T);
+        if(rc != CARE_OK) return rc;
+    }
+    const size_t slice = (bytes + T - 1) / T;
+    std::vector<hipError_t> errs(T, hipSuccess);
+    auto work = [&](int t){
+        hipError_t e = hipSetDevice(ctx->device);
+        const size_t beg = std::min(bytes, size_t(t) * slice), end = std::min(bytes, beg + slice);
+        int pb = -1;                                  // buffer of the chunk in flight
+        size_t poff = 0, plen = 0;
+        int k = 0;
+        for(size_t off = beg; e == hipSuccess && off < end; off += kUpChunk, k++){
+            const int b = k & 1;
+            const size_t n = std::min(kUpChunk, end - off);
+            char* buf = static_cast<char*>(ctx->upPinned) + (size_t(t) * 2 + size_t(b)) * kUpChunk;
+            e = hipMemcpyAsync(buf, static_cast<const char*>(src) + off, n, hipMemcpyDeviceToHost, ctx->upStream[t]);
+            if(e == hipSuccess) e = hipEventRecord(ctx->upEv[t][b], ctx->upStream[t]);
+            if(e == hipSuccess && pb >= 0){
+                e = hipEventSynchronize(ctx->upEv[t][pb]);
+                if(e == hipSuccess)
+                    std::memcpy(static_cast<char*>(dst) + poff,
+                                static_cast<char*>(ctx->upPinned) + (size_t(t) * 2 + size_t(pb)) * kUpChunk, plen);
+            }
+            pb = b; poff = off; plen = n;
+        }
+        if(e == hipSuccess && pb >= 0){
+            e = hipEventSynchronize(ctx->upEv[t][pb]);
+            if(e == hipSuccess)
+                std::memcpy(static_cast<char*>(dst) + poff,
+                            static_cast<char*>(ctx->upPinned) + (size_t(t) * 2 + size_t(pb)) * kUpChunk, plen);
+        }
+        const hipError_t e2 = hipStreamSynchronize(ctx->upStream[t]);
+        errs[t] = e != hipSuccess ? e : e2;
+    };
+    {
+        const int rc = onTransferThreads(ctx, T, work);
+        if(rc != CARE_OK) return rc;
+    }
+    for(int t = 0; t < T; t++){
+        if(errs[t] != hipSuccess) return ctx->fail(CARE_EHIP, "download failed: %s", hipGetErrorString(errs[t]));
+    }
+    return CARE_OK;
+}
+
 // Linear host-to-device copy, complete on the device when it returns: small ones as one
 // copy on `st`, large ones through the upload workers.
-int uploadH2D(care_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st){
+static int uploadH2D(care_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st){
     if(bytes < kUpMin){
         HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st));
         HIPCHECK(hipStreamSynchronize(st));
@@ -5402,8 +5482,7 @@ static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size
         *written = raw;
         if(!dst) return CARE_OK;
         if(cap < raw) return ctx->fail(CARE_EINVAL, "output needs %zu bytes, cap is %zu", raw, cap);
-        if(raw) HIPCHECK(hipMemcpy(dst, ctx->fqText[file].p, raw, hipMemcpyDeviceToHost));
-        return CARE_OK;
+        return raw ? downloadD2H(ctx, dst, ctx->fqText[file].p, raw, ctx->stream) : CARE_OK;
     }
     if(m64 >= uint64_t(INT_MAX)) return ctx->fail(CARE_EINVAL, "too many records");
     const uint32_t m = uint32_t(m64);
@@ -5474,8 +5553,10 @@ static int care_hip_write_fastq_body(care_ctx* ctx, const uint8_t* records, size
     k_merge_write<<<gridFor(count * 64, 256), 256, 0, st>>>(mp);
     HIPCHECK(hipGetLastError());
     HIPCHECK(hipEventRecord(ctx->ev[7], st));
-    HIPCHECK(hipMemcpyAsync(dst, ctx->mText.p, total, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    {
+        const int rc = downloadD2H(ctx, dst, ctx->mText.p, total, st);
+        if(rc != CARE_OK) return rc;
+    }
     float ms = 0;
     HIPCHECK(hipEventElapsedTime(&ms, ctx->ev[6], ctx->ev[7]));
     ctx->mergeMs = ms;

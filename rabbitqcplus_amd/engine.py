@@ -221,9 +221,10 @@ class CareEngine:
         self._check(rc, "care_hip_set_reads_fastq")
         return self.read_layout()[0]
 
-    def write_fastq(self, records: bytes, file: int = 0, quality_labels: bool = False) -> bytes:
+    def write_fastq(self, records: bytes, file: int = 0, quality_labels: bool = False, as_array: bool = False):
         """Corrected FASTQ text of input file `file` from serialized records (output merge
-        on the device; needs reads set by set_reads_fastq)."""
+        on the device; needs reads set by set_reads_fastq): bytes, or with as_array a uint8
+        array holding the text without a further copy (to write to a file, say)."""
         rec = np.frombuffer(records, np.uint8) if len(records) else np.zeros(1, np.uint8)
         size = ctypes.c_size_t(0)
         # every read's output is at most its input text, a label and its records' bytes
@@ -238,7 +239,7 @@ class CareEngine:
         out = np.empty(max(bound, 1), np.uint8)
         self._check(self.lib.care_hip_write_fastq(self.h, rec.ctypes.data, len(records), file, int(quality_labels),
                                                   out.ctypes.data, out.size, ctypes.byref(size)), "care_hip_write_fastq")
-        return out[:size.value].tobytes()
+        return out[:size.value] if as_array else out[:size.value].tobytes()
 
     def save_reads(self, path: str) -> None:
         """Writes the resident read storage as the reference's preprocessed-reads file."""
