@@ -4607,8 +4607,10 @@ static int onTransferThreads(care_ctx* ctx, int T, F&& work){
 
 static int uploadRowsH2D(care_ctx* ctx, void* dst, size_t dpitch, const void* src, size_t spitch, uint64_t rows,
                   hipStream_t st){
-    HIPCHECK(hipStreamSynchronize(st));
     const bool linear = rows == 1 && dpitch == spitch;
+    if(!linear && (dpitch > kUpChunk || spitch > dpitch))
+        return ctx->fail(CARE_EINVAL, "upload rows of %zu bytes at pitch %zu do not fit a transfer chunk", spitch, dpitch);
+    HIPCHECK(hipStreamSynchronize(st));
     const size_t bytes = size_t(rows) * dpitch;
     int T = 0;
     {
