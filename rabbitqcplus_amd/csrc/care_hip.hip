@@ -1506,6 +1506,7 @@ __host__ __device__ inline int tileRowWords(int maxLen){ return (maxLen + 15) >>
 // the slot of qStride(maxLen) bytes holds the whole 16-byte-chunked row).
 __device__ __forceinline__ int rcQualityOffset(const MsaParams& p){ return qStride(p.maxLen) - int(p.qpitch); }
 
+template<int G>
 __device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile& t, int tn, int tid, int nthreads);
 
 // Stages candidates [tb, tb+tn) of the working list into the LDS tile: one thread per
@@ -1522,19 +1523,47 @@ __device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile
 // vec holds (W even, RW and the enc pitch multiples of 4 words: the enc rows, the
 // reverse-complement rows and the tile slots start on 16-byte boundaries) and the group of
 // four words lies inside the row's RW words.
+// Loads go out in groups (G 16-byte or 4G 4-byte loads) before their stores, so a group
+// costs one memory round trip (G = 1 where the registers of a larger group would spill).
+template<int G>
 __device__ __forceinline__ void stageRow(const uint32_t* __restrict__ src, uint32_t* dst, int len, int RW, bool varLen,
                                          bool vec){
-    if(!varLen){
-        for(int k = 0; k < RW; k++) dst[k] = src[k];
+    if constexpr(G == 1){
+        if(!varLen){
+            for(int k = 0; k < RW; k++) dst[k] = src[k];
+            return;
+        }
+        const int nw = min((len + 15) >> 4, RW);
+        int k = 0;
+        if(vec){
+            for(; k < nw && k + 4 <= RW; k += 4)
+                *reinterpret_cast<uint4*>(dst + k) = *reinterpret_cast<const uint4*>(src + k);
+        }
+        for(; k < nw; k++) dst[k] = src[k];
         return;
     }
-    const int nw = min((len + 15) >> 4, RW);
-    int k = 0;
+    const int nw = varLen ? min((len + 15) >> 4, RW) : RW;
     if(vec){
-        for(; k < nw && k + 4 <= RW; k += 4)
-            *reinterpret_cast<uint4*>(dst + k) = *reinterpret_cast<const uint4*>(src + k);
+        // (RW is a multiple of 4: the 16-byte groups cover the nw words and stay in the row)
+        const uint4* s4 = reinterpret_cast<const uint4*>(src);
+        uint4* d4 = reinterpret_cast<uint4*>(dst);
+        const int nv = (nw + 3) >> 2;
+        for(int g = 0; g < nv; g += G){
+            uint4 r[G];
+            #pragma unroll
+            for(int i = 0; i < G; i++) if(g + i < nv) r[i] = s4[g + i];
+            #pragma unroll
+            for(int i = 0; i < G; i++) if(g + i < nv) d4[g + i] = r[i];
+        }
+        return;
     }
-    for(; k < nw; k++) dst[k] = src[k];
+    for(int g = 0; g < nw; g += 4 * G){
+        uint32_t r[4 * G];
+        #pragma unroll
+        for(int i = 0; i < 4 * G; i++) if(g + i < nw) r[i] = src[g + i];
+        #pragma unroll
+        for(int i = 0; i < 4 * G; i++) if(g + i < nw) dst[g + i] = r[i];
+    }
 }
 
 // Whether 16-byte chunk c of a candidate's quality row holds a byte the MSA reads: a
@@ -1544,6 +1573,7 @@ __device__ __forceinline__ bool qualityChunkUsed(int c, int len, bool rc, int qp
     return rc ? (16 * c + 16 > qpitch - len && 16 * c < qpitch) : 16 * c < len;
 }
 
+template<int G>
 __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists& lst, const MsaTile& t, int beg,
                                           int tb, int tn, int tid, int nthreads){
     const int W = p.W;
@@ -1559,33 +1589,68 @@ __device__ __forceinline__ void stageTile(const MsaParams& p, const AnchorLists&
         // the oriented row (the enc pitch holds at least RW words)
         const uint32_t* src = orient == 1 ? p.enc + p.pitch * id : p.rcenc + uint64_t(id) * uint64_t(2 * W);
         const int RW = tileRowWords(p.maxLen);
-        stageRow(src, t.rows + q * 2 * W, t.len[q], RW, p.varLen != 0, (W & 1) == 0 && (RW & 3) == 0 && (p.pitch & 3) == 0);
+        stageRow<G>(src, t.rows + q * 2 * W, t.len[q], RW, p.varLen != 0, (W & 1) == 0 && (RW & 3) == 0 && (p.pitch & 3) == 0);
     }
     if(!p.useQ) return;
     __syncthreads();
-    stageQualities(p, t, tn, tid, nthreads);
+    stageQualities<G>(p, t, tn, tid, nthreads);
 }
 
 // Quality rows of the tile's tn candidates (ids and orientations already in the tile).
+template<int G>
 __device__ __forceinline__ void stageQualities(const MsaParams& p, const MsaTile& t, int tn, int tid, int nthreads){
     const int qs = qStride(p.maxLen);
     const int nc = qs >> 4;              // chunks per row; device rows are qdpitch >= qs bytes
     // u / nc as a multiply-high: exact for u * nc < 2^32
     const uint64_t magic = ((uint64_t(1) << 32) + uint64_t(nc) - 1) / uint64_t(nc);
-    for(int u = tid; u < tn * nc; u += nthreads){
-        const int j = int((uint64_t(u) * magic) >> 32);
-        const int c = u - j * nc;
-        if(p.varLen && !qualityChunkUsed(c, t.len[j], t.orient[j] != 1, int(p.qpitch))) continue;
-        const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * t.id[j] + 16 * c);
-        uint8_t* dst = t.q + j * qs;
-        if(t.orient[j] == 1){
-            *reinterpret_cast<uint4*>(dst + 16 * c) = v;
-        }else{
-            // byte s of the row goes to qs - 1 - s
-            uint4 r;
-            r.x = __builtin_bswap32(v.w); r.y = __builtin_bswap32(v.z);
-            r.z = __builtin_bswap32(v.y); r.w = __builtin_bswap32(v.x);
-            *reinterpret_cast<uint4*>(dst + qs - 16 * (c + 1)) = r;
+    if constexpr(G == 1){
+        for(int u = tid; u < tn * nc; u += nthreads){
+            const int j = int((uint64_t(u) * magic) >> 32);
+            const int c = u - j * nc;
+            if(p.varLen && !qualityChunkUsed(c, t.len[j], t.orient[j] != 1, int(p.qpitch))) continue;
+            const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * t.id[j] + 16 * c);
+            uint8_t* dst = t.q + j * qs;
+            if(t.orient[j] == 1){
+                *reinterpret_cast<uint4*>(dst + 16 * c) = v;
+            }else{
+                // byte s of the row goes to qs - 1 - s
+                uint4 r;
+                r.x = __builtin_bswap32(v.w); r.y = __builtin_bswap32(v.z);
+                r.z = __builtin_bswap32(v.y); r.w = __builtin_bswap32(v.x);
+                *reinterpret_cast<uint4*>(dst + qs - 16 * (c + 1)) = r;
+            }
+        }
+        return;
+    }
+    // G chunks per thread in flight before their stores
+    for(int base = tid; base < tn * nc; base += G * nthreads){
+        uint4 v[G];
+        int doff[G];           // byte offset of the chunk's slot in t.q, or -1
+        bool rcv[G];
+        #pragma unroll
+        for(int i = 0; i < G; i++){
+            const int u = base + i * nthreads;
+            doff[i] = -1;
+            rcv[i] = false;
+            v[i] = make_uint4(0u, 0u, 0u, 0u);
+            if(u >= tn * nc) continue;
+            const int j = int((uint64_t(u) * magic) >> 32);
+            const int c = u - j * nc;
+            if(p.varLen && !qualityChunkUsed(c, t.len[j], t.orient[j] != 1, int(p.qpitch))) continue;
+            rcv[i] = t.orient[j] != 1;
+            // a reverse-complement chunk: byte s of the row goes to qs - 1 - s
+            doff[i] = j * qs + (rcv[i] ? qs - 16 * (c + 1) : 16 * c);
+            v[i] = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * t.id[j] + 16 * c);
+        }
+        #pragma unroll
+        for(int i = 0; i < G; i++){
+            if(doff[i] < 0) continue;
+            uint4 r = v[i];
+            if(rcv[i]){
+                r.x = __builtin_bswap32(v[i].w); r.y = __builtin_bswap32(v[i].z);
+                r.z = __builtin_bswap32(v[i].y); r.w = __builtin_bswap32(v[i].x);
+            }
+            *reinterpret_cast<uint4*>(t.q + doff[i]) = r;
         }
     }
 }
@@ -1602,6 +1667,7 @@ struct LaneCand {
 // its lane (ds_bpermute, every lane active), so the row and quality loads are independent.
 // Same rows and qualities as stageTile; the caller reads shift, length and weight from the
 // lanes (the tile's metadata arrays are not written).
+template<int G>
 __device__ __forceinline__ void stageTileLanes(const MsaParams& p, const MsaTile& t, const LaneCand& c, int tb, int tn,
                                                int lane){
     const int W = p.W;
@@ -1609,33 +1675,67 @@ __device__ __forceinline__ void stageTileLanes(const MsaParams& p, const MsaTile
     if(q >= 0 && q < tn){
         const uint32_t* src = c.orient == 1 ? p.enc + p.pitch * c.id : p.rcenc + uint64_t(c.id) * uint64_t(2 * W);
         const int RW = tileRowWords(p.maxLen);
-        stageRow(src, t.rows + q * 2 * W, c.len, RW, p.varLen != 0, (W & 1) == 0 && (RW & 3) == 0 && (p.pitch & 3) == 0);
+        stageRow<G>(src, t.rows + q * 2 * W, c.len, RW, p.varLen != 0, (W & 1) == 0 && (RW & 3) == 0 && (p.pitch & 3) == 0);
     }
     if(!p.useQ) return;
     const int qs = qStride(p.maxLen);
     const int nc = qs >> 4;              // chunks per row; device rows are qdpitch >= qs bytes
     const uint64_t magic = ((uint64_t(1) << 32) + uint64_t(nc) - 1) / uint64_t(nc);   // u / nc, exact here
     const int total = tn * nc;
-    for(int base = 0; base < total; base += kWave){
-        const int u = base + lane;
-        const int j = min(int((uint64_t(u) * magic) >> 32), tn - 1);
-        const uint32_t id = uint32_t(__builtin_amdgcn_ds_bpermute(4 * (tb + j), int(c.id)));
-        const int orient = __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.orient);
-        // reads of one length use every chunk (fixed length: no test, no third permute)
-        const int len = p.varLen ? __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.len) : int(p.qpitch);
-        if(u < total && (!p.varLen || qualityChunkUsed(u - j * nc, len, orient != 1, int(p.qpitch)))){
-            const int ch = u - j * nc;
-            const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * id + 16 * ch);
-            uint8_t* dst = t.q + j * qs;
-            if(orient == 1){
-                *reinterpret_cast<uint4*>(dst + 16 * ch) = v;
-            }else{
-                // byte s of the row goes to qs - 1 - s
-                uint4 r;
-                r.x = __builtin_bswap32(v.w); r.y = __builtin_bswap32(v.z);
-                r.z = __builtin_bswap32(v.y); r.w = __builtin_bswap32(v.x);
-                *reinterpret_cast<uint4*>(dst + qs - 16 * (ch + 1)) = r;
+    if constexpr(G == 1){
+        for(int base = 0; base < total; base += kWave){
+            const int u = base + lane;
+            const int j = min(int((uint64_t(u) * magic) >> 32), tn - 1);
+            const uint32_t id = uint32_t(__builtin_amdgcn_ds_bpermute(4 * (tb + j), int(c.id)));
+            const int orient = __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.orient);
+            // reads of one length use every chunk (fixed length: no test, no third permute)
+            const int len = p.varLen ? __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.len) : int(p.qpitch);
+            if(u < total && (!p.varLen || qualityChunkUsed(u - j * nc, len, orient != 1, int(p.qpitch)))){
+                const int ch = u - j * nc;
+                const uint4 v = *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * id + 16 * ch);
+                uint8_t* dst = t.q + j * qs;
+                if(orient == 1){
+                    *reinterpret_cast<uint4*>(dst + 16 * ch) = v;
+                }else{
+                    // byte s of the row goes to qs - 1 - s
+                    uint4 r;
+                    r.x = __builtin_bswap32(v.w); r.y = __builtin_bswap32(v.z);
+                    r.z = __builtin_bswap32(v.y); r.w = __builtin_bswap32(v.x);
+                    *reinterpret_cast<uint4*>(dst + qs - 16 * (ch + 1)) = r;
+                }
             }
+        }
+        return;
+    }
+    // G chunks per lane in flight before their stores (one memory round trip per group)
+    for(int base = 0; base < total; base += G * kWave){
+        uint4 v[G];
+        int doff[G];           // byte offset of the chunk's slot in t.q, or -1
+        bool rcv[G];
+        #pragma unroll
+        for(int i = 0; i < G; i++){
+            const int u = base + i * kWave + lane;
+            const int j = min(int((uint64_t(u) * magic) >> 32), tn - 1);
+            const uint32_t id = uint32_t(__builtin_amdgcn_ds_bpermute(4 * (tb + j), int(c.id)));
+            const int orient = __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.orient);
+            // reads of one length use every chunk (fixed length: no test, no third permute)
+            const int len = p.varLen ? __builtin_amdgcn_ds_bpermute(4 * (tb + j), c.len) : int(p.qpitch);
+            const int ch = u - j * nc;
+            const bool ok = u < total && (!p.varLen || qualityChunkUsed(ch, len, orient != 1, int(p.qpitch)));
+            rcv[i] = orient != 1;
+            // a reverse-complement chunk: byte s of the row goes to qs - 1 - s
+            doff[i] = ok ? j * qs + (rcv[i] ? qs - 16 * (ch + 1) : 16 * ch) : -1;
+            v[i] = ok ? *reinterpret_cast<const uint4*>(p.qual + p.qdpitch * id + 16 * ch) : make_uint4(0u, 0u, 0u, 0u);
+        }
+        #pragma unroll
+        for(int i = 0; i < G; i++){
+            if(doff[i] < 0) continue;
+            uint4 r = v[i];
+            if(rcv[i]){
+                r.x = __builtin_bswap32(v[i].w); r.y = __builtin_bswap32(v[i].z);
+                r.z = __builtin_bswap32(v[i].y); r.w = __builtin_bswap32(v[i].x);
+            }
+            *reinterpret_cast<uint4*>(t.q + doff[i]) = r;
         }
     }
 }
@@ -1682,7 +1782,7 @@ __device__ bool buildMsa(const MsaParams& p, MsaShared& sh, uint64_t a, int La, 
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();   // previous tile fully consumed
-        stageTile(p, listsOf(p), t, beg, tb, tn, threadIdx.x, kBlock);
+        stageTile<1>(p, listsOf(p), t, beg, tb, tn, threadIdx.x, kBlock);
         __syncthreads();
         for(int c = threadIdx.x; c < ncol; c += kBlock){
             int n0 = cnt[4 * c + 0], n1 = cnt[4 * c + 1], n2 = cnt[4 * c + 2], n3 = cnt[4 * c + 3];
@@ -2130,8 +2230,10 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();
-        if(inLanes) stageTileLanes(p, t, lc, tb, tn, lane);
-        else stageTile(p, lst, t, beg, tb, tn, lane, kWave);
+        // two loads in flight per lane where the column registers leave room (KC >= 4)
+        constexpr int G = KC >= 4 ? 2 : 1;
+        if(inLanes) stageTileLanes<G>(p, t, lc, tb, tn, lane);
+        else stageTile<G>(p, lst, t, beg, tb, tn, lane, kWave);
         __syncthreads();
         // lane src+q holds candidate tb+q's shift, length, weight and orientation; the loop
         // below reads them with readlane (no LDS round trip per candidate)
