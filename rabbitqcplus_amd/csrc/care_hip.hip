@@ -2230,8 +2230,9 @@ __device__ __forceinline__ bool buildMsaWave(const MsaParams& p, const AnchorLis
     for(int tb = 0; tb < nk; tb += T){
         const int tn = min(T, nk - tb);
         __syncthreads();
-        // two loads in flight per lane where the column registers leave room (KC >= 4)
-        constexpr int G = KC >= 4 ? 2 : 1;
+        // two loads in flight per lane from 3 column slots on (at 2 slots the 8-wave budget
+        // leaves no room)
+        constexpr int G = KC >= 3 ? 2 : 1;
         if(inLanes) stageTileLanes<G>(p, t, lc, tb, tn, lane);
         else stageTile<G>(p, lst, t, beg, tb, tn, lane, kWave);
         __syncthreads();
