@@ -195,15 +195,25 @@ struct OpOr  { __device__ int operator()(int a, int b) const { return a | b; } }
 // the first k-mer that attains the minimum, times 2, plus 1 when its canonical form is the
 // reverse complement (0 for reads shorter than k). Seeds place a shared k-mer in both reads
 // of a candidate pair (seeded alignment).
+// A work item is one read and kSigGroup consecutive maps (items group-major: consecutive
+// threads take consecutive reads of one group, so the signature stores coalesce): the
+// k-mers are walked once per group, and a thread holds kSigGroup minima instead of all
+// maps', which keeps the kernel at a high occupancy.
+constexpr int kSigGroup = 8;
+__host__ __device__ inline int sigGroups(int M){ return (M + kSigGroup - 1) / kSigGroup; }
+
 __global__ void __launch_bounds__(256)
 k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restrict__ len, uint64_t n,
              int k, int M, uint64_t* __restrict__ sig, uint16_t* __restrict__ seeds){
     const uint64_t mask = ~0ull >> ((32 - k) * 2);
-    FOR_GRID(r, n){
-    uint64_t mins[kMaxMaps];
-    uint32_t seed[kMaxMaps];
+    FOR_GRID(t, n * uint64_t(sigGroups(M))){
+    const uint64_t g = t / n;
+    const uint64_t r = t - g * n;
+    const int m0 = int(g) * kSigGroup;
+    uint64_t mins[kSigGroup];
+    uint32_t seed[kSigGroup];
     #pragma unroll
-    for(int m = 0; m < kMaxMaps; m++){ mins[m] = ~0ull; seed[m] = 0; }
+    for(int j = 0; j < kSigGroup; j++){ mins[j] = ~0ull; seed[j] = 0; }
     const int L = len[r];
     const uint32_t* row = enc + pitch * r;
     if(L >= k){
@@ -219,20 +229,21 @@ k_signatures(const uint32_t* __restrict__ enc, size_t pitch, const int* __restri
                 const uint64_t canon = fwd < rc ? fwd : rc;
                 const uint32_t here = (uint32_t(i - k + 1) << 1) | uint32_t(rc < fwd);
                 #pragma unroll
-                for(int m = 0; m < kMaxMaps; m++){
-                    if(m < M){
-                        const uint64_t h = fmix64(canon + uint64_t(m));
-                        if(h < mins[m]){ mins[m] = h; seed[m] = here; }
+                for(int j = 0; j < kSigGroup; j++){
+                    if(m0 + j < M){
+                        const uint64_t h = fmix64(canon + uint64_t(m0 + j));
+                        if(h < mins[j]){ mins[j] = h; seed[j] = here; }
                     }
                 }
             }
         }
     }
     #pragma unroll
-    for(int m = 0; m < kMaxMaps; m++){
+    for(int j = 0; j < kSigGroup; j++){
+        const int m = m0 + j;
         if(m < M){
-            sig[uint64_t(m) * n + r] = mins[m] & mask;
-            if(seeds) seeds[r * uint64_t(M) + uint64_t(m)] = uint16_t(seed[m]);
+            sig[uint64_t(m) * n + r] = mins[j] & mask;
+            if(seeds) seeds[r * uint64_t(M) + uint64_t(m)] = uint16_t(seed[j]);
         }
     }
     }
@@ -6024,8 +6035,9 @@ static int care_hip_build_tables_body(care_ctx* ctx, int* maps_built){
     hipStream_t st = ctx->stream;
     HIPCHECK(ctx->sig.ensure(ctx->n * ctx->M * 8));
     HIPCHECK(ctx->seeds.ensure(std::max<uint64_t>(1, ctx->n * ctx->M) * 2));
-    k_signatures<<<gridFor(ctx->n, 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
-                                                                   ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>(), ctx->seeds.as<uint16_t>());
+    k_signatures<<<gridFor(ctx->n * uint64_t(sigGroups(ctx->M)), 256), dim3(256), 0, st>>>(
+        ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(), ctx->n, ctx->k, ctx->M, ctx->sig.as<uint64_t>(),
+        ctx->seeds.as<uint16_t>());
     HIPCHECK(hipGetLastError());
     int rc = buildTables(ctx);
     if(rc != CARE_OK) return rc;
@@ -6131,8 +6143,9 @@ static int care_hip_load_tables_body(care_ctx* ctx, const char* path, int* maps_
     ctx->M = M;
     HIPCHECK(ctx->sig.ensure(ctx->n * M * 8));
     HIPCHECK(ctx->seeds.ensure(std::max<uint64_t>(1, ctx->n * M) * 2));
-    k_signatures<<<gridFor(ctx->n, 256), dim3(256), 0, st>>>(ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(),
-                                                                   ctx->n, ctx->k, M, ctx->sig.as<uint64_t>(), ctx->seeds.as<uint16_t>());
+    k_signatures<<<gridFor(ctx->n * uint64_t(sigGroups(M)), 256), dim3(256), 0, st>>>(
+        ctx->enc.as<uint32_t>(), ctx->pitch, ctx->len.as<int>(), ctx->n, ctx->k, M, ctx->sig.as<uint64_t>(),
+        ctx->seeds.as<uint16_t>());
     HIPCHECK(hipGetLastError());
     HIPCHECK(ctx->values.ensure(std::max<uint64_t>(1, ctx->n) * M * sizeof(uint32_t)));
     ctx->slotBufs.resize(M);
