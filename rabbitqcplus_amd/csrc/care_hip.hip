@@ -368,14 +368,30 @@ __device__ __forceinline__ int batchAnchor(const uint32_t* order, int B){
 // signature was inserted into every map, so the answer is the bucket the read belongs to
 // (or nothing when that bucket was not kept); resolving it once per table set turns each
 // pass's 48 signature loads and hash probes per anchor into one 384-byte read.
-__global__ void k_bucket_of(const uint64_t* __restrict__ sig, uint64_t n, int M, const Slot* const* slots,
-                            const uint32_t* __restrict__ caps, uint2* __restrict__ out){
-    FOR_GRID(t, n * uint64_t(M)){
-        const int m = int(t / n);
-        const uint64_t a = t - uint64_t(m) * n;
-        uint32_t off = 0;
-        const uint32_t cnt = lookup(slots[m], caps[m], sig[t], &off);
-        out[a * uint64_t(M) + uint64_t(m)] = make_uint2(cnt ? off : 0u, cnt);
+// A workgroup resolves kBucketTile reads at a time: lookups map by map (signatures of
+// consecutive reads are consecutive), results transposed in LDS, rows stored read-major
+// as one contiguous stretch.
+constexpr int kBucketTile = 64;
+__global__ void __launch_bounds__(256)
+k_bucket_of(const uint64_t* __restrict__ sig, uint64_t n, int M, const Slot* const* slots,
+            const uint32_t* __restrict__ caps, uint2* __restrict__ out){
+    __shared__ uint2 tile[kBucketTile * kMaxMaps];
+    const uint64_t ntiles = (n + kBucketTile - 1) / kBucketTile;
+    for(uint64_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x){
+        const uint64_t a0 = tb * kBucketTile;
+        const int rows = int(min(uint64_t(kBucketTile), n - a0));
+        for(int q = threadIdx.x; q < kBucketTile * M; q += blockDim.x){
+            const int m = q / kBucketTile;
+            const int i = q - m * kBucketTile;
+            if(i < rows){
+                uint32_t off = 0;
+                const uint32_t cnt = lookup(slots[m], caps[m], sig[uint64_t(m) * n + a0 + uint64_t(i)], &off);
+                tile[i * M + m] = make_uint2(cnt ? off : 0u, cnt);
+            }
+        }
+        __syncthreads();
+        for(int q = threadIdx.x; q < rows * M; q += blockDim.x) out[a0 * uint64_t(M) + uint64_t(q)] = tile[q];
+        __syncthreads();            // the tile is written again for the next reads
     }
 }
 
@@ -6003,7 +6019,7 @@ static int resolveBuckets(care_ctx* ctx){
     hipStream_t st = ctx->stream;
     const uint64_t total = ctx->n * uint64_t(ctx->M);
     HIPCHECK(ctx->bucketOf.ensure(total * sizeof(uint2)));
-    k_bucket_of<<<gridFor(total, 256), 256, 0, st>>>(ctx->sig.as<uint64_t>(), ctx->n, ctx->M,
+    k_bucket_of<<<gridFor((ctx->n + kBucketTile - 1) / kBucketTile, 1), 256, 0, st>>>(ctx->sig.as<uint64_t>(), ctx->n, ctx->M,
                                                                     ctx->slotPtrs.as<const Slot*>(),
                                                                     ctx->capsDev.as<uint32_t>(), ctx->bucketOf.as<uint2>());
     HIPCHECK(hipGetLastError());
